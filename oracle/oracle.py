@@ -1,0 +1,274 @@
+"""ctypes bindings for the CPU checkers (TEST INFRASTRUCTURE ONLY).
+
+* ``Oracle``   -- oracle/libmgcm_oracle.so, the C FP64 restatement (OpenMP batch).
+* ``Reference`` -- oracle/_ref/libmgcm_ref.so, the reference Fortran routines
+  compiled from their own sources by oracle/build_ref.sh (only where that
+  library was built).
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import this
+module; the product library never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "libmgcm_oracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libmgcm_ref.so")
+
+NV, NI, NG = 7, 5, 17
+DP = C.POINTER(C.c_double)
+
+
+def _p(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"], (a.dtype, a.flags)
+    return a.ctypes.data_as(DP)
+
+
+class _Tables(C.Structure):
+    _fields_ = [("nlay", C.c_int)] + [(n, DP) for n in (
+        "co2v", "co2i", "pfgasref", "tgasref", "wrefh2o", "gwv", "gwi", "fzerov", "fzeroi",
+        "tauray", "dwni", "planckir",
+        "qxvdst", "qsvdst", "gvdst", "qxvcld", "qsvcld", "gvcld",
+        "qxidst", "qsidst", "gidst", "qxicld", "qsicld", "gicld",
+        "qextrefdst", "qextrefcld")]
+
+
+class _BatchIn(C.Structure):
+    _fields_ = [("ncol", C.c_int)] + [(n, DP) for n in (
+        "plev", "tlev", "pmid", "tmid", "qh2o", "tauref", "taurefcld", "als", "albi", "acosz")]
+
+
+class _BatchOut(C.Structure):
+    _fields_ = [(n, DP) for n in (
+        "fmnetv", "fluxupv", "fluxdnv", "fmneti", "fluxupi", "fluxdni",
+        "nfluxtopv", "nfluxtopi", "diffvt", "detau", "cumtauv", "hr")]
+
+
+class _OpCount(C.Structure):
+    _fields_ = [("flops", C.c_double), ("exps", C.c_double), ("sqrts", C.c_double), ("logs", C.c_double)]
+
+
+OUT_KEYS = ("fmnetv", "fluxupv", "fluxdnv", "fmneti", "fluxupi", "fluxdni",
+            "nfluxtopv", "nfluxtopi", "diffvt", "detau", "cumtauv", "hr")
+
+
+def alloc_outputs(ncol: int, nlay: int, fill: float = 0.0) -> dict:
+    nl = nlay + 1
+    o = {}
+    for k in ("fmnetv", "fluxupv", "fluxdnv", "fmneti", "fluxupi", "fluxdni"):
+        o[k] = np.full((ncol, nl), fill)
+    for k in ("nfluxtopv", "nfluxtopi", "diffvt"):
+        o[k] = np.full(ncol, fill)
+    o["detau"] = np.full((ncol, NG, NV), fill)    # Fortran (7,17) per column
+    o["cumtauv"] = np.full((ncol, NG, NV), fill)
+    o["hr"] = np.full((ncol, nlay + 1), fill)
+    return o
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Oracle:
+    def __init__(self, path: str = ORACLE_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} not built (run oracle/build_oracle.sh)")
+        self.lib = C.CDLL(path)
+        self.lib.or_batch_run.argtypes = [C.POINTER(_Tables), DP, C.POINTER(_BatchIn), C.POINTER(_BatchOut), C.c_int]
+        self.lib.or_batch_count.argtypes = [C.POINTER(_Tables), DP, C.POINTER(_BatchIn), C.POINTER(_OpCount)]
+        self.lib.or_tpindex.argtypes = [C.c_double, C.c_double, C.c_double, DP, DP, DP, DP,
+                                        C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), DP]
+        self.lib.or_dtridgl.argtypes = [C.c_int, DP, DP, DP, DP, DP]
+        self.lib.or_dsolver.argtypes = [C.c_int] + [DP] * 9 + [C.c_double] * 3 + [DP, DP]
+        self.lib.or_lagrange.argtypes = [C.c_double, DP, DP, DP]
+        self.lib.or_laginterp.argtypes = [DP] * 6
+        self.lib.or_planck_table.argtypes = [DP, DP, DP]
+        self.lib.or_setspv.argtypes = [DP] * 4
+
+    @staticmethod
+    def _tables(t: dict, keep: list) -> _Tables:
+        s = _Tables()
+        s.nlay = int(t["nlay"])
+        for name, _ in _Tables._fields_[1:]:
+            a = _c(t[name])
+            keep.append(a)
+            setattr(s, name, _p(a))
+        return s
+
+    @staticmethod
+    def _inputs(c: dict, keep: list) -> _BatchIn:
+        s = _BatchIn()
+        s.ncol = int(c["ncol"])
+        for name, _ in _BatchIn._fields_[1:]:
+            a = _c(c[name])
+            keep.append(a)
+            setattr(s, name, _p(a))
+        return s
+
+    def run(self, t: dict, c: dict, nthreads: int = 0, out: dict | None = None) -> dict:
+        keep: list = []
+        ts = self._tables(t, keep)
+        ins = self._inputs(c, keep)
+        o = out if out is not None else alloc_outputs(int(c["ncol"]), int(t["nlay"]))
+        os_ = _BatchOut(*[_p(o[k]) for k in OUT_KEYS])
+        sol = _c(t["sol"])
+        self.lib.or_batch_run(C.byref(ts), _p(sol), C.byref(ins), C.byref(os_), int(nthreads))
+        return o
+
+    def count(self, t: dict, c: dict) -> dict:
+        keep: list = []
+        ts = self._tables(t, keep)
+        ins = self._inputs(c, keep)
+        n = _OpCount()
+        sol = _c(t["sol"])
+        self.lib.or_batch_count(C.byref(ts), _p(sol), C.byref(ins), C.byref(n))
+        return {"flops": n.flops, "exps": n.exps, "sqrts": n.sqrts, "logs": n.logs}
+
+    def tpindex(self, pw, tw, q, pref, tref, wref):
+        lc = np.zeros(4)
+        mt, mp, nh = C.c_int(), C.c_int(), C.c_int()
+        wr = np.zeros(1)
+        self.lib.or_tpindex(pw, tw, q, _p(_c(pref)), _p(_c(tref)), _p(_c(wref)), _p(lc),
+                            C.byref(mt), C.byref(mp), C.byref(nh), _p(wr))
+        return lc, mt.value, mp.value, nh.value, wr[0]
+
+    def dtridgl(self, af, bf, cf, df):
+        n = len(af)
+        pad = lambda a: _c(np.concatenate([[0.0], a]))  # noqa: E731  (1-based)
+        xk = np.zeros(n + 1)
+        self.lib.or_dtridgl(n, _p(pad(af)), _p(pad(bf)), _p(pad(cf)), _p(pad(df)), _p(xk))
+        return xk[1:]
+
+    def dsolver(self, gama, cp, cm, cpm1, cmm1, e1, e2, e3, e4, btop, bsurf, rsf):
+        nl = len(gama)
+        pad = lambda a: _c(np.concatenate([[0.0], a]))  # noqa: E731
+        xk1, xk2 = np.zeros(nl + 1), np.zeros(nl + 1)
+        arrs = [pad(a) for a in (gama, cp, cm, cpm1, cmm1, e1, e2, e3, e4)]
+        self.lib.or_dsolver(nl, *[_p(a) for a in arrs], btop, bsurf, rsf, _p(xk1), _p(xk2))
+        return xk1[1:], xk2[1:]
+
+    def lagrange(self, x, xi, yi):
+        ans = np.zeros(1)
+        self.lib.or_lagrange(x, _p(_c(xi)), _p(_c(yi)), _p(ans))
+        return ans[0]
+
+    def laginterp(self, pgref11, co2v8, co2i8):
+        pint = np.zeros(51)
+        co2v = np.zeros((NG, NV, 10, 51, 7))
+        co2i = np.zeros((NG, NI, 10, 51, 7))
+        self.lib.or_laginterp(_p(_c(pgref11)), _p(_c(co2v8)), _p(_c(co2i8)), _p(pint), _p(co2v), _p(co2i))
+        return pint, co2v, co2i
+
+    def planck_table(self):
+        p = np.zeros((8501, NI))
+        dwni, wnoi = np.zeros(NI), np.zeros(NI)
+        self.lib.or_planck_table(_p(p), _p(dwni), _p(wnoi))
+        return p, dwni, wnoi
+
+    def setspv(self):
+        a = [np.zeros(NV) for _ in range(4)]
+        self.lib.or_setspv(*[_p(x) for x in a])
+        return tuple(a)
+
+
+class Reference:
+    """The reference Fortran (amdflang build), one column per call; 24 layers."""
+
+    def __init__(self, path: str = REF_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} not built (run oracle/build_ref.sh)")
+        self.lib = C.CDLL(path)
+        n = C.c_int()
+        self.lib.ref_nlay(C.byref(n))
+        self.nlay = n.value
+        self.lib.ref_column.argtypes = [DP] * 33 + [C.c_double] * 3 + [DP] * 11
+        self.lib.ref_set_planck.argtypes = [DP]
+        self.lib.ref_get_planck.argtypes = [DP]
+        self.lib.ref_setspv.argtypes = [DP] * 5
+        self.lib.ref_setspi.argtypes = [DP] * 3
+        self.lib.ref_setrad.argtypes = [DP] * 14
+        self.lib.ref_dsolver.argtypes = [C.c_int] + [DP] * 9 + [C.c_double] * 3 + [DP, DP]
+        self.lib.ref_tpindex.argtypes = [C.c_double] * 3 + [DP] * 4 + [C.POINTER(C.c_int)] * 3 + [DP]
+
+    def run(self, t: dict, c: dict, cols=None) -> dict:
+        assert int(t["nlay"]) == self.nlay, "reference build is compiled for %d layers" % self.nlay
+        ncol = int(c["ncol"])
+        cols = range(ncol) if cols is None else cols
+        o = alloc_outputs(ncol, self.nlay)
+        self.lib.ref_set_planck(_p(_c(t["planckir"])))
+        tabs = [_c(t[k]) for k in ("co2v", "co2i", "pfgasref", "tgasref", "wrefh2o", "gwv", "gwi",
+                                   "fzerov", "fzeroi", "tauray", "dwni", "sol",
+                                   "qxvdst", "qsvdst", "gvdst", "qxvcld", "qsvcld", "gvcld",
+                                   "qxidst", "qsidst", "gidst", "qxicld", "qsicld", "gicld",
+                                   "qextrefdst", "qextrefcld")]
+        for i in cols:
+            prof = [_c(c[k][i]) for k in ("plev", "tlev", "pmid", "tmid", "qh2o", "tauref", "taurefcld")]
+            outs = [o[k][i] for k in ("fmnetv", "fluxupv", "fluxdnv", "fmneti", "fluxupi", "fluxdni")]
+            sc = [o[k][i:i + 1] for k in ("nfluxtopv", "nfluxtopi", "diffvt")]
+            self.lib.ref_column(*[_p(a) for a in tabs], *[_p(a) for a in prof],
+                                float(c["als"][i]), float(c["albi"][i]), float(c["acosz"][i]),
+                                *[_p(a) for a in outs], *[_p(a) for a in sc],
+                                _p(o["detau"][i]), _p(o["cumtauv"][i]))
+        _heating(t, c, o)
+        return o
+
+    def tables(self, rundir: str):
+        """SETSPV/SETSPI/SETRAD (+LAGINTERP) as the reference computes them at start-up."""
+        wnov, dwnv, wavev, solarf, tauray = (np.zeros(NV) for _ in range(5))
+        self.lib.ref_setspv(_p(wnov), _p(dwnv), _p(wavev), _p(solarf), _p(tauray))
+        wnoi, dwni, wavei = (np.zeros(NI) for _ in range(3))
+        self.lib.ref_setspi(_p(wnoi), _p(dwni), _p(wavei))
+        planck = np.zeros((8501, NI))
+        self.lib.ref_get_planck(_p(planck))
+        tg, pf = np.zeros(7), np.zeros(51)
+        co2v, co2i = np.zeros((NG, NV, 10, 51, 7)), np.zeros((NG, NI, 10, 51, 7))
+        misc = [np.zeros(NV) for _ in range(4)] + [np.zeros(NI) for _ in range(4)] + [np.zeros(NI), np.zeros(NV)]
+        cwd = os.getcwd()
+        os.chdir(rundir)
+        try:
+            self.lib.ref_setrad(_p(tg), _p(pf), _p(co2v), _p(co2i), *[_p(a) for a in misc])
+        finally:
+            os.chdir(cwd)
+        return dict(wnov=wnov, dwnv=dwnv, solarf=solarf, tauray=tauray, wnoi=wnoi, dwni=dwni,
+                    planckir=planck, tgasref=tg, pfgasref=pf, co2v=co2v, co2i=co2i,
+                    fzeroi=misc[8], fzerov=misc[9])
+
+    def dsolver(self, gama, cp, cm, cpm1, cmm1, e1, e2, e3, e4, btop, bsurf, rsf):
+        nl = len(gama)
+        xk1, xk2 = np.zeros(nl), np.zeros(nl)
+        arrs = [_c(a) for a in (gama, cp, cm, cpm1, cmm1, e1, e2, e3, e4)]
+        self.lib.ref_dsolver(nl, *[_p(a) for a in arrs], btop, bsurf, rsf, _p(xk1), _p(xk2))
+        return xk1, xk2
+
+    def tpindex(self, pw, tw, q, pref, tref, wref):
+        lc = np.zeros(4)
+        mt, mp, nh = C.c_int(), C.c_int(), C.c_int()
+        wr = np.zeros(1)
+        self.lib.ref_tpindex(pw, tw, q, _p(_c(pref)), _p(_c(tref)), _p(_c(wref)), _p(lc),
+                             C.byref(mt), C.byref(mp), C.byref(nh), _p(wr))
+        return lc, mt.value, mp.value, nh.value, wr[0]
+
+
+def _heating(t: dict, c: dict, o: dict) -> None:
+    """Layer heating (K/sol) from net fluxes, same statement order as the oracle's."""
+    nlay = int(t["nlay"])
+    grav, cpair, scalep, sps = 3.72, 7.3594e2, 1.0e2, 88775.0
+    acosz = np.asarray(c["acosz"])
+    up = acosz >= np.float64(np.float32(1.0e-5))
+    fv, fi = o["fmnetv"], o["fmneti"]
+    sun = np.zeros_like(fv)
+    sun[:, 0] = fv[:, 0] - o["nfluxtopv"]
+    sun[:, 1:] = fv[:, 1:] - fv[:, :-1]
+    sun[~up] = 0.0
+    ir = np.empty_like(fi)
+    ir[:, 0] = fi[:, 0] - o["nfluxtopi"]
+    ir[:, 1:] = fi[:, 1:] - fi[:, :-1]
+    plev = np.asarray(c["plev"])
+    for l in range(nlay + 1):
+        dp = (plev[:, 0] + plev[:, 1]) if l == 0 else (plev[:, 2 * l + 2] - plev[:, 2 * l])
+        df = sun[:, l] + ir[:, l]
+        o["hr"][:, l] = ((df * grav) / ((cpair * scalep) * dp)) * sps
