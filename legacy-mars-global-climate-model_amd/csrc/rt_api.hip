@@ -1,0 +1,489 @@
+// rt_api.hip -- the C ABI of include/mgcm_rt.h: context set-up, runs, and the
+// init-time table builders (Planck table, LAGINTERP) as device kernels.
+#include <new>
+#include <stdlib.h>
+#include <string.h>
+#include <vector>
+
+#include "../../include/mgcm_rt.h"
+#include "rt_internal.h"
+#include "rt_libm.h"
+
+namespace mgcm {
+hipError_t launch_plain(const DevTables &dt, const RunArgs &a, hipStream_t st);
+hipError_t launch_mixed(const DevTables &dt, const RunArgs &a, hipStream_t st);
+int run_batched_impl(struct ::mgcm_rt_ctx *ctx, int ngrids, const mgcm_rt_columns *grids,
+                     const mgcm_rt_fluxes *outs, int precision, hipStream_t st);
+}  // namespace mgcm
+void batch_free(void *);
+
+using namespace mgcm;
+
+struct mgcm_rt_ctx {
+  int device;
+  RtConst *d_const;
+  double *d_kv, *d_ki, *d_planck;
+  float *d_kv32, *d_ki32;
+  DevTables dt;
+  // workspace (grow-only)
+  double *d_ws;
+  size_t ws_bytes;
+  int *d_flag;
+  // device staging of the host-pointer path
+  size_t host_bytes;
+  double *d_host;
+  // batched-path state (owned by rt_batched.hip)
+  void *batch;
+};
+
+#define HIP_TRY(x)                                   \
+  do {                                               \
+    if ((x) != hipSuccess) return MGCM_RT_ERR_DEVICE; \
+  } while (0)
+
+extern "C" const char *mgcm_rt_strerror(int s) {
+  switch (s) {
+    case MGCM_RT_OK: return "ok";
+    case MGCM_RT_ERR_ARG: return "invalid argument (null pointer or negative size)";
+    case MGCM_RT_ERR_NLAY: return "layer count outside 1..64";
+    case MGCM_RT_ERR_DEVICE: return "HIP runtime error";
+    case MGCM_RT_ERR_NOMEM: return "out of device memory";
+    case MGCM_RT_ERR_RANGE: return "temperature outside the Planck table (50.0..900.0 K)";
+    case MGCM_RT_ERR_PRECISION: return "unknown precision mode";
+    default: return "unknown status";
+  }
+}
+
+// Re-lay a Fortran k-table CO2(7,51,10,nb,17) into [nb][51][10][7][16] (gas
+// Gauss points fastest): the 16 points of one (T, p, H2O) cell share a
+// 128-byte line, which the 16 lanes of a column read together.
+static void relayout_ktable(const double *src, int nb, double *dst, float *dst32) {
+  for (int nw = 0; nw < nb; nw++)
+    for (int p = 0; p < NP; p++)
+      for (int h = 0; h < NH; h++)
+        for (int t = 0; t < NT; t++)
+          for (int g = 0; g < NGAS; g++) {
+            const size_t s = (size_t)t + NT * ((size_t)p + NP * ((size_t)h + NH * ((size_t)nw + (size_t)nb * g)));
+            const size_t d = ((((size_t)nw * NP + p) * NH + h) * NT + t) * NGAS + g;
+            dst[d] = src[s];
+            dst32[d] = (float)src[s];
+          }
+}
+
+extern "C" int mgcm_rt_init(const mgcm_rt_tables *t, mgcm_rt_ctx **out) {
+  if (!t || !out) return MGCM_RT_ERR_ARG;
+  const double *req[] = {t->co2v, t->co2i, t->pfgasref, t->tgasref, t->wrefh2o, t->gwv, t->gwi,
+                         t->fzerov, t->fzeroi, t->tauray, t->dwni, t->planckir,
+                         t->qxv_dst, t->qsv_dst, t->gv_dst, t->qxi_dst, t->qsi_dst, t->gi_dst,
+                         t->qxv_cld, t->qsv_cld, t->gv_cld, t->qxi_cld, t->qsi_cld, t->gi_cld};
+  for (const double *p : req)
+    if (!p) return MGCM_RT_ERR_ARG;
+  *out = nullptr;
+  mgcm_rt_ctx *c = new (std::nothrow) mgcm_rt_ctx();
+  if (!c) return MGCM_RT_ERR_NOMEM;
+  if (hipGetDevice(&c->device) != hipSuccess) { delete c; return MGCM_RT_ERR_DEVICE; }
+
+  RtConst h;
+  memset(&h, 0, sizeof(h));
+  memcpy(h.pfgasref, t->pfgasref, sizeof(h.pfgasref));
+  memcpy(h.tgasref, t->tgasref, sizeof(h.tgasref));
+  memcpy(h.wrefh2o, t->wrefh2o, sizeof(h.wrefh2o));
+  for (int b = 0; b < NB; b++)
+    for (int g = 0; g < NG; g++) h.gw[b][g] = (b < NV) ? t->gwv[g + NG * b] : t->gwi[g + NG * (b - NV)];
+  for (int b = 0; b < NV; b++) {
+    h.fzero[b] = t->fzerov[b];
+    h.tauray[b] = t->tauray[b];
+    h.qx_dst[b] = t->qxv_dst[b]; h.qs_dst[b] = t->qsv_dst[b]; h.g_dst[b] = t->gv_dst[b];
+    h.qx_cld[b] = t->qxv_cld[b]; h.qs_cld[b] = t->qsv_cld[b]; h.g_cld[b] = t->gv_cld[b];
+  }
+  for (int b = 0; b < NI; b++) {
+    h.fzero[NV + b] = t->fzeroi[b];
+    h.dwni[b] = t->dwni[b];
+    h.qx_dst[NV + b] = t->qxi_dst[b]; h.qs_dst[NV + b] = t->qsi_dst[b]; h.g_dst[NV + b] = t->gi_dst[b];
+    h.qx_cld[NV + b] = t->qxi_cld[b]; h.qs_cld[NV + b] = t->qsi_cld[b]; h.g_cld[NV + b] = t->gi_cld[b];
+  }
+  h.qextref_vis = t->qxv_dst[5];  // QXVDST(K, L_NREFV) with L_NREFV = 6 (code/optcv.f90:114)
+  h.qextref_dst = t->qextref_dst;
+  h.qextref_cld = t->qextref_cld;
+
+  const size_t nkv = (size_t)NV * NP * NH * NT * NGAS, nki = (size_t)NI * NP * NH * NT * NGAS;
+  std::vector<double> kv(nkv), ki(nki);
+  std::vector<float> kv32(nkv), ki32(nki);
+  relayout_ktable(t->co2v, NV, kv.data(), kv32.data());
+  relayout_ktable(t->co2i, NI, ki.data(), ki32.data());
+  bool ok = hipMalloc(&c->d_const, sizeof(RtConst)) == hipSuccess &&
+            hipMalloc(&c->d_kv, nkv * sizeof(double)) == hipSuccess &&
+            hipMalloc(&c->d_ki, nki * sizeof(double)) == hipSuccess &&
+            hipMalloc(&c->d_kv32, nkv * sizeof(float)) == hipSuccess &&
+            hipMalloc(&c->d_ki32, nki * sizeof(float)) == hipSuccess &&
+            hipMalloc(&c->d_planck, (size_t)NPLANCK * NI * sizeof(double)) == hipSuccess &&
+            hipMalloc(&c->d_flag, sizeof(int)) == hipSuccess;
+  if (ok)
+    ok = hipMemcpy(c->d_const, &h, sizeof(h), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(c->d_kv, kv.data(), nkv * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(c->d_ki, ki.data(), nki * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(c->d_kv32, kv32.data(), nkv * sizeof(float), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(c->d_ki32, ki32.data(), nki * sizeof(float), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(c->d_planck, t->planckir, (size_t)NPLANCK * NI * sizeof(double), hipMemcpyHostToDevice) ==
+             hipSuccess &&
+         hipMemset(c->d_flag, 0, sizeof(int)) == hipSuccess;
+  if (!ok) {
+    mgcm_rt_destroy(c);
+    return MGCM_RT_ERR_DEVICE;
+  }
+  c->dt.c = c->d_const;
+  c->dt.kv = c->d_kv;
+  c->dt.ki = c->d_ki;
+  c->dt.kv32_unused = nullptr;
+  c->dt.planck = c->d_planck;
+  c->dt.kv32 = c->d_kv32;
+  c->dt.ki32 = c->d_ki32;
+  *out = c;
+  return MGCM_RT_OK;
+}
+
+extern "C" int mgcm_rt_destroy(mgcm_rt_ctx *c) {
+  if (!c) return MGCM_RT_ERR_ARG;
+  void *bufs[] = {c->d_const, c->d_kv, c->d_ki, c->d_kv32, c->d_ki32, c->d_planck, c->d_ws, c->d_flag, c->d_host};
+  for (void *p : bufs)
+    if (p) (void)hipFree(p);
+  if (c->batch) batch_free(c->batch);
+  delete c;
+  return MGCM_RT_OK;
+}
+
+namespace mgcm {
+// Workspace for one grid: gas and continuum partial sums + skip masks.
+size_t workspace_bytes(int ncol, int nlay) {
+  const size_t nf = 3 * (size_t)(nlay + 1) + 2;
+  return 2 * (size_t)NB * nf * ncol * sizeof(double) + (size_t)NB * ncol * sizeof(uint32_t) + 256;
+}
+
+void bind_workspace(RunArgs &a, void *ws) {
+  const size_t nf = 3 * (size_t)(a.nlay + 1) + 2;
+  a.gas = (double *)ws;
+  a.clear = a.gas + (size_t)NB * nf * a.ncol;
+  a.skip = (uint32_t *)(a.clear + (size_t)NB * nf * a.ncol);
+}
+
+int check_columns(const mgcm_rt_columns *in, const mgcm_rt_fluxes *out) {
+  if (!in || !out) return MGCM_RT_ERR_ARG;
+  if (in->ncol < 0) return MGCM_RT_ERR_ARG;
+  if (in->nlay < 1 || in->nlay > MAX_NLAY) return MGCM_RT_ERR_NLAY;
+  if (in->ncol == 0) return MGCM_RT_OK;
+  const void *req[] = {in->plev, in->tlev, in->pmid, in->tmid, in->qh2o, in->tauref, in->taurefcld,
+                       in->als, in->albi, in->acosz, out->fmnetv, out->fluxupv, out->fluxdnv,
+                       out->fmneti, out->fluxupi, out->fluxdni, out->nfluxtopv, out->nfluxtopi,
+                       out->diffvt, out->detau, out->cumtauv};
+  for (const void *p : req)
+    if (!p) return MGCM_RT_ERR_ARG;
+  return MGCM_RT_OK;
+}
+
+void fill_args(RunArgs &a, const mgcm_rt_columns *in, const mgcm_rt_fluxes *out, int *flag) {
+  a.ncol = in->ncol;
+  a.nlay = in->nlay;
+  a.plev = in->plev; a.tlev = in->tlev; a.pmid = in->pmid; a.tmid = in->tmid; a.qh2o = in->qh2o;
+  a.tauref = in->tauref; a.taurefcld = in->taurefcld;
+  a.als = in->als; a.albi = in->albi; a.acosz = in->acosz;
+  for (int b = 0; b < NV; b++) a.sol[b] = in->sol[b];
+  a.fmnetv = out->fmnetv; a.fluxupv = out->fluxupv; a.fluxdnv = out->fluxdnv;
+  a.fmneti = out->fmneti; a.fluxupi = out->fluxupi; a.fluxdni = out->fluxdni;
+  a.nfluxtopv = out->nfluxtopv; a.nfluxtopi = out->nfluxtopi; a.diffvt = out->diffvt;
+  a.detau = out->detau; a.cumtauv = out->cumtauv; a.hr = out->hr;
+  a.flag = flag;
+}
+}  // namespace mgcm
+
+static int ensure_ws(mgcm_rt_ctx *c, size_t bytes) {
+  if (bytes <= c->ws_bytes) return MGCM_RT_OK;
+  if (c->d_ws) {
+    if (hipDeviceSynchronize() != hipSuccess) return MGCM_RT_ERR_DEVICE;
+    (void)hipFree(c->d_ws);
+    c->d_ws = nullptr;
+    c->ws_bytes = 0;
+  }
+  if (hipMalloc(&c->d_ws, bytes) != hipSuccess) return MGCM_RT_ERR_NOMEM;
+  c->ws_bytes = bytes;
+  return MGCM_RT_OK;
+}
+
+extern "C" int mgcm_rt_run(mgcm_rt_ctx *c, const mgcm_rt_columns *in, const mgcm_rt_fluxes *out,
+                           int precision, void *stream) {
+  if (!c) return MGCM_RT_ERR_ARG;
+  int s = check_columns(in, out);
+  if (s != MGCM_RT_OK) return s;
+  if (precision != MGCM_RT_FP64 && precision != MGCM_RT_MIXED) return MGCM_RT_ERR_PRECISION;
+  if (in->ncol == 0) return MGCM_RT_OK;
+  s = ensure_ws(c, workspace_bytes(in->ncol, in->nlay));
+  if (s != MGCM_RT_OK) return s;
+  RunArgs a;
+  fill_args(a, in, out, c->d_flag);
+  bind_workspace(a, c->d_ws);
+  const hipError_t e = (precision == MGCM_RT_FP64) ? launch_plain(c->dt, a, (hipStream_t)stream)
+                                                   : launch_mixed(c->dt, a, (hipStream_t)stream);
+  return e == hipSuccess ? MGCM_RT_OK : MGCM_RT_ERR_DEVICE;
+}
+
+extern "C" int mgcm_rt_check(mgcm_rt_ctx *c) {
+  if (!c) return MGCM_RT_ERR_ARG;
+  int f = 0;
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(&f, c->d_flag, sizeof(int), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemset(c->d_flag, 0, sizeof(int)));
+  return f ? MGCM_RT_ERR_RANGE : MGCM_RT_OK;
+}
+
+extern "C" int mgcm_rt_run_host(mgcm_rt_ctx *c, const mgcm_rt_columns *in, const mgcm_rt_fluxes *out,
+                                int precision) {
+  if (!c) return MGCM_RT_ERR_ARG;
+  int s = check_columns(in, out);
+  if (s != MGCM_RT_OK) return s;
+  if (precision != MGCM_RT_FP64 && precision != MGCM_RT_MIXED) return MGCM_RT_ERR_PRECISION;
+  if (in->ncol == 0) return MGCM_RT_OK;
+  const size_t ncol = in->ncol, nlev = 2 * (size_t)in->nlay + 3, nl = in->nlay + 1;
+  const size_t nin = ncol * (5 * nlev + 2 * (nlev + 1) + 3);
+  const size_t nout = ncol * (6 * nl + 3 + 2 * NG * NV + nl);
+  const size_t bytes = (nin + nout) * sizeof(double);
+  if (bytes > c->host_bytes) {
+    if (c->d_host) (void)hipFree(c->d_host);
+    c->d_host = nullptr;
+    c->host_bytes = 0;
+    if (hipMalloc(&c->d_host, bytes) != hipSuccess) return MGCM_RT_ERR_NOMEM;
+    c->host_bytes = bytes;
+  }
+  double *p = c->d_host;
+  mgcm_rt_columns din = *in;
+  mgcm_rt_fluxes dout;
+  double *dplev = p; p += ncol * nlev;
+  double *dtlev = p; p += ncol * nlev;
+  double *dpmid = p; p += ncol * nlev;
+  double *dtmid = p; p += ncol * nlev;
+  double *dqh2o = p; p += ncol * nlev;
+  double *dtauref = p; p += ncol * (nlev + 1);
+  double *dtaurefcld = p; p += ncol * (nlev + 1);
+  double *dals = p; p += ncol;
+  double *dalbi = p; p += ncol;
+  double *dacosz = p; p += ncol;
+  HIP_TRY(hipMemcpy(dplev, in->plev, ncol * nlev * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(dtlev, in->tlev, ncol * nlev * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(dpmid, in->pmid, ncol * nlev * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(dtmid, in->tmid, ncol * nlev * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(dqh2o, in->qh2o, ncol * nlev * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(dtauref, in->tauref, ncol * (nlev + 1) * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(dtaurefcld, in->taurefcld, ncol * (nlev + 1) * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(dals, in->als, ncol * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(dalbi, in->albi, ncol * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(dacosz, in->acosz, ncol * 8, hipMemcpyHostToDevice));
+  din.plev = dplev; din.tlev = dtlev; din.pmid = dpmid; din.tmid = dtmid; din.qh2o = dqh2o;
+  din.tauref = dtauref; din.taurefcld = dtaurefcld; din.als = dals; din.albi = dalbi; din.acosz = dacosz;
+  dout.fmnetv = p; p += ncol * nl;
+  dout.fluxupv = p; p += ncol * nl;
+  dout.fluxdnv = p; p += ncol * nl;
+  dout.fmneti = p; p += ncol * nl;
+  dout.fluxupi = p; p += ncol * nl;
+  dout.fluxdni = p; p += ncol * nl;
+  dout.nfluxtopv = p; p += ncol;
+  dout.nfluxtopi = p; p += ncol;
+  dout.diffvt = p; p += ncol;
+  dout.detau = p; p += ncol * NG * NV;
+  dout.cumtauv = p; p += ncol * NG * NV;
+  dout.hr = out->hr ? p : nullptr;
+  // detau/cumtauv are in/out: sun-down columns keep the caller's values
+  HIP_TRY(hipMemcpy(dout.detau, out->detau, ncol * NG * NV * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(dout.cumtauv, out->cumtauv, ncol * NG * NV * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemset(c->d_flag, 0, sizeof(int)));
+  s = mgcm_rt_run(c, &din, &dout, precision, nullptr);
+  if (s != MGCM_RT_OK) return s;
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(out->fmnetv, dout.fmnetv, ncol * nl * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(out->fluxupv, dout.fluxupv, ncol * nl * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(out->fluxdnv, dout.fluxdnv, ncol * nl * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(out->fmneti, dout.fmneti, ncol * nl * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(out->fluxupi, dout.fluxupi, ncol * nl * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(out->fluxdni, dout.fluxdni, ncol * nl * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(out->nfluxtopv, dout.nfluxtopv, ncol * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(out->nfluxtopi, dout.nfluxtopi, ncol * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(out->diffvt, dout.diffvt, ncol * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(out->detau, dout.detau, ncol * NG * NV * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(out->cumtauv, dout.cumtauv, ncol * NG * NV * 8, hipMemcpyDeviceToHost));
+  if (out->hr) HIP_TRY(hipMemcpy(out->hr, dout.hr, ncol * nl * 8, hipMemcpyDeviceToHost));
+  return mgcm_rt_check(c);
+}
+
+extern "C" int mgcm_rt_run_batched(mgcm_rt_ctx *c, int ngrids, const mgcm_rt_columns *grids,
+                                   const mgcm_rt_fluxes *outs, int precision, void *stream) {
+  if (!c || ngrids < 0 || (ngrids > 0 && (!grids || !outs))) return MGCM_RT_ERR_ARG;
+  if (precision != MGCM_RT_FP64 && precision != MGCM_RT_MIXED) return MGCM_RT_ERR_PRECISION;
+  for (int i = 0; i < ngrids; i++) {
+    const int s = check_columns(&grids[i], &outs[i]);
+    if (s != MGCM_RT_OK) return s;
+  }
+  return run_batched_impl(c, ngrids, grids, outs, precision, (hipStream_t)stream);
+}
+
+// ---------------------------------------------------------------------------
+// Init-time tables.
+
+// SETSPI's Planck table (code/setspi.f90:91-105): one thread per (band, T).
+__global__ void k_planck(double *planck, const double *bwni, const double *dwni) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= NPLANCK * NI) return;
+  const int nw = i % NI, n = i / NI;  // n = nt - 500
+  const double x[12] = {-0.981560634246719, -0.904117256370475, -0.769902674194305,
+                        -0.587317954286617, -0.367831498998180, -0.125233408511469,
+                        0.125233408511469,  0.367831498998180,  0.587317954286617,
+                        0.769902674194305,  0.904117256370475,  0.981560634246719};
+  const double w[12] = {0.047175336386512, 0.106939325995318, 0.160078328543346,
+                        0.203167426723066, 0.233492536538355, 0.249147045813403,
+                        0.249147045813403, 0.233492536538355, 0.203167426723066,
+                        0.160078328543346, 0.106939325995318, 0.047175336386512};
+  const double c1 = 3.741832e-16, c2 = 1.438786e-2;
+  const double a = 1.0e-2 / bwni[nw + 1], b = 1.0e-2 / bwni[nw];
+  const double bpa = (b + a) / 2.0, bma = (b - a) / 2.0;
+  const double tt = (double)(n + 500) / 1.0e1;
+  double ans = 0.0;
+  for (int m = 0; m < 12; m++) {
+    const double y = bma * x[m] + bpa;
+    const double y5 = y * y * (y * y) * y;
+    ans = ans + w[m] * c1 / (y5 * (mgcm_libm::mgcm_exp(c2 / (y * tt)) - 1.0));
+  }
+  planck[(size_t)n * NI + nw] = ans * bma / (PI * dwni[nw]);
+}
+
+extern "C" int mgcm_rt_planck_table(double *planckir, double *dwni, double *wnoi) {
+  if (!planckir || !dwni || !wnoi) return MGCM_RT_ERR_ARG;
+  const double bwni[6] = {10.000, 166.667, 416.667, 833.333, 1250.000, 2222.222};  // code/setspi.f90:70-75
+  for (int m = 0; m < NI; m++) {
+    wnoi[m] = 0.5 * (bwni[m + 1] + bwni[m]);
+    dwni[m] = bwni[m + 1] - bwni[m];
+  }
+  double *d = nullptr;
+  const size_t n = (size_t)NPLANCK * NI;
+  HIP_TRY(hipMalloc(&d, (n + 16) * sizeof(double)));
+  double *db = d + n, *dd = d + n + 8;
+  int st = MGCM_RT_OK;
+  if (hipMemcpy(db, bwni, 6 * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dd, dwni, NI * 8, hipMemcpyHostToDevice) != hipSuccess) {
+    st = MGCM_RT_ERR_DEVICE;
+  } else {
+    hipLaunchKernelGGL(k_planck, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, nullptr, d, db, dd);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(planckir, d, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+      st = MGCM_RT_ERR_DEVICE;
+  }
+  (void)hipFree(d);
+  return st;
+}
+
+// LAGINTERP (code/laginterp.f90:74-236) + LAGRANGE (code/lagrange.f90:17-31):
+// one thread per (T, H2O, band, g) table row of 11 pressures -> 51.
+__device__ double lagrange4(double x, const double *xi, const double *yi) {
+  const double fm1 = x - xi[0], fm2 = x - xi[1], fm3 = x - xi[2], fm4 = x - xi[3];
+  return fm2 * fm3 * fm4 * yi[0] / ((xi[0] - xi[1]) * (xi[0] - xi[2]) * (xi[0] - xi[3])) +
+         fm1 * fm3 * fm4 * yi[1] / ((xi[1] - xi[0]) * (xi[1] - xi[2]) * (xi[1] - xi[3])) +
+         fm1 * fm2 * fm4 * yi[2] / ((xi[2] - xi[0]) * (xi[2] - xi[1]) * (xi[2] - xi[3])) +
+         fm1 * fm2 * fm3 * yi[3] / ((xi[3] - xi[0]) * (xi[3] - xi[1]) * (xi[3] - xi[2]));
+}
+
+__global__ void k_laginterp(const double *raw, const double *pref, const double *pint, int nb, double *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;  // (t, h, nw, g) with t fastest
+  const int nrow = NT * NH * nb * NG;
+  if (i >= nrow) return;
+  const int t = i % NT, h = (i / NT) % NH, nw = (i / (NT * NH)) % nb, g = i / (NT * NH * nb);
+  const int npref = 11;
+  auto RAW = [&](int p) {  // log10 with floor -200 (code/laginterp.f90:74-98)
+    const double v = raw[(size_t)t + NT * ((size_t)p + npref * ((size_t)h + NH * ((size_t)nw + (size_t)nb * g)))];
+    return (v > 1.0e-200) ? mgcm_libm::mgcm_log10(v) : -200.0;
+  };
+  auto OUT = [&](int p) -> double & {
+    return out[(size_t)t + NT * ((size_t)p + NP * ((size_t)h + NH * ((size_t)nw + (size_t)nb * g)))];
+  };
+  double xi[4], yi[4];
+  for (int m = 0; m < 5; m++) {  // first interval, one-sided stencil
+    for (int j = 0; j < 4; j++) { xi[j] = pref[j]; yi[j] = RAW(j); }
+    OUT(m) = pow(10.0, lagrange4(pint[m], xi, yi));
+  }
+  for (int n = 2; n <= npref - 2; n++)  // interior (1-based n)
+    for (int m = 1; m <= 5; m++) {
+      const int ii = (n - 1) * 5 + m;
+      for (int j = 0; j < 4; j++) { xi[j] = pref[n - 2 + j]; yi[j] = RAW(n - 2 + j); }
+      OUT(ii - 1) = pow(10.0, lagrange4(pint[ii - 1], xi, yi));
+    }
+  {
+    const int n = npref - 1;  // last interval
+    for (int m = 1; m <= 5; m++) {
+      const int ii = (n - 1) * 5 + m;
+      for (int j = 0; j < 4; j++) { xi[j] = pref[n - 3 + j]; yi[j] = RAW(n - 3 + j); }
+      OUT(ii - 1) = pow(10.0, lagrange4(pint[ii - 1], xi, yi));
+    }
+  }
+  OUT(NP - 1) = pow(10.0, RAW(npref - 1));
+}
+
+extern "C" int mgcm_rt_laginterp(const double *pgref, const double *raw, int nbands, double *pint, double *out) {
+  if (!pgref || !raw || !pint || !out || nbands < 1 || nbands > NB) return MGCM_RT_ERR_ARG;
+  // code/laginterp.f90:30-41: log10 p = -6.0 .. 4.0 in steps of 0.2
+  static const double pin[NP] = {-6.0, -5.8, -5.6, -5.4, -5.2, -5.0, -4.8, -4.6, -4.4, -4.2, -4.0,
+                                 -3.8, -3.6, -3.4, -3.2, -3.0, -2.8, -2.6, -2.4, -2.2, -2.0, -1.8,
+                                 -1.6, -1.4, -1.2, -1.0, -0.8, -0.6, -0.4, -0.2, 0.0,  0.2,  0.4,
+                                 0.6,  0.8,  1.0,  1.2,  1.4,  1.6,  1.8,  2.0,  2.2,  2.4,  2.6,
+                                 2.8,  3.0,  3.2,  3.4,  3.6,  3.8,  4.0};
+  memcpy(pint, pin, sizeof(pin));
+  double pref[11];
+  for (int n = 0; n < 11; n++) pref[n] = log10(pgref[n]);
+  const size_t nraw = (size_t)NT * 11 * NH * nbands * NG, nout = (size_t)NT * NP * NH * nbands * NG;
+  double *d = nullptr;
+  HIP_TRY(hipMalloc(&d, (nraw + nout + 64) * sizeof(double)));
+  double *draw = d, *dout = d + nraw, *dpref = d + nraw + nout, *dpint = dpref;
+  int st = MGCM_RT_OK;
+  double *dpin = nullptr;
+  if (hipMalloc(&dpin, NP * sizeof(double)) != hipSuccess) st = MGCM_RT_ERR_NOMEM;
+  if (st == MGCM_RT_OK &&
+      (hipMemcpy(draw, raw, nraw * 8, hipMemcpyHostToDevice) != hipSuccess ||
+       hipMemcpy(dpref, pref, 11 * 8, hipMemcpyHostToDevice) != hipSuccess ||
+       hipMemcpy(dpin, pin, NP * 8, hipMemcpyHostToDevice) != hipSuccess))
+    st = MGCM_RT_ERR_DEVICE;
+  (void)dpint;
+  if (st == MGCM_RT_OK) {
+    const int nrow = NT * NH * nbands * NG;
+    hipLaunchKernelGGL(k_laginterp, dim3((nrow + 127) / 128), dim3(128), 0, nullptr, draw, dpref, dpin, nbands, dout);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(out, dout, nout * 8, hipMemcpyDeviceToHost) != hipSuccess)
+      st = MGCM_RT_ERR_DEVICE;
+  }
+  if (dpin) (void)hipFree(dpin);
+  (void)hipFree(d);
+  return st;
+}
+
+// Device results of the transcendental functions of the FP64 path, for the
+// parity check against the host C library (mgcm_rt_libm_check).
+__global__ void k_libm(const double *x, int n, double *e, double *l, double *q) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  e[i] = mgcm_libm::mgcm_exp(x[i]);
+  l[i] = (x[i] > 0.0) ? mgcm_libm::mgcm_log10(x[i]) : 0.0;
+  q[i] = (x[i] >= 0.0) ? sqrt(x[i]) : 0.0;
+}
+
+extern "C" int mgcm_rt_libm_check(const double *x, int n, double *exp_out, double *log10_out, double *sqrt_out) {
+  if (n < 0 || (n > 0 && (!x || !exp_out || !log10_out || !sqrt_out))) return MGCM_RT_ERR_ARG;
+  if (n == 0) return MGCM_RT_OK;
+  double *d = nullptr;
+  const size_t b = (size_t)n * sizeof(double);
+  HIP_TRY(hipMalloc(&d, 4 * b));
+  int st = MGCM_RT_OK;
+  if (hipMemcpy(d, x, b, hipMemcpyHostToDevice) != hipSuccess) st = MGCM_RT_ERR_DEVICE;
+  if (st == MGCM_RT_OK) {
+    hipLaunchKernelGGL(k_libm, dim3((n + 255) / 256), dim3(256), 0, nullptr, d, n, d + n, d + 2 * (size_t)n,
+                       d + 3 * (size_t)n);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(exp_out, d + n, b, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(log10_out, d + 2 * (size_t)n, b, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(sqrt_out, d + 3 * (size_t)n, b, hipMemcpyDeviceToHost) != hipSuccess)
+      st = MGCM_RT_ERR_DEVICE;
+  }
+  (void)hipFree(d);
+  return st;
+}

@@ -1,0 +1,245 @@
+"""Python host binding of the MI355X column radiative transfer (include/mgcm_rt.h).
+
+The library libmgcm_rt.so is built in-tree by ../build.py.  Loading fails loudly
+when it is missing: there is no CPU path behind this module.
+
+    from mgcm_rt import Radiation, synth
+    tables, cols = synth.make_workload(seed=1, ncol=2160, nlay=24)
+    rt = Radiation(tables)
+    out = rt.run_host(cols)              # numpy in, numpy out (copies + sync)
+    rt.run_device(dev_cols, dev_out)      # device pointers (torch tensors), async
+
+Argument meaning follows the reference's OPTCV/SFLUXV/OPTCI/SFLUXI as called by
+COMP3 (code/comp3.f:637-738): PLEV/TLEV/PMID/TMID/QH2O per level, TAUREF and
+TAUREFCLD per sub-level, surface albedo ALS, IR albedo ALBI = 1 - emissivity,
+cos(zenith) ACOSZ, SOL = SOLARF/RSDIST.  Outputs are FMNETV/FLUXUPV/FLUXDNV,
+FMNETI/FLUXUPI/FLUXDNI per radiation layer, NFLUXTOPV/I, DIFFVT, DETAU and
+CUMTAUV per (band, Gauss point), and the layer heating rates in K/sol.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import synth  # noqa: F401
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmgcm_rt.so")
+
+NV, NI, NG = 7, 5, 17
+FP64, MIXED = 0, 1
+DP = C.POINTER(C.c_double)
+
+STATUS = {0: "ok", -1: "invalid argument", -2: "layer count outside 1..64", -3: "HIP runtime error",
+          -4: "out of device memory", -5: "temperature outside the Planck table", -6: "unknown precision"}
+
+
+class RadiationError(RuntimeError):
+    def __init__(self, status: int, what: str):
+        super().__init__(f"{what}: {STATUS.get(status, status)} ({status})")
+        self.status = status
+
+
+class Tables(C.Structure):
+    _fields_ = [(n, DP) for n in (
+        "co2v", "co2i", "pfgasref", "tgasref", "wrefh2o", "gwv", "gwi", "fzerov", "fzeroi",
+        "tauray", "dwni", "planckir",
+        "qxv_dst", "qsv_dst", "gv_dst", "qxi_dst", "qsi_dst", "gi_dst",
+        "qxv_cld", "qsv_cld", "gv_cld", "qxi_cld", "qsi_cld", "gi_cld")] + [
+        ("qextref_dst", C.c_double), ("qextref_cld", C.c_double)]
+
+
+class Columns(C.Structure):
+    _fields_ = [("ncol", C.c_int), ("nlay", C.c_int)] + [(n, C.c_void_p) for n in (
+        "plev", "tlev", "pmid", "tmid", "qh2o", "tauref", "taurefcld", "als", "albi", "acosz")] + [
+        ("sol", C.c_double * NV)]
+
+
+class Fluxes(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in (
+        "fmnetv", "fluxupv", "fluxdnv", "fmneti", "fluxupi", "fluxdni",
+        "nfluxtopv", "nfluxtopi", "diffvt", "detau", "cumtauv", "hr")]
+
+
+IN_KEYS = ("plev", "tlev", "pmid", "tmid", "qh2o", "tauref", "taurefcld", "als", "albi", "acosz")
+OUT_KEYS = ("fmnetv", "fluxupv", "fluxdnv", "fmneti", "fluxupi", "fluxdni",
+            "nfluxtopv", "nfluxtopi", "diffvt", "detau", "cumtauv", "hr")
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Load libmgcm_rt.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} is missing: build it with build.py (no CPU fallback exists)")
+        lib = C.CDLL(path)
+        lib.mgcm_rt_init.argtypes = [C.POINTER(Tables), C.POINTER(C.c_void_p)]
+        lib.mgcm_rt_destroy.argtypes = [C.c_void_p]
+        lib.mgcm_rt_run.argtypes = [C.c_void_p, C.POINTER(Columns), C.POINTER(Fluxes), C.c_int, C.c_void_p]
+        lib.mgcm_rt_run_host.argtypes = [C.c_void_p, C.POINTER(Columns), C.POINTER(Fluxes), C.c_int]
+        lib.mgcm_rt_run_batched.argtypes = [C.c_void_p, C.c_int, C.POINTER(Columns), C.POINTER(Fluxes),
+                                            C.c_int, C.c_void_p]
+        lib.mgcm_rt_check.argtypes = [C.c_void_p]
+        lib.mgcm_rt_planck_table.argtypes = [DP, DP, DP]
+        lib.mgcm_rt_laginterp.argtypes = [DP, DP, C.c_int, DP, DP]
+        lib.mgcm_rt_libm_check.argtypes = [DP, C.c_int, DP, DP, DP]
+        lib.mgcm_rt_strerror.restype = C.c_char_p
+        lib.mgcm_rt_strerror.argtypes = [C.c_int]
+        _lib = lib
+    return _lib
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(DP)
+
+
+def _check(status: int, what: str) -> None:
+    if status != 0:
+        raise RadiationError(status, what)
+
+
+def alloc_outputs(ncol: int, nlay: int) -> dict:
+    nl = nlay + 1
+    o = {k: np.zeros((ncol, nl)) for k in ("fmnetv", "fluxupv", "fluxdnv", "fmneti", "fluxupi", "fluxdni")}
+    for k in ("nfluxtopv", "nfluxtopi", "diffvt"):
+        o[k] = np.zeros(ncol)
+    o["detau"] = np.zeros((ncol, NG, NV))
+    o["cumtauv"] = np.zeros((ncol, NG, NV))
+    o["hr"] = np.zeros((ncol, nlay + 1))
+    return o
+
+
+def column_struct(cols: dict, ptr=lambda a: a.ctypes.data, sol=None) -> Columns:
+    s = Columns()
+    s.ncol = int(cols["ncol"])
+    s.nlay = int(cols["nlay"])
+    for k in IN_KEYS:
+        setattr(s, k, ptr(cols[k]))
+    for i in range(NV):
+        s.sol[i] = float(sol[i])
+    return s
+
+
+def flux_struct(out: dict, ptr=lambda a: a.ctypes.data) -> Fluxes:
+    s = Fluxes()
+    for k in OUT_KEYS:
+        v = out.get(k)
+        setattr(s, k, None if v is None else ptr(v))
+    return s
+
+
+class Radiation:
+    """One radiation context (tables resident on the current HIP device)."""
+
+    def __init__(self, tables: dict):
+        self.lib = load_library()
+        self.tables = tables
+        self.sol = np.asarray(tables["sol"], dtype=np.float64)
+        self._keep = {}
+        t = Tables()
+        for name in ("co2v", "co2i", "pfgasref", "tgasref", "wrefh2o", "gwv", "gwi", "fzerov", "fzeroi",
+                     "tauray", "dwni", "planckir"):
+            a = _f64(tables[name])
+            self._keep[name] = a
+            setattr(t, name, _dp(a))
+        # per-band aerosol constants (the tables' per-level arrays are level-constant)
+        for src, dst in (("qxvdst", "qxv_dst"), ("qsvdst", "qsv_dst"), ("gvdst", "gv_dst"),
+                         ("qxidst", "qxi_dst"), ("qsidst", "qsi_dst"), ("gidst", "gi_dst"),
+                         ("qxvcld", "qxv_cld"), ("qsvcld", "qsv_cld"), ("gvcld", "gv_cld"),
+                         ("qxicld", "qxi_cld"), ("qsicld", "qsi_cld"), ("gicld", "gi_cld")):
+            a = _f64(np.asarray(tables[src])[:, 0])
+            self._keep[dst] = a
+            setattr(t, dst, _dp(a))
+        t.qextref_dst = float(np.asarray(tables["qextrefdst"])[0])
+        t.qextref_cld = float(np.asarray(tables["qextrefcld"])[0])
+        h = C.c_void_p()
+        _check(self.lib.mgcm_rt_init(C.byref(t), C.byref(h)), "mgcm_rt_init")
+        self.ctx = h
+
+    def close(self) -> None:
+        if getattr(self, "ctx", None):
+            self.lib.mgcm_rt_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run_host(self, cols: dict, precision: int = FP64, out: dict | None = None) -> dict:
+        """Host arrays in, host arrays out (copies, runs, synchronises)."""
+        c = {k: _f64(cols[k]) for k in IN_KEYS}
+        c["ncol"], c["nlay"] = int(cols["ncol"]), int(cols["nlay"])
+        o = out if out is not None else alloc_outputs(c["ncol"], c["nlay"])
+        cs = column_struct(c, sol=self.sol)
+        fs = flux_struct(o)
+        _check(self.lib.mgcm_rt_run_host(self.ctx, C.byref(cs), C.byref(fs), int(precision)), "mgcm_rt_run_host")
+        return o
+
+    def run_device(self, cols: dict, out: dict, precision: int = FP64, stream: int = 0) -> None:
+        """Device tensors (torch, float64, contiguous) in and out; asynchronous."""
+        ptr = lambda t: t.data_ptr()  # noqa: E731
+        cs = column_struct(cols, ptr=ptr, sol=self.sol)
+        fs = flux_struct(out, ptr=ptr)
+        _check(self.lib.mgcm_rt_run(self.ctx, C.byref(cs), C.byref(fs), int(precision), C.c_void_p(stream)),
+               "mgcm_rt_run")
+
+    def run_batched(self, grids: list, outs: list, precision: int = FP64, stream: int = 0) -> None:
+        ptr = lambda t: t.data_ptr()  # noqa: E731
+        n = len(grids)
+        cs = (Columns * n)(*[column_struct(g, ptr=ptr, sol=self.sol) for g in grids])
+        fs = (Fluxes * n)(*[flux_struct(o, ptr=ptr) for o in outs])
+        _check(self.lib.mgcm_rt_run_batched(self.ctx, n, cs, fs, int(precision), C.c_void_p(stream)),
+               "mgcm_rt_run_batched")
+
+    def check(self) -> None:
+        _check(self.lib.mgcm_rt_check(self.ctx), "mgcm_rt_check")
+
+
+def planck_table():
+    """PLANCKIR(5,8501) as C (8501, 5), DWNI, WNOI -- built on the device."""
+    lib = load_library()
+    p, dwni, wnoi = np.zeros((8501, NI)), np.zeros(NI), np.zeros(NI)
+    _check(lib.mgcm_rt_planck_table(_dp(p), _dp(dwni), _dp(wnoi)), "mgcm_rt_planck_table")
+    return p, dwni, wnoi
+
+
+def laginterp(pgref11, raw) -> tuple:
+    """LAGINTERP on the device: raw (17, nb, 10, 11, 7) -> (pint[51], (17, nb, 10, 51, 7))."""
+    lib = load_library()
+    raw = _f64(raw)
+    nb = raw.shape[1]
+    out = np.zeros((NG, nb, 10, 51, 7))
+    pint = np.zeros(51)
+    _check(lib.mgcm_rt_laginterp(_dp(_f64(pgref11)), _dp(raw), nb, _dp(pint), _dp(out)), "mgcm_rt_laginterp")
+    return pint, out
+
+
+def libm_check(x) -> tuple:
+    """Device exp/log10/sqrt of x (diagnostic entry of the library)."""
+    lib = load_library()
+    x = _f64(x)
+    e, l, q = np.zeros_like(x), np.zeros_like(x), np.zeros_like(x)
+    _check(lib.mgcm_rt_libm_check(_dp(x), x.size, _dp(e), _dp(l), _dp(q)), "mgcm_rt_libm_check")
+    return e, l, q
+
+
+def to_device(d: dict, device="cuda:0") -> dict:
+    """Copy a dict of numpy arrays to contiguous float64 torch tensors (plumbing only)."""
+    import torch
+    r = {}
+    for k, v in d.items():
+        if isinstance(v, np.ndarray):
+            r[k] = torch.from_numpy(np.ascontiguousarray(v, dtype=np.float64)).to(device)
+        else:
+            r[k] = v
+    return r

@@ -1,0 +1,56 @@
+"""FP64 plain path on the GPU against the oracle (rel <= 1e-10), through the C ABI."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10  # plain FP64 path vs the reference's FP64 arithmetic
+KEYS = ("fmnetv", "fluxupv", "fluxdnv", "fmneti", "fluxupi", "fluxdni", "nfluxtopv", "nfluxtopi",
+        "diffvt", "detau", "cumtauv", "hr")
+
+
+def compare(got, want, keys=KEYS, tol=TOL):
+    errs = {k: rel_err(got[k], want[k]) for k in keys}
+    bad = {k: v for k, v in errs.items() if not v <= tol}
+    assert not bad, f"rel errors above {tol}: {bad}"
+    return errs
+
+
+@pytest.mark.parametrize("nlay,weights,seed", [(24, "dirichlet", 11), (24, "reference", 12), (36, "dirichlet", 13)])
+def test_plain_matches_oracle(oracle, nlay, weights, seed):
+    from mgcm_rt import Radiation, synth
+    t, c = synth.make_workload(seed, 300, nlay, clouds=(nlay == 36), weights=weights)
+    want = oracle.run(t, c)
+    rt = Radiation(t)
+    got = rt.run_host(c)
+    errs = compare(got, want)
+    print(nlay, weights, {k: f"{v:.1e}" for k, v in errs.items()}, flush=True)
+
+
+def test_device_libm_matches_host_libm():
+    """exp bit-identical to the host C library; log10 identical except where the host's
+    log misrounds (bounded rate); sqrt correctly rounded."""
+    import ctypes
+    import math
+    from mgcm_rt import libm_check
+    rng = np.random.Generator(np.random.PCG64(77))
+    x = np.concatenate([rng.uniform(-40, 40, 200000), rng.uniform(-745, 709, 50000),
+                        10 ** rng.uniform(-3.6, 1.2, 100000), [0.0, 1.0, -35.0, 35.0, 1e-300]])
+    e, l, q = libm_check(x)
+    libm = ctypes.CDLL("libm.so.6")
+    for f in ("exp", "log10"):
+        getattr(libm, f).restype = ctypes.c_double
+        getattr(libm, f).argtypes = [ctypes.c_double]
+    he = np.array([libm.exp(v) for v in x])
+    assert np.array_equal(e, he), int(np.sum(e != he))
+    pos = x > 0
+    hl = np.array([libm.log10(v) for v in x[pos]])
+    nbad = int(np.sum(l[pos] != hl))
+    assert nbad <= 1e-3 * pos.sum(), nbad
+    nn = x >= 0
+    assert np.array_equal(q[nn], np.sqrt(x[nn]))
+    print("log10 differences", nbad, "of", int(pos.sum()), flush=True)
