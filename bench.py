@@ -1,0 +1,211 @@
+"""Radiative columns/s of the MI355X column radiation on one GPU.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--ncol 64800] [--nlay 24]
+
+A step is one mgcm_rt_run over the whole grid: 64,800 seeded synthetic columns
+(180x360), 24 layers, 7 VIS + 5 IR bands x 17 Gauss points, inputs resident in
+HBM.  Prints one JSON line.  Besides the plain FP64 path it reports the
+mixed-precision and surrogate forms when they are built, each with its error
+against the FP64 path, the FP64 kernel's roofline position, the reference's
+own Fortran (oracle/_ref, built from its sources) timed on one host core over a
+bounded sample of the same columns, and the C restatement with OpenMP on all
+host cores over the whole grid.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "legacy-mars-global-climate-model_amd")]
+
+FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 vector/matrix peak
+HBM_PEAK_GBS = 8000.0
+SECONDS_PER_SOL = 88775.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--ncol", type=int, default=64800)
+    p.add_argument("--nlay", type=int, default=24)
+    p.add_argument("--seed", type=int, default=2024)
+    p.add_argument("--cpu-sample", type=int, default=20000,
+                   help="columns of the same grid the CPU baseline runs (about 10 s on one core)")
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--copy-calib", action="store_true",
+                   help="also run a 512 MiB device copy (known HBM bytes) for scaling FETCH_SIZE/WRITE_SIZE")
+    return p.parse_args()
+
+
+def timed_steps(fn, steps, warmup, stream):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return e0.elapsed_time(e1) / steps, wall / steps * 1e3
+
+
+def flops_per_column(tables, cols, nsample=1024):
+    """Executed FP64 +-*/ per column, counted by the CPU restatement on a sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle
+    sub = {k: (v[:nsample] if isinstance(v, np.ndarray) else v) for k, v in cols.items()}
+    sub["ncol"] = min(nsample, int(cols["ncol"]))
+    n = Oracle().count(tables, sub)
+    return {k: v / sub["ncol"] for k, v in n.items()}
+
+
+def _sample(cols, n):
+    sub = {k: (v[:n] if isinstance(v, np.ndarray) else v) for k, v in cols.items()}
+    sub["ncol"] = min(n, int(cols["ncol"]))
+    return sub
+
+
+def cpu_baseline(seed, cols, nsample):
+    """The reference's own Fortran (oracle/_ref, built from its sources) on one core.
+
+    It runs with the reference's single GWEIGHT set (one SFLUXV/SFLUXI per
+    column, its native cost); the GPU time does not depend on the weights.
+    """
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Reference
+    from mgcm_rt import synth
+    ref = Reference()
+    if ref.nlay != int(cols["nlay"]):
+        raise RuntimeError(f"reference build has {ref.nlay} layers")
+    t = synth.make_tables(seed, int(cols["nlay"]), weights="reference")
+    ref.run_grid(t, _sample(cols, 64))
+    sub = _sample(cols, nsample)
+    t0 = time.perf_counter()
+    ref.run_grid(t, sub)
+    dt = time.perf_counter() - t0
+    return {"value": sub["ncol"] / dt, "unit": "columns/s", "cores": 1, "kind": "reference",
+            "sample": f"first {sub['ncol']} of the {cols['ncol']} seeded columns, {int(cols['nlay'])} layers, "
+                      f"reference OPTCV/SFLUXV/OPTCI/SFLUXI (amdflang -O2 build), one core, {dt:.2f} s wall"}
+
+
+def cpu_port(tables, cols):
+    """The C restatement with OpenMP on every host core, over the whole grid."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    o = Oracle()
+    o.run(tables, _sample(cols, 64), nthreads=threads)
+    t0 = time.perf_counter()
+    o.run(tables, cols, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": int(cols["ncol"]) / dt, "unit": "columns/s", "cores": threads, "kind": "port",
+            "sample": f"all {cols['ncol']} columns, C restatement with OpenMP, {dt:.2f} s wall"}
+
+
+def traffic_from_profile(ncol, nlay):
+    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary."""
+    path = os.path.join(ROOT, "profiles", f"k_rt_fp64_{ncol}x{nlay}_hbm.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch"):
+            return float(line.split("=")[1].split()[0])
+    return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(columns are independent; the multi-GPU split is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from mgcm_rt import FP64, MIXED, Radiation, alloc_outputs, synth, to_device
+
+    tables, cols = synth.make_workload(args.seed, args.ncol, args.nlay)
+    rt = Radiation(tables)
+    dc = to_device(cols)
+    do = to_device(alloc_outputs(args.ncol, args.nlay))
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+
+    ms, wall_ms = timed_steps(lambda: rt.run_device(dc, do, FP64, sh), args.steps, args.warmup, stream)
+    value = args.ncol / (ms / 1e3)
+
+    fpc = flops_per_column(tables, cols)
+    achieved = fpc["flops"] * args.ncol / (ms / 1e3) / 1e12
+    traffic = traffic_from_profile(args.ncol, args.nlay)
+    result = {
+        "metric": "radiative columns/sec (plain FP64 column radiation, 64,800 columns x 24 layers)",
+        "value": value,
+        "unit": "columns/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded PCG64 generator, mgcm_rt/synth.py)",
+        "config": {"workload": f"plain FP64 radiation step, {args.ncol} columns x {args.nlay} layers, "
+                               "7 VIS + 5 IR bands x 17 Gauss points",
+                   "ncol": args.ncol, "nlay": args.nlay, "seed": args.seed, "launches_per_step": 1},
+        "roofline": {"bound": "mfma", "pipe": "FP64 vector ALU (MI355X FP64 vector peak = matrix peak)",
+                     "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+                     "hbm_GBps": None if traffic is None else traffic / (ms / 1e3) / 1e9,
+                     "flops_per_column": fpc["flops"], "exp_per_column": fpc["exps"],
+                     "sqrt_per_column": fpc["sqrts"], "kernel_ms": ms},
+        "wall_ms_per_step": wall_ms,
+    }
+
+    if args.copy_calib:  # known traffic for scaling the FETCH_SIZE/WRITE_SIZE counters
+        x = torch.empty(64 * 2 ** 20, dtype=torch.float64, device="cuda")
+        x.fill_(1.0)         # writes 512 MiB
+        y = x.clone()        # reads 512 MiB, writes 512 MiB
+        torch.cuda.synchronize()
+        result["copy_calib_bytes"] = int(x.numel() * 8)
+        del x, y
+
+    # other forms, each against the FP64 path on the same columns
+    ref = {k: v.clone() for k, v in do.items() if hasattr(v, "clone")}
+    forms = {}
+    try:
+        dm = to_device(alloc_outputs(args.ncol, args.nlay))
+        msm, _ = timed_steps(lambda: rt.run_device(dc, dm, MIXED, sh), args.steps, args.warmup, stream)
+        err = (dm["hr"] - ref["hr"]).abs()
+        forms["mixed"] = {"value": args.ncol / (msm / 1e3), "ms_per_step": msm,
+                          "speedup_vs_fp64": ms / msm, "max_err_K_per_sol": float(err.max()),
+                          "rms_err_K_per_sol": float(err.pow(2).mean().sqrt()),
+                          "max_flux_err_W_m2": float(max((dm[k] - ref[k]).abs().max() for k in
+                                                         ("fmnetv", "fmneti")))}
+    except Exception as e:  # the form is reported as missing, never substituted
+        forms["mixed"] = {"error": str(e)[:200]}
+    result["forms"] = forms
+
+    if not args.no_cpu:
+        try:
+            result["cpu_baseline"] = cpu_baseline(args.seed, cols, args.cpu_sample)
+        except (FileNotFoundError, RuntimeError) as e:
+            print(f"bench.py: reference build unavailable ({e}); timing the C restatement", file=sys.stderr)
+            result["cpu_baseline"] = cpu_port(tables, _sample(cols, args.cpu_sample))
+        result["cpu_port"] = cpu_port(tables, cols)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -153,17 +153,14 @@ extern "C" int mgcm_rt_destroy(mgcm_rt_ctx *c) {
 }
 
 namespace mgcm {
-// Workspace for one grid: gas and continuum partial sums + skip masks.
-size_t workspace_bytes(int ncol, int nlay) {
-  const size_t nf = 3 * (size_t)(nlay + 1) + 2;
-  return 2 * (size_t)NB * nf * ncol * sizeof(double) + (size_t)NB * ncol * sizeof(uint32_t) + 256;
-}
+// Device workspace of one grid.  The fused kernels keep every intermediate in
+// LDS and registers; the workspace only holds the per-call range flag slot.
+size_t workspace_bytes(int, int) { return 256; }
 
 void bind_workspace(RunArgs &a, void *ws) {
-  const size_t nf = 3 * (size_t)(a.nlay + 1) + 2;
-  a.gas = (double *)ws;
-  a.clear = a.gas + (size_t)NB * nf * a.ncol;
-  a.skip = (uint32_t *)(a.clear + (size_t)NB * nf * a.ncol);
+  a.gas = nullptr;
+  a.clear = nullptr;
+  a.skip = (uint32_t *)ws;
 }
 
 int check_columns(const mgcm_rt_columns *in, const mgcm_rt_fluxes *out) {

@@ -1,0 +1,431 @@
+// rt_item.h -- one two-stream item of the column radiative transfer on the
+// device: the aerosol/Rayleigh optics of a sub-level and of a
+// layer, and the VIS (GFLUXV) and IR (GFLUXI) solves with DSOLVER/DTRIDGL.
+// Every FP64 statement follows the reference's evaluation order; files that
+// include this header are compiled with -ffp-contract=off.
+#pragma once
+#include "rt_internal.h"
+#include "rt_libm.h"
+
+namespace mgcm {
+
+// Aerosol and Rayleigh optical depths of one sub-level K (1-based, 2..NLEV),
+// OPTCV code/optcv.f90:103-123,184-185 and OPTCI code/optci.f90:113-130,184-185.
+struct SubAer {
+  double tray, taeros, tcloud, tlk, tck;
+};
+
+template <bool VIS>
+__device__ __forceinline__ SubAer sub_aerosol(const RtConst *__restrict__ rc, int b,
+                                              const double *__restrict__ plev,
+                                              const double *__restrict__ tauref,
+                                              const double *__restrict__ taurefcld, int K) {
+  SubAer s;
+  const double dpr = plev[K - 1] - plev[K - 2];
+  const double tr = tauref[K - 1] / (VIS ? rc->qextref_vis : rc->qextref_dst);
+  const double tc = taurefcld[K - 1] / rc->qextref_cld;
+  s.tray = VIS ? rc->tauray[b] * dpr : 0.0;
+  s.taeros = tr * rc->qx_dst[b];
+  s.tcloud = tc * rc->qx_cld[b];
+  s.tlk = tr * rc->qs_dst[b];
+  s.tck = tc * rc->qs_cld[b];
+  return s;
+}
+
+// IR padding level L_LEVELS+1 (code/optci.f90:89-95,182-187): TAUREF there is
+// not divided by the reference extinction.
+__device__ __forceinline__ void ir_pad(const RtConst *__restrict__ rc, int b,
+                                       const double *__restrict__ tauref,
+                                       const double *__restrict__ taurefcld, int nlev,
+                                       double &tlk, double &tck) {
+  tlk = tauref[nlev] * rc->qs_dst[b];
+  tck = taurefcld[nlev] * rc->qs_cld[b];
+}
+
+// Layer scattering optical depth and asymmetry of the continuum channel:
+// VIS code/optcv.f90:194-222, IR code/optci.f90:195-219.
+template <bool VIS>
+__device__ void layer_aerosol(const RtConst *__restrict__ rc, int b, const double *__restrict__ plev,
+                              const double *__restrict__ tauref, const double *__restrict__ taurefcld,
+                              int nl, int L, double &scat, double &cosb) {
+  const int nlev = 2 * nl + 1;
+  const int K = 2 * L + 1;
+  const double gd = rc->g_dst[b], gc = rc->g_cld[b];
+  if (VIS) {
+    const double f9999 = F32LIT(0.9999);
+    const SubAer a = sub_aerosol<true>(rc, b, plev, tauref, taurefcld, K);
+    if (L < nl) {
+      const SubAer a2 = sub_aerosol<true>(rc, b, plev, tauref, taurefcld, K + 1);
+      cosb = (gd * a.tlk + gd * a2.tlk + gc * a.tck + gc * a2.tck) /
+             (a.tray + a2.tray + a.tlk + a2.tlk + a.tck + a2.tck);
+      scat = a.tlk + a2.tlk + a.tck + a2.tck + (a.tray + a2.tray) * f9999;
+    } else {
+      cosb = (gd * a.tlk + gc * a.tck) / (a.tray + a.tlk + a.tck);
+      scat = a.tlk + a.tck + a.tray * f9999;
+    }
+  } else {
+    const SubAer a = sub_aerosol<false>(rc, b, plev, tauref, taurefcld, K);
+    double tlk2, tck2;
+    if (K + 1 <= nlev) {
+      const SubAer a2 = sub_aerosol<false>(rc, b, plev, tauref, taurefcld, K + 1);
+      tlk2 = a2.tlk;
+      tck2 = a2.tck;
+    } else {
+      ir_pad(rc, b, tauref, taurefcld, nlev, tlk2, tck2);
+    }
+    scat = a.tlk + tlk2 + a.tck + tck2;
+    cosb = (scat > 0.0) ? (gd * a.tlk + gd * tlk2 + gc * a.tck + gc * tck2) / scat : 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// One two-stream item.  NLM bounds the radiation layer count (arrays are
+// 1-based).  Inputs arrive through the Src functor: dtk(K) gives the sub-level
+// optical depth DTAUK(K) and adds the gas part to taugsurf.
+
+// Per-item state of a VIS item (1-based layer arrays): scaled single-scattering
+// albedo W0, scaled layer optical depth DTAU, the scaled optical depth TAU at
+// each layer top, the scaled optical depth HM at each layer mid-point
+// (TAUCUM(2L+1) of code/gfluxv.f90:76-103) and the reverse-Thomas
+// coefficients AS/DS of the 2*NL rows.  TAUCUM(2L) = TAU(L) for L >= 2; TM1
+// holds TAUCUM(2).
+template <int NLM>
+struct VisState {
+  double w0[NLM + 1], dtau[NLM + 1], tau[NLM + 2], hm[NLM + 1];
+  double tm1;
+  double as[2 * NLM + 1], ds[2 * NLM + 1];
+};
+
+// Two-stream constants of one VIS layer (code/gfluxv.f90:108-135): the
+// quadrature gammas, LAMDA, GAMA and the particular-solution amplitudes AM/AP.
+// Evaluated in the upward elimination and again in the downward flux pass;
+// identical inputs give identical bits.
+struct VisLayer {
+  double g1, g2, g3, lam, gama, am, ap;
+};
+__device__ __forceinline__ VisLayer vis_layer(double w0, double cb, double u0, double f0pi) {
+  const double s3 = 0x1.bb67aep+0;   // SQRT(3.0) in REAL(4) (code/gfluxv.f90:114-116)
+  const double s3h = 0x1.bb67aep-1;  // SQRT(3.0)*0.5 in REAL(4)
+  VisLayer v;
+  v.g1 = s3h * (2.0 - w0 * (1.0 + cb));
+  v.g2 = (s3 * w0 * 0.5) * (1.0 - cb);
+  v.g3 = 0.5 * (1.0 - s3 * cb * u0);
+  v.lam = sqrt(v.g1 * v.g1 - v.g2 * v.g2);
+  v.gama = (v.g1 - v.lam) / v.g2;
+  const double g4 = 1.0 - v.g3;
+  double denom = v.lam * v.lam - 1.0 / (u0 * u0);
+  if (denom == 0.0) denom = F32LIT(1.e-10);
+  v.am = f0pi * w0 * (g4 * (v.g1 + 1.0 / u0) + v.g2 * v.g3) / denom;
+  v.ap = f0pi * w0 * (v.g3 * (v.g1 - 1.0 / u0) + v.g2 * g4) / denom;
+  return v;
+}
+
+// VIS optics of one item + delta-Eddington scaling (OPTCV layer assembly
+// code/optcv.f90:189-274 fused with GETDETAU/GFLUXV code/gfluxv.f90:76-103).
+// Returns DETAU; cumtau receives TAUCUMV(L_LEVELS).
+template <int NLM, typename Src>
+__device__ __forceinline__ double vis_optics(int nl, Src &src, const double *__restrict__ scat,
+                             const double *__restrict__ cosb, VisState<NLM> &s, double &cumtau) {
+  // sub-level 2 (top half of the cap)
+  const double d2 = src.dtk(2);
+  double tcv = d2;  // TAUCUMV(2)
+  double fprev = 0.0, tcv_prev_level = 0.0, tauL = 0.0;  // tauL = TAU(L), carried in a register
+  s.tau[1] = 0.0;
+  for (int L = 1; L <= nl - 1; L++) {
+    const double a = src.dtk(2 * L + 1), b = src.dtk(2 * L + 2);
+    const double dtauv = a + b;
+    const double tcA = tcv + a;  // TAUCUMV(2L+1)
+    const double tcB = tcA + b;  // TAUCUMV(2L+2)
+    const double W = scat[L - 1] / dtauv, C = cosb[L - 1];
+    const double F = 1.0 - W * (C * C);
+    if (L == 1) {
+      tauL = 0.0 * F;      // TAU(1) = TDEL(1)*FACTOR with TDEL(1) = 0
+      s.tau[1] = tauL;
+      s.tm1 = tcv * F;     // TAUCUM(2)
+      s.hm[1] = s.tm1 + (tcA - tcv) * F;
+    } else {
+      // TAUCUM(2L) = TAU(L); TAUCUM(2L+1) uses the factor of layer L-1
+      s.hm[L] = tauL + (tcA - tcv_prev_level) * fprev;
+    }
+    s.w0[L] = W * (1.0 - C * C) / F;
+    const double dtau = dtauv * F;
+    s.dtau[L] = dtau;
+    tauL = tauL + dtau;
+    s.tau[L + 1] = tauL;
+    fprev = F;
+    tcv_prev_level = tcB;  // TAUCUMV(2L+2), the base of the next layer's mid level
+    tcv = tcB;
+  }
+  {
+    const int L = nl;
+    const double a = src.dtk(2 * L + 1);
+    const double dtauv = a;
+    cumtau = tcv + a;  // TAUCUMV(L_LEVELS)
+    const double W = scat[L - 1] / dtauv, C = cosb[L - 1];
+    const double F = 1.0 - W * (C * C);
+    s.w0[L] = W * (1.0 - C * C) / F;
+    const double dtau = dtauv * F;
+    s.dtau[L] = dtau;
+    tauL = tauL + dtau;
+    s.tau[L + 1] = tauL;
+  }
+  return tauL;
+}
+
+// GFLUXV (code/gfluxv.f90:108-280) + DSOLVER/DTRIDGL (code/dsolver.f90,
+// code/dtridgl.f90) on a scaled item.  Calls acc(L, fmidp, fmidm) for each
+// layer mid-point and returns the top fluxes and DIFFV.
+template <int NLM, typename Acc>
+__device__ __forceinline__ void vis_solve(int nl, VisState<NLM> &s, const double *__restrict__ cbar, double u0,
+                                          double f0pi, double rsf, double btop,
+                          double bsurf, Acc &acc, double &fluxup, double &fluxdn, double &diffv) {
+  // bottom-up: layer constants and the reverse-Thomas elimination.  AS/DS of
+  // the row below and the transmission at the layer bottom (the top of the
+  // layer below, the same argument) are carried in registers.
+  double gama_n1 = 0.0, cpm1_n1 = 0.0, cmm1_n1 = 0.0, as1 = 0.0, ds1 = 0.0;
+  double xbot = mgcm_libm::mgcm_exp(-dmin(s.tau[nl + 1] / u0, MAXEXP));
+  for (int n = nl; n >= 1; n--) {
+    const VisLayer v = vis_layer(s.w0[n], cbar[n - 1], u0, f0pi);
+    const double lam = v.lam, gama = v.gama, am = v.am, ap = v.ap;
+    const double xtop = mgcm_libm::mgcm_exp(-dmin(s.tau[n] / u0, MAXEXP));
+    const double cpm1 = ap * xtop, cmm1 = am * xtop;
+    const double cp = ap * xbot, cm = am * xbot;
+    const double ep = mgcm_libm::mgcm_exp(dmin(MAXEXP, lam * s.dtau[n]));
+    const double em = 1.0 / ep;
+    const double e1 = ep + gama * em, e2 = ep - gama * em;
+    const double e3 = gama * ep + em, e4 = gama * ep - em;
+    if (n == nl) {  // surface row (dsolver.f90:82-85)
+      const int i = 2 * nl;
+      const double af = e1 - rsf * e3, bf = e2 - rsf * e4;
+      const double df = bsurf - cp + rsf * cm;
+      as1 = af / bf;
+      ds1 = df / bf;
+      s.as[i] = as1;
+      s.ds[i] = ds1;
+    } else {
+      {  // odd row 2n+1 (dsolver.f90:74-80)
+        const int i = 2 * n + 1;
+        const double af = 2.0 * (1.0 - gama * gama);
+        const double bf = (e1 - e3) * (1.0 + gama_n1);
+        const double cf = (e1 + e3) * (gama_n1 - 1.0);
+        const double df = e3 * (cpm1_n1 - cp) + e1 * (cm - cmm1_n1);
+        const double x = 1.0 / (bf - cf * as1);
+        as1 = af * x;
+        ds1 = (df - cf * ds1) * x;
+        s.as[i] = as1;
+        s.ds[i] = ds1;
+      }
+      {  // even row 2n (dsolver.f90:63-70)
+        const int i = 2 * n;
+        const double af = (e1 + e3) * (gama_n1 - 1.0);
+        const double bf = (e2 + e4) * (gama_n1 - 1.0);
+        const double cf = 2.0 * (1.0 - gama_n1 * gama_n1);
+        const double df = (gama_n1 - 1.0) * (cpm1_n1 - cp) + (1.0 - gama_n1) * (cm - cmm1_n1);
+        const double x = 1.0 / (bf - cf * as1);
+        as1 = af * x;
+        ds1 = (df - cf * ds1) * x;
+        s.as[i] = as1;
+        s.ds[i] = ds1;
+      }
+    }
+    if (n == 1) {  // top row (dsolver.f90:54-57)
+      const double af = 0.0, bf = gama + 1.0, cf = gama - 1.0, df = btop - cmm1;
+      const double x = 1.0 / (bf - cf * as1);
+      s.as[1] = af * x;
+      s.ds[1] = (df - cf * ds1) * x;
+    }
+    gama_n1 = gama;
+    cpm1_n1 = cpm1;
+    cmm1_n1 = cmm1;
+    xbot = xtop;
+  }
+  // top-down: substitution, unmixing and the layer fluxes
+  double xkprev = s.ds[1];
+  double xk1_1 = 0.0, xk2_1 = 0.0, gama_1 = 0.0, am_1 = 0.0, ap_1 = 0.0;
+  for (int n = 1; n <= nl; n++) {
+    const double xa = (n == 1) ? s.ds[1] : s.ds[2 * n - 1] - s.as[2 * n - 1] * xkprev;
+    const double xb = s.ds[2 * n] - s.as[2 * n] * xa;
+    xkprev = xb;
+    const double xk1 = xa + xb;
+    double xk2 = xa - xb;
+    if (xk2 != 0.0 && fabs(xk2 / (xa + F32LIT(1.e-20))) < F32LIT(1.e-30)) xk2 = 0.0;
+    const VisLayer v = vis_layer(s.w0[n], cbar[n - 1], u0, f0pi);
+    const double lam = v.lam, gama = v.gama, am = v.am, ap = v.ap;
+    double fmidp, fmidm;
+    if (n < nl) {  // gfluxv.f90:170-206
+      const double ep = mgcm_libm::mgcm_exp(dmin(MAXEXP, lam * (s.hm[n] - (n == 1 ? s.tm1 : s.tau[n]))));
+      const double em = 1.0 / ep;
+      const double taumid = s.hm[n];
+      const double xm = mgcm_libm::mgcm_exp(-dmin(taumid / u0, MAXEXP));
+      const double cpmid = ap * xm, cmmid = am * xm;
+      fmidp = xk1 * ep + gama * xk2 * em + cpmid;
+      fmidm = xk1 * ep * gama + xk2 * em + cmmid;
+      fmidm = fmidm + u0 * f0pi * xm;
+    } else {  // bottom layer, full sub-layer (gfluxv.f90:243-280)
+      const double ep = mgcm_libm::mgcm_exp(dmin(MAXEXP, lam * (s.tau[nl + 1] - s.tau[nl])));
+      const double em = 1.0 / ep;
+      const double taumid = dmin(s.tau[nl + 1], MAXEXP);
+      const double xm = mgcm_libm::mgcm_exp(-dmin(taumid / u0, MAXEXP));
+      const double cpmid = ap * xm, cmmid = am * xm;
+      fmidp = xk1 * ep + gama * xk2 * em + cpmid;
+      fmidm = xk1 * ep * gama + xk2 * em + cmmid;
+      diffv = fmidm;
+      fmidm = fmidm + u0 * f0pi * xm;
+    }
+    acc(n, fmidp, fmidm);
+    if (n == 1) { xk1_1 = xk1; xk2_1 = xk2; gama_1 = gama; am_1 = am; ap_1 = ap; }
+  }
+  {  // top of the atmosphere (gfluxv.f90:210-238)
+    const double ep = 1.0, em = 1.0;
+    fluxup = xk1_1 * ep + gama_1 * xk2_1 * em + ap_1;
+    fluxdn = xk1_1 * ep * gama_1 + xk2_1 * em + am_1;
+    fluxdn = fluxdn + u0 * f0pi * mgcm_libm::mgcm_exp(-dmin(0.0 / u0, MAXEXP));
+  }
+}
+
+// Per-item state of an IR item: W0 (= WBARI), DTAU (= DTAUI), the half-layer
+// optical depth TAUCUMI(2L+1)-TAUCUMI(2L) and the reverse-Thomas AS/DS.
+template <int NLM>
+struct IrState {
+  double w0[NLM + 1], dtau[NLM + 1], dmid[NLM + 1];
+  double as[2 * NLM + 1], ds[2 * NLM + 1];
+};
+
+// Hemispheric-mean constants of one IR layer (code/gfluxi.f90:69-93): LAMDA,
+// GAMA, TERM and the linear-in-tau Planck source B0 + B1*tau.  bt/bb: Planck
+// values at the top and bottom temperature levels of the layer.
+struct IrLayer {
+  double lam, gama, term, b0, b1;
+};
+__device__ __forceinline__ IrLayer ir_layer(double w0, double cb, double dtau, double bt, double bb) {
+  IrLayer v;
+  const double alpha = sqrt((1.0 - w0) / (1.0 - w0 * cb));
+  v.lam = alpha * (1.0 - w0 * cb) * 2.0;  // /UBARI with UBARI = 0.5: exact
+  v.b1 = (bb - bt) / dtau;
+  v.b0 = bt;
+  v.gama = (1.0 - alpha) / (1.0 + alpha);
+  v.term = UBARI / (1.0 - w0 * cb);
+  return v;
+}
+
+// IR optics of one item (OPTCI code/optci.f90:189-269).
+template <int NLM, typename Src>
+__device__ __forceinline__ void ir_optics(int nl, Src &src, const double *__restrict__ scat, IrState<NLM> &s) {
+  const double f1em9 = F32LIT(1.0e-9);
+  double tcv = src.dtk(2);  // TAUCUMI(2)
+  for (int L = 1; L <= nl; L++) {
+    const double a = src.dtk(2 * L + 1);
+    const double b = (L < nl) ? src.dtk(2 * L + 2) : 0.0;  // DTAUKI(L_LEVELS+1) = 0
+    double dt = a + b + 1.e-50;
+    double w;
+    if (dt > f1em9) {
+      w = scat[L - 1] / dt;
+    } else {
+      w = 0.0;
+      dt = f1em9;
+    }
+    const double tcA = tcv + a;  // TAUCUMI(2L+1)
+    s.dmid[L] = tcA - tcv;       // TAUCUMI(2L+1) - TAUCUMI(2L)
+    tcv = (L < nl) ? tcA + b : tcA;
+    s.w0[L] = w;
+    s.dtau[L] = dt;
+  }
+}
+
+// GFLUXI (code/gfluxi.f90:69-194) + DSOLVER/DTRIDGL.  bk(K) = Planck value of
+// TLEV(K).  Calls acc(L, fmidp, fmidm); returns FTOPUP.
+template <int NLM, typename Acc>
+__device__ __forceinline__ double ir_solve(int nl, IrState<NLM> &s, const double *__restrict__ cosb,
+                                           const double *__restrict__ bk, double rsf,
+                           double btop, double bsurf, Acc &acc) {
+  double gama_n1 = 0.0, cpm1_n1 = 0.0, cmm1_n1 = 0.0, as1 = 0.0, ds1 = 0.0;
+  for (int n = nl; n >= 1; n--) {
+    const double dtau = s.dtau[n];
+    // TLEV(2L), TLEV(2L+2); the bottom layer uses TLEV(2L), TLEV(2L+1)
+    const IrLayer v = ir_layer(s.w0[n], cosb[n - 1], dtau, bk[2 * n - 1], (n < nl) ? bk[2 * n + 1] : bk[2 * n]);
+    const double lam = v.lam, gama = v.gama, term = v.term, b0 = v.b0, b1 = v.b1;
+    const double cp = b0 + b1 * dtau + b1 * term;
+    const double cm = b0 + b1 * dtau - b1 * term;
+    const double cpm1 = b0 + b1 * term;
+    const double cmm1 = b0 - b1 * term;
+    const double ep = mgcm_libm::mgcm_exp(dmin(lam * dtau, MAXEXP));
+    const double em = 1.0 / ep;
+    const double e1 = ep + gama * em, e2 = ep - gama * em;
+    const double e3 = gama * ep + em, e4 = gama * ep - em;
+    if (n == nl) {
+      const int i = 2 * nl;
+      const double af = e1 - rsf * e3, bf = e2 - rsf * e4;
+      const double df = bsurf - cp + rsf * cm;
+      as1 = af / bf;
+      ds1 = df / bf;
+      s.as[i] = as1;
+      s.ds[i] = ds1;
+    } else {
+      {
+        const int i = 2 * n + 1;
+        const double af = 2.0 * (1.0 - gama * gama);
+        const double bf = (e1 - e3) * (1.0 + gama_n1);
+        const double cf = (e1 + e3) * (gama_n1 - 1.0);
+        const double df = e3 * (cpm1_n1 - cp) + e1 * (cm - cmm1_n1);
+        const double x = 1.0 / (bf - cf * as1);
+        as1 = af * x;
+        ds1 = (df - cf * ds1) * x;
+        s.as[i] = as1;
+        s.ds[i] = ds1;
+      }
+      {
+        const int i = 2 * n;
+        const double af = (e1 + e3) * (gama_n1 - 1.0);
+        const double bf = (e2 + e4) * (gama_n1 - 1.0);
+        const double cf = 2.0 * (1.0 - gama_n1 * gama_n1);
+        const double df = (gama_n1 - 1.0) * (cpm1_n1 - cp) + (1.0 - gama_n1) * (cm - cmm1_n1);
+        const double x = 1.0 / (bf - cf * as1);
+        as1 = af * x;
+        ds1 = (df - cf * ds1) * x;
+        s.as[i] = as1;
+        s.ds[i] = ds1;
+      }
+    }
+    if (n == 1) {
+      const double af = 0.0, bf = gama + 1.0, cf = gama - 1.0, df = btop - cmm1;
+      const double x = 1.0 / (bf - cf * as1);
+      s.as[1] = af * x;
+      s.ds[1] = (df - cf * ds1) * x;
+    }
+    gama_n1 = gama;
+    cpm1_n1 = cpm1;
+    cmm1_n1 = cmm1;
+  }
+  double xkprev = s.ds[1];
+  double xk1_1 = 0.0, xk2_1 = 0.0, gama_1 = 0.0, b0_1 = 0.0, b1_1 = 0.0, term_1 = 0.0;
+  for (int n = 1; n <= nl; n++) {
+    const double xa = (n == 1) ? s.ds[1] : s.ds[2 * n - 1] - s.as[2 * n - 1] * xkprev;
+    const double xb = s.ds[2 * n] - s.as[2 * n] * xa;
+    xkprev = xb;
+    const double xk1 = xa + xb;
+    double xk2 = xa - xb;
+    if (xk2 != 0.0 && fabs(xk2 / (xa + F32LIT(1.e-20))) < F32LIT(1.e-30)) xk2 = 0.0;
+    const IrLayer v = ir_layer(s.w0[n], cosb[n - 1], s.dtau[n], bk[2 * n - 1], (n < nl) ? bk[2 * n + 1] : bk[2 * n]);
+    const double lam = v.lam, gama = v.gama, b0 = v.b0, b1 = v.b1, term = v.term;
+    const double dk = (n < nl) ? s.dmid[n] : s.dtau[n];
+    const double ep = mgcm_libm::mgcm_exp(dmin(lam * dk, MAXEXP));
+    const double em = 1.0 / ep;
+    const double cpmid = b0 + b1 * dk + b1 * term;
+    const double cmmid = b0 + b1 * dk - b1 * term;
+    double fmidp = xk1 * ep + gama * xk2 * em + cpmid;
+    double fmidm = xk1 * ep * gama + xk2 * em + cmmid;
+    fmidp = fmidp * PI;
+    fmidm = fmidm * PI;
+    acc(n, fmidp, fmidm);
+    if (n == 1) { xk1_1 = xk1; xk2_1 = xk2; gama_1 = gama; b0_1 = b0; b1_1 = b1; term_1 = term; }
+  }
+  const double ep = 1.0, em = 1.0;
+  const double cpmid = b0_1 + b1_1 * term_1;
+  const double cmmid = b0_1 - b1_1 * term_1;
+  const double fluxup = xk1_1 * ep + gama_1 * xk2_1 * em + cpmid;
+  const double fluxdn = xk1_1 * ep * gama_1 + xk2_1 * em + cmmid;
+  return (fluxup - fluxdn) * PI;
+}
+
+// ---------------------------------------------------------------------------
+}  // namespace mgcm

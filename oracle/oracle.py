@@ -186,6 +186,7 @@ class Reference:
         self.lib.ref_nlay(C.byref(n))
         self.nlay = n.value
         self.lib.ref_column.argtypes = [DP] * 33 + [C.c_double] * 3 + [DP] * 11
+        self.lib.ref_grid.argtypes = [C.c_int] + [DP] * 47
         self.lib.ref_set_planck.argtypes = [DP]
         self.lib.ref_get_planck.argtypes = [DP]
         self.lib.ref_setspv.argtypes = [DP] * 5
@@ -213,6 +214,25 @@ class Reference:
                                 float(c["als"][i]), float(c["albi"][i]), float(c["acosz"][i]),
                                 *[_p(a) for a in outs], *[_p(a) for a in sc],
                                 _p(o["detau"][i]), _p(o["cumtauv"][i]))
+        _heating(t, c, o)
+        return o
+
+    def run_grid(self, t: dict, c: dict) -> dict:
+        """All columns in one call of the harness's column loop (no per-column Python)."""
+        assert int(t["nlay"]) == self.nlay, "reference build is compiled for %d layers" % self.nlay
+        ncol = int(c["ncol"])
+        o = alloc_outputs(ncol, self.nlay)
+        self.lib.ref_set_planck(_p(_c(t["planckir"])))
+        tabs = [_c(t[k]) for k in ("co2v", "co2i", "pfgasref", "tgasref", "wrefh2o", "gwv", "gwi",
+                                   "fzerov", "fzeroi", "tauray", "dwni", "sol",
+                                   "qxvdst", "qsvdst", "gvdst", "qxvcld", "qsvcld", "gvcld",
+                                   "qxidst", "qsidst", "gidst", "qxicld", "qsicld", "gicld",
+                                   "qextrefdst", "qextrefcld")]
+        prof = [_c(c[k][:ncol]) for k in ("plev", "tlev", "pmid", "tmid", "qh2o", "tauref", "taurefcld",
+                                          "als", "albi", "acosz")]
+        outs = [o[k] for k in ("fmnetv", "fluxupv", "fluxdnv", "fmneti", "fluxupi", "fluxdni",
+                               "nfluxtopv", "nfluxtopi", "diffvt", "detau", "cumtauv")]
+        self.lib.ref_grid(ncol, *[_p(a) for a in tabs], *[_p(a) for a in prof], *[_p(a) for a in outs])
         _heating(t, c, o)
         return o
 

@@ -178,6 +178,60 @@
       end if
       end subroutine ref_column
 
+!     A grid of ncol columns in one call (the per-column loop of COMP3 over
+!     the grid, code/comp3.f:637-738), so that the reference's own cost can be
+!     timed without per-call overhead.  Arrays: one column after another.
+      subroutine ref_grid(ncol, co2v, co2i, pfgasref, tgasref, wrefh2o,  &
+           gwv, gwi, fzerov, fzeroi, tauray, dwni, sol,                  &
+           qxvdst, qsvdst, gvdst, qxvcld, qsvcld, gvcld,                 &
+           qxidst, qsidst, gidst, qxicld, qsicld, gicld,                 &
+           qextrefdst, qextrefcld,                                       &
+           plev, tlev, pmid, tmid, qh2o, tauref, taurefcld,              &
+           als, albi, acosz,                                             &
+           fmnetv, fluxupv, fluxdnv, fmneti, fluxupi, fluxdni,           &
+           nfluxtopv, nfluxtopi, diffvt, detau_o, cumtauv_o)             &
+           bind(C, name="ref_grid")
+      integer(c_int), value :: ncol
+      real(c_double), intent(in) :: co2v(L_NTREF,L_PINT,L_REFH2O,L_NSPECTV,L_NGAUSS)
+      real(c_double), intent(in) :: co2i(L_NTREF,L_PINT,L_REFH2O,L_NSPECTI,L_NGAUSS)
+      real(c_double), intent(in) :: pfgasref(L_PINT), tgasref(L_NTREF), wrefh2o(L_REFH2O)
+      real(c_double), intent(in) :: gwv(L_NGAUSS,L_NSPECTV), gwi(L_NGAUSS,L_NSPECTI)
+      real(c_double), intent(in) :: fzerov(L_NSPECTV), fzeroi(L_NSPECTI)
+      real(c_double), intent(in) :: tauray(L_NSPECTV), dwni(L_NSPECTI), sol(L_NSPECTV)
+      real(c_double), intent(in) :: qxvdst(L_LEVELS+1,L_NSPECTV), qsvdst(L_LEVELS+1,L_NSPECTV)
+      real(c_double), intent(in) :: gvdst(L_LEVELS+1,L_NSPECTV), qxvcld(L_LEVELS+1,L_NSPECTV)
+      real(c_double), intent(in) :: qsvcld(L_LEVELS+1,L_NSPECTV), gvcld(L_LEVELS+1,L_NSPECTV)
+      real(c_double), intent(in) :: qxidst(L_LEVELS+1,L_NSPECTI), qsidst(L_LEVELS+1,L_NSPECTI)
+      real(c_double), intent(in) :: gidst(L_LEVELS+1,L_NSPECTI), qxicld(L_LEVELS+1,L_NSPECTI)
+      real(c_double), intent(in) :: qsicld(L_LEVELS+1,L_NSPECTI), gicld(L_LEVELS+1,L_NSPECTI)
+      real(c_double), intent(in) :: qextrefdst(L_LEVELS+1), qextrefcld(L_LEVELS+1)
+      real(c_double), intent(in) :: plev(L_LEVELS,ncol), tlev(L_LEVELS,ncol), pmid(L_LEVELS,ncol)
+      real(c_double), intent(in) :: tmid(L_LEVELS,ncol), qh2o(L_LEVELS,ncol)
+      real(c_double), intent(in) :: tauref(L_LEVELS+1,ncol), taurefcld(L_LEVELS+1,ncol)
+      real(c_double), intent(in) :: als(ncol), albi(ncol), acosz(ncol)
+      real(c_double), intent(out) :: fmnetv(L_NLAYRAD,ncol), fluxupv(L_NLAYRAD,ncol)
+      real(c_double), intent(out) :: fluxdnv(L_NLAYRAD,ncol), fmneti(L_NLAYRAD,ncol)
+      real(c_double), intent(out) :: fluxupi(L_NLAYRAD,ncol), fluxdni(L_NLAYRAD,ncol)
+      real(c_double), intent(out) :: nfluxtopv(ncol), nfluxtopi(ncol), diffvt(ncol)
+      real(c_double), intent(inout) :: detau_o(L_NSPECTV,L_NGAUSS,ncol)
+      real(c_double), intent(inout) :: cumtauv_o(L_NSPECTV,L_NGAUSS,ncol)
+      integer :: i
+      do i = 1, ncol
+        call ref_column(co2v, co2i, pfgasref, tgasref, wrefh2o,         &
+             gwv, gwi, fzerov, fzeroi, tauray, dwni, sol,                &
+             qxvdst, qsvdst, gvdst, qxvcld, qsvcld, gvcld,               &
+             qxidst, qsidst, gidst, qxicld, qsicld, gicld,               &
+             qextrefdst, qextrefcld,                                     &
+             plev(:,i), tlev(:,i), pmid(:,i), tmid(:,i), qh2o(:,i),      &
+             tauref(:,i), taurefcld(:,i),                                &
+             als(i), albi(i), acosz(i),                                  &
+             fmnetv(:,i), fluxupv(:,i), fluxdnv(:,i),                    &
+             fmneti(:,i), fluxupi(:,i), fluxdni(:,i),                    &
+             nfluxtopv(i), nfluxtopi(i), diffvt(i),                      &
+             detau_o(:,:,i), cumtauv_o(:,:,i))
+      end do
+      end subroutine ref_grid
+
 !     Init-time tables from the reference's own set-up routines.
       subroutine ref_setspv(wnov, dwnv, wavev, solarf, tauray)          &
            bind(C, name="ref_setspv")
