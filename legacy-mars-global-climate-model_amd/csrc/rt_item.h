@@ -184,13 +184,22 @@ __device__ __forceinline__ void vis_solve(int nl, VisState<NLM> &s, const double
   // layer below, the same argument) are carried in registers.
   double gama_n1 = 0.0, cpm1_n1 = 0.0, cmm1_n1 = 0.0, as1 = 0.0, ds1 = 0.0;
   double xbot = mgcm_libm::mgcm_exp(-dmin(s.tau[nl + 1] / u0, MAXEXP));
+  // the next layer's state is loaded one iteration ahead
+  double w0_n = s.w0[nl], cb_n = cbar[nl - 1], tau_n = s.tau[nl], dtau_n = s.dtau[nl];
   for (int n = nl; n >= 1; n--) {
-    const VisLayer v = vis_layer(s.w0[n], cbar[n - 1], u0, f0pi);
+    const double w0c = w0_n, cbc = cb_n, tauc = tau_n, dtauc = dtau_n;
+    if (n > 1) {
+      w0_n = s.w0[n - 1];
+      cb_n = cbar[n - 2];
+      tau_n = s.tau[n - 1];
+      dtau_n = s.dtau[n - 1];
+    }
+    const VisLayer v = vis_layer(w0c, cbc, u0, f0pi);
     const double lam = v.lam, gama = v.gama, am = v.am, ap = v.ap;
-    const double xtop = mgcm_libm::mgcm_exp(-dmin(s.tau[n] / u0, MAXEXP));
+    const double xtop = mgcm_libm::mgcm_exp(-dmin(tauc / u0, MAXEXP));
     const double cpm1 = ap * xtop, cmm1 = am * xtop;
     const double cp = ap * xbot, cm = am * xbot;
-    const double ep = mgcm_libm::mgcm_exp(dmin(MAXEXP, lam * s.dtau[n]));
+    const double ep = mgcm_libm::mgcm_exp(dmin(MAXEXP, lam * dtauc));
     const double em = 1.0 / ep;
     const double e1 = ep + gama * em, e2 = ep - gama * em;
     const double e3 = gama * ep + em, e4 = gama * ep - em;
@@ -242,29 +251,47 @@ __device__ __forceinline__ void vis_solve(int nl, VisState<NLM> &s, const double
   // top-down: substitution, unmixing and the layer fluxes
   double xkprev = s.ds[1];
   double xk1_1 = 0.0, xk2_1 = 0.0, gama_1 = 0.0, am_1 = 0.0, ap_1 = 0.0;
+  // row pair, layer state and sub-level depth of the next layer, loaded one iteration ahead
+  double as1_n = s.as[1], ds1_n = s.ds[1], as2_n = s.as[2], ds2_n = s.ds[2];
+  w0_n = s.w0[1];
+  cb_n = cbar[0];
+  double hm_n = s.hm[1], tt_n = s.tm1;
+  const double tau_bot = s.tau[nl + 1], tau_lastl = s.tau[nl];
   for (int n = 1; n <= nl; n++) {
-    const double xa = (n == 1) ? s.ds[1] : s.ds[2 * n - 1] - s.as[2 * n - 1] * xkprev;
-    const double xb = s.ds[2 * n] - s.as[2 * n] * xa;
+    const double as1c = as1_n, ds1c = ds1_n, as2c = as2_n, ds2c = ds2_n;
+    const double w0c = w0_n, cbc = cb_n, hmc = hm_n, ttc = tt_n;
+    if (n < nl) {
+      as1_n = s.as[2 * n + 1];
+      ds1_n = s.ds[2 * n + 1];
+      as2_n = s.as[2 * n + 2];
+      ds2_n = s.ds[2 * n + 2];
+      w0_n = s.w0[n + 1];
+      cb_n = cbar[n];
+      hm_n = s.hm[n + 1];
+      tt_n = s.tau[n + 1];
+    }
+    const double xa = (n == 1) ? ds1c : ds1c - as1c * xkprev;
+    const double xb = ds2c - as2c * xa;
     xkprev = xb;
     const double xk1 = xa + xb;
     double xk2 = xa - xb;
     if (xk2 != 0.0 && fabs(xk2 / (xa + F32LIT(1.e-20))) < F32LIT(1.e-30)) xk2 = 0.0;
-    const VisLayer v = vis_layer(s.w0[n], cbar[n - 1], u0, f0pi);
+    const VisLayer v = vis_layer(w0c, cbc, u0, f0pi);
     const double lam = v.lam, gama = v.gama, am = v.am, ap = v.ap;
     double fmidp, fmidm;
     if (n < nl) {  // gfluxv.f90:170-206
-      const double ep = mgcm_libm::mgcm_exp(dmin(MAXEXP, lam * (s.hm[n] - (n == 1 ? s.tm1 : s.tau[n]))));
+      const double ep = mgcm_libm::mgcm_exp(dmin(MAXEXP, lam * (hmc - ttc)));
       const double em = 1.0 / ep;
-      const double taumid = s.hm[n];
+      const double taumid = hmc;
       const double xm = mgcm_libm::mgcm_exp(-dmin(taumid / u0, MAXEXP));
       const double cpmid = ap * xm, cmmid = am * xm;
       fmidp = xk1 * ep + gama * xk2 * em + cpmid;
       fmidm = xk1 * ep * gama + xk2 * em + cmmid;
       fmidm = fmidm + u0 * f0pi * xm;
     } else {  // bottom layer, full sub-layer (gfluxv.f90:243-280)
-      const double ep = mgcm_libm::mgcm_exp(dmin(MAXEXP, lam * (s.tau[nl + 1] - s.tau[nl])));
+      const double ep = mgcm_libm::mgcm_exp(dmin(MAXEXP, lam * (tau_bot - tau_lastl)));
       const double em = 1.0 / ep;
-      const double taumid = dmin(s.tau[nl + 1], MAXEXP);
+      const double taumid = dmin(tau_bot, MAXEXP);
       const double xm = mgcm_libm::mgcm_exp(-dmin(taumid / u0, MAXEXP));
       const double cpmid = ap * xm, cmmid = am * xm;
       fmidp = xk1 * ep + gama * xk2 * em + cpmid;
@@ -339,10 +366,19 @@ __device__ __forceinline__ double ir_solve(int nl, IrState<NLM> &s, const double
                                            const double *__restrict__ bk, double rsf,
                            double btop, double bsurf, Acc &acc) {
   double gama_n1 = 0.0, cpm1_n1 = 0.0, cmm1_n1 = 0.0, as1 = 0.0, ds1 = 0.0;
+  // the next layer's state is loaded one iteration ahead
+  double dtau_n = s.dtau[nl], w0_n = s.w0[nl], cb_n = cosb[nl - 1], bt_n = bk[2 * nl - 1], bb_n = bk[2 * nl];
   for (int n = nl; n >= 1; n--) {
-    const double dtau = s.dtau[n];
+    const double dtau = dtau_n, w0c = w0_n, cbc = cb_n, btc = bt_n, bbc = bb_n;
+    if (n > 1) {
+      dtau_n = s.dtau[n - 1];
+      w0_n = s.w0[n - 1];
+      cb_n = cosb[n - 2];
+      bt_n = bk[2 * n - 3];
+      bb_n = bk[2 * n - 1];
+    }
     // TLEV(2L), TLEV(2L+2); the bottom layer uses TLEV(2L), TLEV(2L+1)
-    const IrLayer v = ir_layer(s.w0[n], cosb[n - 1], dtau, bk[2 * n - 1], (n < nl) ? bk[2 * n + 1] : bk[2 * n]);
+    const IrLayer v = ir_layer(w0c, cbc, dtau, btc, bbc);
     const double lam = v.lam, gama = v.gama, term = v.term, b0 = v.b0, b1 = v.b1;
     const double cp = b0 + b1 * dtau + b1 * term;
     const double cm = b0 + b1 * dtau - b1 * term;
@@ -398,16 +434,36 @@ __device__ __forceinline__ double ir_solve(int nl, IrState<NLM> &s, const double
   }
   double xkprev = s.ds[1];
   double xk1_1 = 0.0, xk2_1 = 0.0, gama_1 = 0.0, b0_1 = 0.0, b1_1 = 0.0, term_1 = 0.0;
+  double as1_n = s.as[1], ds1_n = s.ds[1], as2_n = s.as[2], ds2_n = s.ds[2];
+  w0_n = s.w0[1];
+  cb_n = cosb[0];
+  bt_n = bk[1];
+  bb_n = (1 < nl) ? bk[3] : bk[2];
+  double dt_n = s.dtau[1], dm_n = s.dmid[1];
   for (int n = 1; n <= nl; n++) {
-    const double xa = (n == 1) ? s.ds[1] : s.ds[2 * n - 1] - s.as[2 * n - 1] * xkprev;
-    const double xb = s.ds[2 * n] - s.as[2 * n] * xa;
+    const double as1c = as1_n, ds1c = ds1_n, as2c = as2_n, ds2c = ds2_n;
+    const double w0c = w0_n, cbc = cb_n, dtc = dt_n, dmc = dm_n, btc = bt_n, bbc = bb_n;
+    if (n < nl) {
+      as1_n = s.as[2 * n + 1];
+      ds1_n = s.ds[2 * n + 1];
+      as2_n = s.as[2 * n + 2];
+      ds2_n = s.ds[2 * n + 2];
+      w0_n = s.w0[n + 1];
+      cb_n = cosb[n];
+      dt_n = s.dtau[n + 1];
+      dm_n = s.dmid[n + 1];
+      bt_n = bk[2 * n + 1];
+      bb_n = (n + 1 < nl) ? bk[2 * n + 3] : bk[2 * n + 2];
+    }
+    const double xa = (n == 1) ? ds1c : ds1c - as1c * xkprev;
+    const double xb = ds2c - as2c * xa;
     xkprev = xb;
     const double xk1 = xa + xb;
     double xk2 = xa - xb;
     if (xk2 != 0.0 && fabs(xk2 / (xa + F32LIT(1.e-20))) < F32LIT(1.e-30)) xk2 = 0.0;
-    const IrLayer v = ir_layer(s.w0[n], cosb[n - 1], s.dtau[n], bk[2 * n - 1], (n < nl) ? bk[2 * n + 1] : bk[2 * n]);
+    const IrLayer v = ir_layer(w0c, cbc, dtc, btc, bbc);
     const double lam = v.lam, gama = v.gama, b0 = v.b0, b1 = v.b1, term = v.term;
-    const double dk = (n < nl) ? s.dmid[n] : s.dtau[n];
+    const double dk = (n < nl) ? dmc : dtc;
     const double ep = mgcm_libm::mgcm_exp(dmin(lam * dk, MAXEXP));
     const double em = 1.0 / ep;
     const double cpmid = b0 + b1 * dk + b1 * term;

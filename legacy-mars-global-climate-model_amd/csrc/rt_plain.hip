@@ -144,13 +144,25 @@ struct ClearSrc {
   }
 };
 
-// 16-lane fixed-order sum; every lane of the group returns lane 0's value.
+// Cross-lane move of a double inside 16-lane rows (DPP, no LDS round trip).
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)b, CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// 16-lane fixed-order sum over a row: pairs (lane, lane^1), then quads
+// (lane^2), then the two quads of each half (half-mirror), then the two halves
+// (mirror).  Each step adds two equal-order partial sums, so every lane of the
+// row ends with the same bits.  Needs all 64 lanes active.
 __device__ __forceinline__ double group16_sum(double v) {
-  v = v + __shfl_xor(v, 8, 16);
-  v = v + __shfl_xor(v, 4, 16);
-  v = v + __shfl_xor(v, 2, 16);
-  v = v + __shfl_xor(v, 1, 16);
-  return __shfl(v, 0, 16);
+  v = v + dpp_f64<0xB1>(v);   // quad_perm [1,0,3,2]
+  v = v + dpp_f64<0x4E>(v);   // quad_perm [2,3,0,1]
+  v = v + dpp_f64<0x141>(v);  // row_half_mirror
+  v = v + dpp_f64<0x140>(v);  // row_mirror
+  return v;
 }
 
 __device__ __forceinline__ int planck_index(double T, int *flag) {
