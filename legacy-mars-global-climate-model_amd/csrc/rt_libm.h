@@ -78,7 +78,8 @@ MGCM_HD double exp_special(double tmp, uint64_t sbits, uint64_t ki) {
   return 0x1p-1022 * y;
 }
 
-MGCM_HD double mgcm_exp(double x) {
+template <class Tab>
+MGCM_HD double exp_core(double x, Tab tab) {
   const double InvLn2N = 0x1.71547652b82fep7, NegLn2hiN = -0x1.62e42fefa0000p-8,
                NegLn2loN = -0x1.cf79abc9e3b3ap-47, Shift = 0x1.8p52;
   const double C2 = 0x1.ffffffffffdbdp-2, C3 = 0x1.555555555543cp-3, C4 = 0x1.55555cf172b91p-5,
@@ -99,8 +100,8 @@ MGCM_HD double mgcm_exp(double x) {
   const double r = MGCM_FMA(kd, NegLn2loN, MGCM_FMA(kd, NegLn2hiN, x));
   const int idx = (int)(2 * (ki % 128));
   const uint64_t top = ki << 45;
-  const double tail = asd(exp_tab(idx));
-  const uint64_t sbits = exp_tab(idx + 1) + top;
+  const double tail = asd(tab(idx));
+  const uint64_t sbits = tab(idx + 1) + top;
   const double r2 = r * r;
   const double p1 = MGCM_FMA(r, C3, C2), p2 = MGCM_FMA(r, C5, C4);
   const double tmp = MGCM_FMA(r2 * r2, p2, MGCM_FMA(r2, p1, tail + r));
@@ -108,6 +109,28 @@ MGCM_HD double mgcm_exp(double x) {
   const double scale = asd(sbits);
   return MGCM_FMA(scale, tmp, scale);
 }
+
+struct TabConst {
+  MGCM_HD uint64_t operator()(int i) const { return exp_tab(i); }
+};
+MGCM_HD double mgcm_exp(double x) { return exp_core(x, TabConst{}); }
+
+#if defined(__HIPCC__)
+// The same exp with its 2 KiB table read from the first 256 entries of the
+// launching kernel's dynamic LDS, which the kernel fills with exp_tab() before
+// the first call (load_lds_exptab).  LDS reads keep the table lookups out of
+// the vector-memory queue, where they would wait behind pending stores.
+extern __shared__ uint64_t mgcm_lds_exptab[];
+struct TabLds {
+  __device__ __forceinline__ uint64_t operator()(int i) const { return mgcm_lds_exptab[i]; }
+};
+__device__ __forceinline__ double mgcm_exp_lds(double x) { return exp_core(x, TabLds{}); }
+// Fills the table; every thread of the block calls it, then the block synchronises.
+__device__ __forceinline__ void load_lds_exptab(int tid, int nthreads) {
+  for (int i = tid; i < 256; i += nthreads) mgcm_lds_exptab[i] = exp_tab(i);
+}
+constexpr int LDS_EXPTAB_BYTES = 256 * 8;
+#endif
 
 // ---- double-double helpers ----
 struct DD {

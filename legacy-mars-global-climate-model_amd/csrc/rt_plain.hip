@@ -32,13 +32,18 @@ struct Stage {
   uint32_t *skip;              // [4][NB]    TLIMIT skip masks
 };
 
+// Dynamic LDS of one wave: the exp table, then the staging.  The continuum
+// pass's hand-over area (64 x 3) reuses c1, which that pass does not read
+// (4 * nsub >= 192 for every nlay >= 23; smaller layer counts get their own).
 size_t stage_bytes(int nlay) {
   const size_t nl = nlay + 1, nlev = 2 * (size_t)nlay + 3, nsub = nlev - 1;
-  return sizeof(double) * (4 * nsub * 6 + 4 * nl * 3 + 4 * nlev + 64 * 3) +
+  const size_t xfer = 4 * nsub >= 64 * 3 ? 0 : 64 * 3;
+  return mgcm_libm::LDS_EXPTAB_BYTES + sizeof(double) * (4 * nsub * 6 + 4 * nl * 3 + 4 * nlev + xfer) +
          sizeof(int) * (4 * nsub + 4 * NB);
 }
 
 __device__ inline Stage carve(double *p, int nsub, int nl, int nlev) {
+  p += mgcm_libm::LDS_EXPTAB_BYTES / sizeof(double);
   Stage S;
   S.t = p; p += 4 * nsub;
   S.u = p; p += 4 * nsub;
@@ -50,7 +55,12 @@ __device__ inline Stage carve(double *p, int nsub, int nl, int nlev) {
   S.cosb = p; p += 4 * nl;
   S.cbar = p; p += 4 * nl;
   S.bk = p; p += 4 * nlev;
-  S.xfer = p; p += 64 * 3;
+  if (4 * nsub >= 64 * 3) {
+    S.xfer = S.c1;
+  } else {
+    S.xfer = p;
+    p += 64 * 3;
+  }
   S.base = (int *)p;
   S.skip = (uint32_t *)(S.base + 4 * nsub);
   return S;
@@ -202,7 +212,7 @@ __device__ __forceinline__ double acc_get(const double *arr, int n) {
 }
 
 #ifndef MGCM_FP64_WAVES
-#define MGCM_FP64_WAVES 3
+#define MGCM_FP64_WAVES 2
 #endif
 template <int NLM>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MGCM_FP64_WAVES)))
@@ -217,6 +227,8 @@ void k_rt_fp64(DevTables dt, RunArgs a) {
   const int colc = valid ? col : a.ncol - 1;
   const RtConst *rc = dt.c;
   Stage S = carve(smem, nsub, nl, nlev);
+  mgcm_libm::load_lds_exptab(lane, 64);
+  __syncthreads();
 
   // 1. TPINDEX of every sub-level K = 2..L_LEVELS of the 4 columns
   for (int t = lane; t < 4 * nsub; t += 64) {
@@ -320,7 +332,7 @@ void k_rt_fp64(DevTables dt, RunArgs a) {
           acc_add<SL>(acc.vdn, n, g0, group16_sum(dn));
         };
         double fluxup = 0.0, fluxdn = 0.0, diffv = 0.0;
-        const double bsurf = a.als[colc] * u0 * a.sol[nw] * mgcm_libm::mgcm_exp(-dmin(detau / u0, MAXEXP));
+        const double bsurf = a.als[colc] * u0 * a.sol[nw] * rt_exp(-dmin(detau / u0, MAXEXP));
         vis_solve<NLM>(nl, s, cbar, u0, a.sol[nw], a.als[colc], 0.0, bsurf, accf, fluxup, fluxdn, diffv);
         topv = topv + group16_sum(skip ? 0.0 : (fluxup - fluxdn) * gw * onemf);
         difv = difv + group16_sum(skip ? 0.0 : diffv * gw * onemf);
@@ -346,7 +358,7 @@ void k_rt_fp64(DevTables dt, RunArgs a) {
         const double rsf = a.albi[colc];
         const double bsurf = (1. - rsf) * bk[nlev - 1];
         const double tautop = s.dtau[1] * plev[1] / (plev[3] - plev[1]);
-        const double btop = (1.0 - mgcm_libm::mgcm_exp(-tautop * 2.0)) * bk[1];  // /UBARI, UBARI = 0.5
+        const double btop = (1.0 - rt_exp(-tautop * 2.0)) * bk[1];  // /UBARI, UBARI = 0.5
         const double ftopup = ir_solve<NLM>(nl, s, cosb, bk, rsf, btop, bsurf, accf);
         topi = topi + group16_sum(skip ? 0.0 : ftopup * dw * gw * onemf);
       }
@@ -426,7 +438,7 @@ void k_rt_fp64(DevTables dt, RunArgs a) {
           xfer_add(n, lsun ? (fp - fm) * fzero : 0.0, lsun ? fp * fzero : 0.0, lsun ? fm * fzero : 0.0,
                    acc.vnet, acc.vup, acc.vdn);
         };
-        const double bsurf = a.als[clc] * mu * a.sol[b] * mgcm_libm::mgcm_exp(-dmin(detau / mu, MAXEXP));
+        const double bsurf = a.als[clc] * mu * a.sol[b] * rt_exp(-dmin(detau / mu, MAXEXP));
         vis_solve<NLM>(nl, s, cbar, mu, a.sol[b], a.als[clc], 0.0, bsurf, accf, fluxup, fluxdn, diffv);
         topv = topv + xfer_scalar(lsun ? (fluxup - fluxdn) * fzero : 0.0);
         difv = difv + xfer_scalar(lsun ? diffv * fzero : 0.0);
@@ -443,7 +455,7 @@ void k_rt_fp64(DevTables dt, RunArgs a) {
       const double rsf = a.albi[clc];
       const double bsurf = (1. - rsf) * bk[nlev - 1];
       const double tautop = s.dtau[1] * pl[1] / (pl[3] - pl[1]);
-      const double btop = (1.0 - mgcm_libm::mgcm_exp(-tautop * 2.0)) * bk[1];
+      const double btop = (1.0 - rt_exp(-tautop * 2.0)) * bk[1];
       auto accf = [&](int n, double fp, double fm) {
         xfer_add(n, ok ? (fp - fm) * dw * fzero : 0.0, ok ? fp * dw * fzero : 0.0, ok ? fm * dw * fzero : 0.0,
                  acc.inet, acc.iup, acc.idn);
