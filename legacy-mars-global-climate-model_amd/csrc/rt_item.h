@@ -85,7 +85,7 @@ __device__ void layer_aerosol(const RtConst *__restrict__ rc, int b, const doubl
 
 // ---------------------------------------------------------------------------
 // One two-stream item.  NLM bounds the radiation layer count (arrays are
-// 1-based).  Inputs arrive through the Src functor: dtk(K) gives the sub-level
+// 1-based).  Inputs arrive through the Src functor: finish(K, fetch(K)) gives the sub-level
 // optical depth DTAUK(K) and adds the gas part to taugsurf.
 
 // Per-item state of a VIS item (1-based layer arrays): scaled single-scattering
@@ -131,13 +131,17 @@ __device__ __forceinline__ VisLayer vis_layer(double w0, double cb, double u0, d
 template <int NLM, typename Src>
 __device__ __forceinline__ double vis_optics(int nl, Src &src, const double *__restrict__ scat,
                              const double *__restrict__ cosb, VisState<NLM> &s, double &cumtau) {
+  // sub-level inputs are fetched one layer ahead of their use
+  typename Src::Raw r2 = src.fetch(2), ra = src.fetch(3), rb = src.fetch(4);
   // sub-level 2 (top half of the cap)
-  const double d2 = src.dtk(2);
+  const double d2 = src.finish(2, r2);
   double tcv = d2;  // TAUCUMV(2)
   double fprev = 0.0, tcv_prev_level = 0.0, tauL = 0.0;  // tauL = TAU(L), carried in a register
   s.tau[1] = 0.0;
   for (int L = 1; L <= nl - 1; L++) {
-    const double a = src.dtk(2 * L + 1), b = src.dtk(2 * L + 2);
+    const double a = src.finish(2 * L + 1, ra), b = src.finish(2 * L + 2, rb);
+    ra = src.fetch(2 * L + 3);                   // L = nl-1: the bottom sub-level L_LEVELS
+    if (L < nl - 1) rb = src.fetch(2 * L + 4);
     const double dtauv = a + b;
     const double tcA = tcv + a;  // TAUCUMV(2L+1)
     const double tcB = tcA + b;  // TAUCUMV(2L+2)
@@ -163,7 +167,7 @@ __device__ __forceinline__ double vis_optics(int nl, Src &src, const double *__r
   }
   {
     const int L = nl;
-    const double a = src.dtk(2 * L + 1);
+    const double a = src.finish(2 * L + 1, ra);
     const double dtauv = a;
     cumtau = tcv + a;  // TAUCUMV(L_LEVELS)
     const double W = scat[L - 1] / dtauv, C = cosb[L - 1];
@@ -344,10 +348,14 @@ __device__ __forceinline__ IrLayer ir_layer(double w0, double cb, double dtau, d
 template <int NLM, typename Src>
 __device__ __forceinline__ void ir_optics(int nl, Src &src, const double *__restrict__ scat, IrState<NLM> &s) {
   const double f1em9 = F32LIT(1.0e-9);
-  double tcv = src.dtk(2);  // TAUCUMI(2)
+  // sub-level inputs are fetched one layer ahead of their use
+  typename Src::Raw r2 = src.fetch(2), ra = src.fetch(3), rb = src.fetch(4);
+  double tcv = src.finish(2, r2);  // TAUCUMI(2)
   for (int L = 1; L <= nl; L++) {
-    const double a = src.dtk(2 * L + 1);
-    const double b = (L < nl) ? src.dtk(2 * L + 2) : 0.0;  // DTAUKI(L_LEVELS+1) = 0
+    const double a = src.finish(2 * L + 1, ra);
+    const double b = (L < nl) ? src.finish(2 * L + 2, rb) : 0.0;  // DTAUKI(L_LEVELS+1) = 0
+    if (L < nl) ra = src.fetch(2 * L + 3);
+    if (L < nl - 1) rb = src.fetch(2 * L + 4);
     double dt = a + b + 1.e-50;
     double w;
     if (dt > f1em9) {

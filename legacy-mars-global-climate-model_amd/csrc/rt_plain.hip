@@ -116,21 +116,25 @@ struct GasSrc {
       : tab(tab_ + g0), t_(S.t + c * nsub - 2), u_(S.u + c * nsub - 2), wr_(S.wr + c * nsub - 2),
         uk_(S.uk + c * nsub - 2), c1_(S.c1 + c * nsub - 2), c2_(S.c2 + c * nsub - 2),
         base_(S.base + c * nsub - 2), taugsurf(0.0) {}
-  __device__ __forceinline__ double dtk(int K) {
+  // the 8 corners (p0,T0),(p1,T0),(p1,T1),(p0,T1) x (NH2O, NH2O+1) of sub-level K;
+  // issued a layer ahead of their use (fetch), combined in K order (finish)
+  struct Raw {
+    double a0, a1, b0, b1, c0, c1, d0, d1;
+  };
+  __device__ __forceinline__ Raw fetch(int K) const {
     const int braw = base_[K];
     const int dnh = (braw & 1) ? 0 : NT * NGAS;  // NH2O = L_REFH2O: WRATIO = 0, upper = itself
-    const double t = t_[K], u = u_[K], wr = wr_[K];
-    const double lc0 = (1.0 - t) * (1.0 - u), lc1 = t * (1.0 - u), lc2 = t * u, lc3 = (1.0 - t) * u;
     const double *p = tab + (braw & ~1);
     const int sT = NGAS, sP = NH * NT * NGAS;
-    const double a0 = p[0], a1 = p[dnh];
-    const double b0 = p[sP], b1 = p[sP + dnh];
-    const double c0 = p[sP + sT], c1 = p[sP + sT + dnh];
-    const double d0 = p[sT], d1 = p[sT + dnh];
-    const double k1 = a0 + wr * (a1 - a0);
-    const double k2 = b0 + wr * (b1 - b0);
-    const double k3 = c0 + wr * (c1 - c0);
-    const double k4 = d0 + wr * (d1 - d0);
+    return Raw{p[0], p[dnh], p[sP], p[sP + dnh], p[sP + sT], p[sP + sT + dnh], p[sT], p[sT + dnh]};
+  }
+  __device__ __forceinline__ double finish(int K, const Raw &r) {
+    const double t = t_[K], u = u_[K], wr = wr_[K];
+    const double lc0 = (1.0 - t) * (1.0 - u), lc1 = t * (1.0 - u), lc2 = t * u, lc3 = (1.0 - t) * u;
+    const double k1 = r.a0 + wr * (r.a1 - r.a0);
+    const double k2 = r.b0 + wr * (r.b1 - r.b0);
+    const double k3 = r.c0 + wr * (r.c1 - r.c0);
+    const double k4 = r.d0 + wr * (r.d1 - r.d0);
     const double ans = lc0 * k1 + lc1 * k2 + lc2 * k3 + lc3 * k4;
     const double taugas = uk_[K] * ans;
     taugsurf = taugsurf + taugas;
@@ -147,7 +151,9 @@ struct ClearSrc {
   int b;
   const double *plev, *tauref, *taurefcld;
   double taugsurf;
-  __device__ __forceinline__ double dtk(int K) {
+  struct Raw {};
+  __device__ __forceinline__ Raw fetch(int) const { return Raw{}; }
+  __device__ __forceinline__ double finish(int K, const Raw &) {
     const SubAer s = sub_aerosol<VIS>(rc, b, plev, tauref, taurefcld, K);
     if (VIS) return s.taeros + s.tray + s.tcloud;
     return s.taeros + s.tcloud;
