@@ -7,6 +7,7 @@ multiply-adds the reference does not make); packed FP32 ops are disabled.
 """
 from __future__ import annotations
 
+import concurrent.futures
 import glob
 import os
 import subprocess
@@ -25,17 +26,23 @@ def build(verbose: bool = False) -> str:
     srcs = sorted(glob.glob(os.path.join(HERE, "csrc", "*.hip")))
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
+    cmds, objs = [], []
     for s in srcs:
         o = os.path.join(objdir, os.path.basename(s) + ".o")
         flags = list(COMMON)
         # the FP64 plain path and the table builders keep every rounding step
         flags += ["-ffp-contract=off"]
-        cmd = [HIPCC] + flags + ["-c", s, "-o", o]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        cmds.append([HIPCC] + flags + ["-c", s, "-o", o])
         objs.append(o)
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "0")) or (os.cpu_count() or 1)))
+    with concurrent.futures.ThreadPoolExecutor(jobs) as ex:
+        for cmd in cmds:
+            if verbose:
+                print(" ".join(cmd), flush=True)
+        for r in ex.map(lambda cmd: subprocess.run(cmd, capture_output=True, text=True), cmds):
+            if r.returncode != 0:
+                sys.stderr.write(r.stderr)
+                raise subprocess.CalledProcessError(r.returncode, r.args)
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs
     subprocess.run(cmd, check=True)

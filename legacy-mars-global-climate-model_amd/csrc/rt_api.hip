@@ -5,36 +5,10 @@
 #include <string.h>
 #include <vector>
 
-#include "../../include/mgcm_rt.h"
-#include "rt_internal.h"
+#include "rt_ctx.h"
 #include "rt_libm.h"
 
-namespace mgcm {
-hipError_t launch_plain(const DevTables &dt, const RunArgs &a, hipStream_t st);
-hipError_t launch_mixed(const DevTables &dt, const RunArgs &a, hipStream_t st);
-int run_batched_impl(struct ::mgcm_rt_ctx *ctx, int ngrids, const mgcm_rt_columns *grids,
-                     const mgcm_rt_fluxes *outs, int precision, hipStream_t st);
-}  // namespace mgcm
-void batch_free(void *);
-
 using namespace mgcm;
-
-struct mgcm_rt_ctx {
-  int device;
-  RtConst *d_const;
-  double *d_kv, *d_ki, *d_planck;
-  float *d_kv32, *d_ki32;
-  DevTables dt;
-  // workspace (grow-only)
-  double *d_ws;
-  size_t ws_bytes;
-  int *d_flag;
-  // device staging of the host-pointer path
-  size_t host_bytes;
-  double *d_host;
-  // batched-path state (owned by rt_batched.hip)
-  void *batch;
-};
 
 #define HIP_TRY(x)                                   \
   do {                                               \
@@ -134,7 +108,6 @@ extern "C" int mgcm_rt_init(const mgcm_rt_tables *t, mgcm_rt_ctx **out) {
   c->dt.c = c->d_const;
   c->dt.kv = c->d_kv;
   c->dt.ki = c->d_ki;
-  c->dt.kv32_unused = nullptr;
   c->dt.planck = c->d_planck;
   c->dt.kv32 = c->d_kv32;
   c->dt.ki32 = c->d_ki32;
@@ -144,25 +117,15 @@ extern "C" int mgcm_rt_init(const mgcm_rt_tables *t, mgcm_rt_ctx **out) {
 
 extern "C" int mgcm_rt_destroy(mgcm_rt_ctx *c) {
   if (!c) return MGCM_RT_ERR_ARG;
-  void *bufs[] = {c->d_const, c->d_kv, c->d_ki, c->d_kv32, c->d_ki32, c->d_planck, c->d_ws, c->d_flag, c->d_host};
+  void *bufs[] = {c->d_const, c->d_kv, c->d_ki, c->d_kv32, c->d_ki32, c->d_planck, c->d_flag, c->d_host};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
-  if (c->batch) batch_free(c->batch);
+  if (c->batch) mgcm::batch_free(c->batch);
   delete c;
   return MGCM_RT_OK;
 }
 
 namespace mgcm {
-// Device workspace of one grid.  The fused kernels keep every intermediate in
-// LDS and registers; the workspace only holds the per-call range flag slot.
-size_t workspace_bytes(int, int) { return 256; }
-
-void bind_workspace(RunArgs &a, void *ws) {
-  a.gas = nullptr;
-  a.clear = nullptr;
-  a.skip = (uint32_t *)ws;
-}
-
 int check_columns(const mgcm_rt_columns *in, const mgcm_rt_fluxes *out) {
   if (!in || !out) return MGCM_RT_ERR_ARG;
   if (in->ncol < 0) return MGCM_RT_ERR_ARG;
@@ -192,19 +155,6 @@ void fill_args(RunArgs &a, const mgcm_rt_columns *in, const mgcm_rt_fluxes *out,
 }
 }  // namespace mgcm
 
-static int ensure_ws(mgcm_rt_ctx *c, size_t bytes) {
-  if (bytes <= c->ws_bytes) return MGCM_RT_OK;
-  if (c->d_ws) {
-    if (hipDeviceSynchronize() != hipSuccess) return MGCM_RT_ERR_DEVICE;
-    (void)hipFree(c->d_ws);
-    c->d_ws = nullptr;
-    c->ws_bytes = 0;
-  }
-  if (hipMalloc(&c->d_ws, bytes) != hipSuccess) return MGCM_RT_ERR_NOMEM;
-  c->ws_bytes = bytes;
-  return MGCM_RT_OK;
-}
-
 extern "C" int mgcm_rt_run(mgcm_rt_ctx *c, const mgcm_rt_columns *in, const mgcm_rt_fluxes *out,
                            int precision, void *stream) {
   if (!c) return MGCM_RT_ERR_ARG;
@@ -212,11 +162,8 @@ extern "C" int mgcm_rt_run(mgcm_rt_ctx *c, const mgcm_rt_columns *in, const mgcm
   if (s != MGCM_RT_OK) return s;
   if (precision != MGCM_RT_FP64 && precision != MGCM_RT_MIXED) return MGCM_RT_ERR_PRECISION;
   if (in->ncol == 0) return MGCM_RT_OK;
-  s = ensure_ws(c, workspace_bytes(in->ncol, in->nlay));
-  if (s != MGCM_RT_OK) return s;
   RunArgs a;
   fill_args(a, in, out, c->d_flag);
-  bind_workspace(a, c->d_ws);
   const hipError_t e = (precision == MGCM_RT_FP64) ? launch_plain(c->dt, a, (hipStream_t)stream)
                                                    : launch_mixed(c->dt, a, (hipStream_t)stream);
   return e == hipSuccess ? MGCM_RT_OK : MGCM_RT_ERR_DEVICE;

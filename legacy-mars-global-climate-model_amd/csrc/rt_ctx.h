@@ -1,0 +1,28 @@
+// rt_ctx.h -- host-side context of the C ABI, shared by the entry-point files.
+#pragma once
+#include "../../include/mgcm_rt.h"
+#include "rt_internal.h"
+
+struct mgcm_rt_ctx {
+  int device;
+  mgcm::RtConst *d_const;
+  double *d_kv, *d_ki, *d_planck;
+  float *d_kv32, *d_ki32;
+  mgcm::DevTables dt;
+  int *d_flag;
+  // device staging of the host-pointer path (grow-only)
+  size_t host_bytes;
+  double *d_host;
+  // batched-path state (rt_batched.hip)
+  void *batch;
+};
+
+namespace mgcm {
+hipError_t launch_plain(const DevTables &dt, const RunArgs &a, hipStream_t st);
+hipError_t launch_mixed(const DevTables &dt, const RunArgs &a, hipStream_t st);
+int check_columns(const mgcm_rt_columns *in, const mgcm_rt_fluxes *out);
+void fill_args(RunArgs &a, const mgcm_rt_columns *in, const mgcm_rt_fluxes *out, int *flag);
+int run_batched_impl(mgcm_rt_ctx *ctx, int ngrids, const mgcm_rt_columns *grids, const mgcm_rt_fluxes *outs,
+                     int precision, hipStream_t st);
+void batch_free(void *state);
+}  // namespace mgcm

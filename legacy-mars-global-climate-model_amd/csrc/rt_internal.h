@@ -53,18 +53,13 @@ struct RunArgs {
   double sol[NV];
   double *fmnetv, *fluxupv, *fluxdnv, *fmneti, *fluxupi, *fluxdni;
   double *nfluxtopv, *nfluxtopi, *diffvt, *detau, *cumtauv, *hr;
-  // workspace
-  double *gas;       // [NB][3*nl+2][ncol]  weighted sums over g = 1..16
-  double *clear;     // [NB][3*nl+2][ncol]  weighted g = 17 contribution
-  uint32_t *skip;    // [NB][ncol]          bit g-1 set: Gauss point g skipped (TLIMIT)
-  int *flag;         // range flag
+  int *flag;  // range flag
 };
 
 struct DevTables {
   const RtConst *c;
   const double *kv;      // [NV][NP][NH][NT][16]  k-coefficients, gas g fastest
   const double *ki;      // [NI][NP][NH][NT][16]
-  const double *kv32_unused;
   const double *planck;  // [NPLANCK][NI]  PLANCKIR(5,8501)
   const float *kv32;     // FP32 copies for the mixed path
   const float *ki32;

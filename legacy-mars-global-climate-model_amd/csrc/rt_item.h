@@ -50,7 +50,7 @@ __device__ __forceinline__ void ir_pad(const RtConst *__restrict__ rc, int b,
 // Layer scattering optical depth and asymmetry of the continuum channel:
 // VIS code/optcv.f90:194-222, IR code/optci.f90:195-219.
 template <bool VIS>
-__device__ void layer_aerosol(const RtConst *__restrict__ rc, int b, const double *__restrict__ plev,
+__device__ __forceinline__ void layer_aerosol(const RtConst *__restrict__ rc, int b, const double *__restrict__ plev,
                               const double *__restrict__ tauref, const double *__restrict__ taurefcld,
                               int nl, int L, double &scat, double &cosb) {
   const int nlev = 2 * nl + 1;

@@ -1,5 +1,10 @@
-// rt_mixed.hip -- mixed-precision path (placeholder until the FP32 optics land).
-#include "rt_internal.h"
+// rt_mixed.hip -- the mixed-precision path: FP32 k-interpolation, optical
+// depths and (below the top FP64 layers) layer constants and transmissions;
+// FP64 elimination, substitution, flux sums and heating.
+#include "rt_column.h"
+
 namespace mgcm {
-hipError_t launch_mixed(const DevTables &, const RunArgs &, hipStream_t) { return hipErrorNotSupported; }
+hipError_t launch_mixed(const DevTables &dt, const RunArgs &a, hipStream_t st) {
+  return launch_column<true>(dt, a, st);
+}
 }  // namespace mgcm

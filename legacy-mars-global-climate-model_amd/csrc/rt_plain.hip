@@ -1,529 +1,9 @@
-// rt_plain.hip -- FP64 plain path of the column radiative transfer on gfx950:
-// one fused launch per grid.
-//
-// One wave (64 lanes) owns 4 columns for the whole radiation call:
-//   1. stages the 4 columns' TPINDEX results (band independent) in LDS;
-//   2. for each of the 7 VIS and 5 IR bands: stages the band's continuum and
-//      layer optics, then lane (c, g) solves the (column c, band, Gauss point
-//      g = 1..16) item -- OPTCV/OPTCI layer assembly, delta-Eddington scaling,
-//      GFLUXV/GFLUXI with DSOLVER/DTRIDGL -- and the 16 lanes of a column sum
-//      their weighted layer fluxes in a fixed order into per-column
-//      accumulators held in registers (lane g keeps the levels L == g+1 mod 16);
-//   3. solves the continuum-only Gauss point 17 of every (column, band) -- the
-//      VIS ones in one pass, the IR ones in a second -- weighted with FZERO
-//      plus the weight of the band's TLIMIT-skipped points (code/sfluxv.f90:
-//      72-75,121-162, code/sfluxi.f90:72-75,114-143);
-//   4. writes the fluxes and forms SUNTOT/IRTOTAL and the layer heating
-//      (code/comp3.f:677-681,735-738).
-// Nothing goes through global memory between the stages; the only HBM traffic
-// is the column inputs, the outputs and the k-tables (L2/MALL resident).
-#include "rt_item.h"
+// rt_plain.hip -- the plain FP64 path: every statement in the reference's
+// FP64 evaluation order (compiled with -ffp-contract=off).
+#include "rt_column.h"
 
 namespace mgcm {
-
-// LDS staging of a wave's 4 columns.
-struct Stage {
-  double *t, *u, *wr, *uk;     // [4][nsub]  TPINDEX weights T (log p) and U (T), WRATIO, Cmk*dp
-  double *c1, *c2;             // [4][nsub]  VIS: TRAY+TAEROS+TCLOUD; IR: TAEROS, TCLOUD
-  double *scat, *cosb, *cbar;  // [4][nl]    layer scattering tau, asymmetry, COSBAR = g/(1+g)
-  double *bk;                  // [4][nlev]  IR: PLANCKIR(nw, int(TLEV(K)*10-499))
-  double *xfer;                // [64][3]    continuum-point hand-over
-  int *base;                   // [4][nsub]  table cell offset; bit 0 set when NH2O = L_REFH2O
-  uint32_t *skip;              // [4][NB]    TLIMIT skip masks
-};
-
-// Dynamic LDS of one wave: the exp table, then the staging.  The continuum
-// pass's hand-over area (64 x 3) reuses c1, which that pass does not read
-// (4 * nsub >= 192 for every nlay >= 23; smaller layer counts get their own).
-size_t stage_bytes(int nlay) {
-  const size_t nl = nlay + 1, nlev = 2 * (size_t)nlay + 3, nsub = nlev - 1;
-  const size_t xfer = 4 * nsub >= 64 * 3 ? 0 : 64 * 3;
-  return mgcm_libm::LDS_EXPTAB_BYTES + sizeof(double) * (4 * nsub * 6 + 4 * nl * 3 + 4 * nlev + xfer) +
-         sizeof(int) * (4 * nsub + 4 * NB);
-}
-
-__device__ inline Stage carve(double *p, int nsub, int nl, int nlev) {
-  p += mgcm_libm::LDS_EXPTAB_BYTES / sizeof(double);
-  Stage S;
-  S.t = p; p += 4 * nsub;
-  S.u = p; p += 4 * nsub;
-  S.wr = p; p += 4 * nsub;
-  S.uk = p; p += 4 * nsub;
-  S.c1 = p; p += 4 * nsub;
-  S.c2 = p; p += 4 * nsub;
-  S.scat = p; p += 4 * nl;
-  S.cosb = p; p += 4 * nl;
-  S.cbar = p; p += 4 * nl;
-  S.bk = p; p += 4 * nlev;
-  if (4 * nsub >= 64 * 3) {
-    S.xfer = S.c1;
-  } else {
-    S.xfer = p;
-    p += 64 * 3;
-  }
-  S.base = (int *)p;
-  S.skip = (uint32_t *)(S.base + 4 * nsub);
-  return S;
-}
-
-// TPINDEX (code/tpindex.f90:72-132) of a sub-level without the four bilinear
-// products; they are formed where used, with the same expressions as
-// code/tpindex.f90:110-113.
-__device__ void tp_stage(const RtConst *__restrict__ rc, double pw, double tw, double qh2o, double &t,
-                         double &u, int &mt, int &mp, int &nh, double &wratio) {
-  if (tw <= rc->tgasref[0]) {
-    mt = 1;
-  } else {
-    mt = NT - 1;
-    for (int n = 1; n <= NT - 1; n++) {
-      if (tw > rc->tgasref[n - 1] && tw <= rc->tgasref[n]) { mt = n; break; }
-    }
-  }
-  u = (tw - rc->tgasref[mt - 1]) / (rc->tgasref[mt] - rc->tgasref[mt - 1]);
-  const double pwl = mgcm_libm::mgcm_log10(pw);
-  mp = NP - 1;
-  for (int n = 2; n <= NP - 1; n++) {
-    if (pwl <= rc->pfgasref[n - 1]) { mp = n - 1; break; }
-  }
-  t = (pwl - rc->pfgasref[mp - 1]) / (rc->pfgasref[mp] - rc->pfgasref[mp - 1]);
-  nh = 1;
-  wratio = 0.0;
-  if (qh2o <= rc->wrefh2o[0]) {
-    nh = 1;
-  } else if (qh2o >= rc->wrefh2o[NH - 1]) {
-    nh = NH;
-  } else {
-    for (int n = 2; n <= NH; n++) {
-      if (qh2o >= rc->wrefh2o[n - 2] && qh2o < rc->wrefh2o[n - 1]) {
-        nh = n - 1;
-        wratio = (qh2o - rc->wrefh2o[n - 2]) / (rc->wrefh2o[n - 1] - rc->wrefh2o[n - 2]);
-        break;
-      }
-    }
-  }
-}
-
-// Gas point source: k-interpolation (code/optcv.f90:142-167, code/optci.f90:
-// 143-167) from the g-fastest table; the 16 lanes of a column read one 128-byte
-// line per corner.
-template <bool VIS>
-struct GasSrc {
-  const double *__restrict__ tab;  // band's table
-  const double *t_, *u_, *wr_, *uk_, *c1_, *c2_;  // this column's staged sub-levels (LDS)
-  const int *base_;
-  double taugsurf;
-  __device__ GasSrc(const double *tab_, const Stage &S, int c, int nsub, int g0)
-      : tab(tab_ + g0), t_(S.t + c * nsub - 2), u_(S.u + c * nsub - 2), wr_(S.wr + c * nsub - 2),
-        uk_(S.uk + c * nsub - 2), c1_(S.c1 + c * nsub - 2), c2_(S.c2 + c * nsub - 2),
-        base_(S.base + c * nsub - 2), taugsurf(0.0) {}
-  // the 8 corners (p0,T0),(p1,T0),(p1,T1),(p0,T1) x (NH2O, NH2O+1) of sub-level K;
-  // issued a layer ahead of their use (fetch), combined in K order (finish)
-  struct Raw {
-    double a0, a1, b0, b1, c0, c1, d0, d1;
-  };
-  __device__ __forceinline__ Raw fetch(int K) const {
-    const int braw = base_[K];
-    const int dnh = (braw & 1) ? 0 : NT * NGAS;  // NH2O = L_REFH2O: WRATIO = 0, upper = itself
-    const double *p = tab + (braw & ~1);
-    const int sT = NGAS, sP = NH * NT * NGAS;
-    return Raw{p[0], p[dnh], p[sP], p[sP + dnh], p[sP + sT], p[sP + sT + dnh], p[sT], p[sT + dnh]};
-  }
-  __device__ __forceinline__ double finish(int K, const Raw &r) {
-    const double t = t_[K], u = u_[K], wr = wr_[K];
-    const double lc0 = (1.0 - t) * (1.0 - u), lc1 = t * (1.0 - u), lc2 = t * u, lc3 = (1.0 - t) * u;
-    const double k1 = r.a0 + wr * (r.a1 - r.a0);
-    const double k2 = r.b0 + wr * (r.b1 - r.b0);
-    const double k3 = r.c0 + wr * (r.c1 - r.c0);
-    const double k4 = r.d0 + wr * (r.d1 - r.d0);
-    const double ans = lc0 * k1 + lc1 * k2 + lc2 * k3 + lc3 * k4;
-    const double taugas = uk_[K] * ans;
-    taugsurf = taugsurf + taugas;
-    if (VIS) return taugas + c1_[K];
-    return taugas + c1_[K] + c2_[K];
-  }
-};
-
-// Continuum-only point: sub-level optical depths straight from the column's
-// profiles (code/optcv.f90:172-173, code/optci.f90:173-174).
-template <bool VIS>
-struct ClearSrc {
-  const RtConst *rc;
-  int b;
-  const double *plev, *tauref, *taurefcld;
-  double taugsurf;
-  struct Raw {};
-  __device__ __forceinline__ Raw fetch(int) const { return Raw{}; }
-  __device__ __forceinline__ double finish(int K, const Raw &) {
-    const SubAer s = sub_aerosol<VIS>(rc, b, plev, tauref, taurefcld, K);
-    if (VIS) return s.taeros + s.tray + s.tcloud;
-    return s.taeros + s.tcloud;
-  }
-};
-
-// Cross-lane move of a double inside 16-lane rows (DPP, no LDS round trip).
-template <int CTRL>
-__device__ __forceinline__ double dpp_f64(double v) {
-  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-  const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)b, CTRL, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), CTRL, 0xf, 0xf, false);
-  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
-}
-
-// 16-lane fixed-order sum over a row: pairs (lane, lane^1), then quads
-// (lane^2), then the two quads of each half (half-mirror), then the two halves
-// (mirror).  Each step adds two equal-order partial sums, so every lane of the
-// row ends with the same bits.  Needs all 64 lanes active.
-__device__ __forceinline__ double group16_sum(double v) {
-  v = v + dpp_f64<0xB1>(v);   // quad_perm [1,0,3,2]
-  v = v + dpp_f64<0x4E>(v);   // quad_perm [2,3,0,1]
-  v = v + dpp_f64<0x141>(v);  // row_half_mirror
-  v = v + dpp_f64<0x140>(v);  // row_mirror
-  return v;
-}
-
-__device__ __forceinline__ int planck_index(double T, int *flag) {
-  int nt = (int)(T * 10.0 - 499);  // code/sfluxi.f90:54-55, code/gfluxi.f90:73-74
-  if (nt < 1 || nt > NPLANCK) {
-    atomicOr(flag, 1);
-    nt = nt < 1 ? 1 : NPLANCK;
-  }
-  return nt;
-}
-
-// Per-column accumulators, distributed over the column's 16 lanes: lane g owns
-// radiation levels n with (n-1) % 16 == g.
-template <int SL>
-struct Acc {
-  double vnet[SL], vup[SL], vdn[SL], inet[SL], iup[SL], idn[SL];
-};
-
-template <int SL>
-__device__ __forceinline__ void acc_add(double *arr, int n, int g0, double v) {
-  const int slot = (n - 1) >> 4;
-  if (((n - 1) & 15) == g0) {
-#pragma unroll
-    for (int j = 0; j < SL; j++)
-      if (j == slot) arr[j] = arr[j] + v;
-  }
-}
-
-template <int SL>
-__device__ __forceinline__ double acc_get(const double *arr, int n) {
-  const int slot = (n - 1) >> 4;
-  double r = 0.0;
-#pragma unroll
-  for (int j = 0; j < SL; j++)
-    if (j == slot) r = arr[j];
-  return r;
-}
-
-#ifndef MGCM_FP64_WAVES
-#define MGCM_FP64_WAVES 2
-#endif
-template <int NLM>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MGCM_FP64_WAVES)))
-void k_rt_fp64(DevTables dt, RunArgs a) {
-  extern __shared__ double smem[];
-  constexpr int SL = (NLM + 15) / 16;
-  const int nl = a.nlay + 1, nlev = 2 * a.nlay + 3, nsub = nlev - 1;
-  const int col0 = blockIdx.x * 4;
-  const int lane = threadIdx.x, c = lane >> 4, g0 = lane & 15;
-  const int col = col0 + c;
-  const bool valid = col < a.ncol;
-  const int colc = valid ? col : a.ncol - 1;
-  const RtConst *rc = dt.c;
-  Stage S = carve(smem, nsub, nl, nlev);
-  mgcm_libm::load_lds_exptab(lane, 64);
-  __syncthreads();
-
-  // 1. TPINDEX of every sub-level K = 2..L_LEVELS of the 4 columns
-  for (int t = lane; t < 4 * nsub; t += 64) {
-    const int cc = t / nsub, K = t % nsub + 2;
-    const int cl = col0 + cc;
-    if (cl >= a.ncol) continue;
-    const size_t o = (size_t)cl * nlev + K - 1;
-    int mt, mp, nh;
-    double tt, uu, wr;
-    tp_stage(rc, a.pmid[o], a.tmid[o], a.qh2o[o], tt, uu, mt, mp, nh, wr);
-    S.t[t] = tt;
-    S.u[t] = uu;
-    S.wr[t] = wr;
-    S.uk[t] = CMK * (a.plev[o] - a.plev[o - 1]);
-    S.base[t] = kcell(0, mt, mp, nh) | (nh == NH ? 1 : 0);
-  }
-
-  Acc<SL> acc;
-#pragma unroll
-  for (int j = 0; j < SL; j++) acc.vnet[j] = acc.vup[j] = acc.vdn[j] = acc.inet[j] = acc.iup[j] = acc.idn[j] = 0.0;
-  double topv = 0.0, difv = 0.0, topi = 0.0;
-
-  const double *plev = a.plev + (size_t)colc * nlev;
-  const double u0 = a.acosz[colc];
-  const bool sun = u0 >= F32LIT(1.0e-5);  // code/comp3.f:637
-
-  // 2. k-distribution points, band by band
-  for (int b = 0; b < NB; b++) {
-    const bool VISB = b < NV;
-    const double fz = rc->fzero[b];
-    __syncthreads();
-    if (fz >= F32LIT(0.99)) {  // code/sfluxv.f90:64-65, code/sfluxi.f90:67-68
-      if (g0 == 0) S.skip[c * NB + b] = 0u;
-      continue;
-    }
-    for (int t = lane; t < 4 * nsub; t += 64) {  // continuum optics of the band
-      const int cc = t / nsub, K = t % nsub + 2;
-      const int cl = col0 + cc;
-      if (cl >= a.ncol) continue;
-      const double *pl = a.plev + (size_t)cl * nlev;
-      const double *tr = a.tauref + (size_t)cl * (nlev + 1);
-      const double *tc = a.taurefcld + (size_t)cl * (nlev + 1);
-      if (VISB) {
-        const SubAer s = sub_aerosol<true>(rc, b, pl, tr, tc, K);
-        S.c1[t] = s.tray + s.taeros + s.tcloud;
-      } else {
-        const SubAer s = sub_aerosol<false>(rc, b, pl, tr, tc, K);
-        S.c1[t] = s.taeros;
-        S.c2[t] = s.tcloud;
-      }
-    }
-    for (int t = lane; t < 4 * nl; t += 64) {
-      const int cc = t / nl, L = t % nl + 1;
-      const int cl = col0 + cc;
-      if (cl >= a.ncol) continue;
-      const double *pl = a.plev + (size_t)cl * nlev;
-      const double *tr = a.tauref + (size_t)cl * (nlev + 1);
-      const double *tc = a.taurefcld + (size_t)cl * (nlev + 1);
-      double sc, cb;
-      if (VISB) layer_aerosol<true>(rc, b, pl, tr, tc, nl, L, sc, cb);
-      else layer_aerosol<false>(rc, b, pl, tr, tc, nl, L, sc, cb);
-      S.scat[t] = sc;
-      S.cosb[t] = cb;
-      S.cbar[t] = cb / (1.0 + cb);  // COSBAR of the delta-Eddington scaling (code/gfluxv.f90:86)
-    }
-    if (!VISB) {
-      for (int t = lane; t < 4 * nlev; t += 64) {
-        const int cc = t / nlev, K = t % nlev + 1;
-        const int cl = col0 + cc;
-        if (cl >= a.ncol) continue;
-        const int nt = planck_index(a.tlev[(size_t)cl * nlev + K - 1], a.flag);
-        S.bk[t] = dt.planck[(size_t)(nt - 1) * NI + (b - NV)];
-      }
-    }
-    __syncthreads();
-
-    const double gw = rc->gw[b][g0];
-    const double onemf = 1.0 - fz;
-    const double *scat = S.scat + c * nl, *cosb = S.cosb + c * nl, *cbar = S.cbar + c * nl;
-    bool skip;
-    if (VISB) {
-      const int nw = b;
-      GasSrc<true> src(dt.kv + (size_t)nw * NP * NH * NT * NGAS, S, c, nsub, g0);
-      VisState<NLM> s;
-      double cumtau = 0.0, detau = 0.0;
-      if (valid && sun) detau = vis_optics<NLM>(nl, src, scat, cosb, s, cumtau);
-      skip = !(valid && sun) || src.taugsurf < TLIMITS;
-      if (valid && sun) {  // GETDETAU output and TAUCUMV(L_LEVELS) (code/comp3.f:664-668)
-        a.detau[(size_t)col * NG * NV + g0 * NV + nw] = detau;
-        a.cumtauv[(size_t)col * NG * NV + g0 * NV + nw] = cumtau;
-      }
-      // the 16-lane sums need the whole wave: the solve runs wave-uniformly and
-      // skipped lanes (TLIMIT, sun down, past the last column) contribute zeros
-      if (__any(!skip)) {
-        auto accf = [&](int n, double fp, double fm) {
-          const double net = skip ? 0.0 : (fp - fm) * gw * onemf;
-          const double up = skip ? 0.0 : fp * gw * onemf;
-          const double dn = skip ? 0.0 : fm * gw * onemf;
-          acc_add<SL>(acc.vnet, n, g0, group16_sum(net));
-          acc_add<SL>(acc.vup, n, g0, group16_sum(up));
-          acc_add<SL>(acc.vdn, n, g0, group16_sum(dn));
-        };
-        double fluxup = 0.0, fluxdn = 0.0, diffv = 0.0;
-        const double bsurf = a.als[colc] * u0 * a.sol[nw] * rt_exp(-dmin(detau / u0, MAXEXP));
-        vis_solve<NLM>(nl, s, cbar, u0, a.sol[nw], a.als[colc], 0.0, bsurf, accf, fluxup, fluxdn, diffv);
-        topv = topv + group16_sum(skip ? 0.0 : (fluxup - fluxdn) * gw * onemf);
-        difv = difv + group16_sum(skip ? 0.0 : diffv * gw * onemf);
-      }
-    } else {
-      const int nw = b - NV;
-      const double dw = rc->dwni[nw];
-      GasSrc<false> src(dt.ki + (size_t)nw * NP * NH * NT * NGAS, S, c, nsub, g0);
-      IrState<NLM> s;
-      if (valid) ir_optics<NLM>(nl, src, scat, s);
-      skip = !valid || src.taugsurf < TLIMITI;
-      if (__any(!skip)) {
-        const double *bk = S.bk + c * nlev;
-        auto accf = [&](int n, double fp, double fm) {
-          const double net = skip ? 0.0 : (fp - fm) * dw * gw * onemf;
-          const double up = skip ? 0.0 : fp * dw * gw * onemf;
-          const double dn = skip ? 0.0 : fm * dw * gw * onemf;
-          acc_add<SL>(acc.inet, n, g0, group16_sum(net));
-          acc_add<SL>(acc.iup, n, g0, group16_sum(up));
-          acc_add<SL>(acc.idn, n, g0, group16_sum(dn));
-        };
-        // boundary conditions (code/sfluxi.f90:61-62,81-82)
-        const double rsf = a.albi[colc];
-        const double bsurf = (1. - rsf) * bk[nlev - 1];
-        const double tautop = s.dtau[1] * plev[1] / (plev[3] - plev[1]);
-        const double btop = (1.0 - rt_exp(-tautop * 2.0)) * bk[1];  // /UBARI, UBARI = 0.5
-        const double ftopup = ir_solve<NLM>(nl, s, cosb, bk, rsf, btop, bsurf, accf);
-        topi = topi + group16_sum(skip ? 0.0 : ftopup * dw * gw * onemf);
-      }
-    }
-    const unsigned long long m = __ballot(skip);
-    if (g0 == 0) S.skip[c * NB + b] = (uint32_t)((m >> (16 * c)) & 0xffffu);
-  }
-  __syncthreads();
-
-  // 3. continuum point g = 17: pass 0 = VIS (lanes 0..27), pass 1 = IR (lanes 0..19)
-  for (int pass = 0; pass < 2; pass++) {
-    const int nbp = pass == 0 ? NV : NI;
-    const bool act = lane < 4 * nbp;
-    const int cc = act ? lane / nbp : 0;
-    const int b = (pass == 0 ? 0 : NV) + (act ? lane % nbp : 0);
-    const int cl = col0 + cc;
-    const bool ok = act && cl < a.ncol;
-    const int clc = ok ? cl : colc;
-    const double *pl = a.plev + (size_t)clc * nlev;
-    const double *tr = a.tauref + (size_t)clc * (nlev + 1);
-    const double *tc = a.taurefcld + (size_t)clc * (nlev + 1);
-    // FZERO plus the weight of the skipped points, in g order
-    double fzero = rc->fzero[b];
-    if (!(fzero >= F32LIT(0.99))) {
-      const uint32_t mk = S.skip[cc * NB + b];
-      for (int g = 0; g < NGAS; g++)
-        if (mk & (1u << g)) fzero = fzero + (1.0 - rc->fzero[b]) * rc->gw[b][g];
-    }
-    double scat[NLM], cosb[NLM], cbar[NLM];
-    for (int L = 1; L <= nl; L++) {
-      if (pass == 0) layer_aerosol<true>(rc, b, pl, tr, tc, nl, L, scat[L - 1], cosb[L - 1]);
-      else layer_aerosol<false>(rc, b, pl, tr, tc, nl, L, scat[L - 1], cosb[L - 1]);
-      cbar[L - 1] = cosb[L - 1] / (1.0 + cosb[L - 1]);
-    }
-    // hand the weighted contributions to the owner lanes of the column, level by level
-    auto xfer_add = [&](int n, double net, double up, double dn, double *an, double *au, double *ad) {
-      __syncthreads();
-      S.xfer[lane * 3 + 0] = net;
-      S.xfer[lane * 3 + 1] = up;
-      S.xfer[lane * 3 + 2] = dn;
-      __syncthreads();
-      if (((n - 1) & 15) == g0) {
-        double sn = 0.0, su = 0.0, sd = 0.0;
-        for (int bb = 0; bb < nbp; bb++) {
-          const int src = c * nbp + bb;
-          sn = sn + S.xfer[src * 3 + 0];
-          su = su + S.xfer[src * 3 + 1];
-          sd = sd + S.xfer[src * 3 + 2];
-        }
-        acc_add<SL>(an, n, g0, sn);
-        acc_add<SL>(au, n, g0, su);
-        acc_add<SL>(ad, n, g0, sd);
-      }
-    };
-    auto xfer_scalar = [&](double v) {  // every lane receives its column's sum over bands
-      __syncthreads();
-      S.xfer[lane * 3] = v;
-      __syncthreads();
-      double sum = 0.0;
-      for (int bb = 0; bb < nbp; bb++) sum = sum + S.xfer[(c * nbp + bb) * 3];
-      return sum;
-    };
-    if (pass == 0) {
-      const double mu = a.acosz[clc];
-      const bool lsun = ok && mu >= F32LIT(1.0e-5);
-      ClearSrc<true> src{rc, b, pl, tr, tc, 0.0};
-      VisState<NLM> s;
-      double cumtau = 0.0, detau = 0.0;
-      if (lsun) {
-        detau = vis_optics<NLM>(nl, src, scat, cosb, s, cumtau);
-        a.detau[(size_t)cl * NG * NV + 16 * NV + b] = detau;
-        a.cumtauv[(size_t)cl * NG * NV + 16 * NV + b] = cumtau;
-      }
-      if (__any(lsun)) {
-        double fluxup = 0.0, fluxdn = 0.0, diffv = 0.0;
-        auto accf = [&](int n, double fp, double fm) {
-          xfer_add(n, lsun ? (fp - fm) * fzero : 0.0, lsun ? fp * fzero : 0.0, lsun ? fm * fzero : 0.0,
-                   acc.vnet, acc.vup, acc.vdn);
-        };
-        const double bsurf = a.als[clc] * mu * a.sol[b] * rt_exp(-dmin(detau / mu, MAXEXP));
-        vis_solve<NLM>(nl, s, cbar, mu, a.sol[b], a.als[clc], 0.0, bsurf, accf, fluxup, fluxdn, diffv);
-        topv = topv + xfer_scalar(lsun ? (fluxup - fluxdn) * fzero : 0.0);
-        difv = difv + xfer_scalar(lsun ? diffv * fzero : 0.0);
-      }
-    } else {
-      const int nw = b - NV;
-      const double dw = rc->dwni[nw];
-      ClearSrc<false> src{rc, b, pl, tr, tc, 0.0};
-      IrState<NLM> s;
-      ir_optics<NLM>(nl, src, scat, s);
-      double bk[2 * NLM + 1];
-      for (int K = 1; K <= nlev; K++)
-        bk[K - 1] = dt.planck[(size_t)(planck_index(a.tlev[(size_t)clc * nlev + K - 1], a.flag) - 1) * NI + nw];
-      const double rsf = a.albi[clc];
-      const double bsurf = (1. - rsf) * bk[nlev - 1];
-      const double tautop = s.dtau[1] * pl[1] / (pl[3] - pl[1]);
-      const double btop = (1.0 - rt_exp(-tautop * 2.0)) * bk[1];
-      auto accf = [&](int n, double fp, double fm) {
-        xfer_add(n, ok ? (fp - fm) * dw * fzero : 0.0, ok ? fp * dw * fzero : 0.0, ok ? fm * dw * fzero : 0.0,
-                 acc.inet, acc.iup, acc.idn);
-      };
-      const double ftopup = ir_solve<NLM>(nl, s, cosb, bk, rsf, btop, bsurf, accf);
-      topi = topi + xfer_scalar(ok ? ftopup * dw * fzero : 0.0);
-    }
-  }
-
-  // 4. outputs, net-flux differences and heating
-  __syncthreads();
-  double *fv = S.t;  // reuse the staging: [4][nl] net VIS, [4][nl] net IR
-  double *fi = S.t + 4 * nl;
-  if (!sun) { topv = 0.0; difv = 0.0; }
-  for (int n = 1; n <= nl; n++) {
-    if (((n - 1) & 15) == g0) {
-      const double nv = sun ? acc_get<SL>(acc.vnet, n) : 0.0;
-      const double uv = sun ? acc_get<SL>(acc.vup, n) : 0.0;
-      const double dv = sun ? acc_get<SL>(acc.vdn, n) : 0.0;
-      const double ni = acc_get<SL>(acc.inet, n), ui = acc_get<SL>(acc.iup, n), di = acc_get<SL>(acc.idn, n);
-      fv[c * nl + n - 1] = nv;
-      fi[c * nl + n - 1] = ni;
-      if (valid) {
-        const size_t o = (size_t)col * nl + n - 1;
-        a.fmnetv[o] = nv; a.fluxupv[o] = uv; a.fluxdnv[o] = dv;
-        a.fmneti[o] = ni; a.fluxupi[o] = ui; a.fluxdni[o] = di;
-      }
-    }
-  }
-  if (g0 == 0 && valid) {
-    a.nfluxtopv[col] = topv;
-    a.nfluxtopi[col] = topi;
-    a.diffvt[col] = difv;
-  }
-  __syncthreads();
-  if (a.hr && valid) {
-    for (int l = g0; l <= a.nlay; l += 16) {
-      const int L = l + 1;  // SUNTOT(2L+1), IRTOTAL(2L+1)
-      const double nv = fv[c * nl + L - 1], ni = fi[c * nl + L - 1];
-      const double sunt = sun ? (L == 1 ? nv - topv : nv - fv[c * nl + L - 2]) : 0.0;
-      const double irt = (L == 1) ? ni - topi : ni - fi[c * nl + L - 2];
-      const double dp = (l == 0) ? (plev[0] + plev[1]) : (plev[2 * l + 2] - plev[2 * l]);
-      a.hr[(size_t)col * nl + l] = (((sunt + irt) * GRAV) / ((CPAIR * SCALEP) * dp)) * SECONDS_PER_SOL;
-    }
-  }
-}
-
-template <int NLM>
-static hipError_t launch_nlm(const DevTables &dt, const RunArgs &a, hipStream_t st) {
-  const size_t lds = stage_bytes(a.nlay);
-  const unsigned gx = (unsigned)((a.ncol + 3) / 4);
-  hipLaunchKernelGGL((k_rt_fp64<NLM>), dim3(gx), dim3(64), lds, st, dt, a);
-  return hipGetLastError();
-}
-
 hipError_t launch_plain(const DevTables &dt, const RunArgs &a, hipStream_t st) {
-  const int nl = a.nlay + 1;
-  if (nl <= 17) return launch_nlm<17>(dt, a, st);
-  if (nl <= 25) return launch_nlm<25>(dt, a, st);
-  if (nl <= 37) return launch_nlm<37>(dt, a, st);
-  if (nl <= 49) return launch_nlm<49>(dt, a, st);
-  return launch_nlm<MAX_NLAY + 1>(dt, a, st);
+  return launch_column<false>(dt, a, st);
 }
-
 }  // namespace mgcm
