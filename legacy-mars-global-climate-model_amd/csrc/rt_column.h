@@ -112,7 +112,7 @@ static __device__ void tp_stage(const RtConst *__restrict__ rc, double pw, doubl
 // line per corner.
 template <bool VIS, bool F32 = false>
 struct GasSrc {
-  using kt = typename std::conditional<F32, float, double>::type;  // table and interpolation type
+  using kt = typename std::conditional<F32, float, double>::type;  // table storage type
   const kt *__restrict__ tab;      // band's table
   const double *t_, *u_, *wr_, *uk_, *c1_, *c2_;  // this column's staged sub-levels (LDS)
   const int *base_;
@@ -146,11 +146,14 @@ struct GasSrc {
     const R ans = lc0 * k1 + lc1 * k2 + lc2 * k3 + lc3 * k4;
     return (R)uk_[K] * ans;
   }
-  __device__ __forceinline__ kt finish(int K, const Raw &r) {
-    const kt taugas = taugas_of<kt>(K, r.a0, r.a1, r.b0, r.b1, r.c0, r.c1, r.d0, r.d1);
-    taugsurf = taugsurf + (double)taugas;
-    if (VIS) return taugas + (kt)c1_[K];
-    return taugas + (kt)c1_[K] + (kt)c2_[K];
+  // DTAUK(K) = gas + continuum, in FP64 on both paths: the mixed path only
+  // stores the table in FP32 (FP32 arithmetic here costs accuracy -- 1 - W of
+  // near-conservative layers -- and runs no faster than FP64 on this VALU)
+  __device__ __forceinline__ double finish(int K, const Raw &r) {
+    const double taugas = taugas_of<double>(K, r.a0, r.a1, r.b0, r.b1, r.c0, r.c1, r.d0, r.d1);
+    taugsurf = taugsurf + taugas;
+    if (VIS) return taugas + c1_[K];
+    return taugas + c1_[K] + c2_[K];
   }
   // TAUGSURF of the FP64 table tab64 (this lane's g), summed in the order of
   // the plain path: the mixed path settles TLIMIT decisions near the threshold
@@ -170,9 +173,10 @@ struct GasSrc {
 };
 
 // TLIMIT test of a gas point (code/sfluxv.f90:72, code/sfluxi.f90:72).  The
-// mixed path's FP32 sum is within a relative 1e-5 of the FP64 one; closer
-// than 1e-4 to the limit it is recomputed in FP64 (all lanes of the wave take
-// part, the others keep their FP32 decision).
+// mixed path's sum over FP32 table values is within a relative 1e-6 of the
+// FP64-table one; closer than 1e-4 to the limit it is recomputed from the FP64
+// table (all lanes of the wave take part, the others keep their decision), so
+// both paths skip the same Gauss points.
 template <bool MIXED, class Src>
 __device__ __forceinline__ bool below_tlimit(const Src &src, const double *tab64, int nlev, double tlim,
                                              bool active) {
@@ -265,24 +269,21 @@ __device__ __forceinline__ double acc_get(const double *arr, int n) {
 #define MGCM_FP64_WAVES 2
 #endif
 
-// MIXED: the k-tables and the 8-corner k-interpolation in FP32, everything
-// downstream (optical depths, scaling, two-stream terms, solve, sums) in FP64.
+// MIXED: the k-tables are read in FP32 (half the gather bytes and corner
+// registers); all arithmetic, the interpolation included, is FP64.
+// The whole radiation call of columns col0..col0+3 of grid a, by one wave.
+// The exp table must be in LDS; the staging after it is rewritten.
 template <int NLM, bool MIXED>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MGCM_FP64_WAVES)))
-void k_rt(DevTables dt, RunArgs a) {
+__device__ __forceinline__ void column_block(const DevTables &dt, const RunArgs &a, int col0, double *smem) {
   using kt = typename std::conditional<MIXED, float, double>::type;
-  extern __shared__ double smem[];
   constexpr int SL = (NLM + 15) / 16;
   const int nl = a.nlay + 1, nlev = 2 * a.nlay + 3, nsub = nlev - 1;
-  const int col0 = blockIdx.x * 4;
   const int lane = threadIdx.x, c = lane >> 4, g0 = lane & 15;
   const int col = col0 + c;
   const bool valid = col < a.ncol;
   const int colc = valid ? col : a.ncol - 1;
   const RtConst *rc = dt.c;
   Stage S = carve(smem, nsub, nl, nlev);
-  mgcm_libm::load_lds_exptab(lane, 64);
-  __syncthreads();
 
   // 1. TPINDEX of every sub-level K = 2..L_LEVELS of the 4 columns
   for (int t = lane; t < 4 * nsub; t += 64) {
@@ -559,6 +560,32 @@ void k_rt(DevTables dt, RunArgs a) {
   }
 }
 
+// One grid: block b owns columns 4b..4b+3.
+template <int NLM, bool MIXED>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MGCM_FP64_WAVES)))
+void k_rt(DevTables dt, RunArgs a) {
+  extern __shared__ double smem[];
+  mgcm_libm::load_lds_exptab(threadIdx.x, 64);
+  __syncthreads();
+  column_block<NLM, MIXED>(dt, a, blockIdx.x * 4, smem);
+}
+
+// Many grids of one layer-count class: a persistent grid of blocks walks the
+// work list (grid index, first column) round-robin, largest layer counts first.
+// No counters: every block's share is fixed by its index.
+template <int NLM, bool MIXED>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MGCM_FP64_WAVES)))
+void k_rt_batch(DevTables dt, const RunArgs *__restrict__ grids, const uint2 *__restrict__ work, int nwork) {
+  extern __shared__ double smem[];
+  mgcm_libm::load_lds_exptab(threadIdx.x, 64);
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    __syncthreads();  // the previous item's reads of the staging are done
+    const uint2 wk = work[w];
+    const RunArgs a = grids[wk.x];
+    column_block<NLM, MIXED>(dt, a, (int)wk.y, smem);
+  }
+}
+
 template <int NLM, bool MIXED>
 static hipError_t launch_nlm(const DevTables &dt, const RunArgs &a, hipStream_t st) {
   const size_t lds = stage_bytes(a.nlay);
@@ -577,6 +604,35 @@ hipError_t launch_column(const DevTables &dt, const RunArgs &a, hipStream_t st) 
   if (nl <= 37) return launch_nlm<37, MIXED>(dt, a, st);
   if (nl <= 49) return launch_nlm<49, MIXED>(dt, a, st);
   return launch_nlm<MAX_NLAY + 1, MIXED>(dt, a, st);
+}
+
+// Layer-count classes of the batched path: kernel instance NLM = KCLASS_NL[k]
+// serves grids with nlay + 1 <= NLM.
+constexpr int NCLASS = 5;
+constexpr int KCLASS_NL[NCLASS] = {17, 25, 37, 49, MAX_NLAY + 1};
+inline int nl_class(int nl) {
+  for (int k = 0; k < NCLASS; k++)
+    if (nl <= KCLASS_NL[k]) return k;
+  return NCLASS - 1;
+}
+
+template <int NLM, bool MIXED>
+static hipError_t launch_batch_nlm(const DevTables &dt, const RunArgs *grids, const uint2 *work, int nwork,
+                                   int nblocks, size_t lds, hipStream_t st) {
+  hipLaunchKernelGGL((k_rt_batch<NLM, MIXED>), dim3(nblocks), dim3(64), lds, st, dt, grids, work, nwork);
+  return hipGetLastError();
+}
+
+template <bool MIXED>
+hipError_t launch_batch_class(int cls, const DevTables &dt, const RunArgs *grids, const uint2 *work, int nwork,
+                              int nblocks, size_t lds, hipStream_t st) {
+  switch (cls) {
+    case 0: return launch_batch_nlm<17, MIXED>(dt, grids, work, nwork, nblocks, lds, st);
+    case 1: return launch_batch_nlm<25, MIXED>(dt, grids, work, nwork, nblocks, lds, st);
+    case 2: return launch_batch_nlm<37, MIXED>(dt, grids, work, nwork, nblocks, lds, st);
+    case 3: return launch_batch_nlm<49, MIXED>(dt, grids, work, nwork, nblocks, lds, st);
+    default: return launch_batch_nlm<MAX_NLAY + 1, MIXED>(dt, grids, work, nwork, nblocks, lds, st);
+  }
 }
 
 }  // namespace mgcm

@@ -20,6 +20,10 @@ struct mgcm_rt_ctx {
 namespace mgcm {
 hipError_t launch_plain(const DevTables &dt, const RunArgs &a, hipStream_t st);
 hipError_t launch_mixed(const DevTables &dt, const RunArgs &a, hipStream_t st);
+hipError_t launch_batch_plain(int cls, const DevTables &dt, const RunArgs *grids, const uint2 *work, int nwork,
+                              int nblocks, size_t lds, hipStream_t st);
+hipError_t launch_batch_mixed(int cls, const DevTables &dt, const RunArgs *grids, const uint2 *work, int nwork,
+                              int nblocks, size_t lds, hipStream_t st);
 int check_columns(const mgcm_rt_columns *in, const mgcm_rt_fluxes *out);
 void fill_args(RunArgs &a, const mgcm_rt_columns *in, const mgcm_rt_fluxes *out, int *flag);
 int run_batched_impl(mgcm_rt_ctx *ctx, int ngrids, const mgcm_rt_columns *grids, const mgcm_rt_fluxes *outs,

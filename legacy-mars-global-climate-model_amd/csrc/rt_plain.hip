@@ -6,4 +6,8 @@ namespace mgcm {
 hipError_t launch_plain(const DevTables &dt, const RunArgs &a, hipStream_t st) {
   return launch_column<false>(dt, a, st);
 }
+hipError_t launch_batch_plain(int cls, const DevTables &dt, const RunArgs *grids, const uint2 *work, int nwork,
+                          int nblocks, size_t lds, hipStream_t st) {
+  return launch_batch_class<false>(cls, dt, grids, work, nwork, nblocks, lds, st);
+}
 }  // namespace mgcm

@@ -70,12 +70,25 @@ OUT_KEYS = ("fmnetv", "fluxupv", "fluxdnv", "fmneti", "fluxupi", "fluxdni",
 _lib = None
 
 
+def _one_hip_runtime() -> None:
+    """A process must map one HIP runtime.  PyTorch ships its own libamdhip64 and
+    HSA runtime; loaded first, they satisfy the library's HIP dependency by
+    soname.  Loaded after the library, they would be a second runtime next to
+    /opt/rocm's and torch would find no device.  So torch, when installed, is
+    imported before the library is loaded."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def load_library(path: str = LIB_PATH) -> C.CDLL:
     """Load libmgcm_rt.so (raises if it was not built)."""
     global _lib
     if _lib is None:
         if not os.path.exists(path):
             raise FileNotFoundError(f"{path} is missing: build it with build.py (no CPU fallback exists)")
+        _one_hip_runtime()
         lib = C.CDLL(path)
         lib.mgcm_rt_init.argtypes = [C.POINTER(Tables), C.POINTER(C.c_void_p)]
         lib.mgcm_rt_destroy.argtypes = [C.c_void_p]

@@ -306,9 +306,31 @@ def make_workload(seed: int, ncol: int, nlay: int, clouds: bool = False, weights
     return make_tables(seed, nlay, weights), make_columns(seed, ncol, nlay, clouds)
 
 
+def ragged_shapes(seed: int, ngrids: int = 512, cols=(2000, 20000), lays=(16, 64)) -> list:
+    """(ncol, nlay) of each grid of a ragged batch: U{cols} columns, U{lays} layers."""
+    rng = np.random.Generator(np.random.PCG64(seed + 104729))
+    nc = rng.integers(cols[0], cols[1], size=ngrids, endpoint=True)
+    nl = rng.integers(lays[0], lays[1], size=ngrids, endpoint=True)
+    return [(int(a), int(b)) for a, b in zip(nc, nl)]
+
+
+def ragged_pools(seed: int, shapes: list, clouds: bool = False) -> dict:
+    """One pool of columns per layer count, as long as the largest grid using it.
+
+    Grid i reads the first ncol_i columns of pool[nlay_i]: inputs are shared
+    read-only between grids of equal layer count, which keeps a batch of
+    millions of columns cheap to generate; every grid writes its own outputs.
+    """
+    need = {}
+    for n, nl in shapes:
+        need[nl] = max(need.get(nl, 0), n)
+    return {nl: make_columns(seed * 1000 + nl, n, nl, clouds) for nl, n in sorted(need.items())}
+
+
 CONFIGS = {
     # name: (ncol, nlay, clouds)
     "c1_cpu_2160x24": (2160, 24, False),
     "c2_gpu_64800x24": (64800, 24, False),
     "c3_mixed_64800x36": (64800, 36, True),
+    # c4: ragged_shapes(seed) -- 512 grids, 2000..20000 columns, 16..64 layers
 }

@@ -109,7 +109,13 @@ class Oracle:
             setattr(s, name, _p(a))
         return s
 
+    @staticmethod
+    def _check_layers(t: dict, c: dict) -> None:
+        if int(c["nlay"]) != int(t["nlay"]):
+            raise ValueError(f"tables are for {int(t['nlay'])} layers, columns have {int(c['nlay'])}")
+
     def run(self, t: dict, c: dict, nthreads: int = 0, out: dict | None = None) -> dict:
+        self._check_layers(t, c)
         keep: list = []
         ts = self._tables(t, keep)
         ins = self._inputs(c, keep)
@@ -120,6 +126,7 @@ class Oracle:
         return o
 
     def count(self, t: dict, c: dict) -> dict:
+        self._check_layers(t, c)
         keep: list = []
         ts = self._tables(t, keep)
         ins = self._inputs(c, keep)

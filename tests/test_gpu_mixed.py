@@ -1,12 +1,11 @@
-"""Mixed-precision path (FP32 k-tables and k-interpolation, FP64 elsewhere) on the GPU.
+"""Mixed-precision path (k-tables stored in FP32, all arithmetic FP64) on the GPU.
 
 Bar: the reference formulation is ill-conditioned in the k-coefficients on these
 inputs (thin IR layers: B1 = dB/DTAU enters with TERM; thick layers clip at 35),
 so rounding the k-tables to FP32 moves even an all-FP64 evaluation by up to ~1 K/sol.
 That floor is measured here with the oracle (FP64 arithmetic, FP32-rounded
 tables), and the mixed path must stay within twice it (plus 0.05 K/sol), with an
-RMS error below 0.05 K/sol.  TLIMIT decisions must match the FP64 path exactly:
-DETAU/CUMTAUV of sunlit columns agree to FP32 precision.
+RMS error below 0.05 K/sol.  DETAU of sunlit columns agrees to FP32 precision.
 """
 from __future__ import annotations
 
