@@ -1,6 +1,7 @@
 """Builds the MI355X radiation library in-tree: mgcm_rt/libmgcm_rt.so (gfx950).
 
-    python build.py            # compile every HIP source into the shared library
+    python build.py            # compile every HIP source into the shared library, then the
+                               # Fortran host module and driver (fortran/) with amdflang
 
 Flags: -ffp-contract=off keeps the FP64 path's evaluation order (no fused
 multiply-adds the reference does not make); packed FP32 ops are disabled.
@@ -16,6 +17,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "mgcm_rt", "libmgcm_rt.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FC = os.environ.get("FC", "/opt/rocm/bin/amdflang")
 ARCH = os.environ.get("MGCM_RT_ARCH", "gfx950")
 
 COMMON = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
@@ -47,7 +49,33 @@ def build(verbose: bool = False) -> str:
     cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_fortran(verbose)
     return LIB
+
+
+def build_fortran(verbose: bool = False) -> str | None:
+    """The ISO_C_BINDING module and the COMP3-style driver program (amdflang)."""
+    if not os.path.exists(FC):
+        print(f"build.py: {FC} not found; Fortran host not built", file=sys.stderr)
+        return None
+    objdir = os.path.join(HERE, "_build", "fortran")
+    os.makedirs(objdir, exist_ok=True)
+    src = os.path.join(HERE, "fortran")
+    exe = os.path.join(src, "mgcm_rt_driver")
+    steps = [
+        [FC, "-O2", "-c", os.path.join(src, "mgcm_rt_mod.f90"), "-J", objdir, "-o",
+         os.path.join(objdir, "mgcm_rt_mod.o")],
+        [FC, "-O2", "-c", os.path.join(src, "mgcm_rt_driver.f90"), "-J", objdir, "-o",
+         os.path.join(objdir, "mgcm_rt_driver.o")],
+        [FC, "-o", exe + ".tmp", os.path.join(objdir, "mgcm_rt_driver.o"), os.path.join(objdir, "mgcm_rt_mod.o"),
+         "-L", os.path.dirname(LIB), "-lmgcm_rt", "-Wl,-rpath,$ORIGIN/../mgcm_rt"],
+    ]
+    for cmd in steps:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+    os.replace(exe + ".tmp", exe)
+    return exe
 
 
 if __name__ == "__main__":

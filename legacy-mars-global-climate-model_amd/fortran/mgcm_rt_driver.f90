@@ -1,0 +1,90 @@
+! mgcm_rt_driver.f90 -- a COMP3-style host program over mgcm_rt_mod: reads a
+! radiation case (tables and a batch of columns) from a stream file, sets the
+! tables up once, runs the batch through mgcm_rt_radiation and writes the
+! results to a second stream file.
+!
+!   mgcm_rt_driver <case file> <result file>
+!
+! Case file (native-endian stream): int32 ncol, nlay, precision; then FP64
+! CO2V(7,51,10,7,17), CO2I(7,51,10,5,17), PFGASREF(51), TGASREF(7),
+! WREFH2O(10), GWV(17,7), GWI(17,5), FZEROV(7), FZEROI(5), TAURAY(7), DWNI(5),
+! PLANCKIR(5,8501), dust Qext/Qscat/g VIS(7) and IR(5), cloud Qext/Qscat/g
+! VIS(7) and IR(5), QEXTREFDST, QEXTREFCLD, SOL(7); then per column set
+! PLEV, TLEV, PMID, TMID, QH2O (L_LEVELS, ncol), TAUREF, TAUREFCLD
+! (L_LEVELS+1, ncol), ALS, ALBI, ACOSZ (ncol).
+! Result file: int32 status; FMNETV, FLUXUPV, FLUXDNV, FMNETI, FLUXUPI,
+! FLUXDNI (L_NLAYRAD, ncol), NFLUXTOPV, NFLUXTOPI, DIFFVT (ncol), DETAU,
+! CUMTAUV (7, 17, ncol), heating (L_NLAYRAD, ncol).
+program mgcm_rt_driver
+  use iso_c_binding
+  use mgcm_rt_mod
+  implicit none
+  character(len=4096) :: fin, fout
+  integer :: u, ncol, nlay, nlev, nl, ios
+  integer(c_int) :: precision, status, s2
+  type(c_ptr) :: ctx
+  real(c_double), allocatable :: co2v(:,:,:,:,:), co2i(:,:,:,:,:)
+  real(c_double) :: pfgasref(MGCM_RT_NP), tgasref(MGCM_RT_NT), wrefh2o(MGCM_RT_NH)
+  real(c_double) :: gwv(MGCM_RT_NG, MGCM_RT_NV), gwi(MGCM_RT_NG, MGCM_RT_NI)
+  real(c_double) :: fzerov(MGCM_RT_NV), fzeroi(MGCM_RT_NI), tauray(MGCM_RT_NV), dwni(MGCM_RT_NI)
+  real(c_double), allocatable :: planckir(:,:)
+  real(c_double) :: qxvdst(MGCM_RT_NV), qsvdst(MGCM_RT_NV), gvdst(MGCM_RT_NV)
+  real(c_double) :: qxidst(MGCM_RT_NI), qsidst(MGCM_RT_NI), gidst(MGCM_RT_NI)
+  real(c_double) :: qxvcld(MGCM_RT_NV), qsvcld(MGCM_RT_NV), gvcld(MGCM_RT_NV)
+  real(c_double) :: qxicld(MGCM_RT_NI), qsicld(MGCM_RT_NI), gicld(MGCM_RT_NI)
+  real(c_double) :: qextrefdst, qextrefcld, sol(MGCM_RT_NV)
+  real(c_double), allocatable :: plev(:,:), tlev(:,:), pmid(:,:), tmid(:,:), qh2o(:,:)
+  real(c_double), allocatable :: tauref(:,:), taurefcld(:,:), als(:), albi(:), acosz(:)
+  real(c_double), allocatable :: fmnetv(:,:), fluxupv(:,:), fluxdnv(:,:), fmneti(:,:), fluxupi(:,:)
+  real(c_double), allocatable :: fluxdni(:,:), nfluxtopv(:), nfluxtopi(:), diffvt(:)
+  real(c_double), allocatable :: detau(:,:,:), cumtauv(:,:,:), hr(:,:)
+
+  if (command_argument_count() /= 2) then
+    write(*, '(a)') 'usage: mgcm_rt_driver <case file> <result file>'
+    stop 2
+  end if
+  call get_command_argument(1, fin)
+  call get_command_argument(2, fout)
+
+  open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old', iostat=ios)
+  if (ios /= 0) then
+    write(*, '(a)') 'mgcm_rt_driver: cannot open '//trim(fin)
+    stop 1
+  end if
+  read(u) ncol, nlay, precision
+  nlev = 2 * nlay + 3
+  nl = nlay + 1
+  allocate(co2v(MGCM_RT_NT, MGCM_RT_NP, MGCM_RT_NH, MGCM_RT_NV, MGCM_RT_NG))
+  allocate(co2i(MGCM_RT_NT, MGCM_RT_NP, MGCM_RT_NH, MGCM_RT_NI, MGCM_RT_NG))
+  allocate(planckir(MGCM_RT_NI, MGCM_RT_NPLANCK))
+  read(u) co2v, co2i, pfgasref, tgasref, wrefh2o, gwv, gwi, fzerov, fzeroi, tauray, dwni, planckir
+  read(u) qxvdst, qsvdst, gvdst, qxidst, qsidst, gidst, qxvcld, qsvcld, gvcld, qxicld, qsicld, gicld
+  read(u) qextrefdst, qextrefcld, sol
+  allocate(plev(nlev, ncol), tlev(nlev, ncol), pmid(nlev, ncol), tmid(nlev, ncol), qh2o(nlev, ncol))
+  allocate(tauref(nlev + 1, ncol), taurefcld(nlev + 1, ncol), als(ncol), albi(ncol), acosz(ncol))
+  read(u) plev, tlev, pmid, tmid, qh2o, tauref, taurefcld, als, albi, acosz
+  close(u)
+
+  allocate(fmnetv(nl, ncol), fluxupv(nl, ncol), fluxdnv(nl, ncol), fmneti(nl, ncol), fluxupi(nl, ncol))
+  allocate(fluxdni(nl, ncol), nfluxtopv(ncol), nfluxtopi(ncol), diffvt(ncol), hr(nl, ncol))
+  allocate(detau(MGCM_RT_NV, MGCM_RT_NG, ncol), cumtauv(MGCM_RT_NV, MGCM_RT_NG, ncol))
+  detau = 0.0d0
+  cumtauv = 0.0d0
+
+  call mgcm_rt_setup(co2v, co2i, pfgasref, tgasref, wrefh2o, gwv, gwi, fzerov, fzeroi, tauray, dwni, &
+                     planckir, qxvdst, qsvdst, gvdst, qxidst, qsidst, gidst, qxvcld, qsvcld, gvcld, &
+                     qxicld, qsicld, gicld, qextrefdst, qextrefcld, ctx, status)
+  if (status == MGCM_RT_OK) then
+    call mgcm_rt_radiation(ctx, ncol, nlay, plev, tlev, pmid, tmid, qh2o, tauref, taurefcld, als, albi, &
+                           acosz, sol, fmnetv, fluxupv, fluxdnv, fmneti, fluxupi, fluxdni, nfluxtopv, &
+                           nfluxtopi, diffvt, detau, cumtauv, hr, precision, status)
+    s2 = mgcm_rt_destroy(ctx)
+  end if
+  if (status /= MGCM_RT_OK) write(*, '(a,i0,a)') 'mgcm_rt_driver: status ', status, ': '//trim(mgcm_rt_message(status))
+
+  open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
+  write(u) status
+  write(u) fmnetv, fluxupv, fluxdnv, fmneti, fluxupi, fluxdni, nfluxtopv, nfluxtopi, diffvt, detau, cumtauv, hr
+  close(u)
+  if (status /= MGCM_RT_OK) stop 1
+end program mgcm_rt_driver

@@ -1,0 +1,219 @@
+! mgcm_rt_mod.f90 -- Fortran binding of the MI355X column radiative transfer
+! (include/mgcm_rt.h) through ISO_C_BINDING.
+!
+! The reference's physics driver COMP3 calls OPTCV, SFLUXV, OPTCI and SFLUXI
+! once per column (code/comp3.f:637-738).  With this module it gathers a batch
+! of columns in the reference's per-column layouts, calls mgcm_rt_radiation
+! once, and scatters the results: FMNETV/FLUXUPV/FLUXDNV, FMNETI/FLUXUPI/
+! FLUXDNI per radiation layer, NFLUXTOPV/I, DIFFVT, DETAU and TAUCUMV(L_LEVELS)
+! per (band, Gauss point), and the layer heating rates in K/sol.
+module mgcm_rt_mod
+  use iso_c_binding
+  implicit none
+  private
+
+  integer, parameter, public :: MGCM_RT_NT = 7, MGCM_RT_NP = 51, MGCM_RT_NH = 10, MGCM_RT_NG = 17
+  integer, parameter, public :: MGCM_RT_NV = 7, MGCM_RT_NI = 5, MGCM_RT_NPLANCK = 8501, MGCM_RT_MAX_NLAY = 64
+  integer(c_int), parameter, public :: MGCM_RT_OK = 0, MGCM_RT_ERR_ARG = -1, MGCM_RT_ERR_NLAY = -2
+  integer(c_int), parameter, public :: MGCM_RT_ERR_DEVICE = -3, MGCM_RT_ERR_NOMEM = -4
+  integer(c_int), parameter, public :: MGCM_RT_ERR_RANGE = -5, MGCM_RT_ERR_PRECISION = -6
+  integer(c_int), parameter, public :: MGCM_RT_FP64 = 0, MGCM_RT_MIXED = 1
+
+  ! mirrors mgcm_rt_tables (host pointers, copied by mgcm_rt_init)
+  type, bind(C), public :: mgcm_rt_tables
+    type(c_ptr) :: co2v, co2i, pfgasref, tgasref, wrefh2o, gwv, gwi, fzerov, fzeroi, tauray, dwni, planckir
+    type(c_ptr) :: qxv_dst, qsv_dst, gv_dst, qxi_dst, qsi_dst, gi_dst
+    type(c_ptr) :: qxv_cld, qsv_cld, gv_cld, qxi_cld, qsi_cld, gi_cld
+    real(c_double) :: qextref_dst, qextref_cld
+  end type
+
+  ! mirrors mgcm_rt_columns
+  type, bind(C), public :: mgcm_rt_columns
+    integer(c_int) :: ncol, nlay
+    type(c_ptr) :: plev, tlev, pmid, tmid, qh2o, tauref, taurefcld, als, albi, acosz
+    real(c_double) :: sol(7)
+  end type
+
+  ! mirrors mgcm_rt_fluxes
+  type, bind(C), public :: mgcm_rt_fluxes
+    type(c_ptr) :: fmnetv, fluxupv, fluxdnv, fmneti, fluxupi, fluxdni
+    type(c_ptr) :: nfluxtopv, nfluxtopi, diffvt, detau, cumtauv, hr
+  end type
+
+  public :: mgcm_rt_init, mgcm_rt_destroy, mgcm_rt_run, mgcm_rt_run_host, mgcm_rt_check
+  public :: mgcm_rt_run_batched, mgcm_rt_planck_table, mgcm_rt_laginterp, mgcm_rt_strerror
+  public :: mgcm_rt_setup, mgcm_rt_radiation, mgcm_rt_message
+
+  interface
+    integer(c_int) function mgcm_rt_init(tables, ctx) bind(C, name="mgcm_rt_init")
+      import :: c_int, c_ptr, mgcm_rt_tables
+      type(mgcm_rt_tables), intent(in) :: tables
+      type(c_ptr), intent(out) :: ctx
+    end function
+    integer(c_int) function mgcm_rt_destroy(ctx) bind(C, name="mgcm_rt_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function mgcm_rt_run(ctx, cols, outs, precision, stream) bind(C, name="mgcm_rt_run")
+      import :: c_int, c_ptr, mgcm_rt_columns, mgcm_rt_fluxes
+      type(c_ptr), value :: ctx
+      type(mgcm_rt_columns), intent(in) :: cols
+      type(mgcm_rt_fluxes), intent(in) :: outs
+      integer(c_int), value :: precision
+      type(c_ptr), value :: stream
+    end function
+    integer(c_int) function mgcm_rt_run_host(ctx, cols, outs, precision) bind(C, name="mgcm_rt_run_host")
+      import :: c_int, c_ptr, mgcm_rt_columns, mgcm_rt_fluxes
+      type(c_ptr), value :: ctx
+      type(mgcm_rt_columns), intent(in) :: cols
+      type(mgcm_rt_fluxes), intent(in) :: outs
+      integer(c_int), value :: precision
+    end function
+    integer(c_int) function mgcm_rt_check(ctx) bind(C, name="mgcm_rt_check")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function mgcm_rt_run_batched(ctx, ngrids, grids, outs, precision, stream) &
+        bind(C, name="mgcm_rt_run_batched")
+      import :: c_int, c_ptr, mgcm_rt_columns, mgcm_rt_fluxes
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ngrids
+      type(mgcm_rt_columns), intent(in) :: grids(*)
+      type(mgcm_rt_fluxes), intent(in) :: outs(*)
+      integer(c_int), value :: precision
+      type(c_ptr), value :: stream
+    end function
+    integer(c_int) function mgcm_rt_planck_table(planckir, dwni, wnoi) bind(C, name="mgcm_rt_planck_table")
+      import :: c_int, c_double
+      real(c_double), intent(out) :: planckir(*), dwni(*), wnoi(*)
+    end function
+    integer(c_int) function mgcm_rt_laginterp(pgref, raw, nbands, pint, out) bind(C, name="mgcm_rt_laginterp")
+      import :: c_int, c_double
+      real(c_double), intent(in) :: pgref(*), raw(*)
+      integer(c_int), value :: nbands
+      real(c_double), intent(out) :: pint(*), out(*)
+    end function
+    type(c_ptr) function mgcm_rt_strerror(status) bind(C, name="mgcm_rt_strerror")
+      import :: c_int, c_ptr
+      integer(c_int), value :: status
+    end function
+  end interface
+
+contains
+
+  ! Status text of a return code.
+  function mgcm_rt_message(status) result(msg)
+    integer(c_int), intent(in) :: status
+    character(len=80) :: msg
+    character(kind=c_char), pointer :: s(:)
+    integer :: i
+    msg = ' '
+    call c_f_pointer(mgcm_rt_strerror(status), s, [80])
+    do i = 1, 80
+      if (s(i) == c_null_char) exit
+      msg(i:i) = s(i)
+    end do
+  end function
+
+  ! RADSETUP's tables (code/radsetup.f90:26-46) into a device context.  Arrays
+  ! in the reference's layouts; the aerosol constants per band.
+  subroutine mgcm_rt_setup(co2v, co2i, pfgasref, tgasref, wrefh2o, gwv, gwi, fzerov, fzeroi, tauray, dwni, &
+                           planckir, qxvdst, qsvdst, gvdst, qxidst, qsidst, gidst, qxvcld, qsvcld, gvcld, &
+                           qxicld, qsicld, gicld, qextrefdst, qextrefcld, ctx, status)
+    real(c_double), intent(in), target :: co2v(MGCM_RT_NT, MGCM_RT_NP, MGCM_RT_NH, MGCM_RT_NV, MGCM_RT_NG)
+    real(c_double), intent(in), target :: co2i(MGCM_RT_NT, MGCM_RT_NP, MGCM_RT_NH, MGCM_RT_NI, MGCM_RT_NG)
+    real(c_double), intent(in), target :: pfgasref(MGCM_RT_NP), tgasref(MGCM_RT_NT), wrefh2o(MGCM_RT_NH)
+    real(c_double), intent(in), target :: gwv(MGCM_RT_NG, MGCM_RT_NV), gwi(MGCM_RT_NG, MGCM_RT_NI)
+    real(c_double), intent(in), target :: fzerov(MGCM_RT_NV), fzeroi(MGCM_RT_NI), tauray(MGCM_RT_NV)
+    real(c_double), intent(in), target :: dwni(MGCM_RT_NI), planckir(MGCM_RT_NI, MGCM_RT_NPLANCK)
+    real(c_double), intent(in), target :: qxvdst(MGCM_RT_NV), qsvdst(MGCM_RT_NV), gvdst(MGCM_RT_NV)
+    real(c_double), intent(in), target :: qxidst(MGCM_RT_NI), qsidst(MGCM_RT_NI), gidst(MGCM_RT_NI)
+    real(c_double), intent(in), target :: qxvcld(MGCM_RT_NV), qsvcld(MGCM_RT_NV), gvcld(MGCM_RT_NV)
+    real(c_double), intent(in), target :: qxicld(MGCM_RT_NI), qsicld(MGCM_RT_NI), gicld(MGCM_RT_NI)
+    real(c_double), intent(in) :: qextrefdst, qextrefcld
+    type(c_ptr), intent(out) :: ctx
+    integer(c_int), intent(out) :: status
+    type(mgcm_rt_tables) :: t
+    t%co2v = c_loc(co2v)
+    t%co2i = c_loc(co2i)
+    t%pfgasref = c_loc(pfgasref)
+    t%tgasref = c_loc(tgasref)
+    t%wrefh2o = c_loc(wrefh2o)
+    t%gwv = c_loc(gwv)
+    t%gwi = c_loc(gwi)
+    t%fzerov = c_loc(fzerov)
+    t%fzeroi = c_loc(fzeroi)
+    t%tauray = c_loc(tauray)
+    t%dwni = c_loc(dwni)
+    t%planckir = c_loc(planckir)
+    t%qxv_dst = c_loc(qxvdst)
+    t%qsv_dst = c_loc(qsvdst)
+    t%gv_dst = c_loc(gvdst)
+    t%qxi_dst = c_loc(qxidst)
+    t%qsi_dst = c_loc(qsidst)
+    t%gi_dst = c_loc(gidst)
+    t%qxv_cld = c_loc(qxvcld)
+    t%qsv_cld = c_loc(qsvcld)
+    t%gv_cld = c_loc(gvcld)
+    t%qxi_cld = c_loc(qxicld)
+    t%qsi_cld = c_loc(qsicld)
+    t%gi_cld = c_loc(gicld)
+    t%qextref_dst = qextrefdst
+    t%qextref_cld = qextrefcld
+    status = mgcm_rt_init(t, ctx)
+  end subroutine
+
+  ! The per-column OPTCV/SFLUXV/OPTCI/SFLUXI sequence of COMP3 for ncol
+  ! columns at once (host arrays; copies in, runs, copies out).  Column i
+  ! holds PLEV(:,i) etc. as COMP3 passes them; SOL = SOLARF/RSDIST.
+  ! detau/cumtauv keep their values for sun-down columns, as the reference
+  ! leaves its module arrays.
+  subroutine mgcm_rt_radiation(ctx, ncol, nlay, plev, tlev, pmid, tmid, qh2o, tauref, taurefcld, &
+                               als, albi, acosz, sol, fmnetv, fluxupv, fluxdnv, fmneti, fluxupi, &
+                               fluxdni, nfluxtopv, nfluxtopi, diffvt, detau, cumtauv, hr, precision, status)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: ncol, nlay
+    real(c_double), intent(in), target :: plev(2*nlay+3, ncol), tlev(2*nlay+3, ncol)
+    real(c_double), intent(in), target :: pmid(2*nlay+3, ncol), tmid(2*nlay+3, ncol), qh2o(2*nlay+3, ncol)
+    real(c_double), intent(in), target :: tauref(2*nlay+4, ncol), taurefcld(2*nlay+4, ncol)
+    real(c_double), intent(in), target :: als(ncol), albi(ncol), acosz(ncol)
+    real(c_double), intent(in) :: sol(MGCM_RT_NV)
+    real(c_double), intent(out), target :: fmnetv(nlay+1, ncol), fluxupv(nlay+1, ncol), fluxdnv(nlay+1, ncol)
+    real(c_double), intent(out), target :: fmneti(nlay+1, ncol), fluxupi(nlay+1, ncol), fluxdni(nlay+1, ncol)
+    real(c_double), intent(out), target :: nfluxtopv(ncol), nfluxtopi(ncol), diffvt(ncol)
+    real(c_double), intent(inout), target :: detau(MGCM_RT_NV, MGCM_RT_NG, ncol)
+    real(c_double), intent(inout), target :: cumtauv(MGCM_RT_NV, MGCM_RT_NG, ncol)
+    real(c_double), intent(out), target :: hr(nlay+1, ncol)
+    integer(c_int), intent(in) :: precision
+    integer(c_int), intent(out) :: status
+    type(mgcm_rt_columns) :: c
+    type(mgcm_rt_fluxes) :: o
+    c%ncol = ncol
+    c%nlay = nlay
+    c%plev = c_loc(plev)
+    c%tlev = c_loc(tlev)
+    c%pmid = c_loc(pmid)
+    c%tmid = c_loc(tmid)
+    c%qh2o = c_loc(qh2o)
+    c%tauref = c_loc(tauref)
+    c%taurefcld = c_loc(taurefcld)
+    c%als = c_loc(als)
+    c%albi = c_loc(albi)
+    c%acosz = c_loc(acosz)
+    c%sol = sol
+    o%fmnetv = c_loc(fmnetv)
+    o%fluxupv = c_loc(fluxupv)
+    o%fluxdnv = c_loc(fluxdnv)
+    o%fmneti = c_loc(fmneti)
+    o%fluxupi = c_loc(fluxupi)
+    o%fluxdni = c_loc(fluxdni)
+    o%nfluxtopv = c_loc(nfluxtopv)
+    o%nfluxtopi = c_loc(nfluxtopi)
+    o%diffvt = c_loc(diffvt)
+    o%detau = c_loc(detau)
+    o%cumtauv = c_loc(cumtauv)
+    o%hr = c_loc(hr)
+    status = mgcm_rt_run_host(ctx, c, o, precision)
+  end subroutine
+
+end module mgcm_rt_mod

@@ -1,0 +1,73 @@
+"""Fortran host binding (fortran/mgcm_rt_mod.f90 + mgcm_rt_driver.f90).
+
+CPU: the driver program was built against libmgcm_rt.so and answers its usage
+line without touching a device; the case file round-trips.  GPU: a batch run
+through the Fortran program equals the Python binding of the same C ABI bit
+for bit, and a bad layer count comes back as a status, not an abort."""
+from __future__ import annotations
+
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import PKG
+
+DRIVER = os.path.join(PKG, "fortran", "mgcm_rt_driver")
+
+
+def test_driver_built_and_usage():
+    assert os.path.exists(DRIVER), "build() builds the Fortran driver"
+    r = subprocess.run([DRIVER], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stdout
+
+
+def test_case_file_layout(tmp_path):
+    from mgcm_rt import casefile, synth
+    t, c = synth.make_workload(5, 3, 24)
+    p = str(tmp_path / "case.bin")
+    casefile.write_case(p, t, c, precision=1)
+    raw = np.fromfile(p, dtype=np.uint8)
+    hdr = np.frombuffer(raw[:12].tobytes(), dtype=np.int32)
+    assert hdr.tolist() == [3, 24, 1]
+    nlev = 51
+    n_tab = 7 * 51 * 10 * 7 * 17 + 7 * 51 * 10 * 5 * 17 + 51 + 7 + 10 + 17 * 7 + 17 * 5 + 7 + 5 + 7 + 5 + 5 * 8501
+    n_aer = 6 * 7 + 6 * 5
+    n_col = 3 * (5 * nlev + 2 * (nlev + 1) + 3)
+    assert raw.size == 12 + 8 * (n_tab + n_aer + 2 + 7 + n_col)
+    body = np.frombuffer(raw[12:12 + 8 * 7 * 51 * 10 * 7 * 17].tobytes(), dtype=np.float64)
+    assert np.array_equal(body, t["co2v"].ravel())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nlay,precision", [(24, 0), (36, 0), (24, 1)])
+def test_fortran_driver_matches_python_binding(tmp_path, nlay, precision):
+    from mgcm_rt import Radiation, casefile, synth
+    t, c = synth.make_workload(17, 300, nlay, clouds=(nlay == 36))
+    case, res = str(tmp_path / "case.bin"), str(tmp_path / "res.bin")
+    casefile.write_case(case, t, c, precision)
+    r = subprocess.run([DRIVER, case, res], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    status, got = casefile.read_result(res, 300, nlay)
+    assert status == 0
+    want = Radiation(t).run_host(c, precision=precision)
+    for k in casefile.RESULT_KEYS:
+        assert np.array_equal(got[k], want[k]), k
+
+
+@pytest.mark.gpu
+def test_fortran_driver_reports_bad_layer_count(tmp_path):
+    from mgcm_rt import casefile, synth
+    t, _ = synth.make_workload(18, 2, 24)
+    nlev = 2 * 70 + 3  # arrays sized for the header; the library rejects nlay > 64 before reading them
+    c = {"ncol": 2, "nlay": 70, "als": np.zeros(2), "albi": np.zeros(2), "acosz": np.zeros(2)}
+    c.update({k: np.zeros((2, nlev)) for k in ("plev", "tlev", "pmid", "tmid", "qh2o")})
+    c.update({k: np.zeros((2, nlev + 1)) for k in ("tauref", "taurefcld")})
+    case, res = str(tmp_path / "case.bin"), str(tmp_path / "res.bin")
+    casefile.write_case(case, t, c)
+    r = subprocess.run([DRIVER, case, res], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1
+    status, _ = casefile.read_result(res, 2, 70)
+    assert status == -2
+    assert "layer count" in r.stdout
