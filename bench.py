@@ -41,6 +41,8 @@ def parse():
                    help="columns of the same grid the CPU baseline runs (the whole grid, about 14 s on one core)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-forms", action="store_true", help="only the FP64 headline path")
+    p.add_argument("--sg-train", type=int, default=2000000, help="surrogate training columns")
+    p.add_argument("--sg-epochs", type=int, default=10)
     p.add_argument("--copy-calib", action="store_true",
                    help="also run a 512 MiB device copy (known HBM bytes) for scaling FETCH_SIZE/WRITE_SIZE")
     return p.parse_args()
@@ -198,6 +200,94 @@ def form_batched(args, rt, stream):
     return out
 
 
+def form_surrogate(args, rt_unused, stream):
+    """Learned IR surrogate with exact fallback on 64,800 fresh columns x 36 layers.
+
+    Training: 2,000,000 clear-sky columns whose IR heating the plain path
+    computes on the device; two networks (seeds 1 and 2), 10 epochs each, by
+    the library's own BF16 matrix-core forward/backward kernels and Adam.  The
+    disagreement threshold is calibrated at 0.05 K/sol on 16,384 held-out
+    columns.  A step = VIS part exact + features + both networks + check +
+    exact IR re-solve of the flagged columns, against the plain FP64 step on
+    the same columns."""
+    import torch
+    from mgcm_rt import FP64, SKIP_IR, Radiation, alloc_outputs, synth, to_device
+    from mgcm_rt.surrogate import (Net, Surrogate, calibrate, exact_ir_heating, features, make_stats,
+                                   raw_features, train)
+    bar, nlay, seed = 0.05, 36, args.seed + 2
+    t = synth.make_tables(seed, nlay)
+    rt = Radiation(t)
+    t0 = time.perf_counter()
+    ntrain = args.sg_train
+    c = synth.make_columns(seed, ntrain, nlay)
+    dc = to_device(c)
+    h = exact_ir_heating(rt, dc)
+    stats = make_stats(raw_features(c), h.cpu().numpy(), "cuda:0")
+    x = torch.empty((ntrain, 147), dtype=torch.float32, device="cuda:0")
+    features(dc, stats["mean"], stats["inv_std"], x)
+    ty = ((torch.asinh(h / 10.0) - stats["tmean"]) / stats["tstd"]).float().contiguous()
+    del dc, c, h
+    nets, hists = [], []
+    for s in (1, 2):
+        net = Net(4096, seed=s)
+        hists.append(train(net, x, ty, epochs=args.sg_epochs, batch=1024, lr=2e-3, seed=s))
+        nets.append(net)
+    del x, ty
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    # calibration on held-out columns
+    cv = to_device(synth.make_columns(seed + 1, 16384, nlay))
+    sg = Surrogate(stats, nets, threshold=0.0)
+    pv, dis, inr = sg.predict(cv)
+    ev = (pv - exact_ir_heating(rt, cv)).abs().amax(dim=1)
+    ev = torch.where(inr, ev, torch.full_like(ev, float("inf")))
+    sg.threshold = calibrate(dis, ev, bar)
+    # evaluation
+    ncol = 64800
+    ce = synth.make_columns(seed + 2, ncol, nlay)
+    de = to_device(ce)
+    d64 = to_device(alloc_outputs(ncol, nlay))
+    dv = to_device(alloc_outputs(ncol, nlay))
+    sh = stream.cuda_stream
+    ms64, _ = timed_steps(lambda: rt.run_device(de, d64, FP64, sh), args.steps, args.warmup, stream)
+    exact = exact_ir_heating(rt, de)
+    res = {}
+
+    def step():
+        rt.run_device(de, dv, FP64 | SKIP_IR, sh)
+        hi, bad = sg.heating(rt, de, sh)
+        res["hr"] = dv["hr"][:, 1:] + hi
+        res["bad"] = bad
+    ms, _ = timed_steps(step, args.steps, args.warmup, stream)
+    rt.check()
+    err = (res["hr"] - d64["hr"][:, 1:]).abs()
+    pred, dise, _ = sg.predict(de)
+    perr = (pred - exact).abs()
+    y = torch.empty((ncol, 36), dtype=torch.float32, device="cuda:0")
+    xe = torch.empty((ncol, 147), dtype=torch.float32, device="cuda:0")
+    features(de, stats["mean"], stats["inv_std"], xe)
+    ms_net, _ = timed_steps(lambda: [nets[0].forward(xe[b:b + 4096], y[b:b + 4096], sh)
+                                     for b in range(0, ncol, 4096)], args.steps, args.warmup, stream)
+    ty_e = (torch.asinh(exact / 10.0) - stats["tmean"]) / stats["tstd"]
+    mse_net = float((y.double() - ty_e).pow(2).mean())
+    mse_mean = float(ty_e.pow(2).mean())
+    flops = 2 * (147 * 512 + 2 * 512 * 512 + 512 * 36)
+    return {"workload": f"{ncol} fresh clear-sky columns x {nlay} layers; trained on {ntrain} columns",
+            "value": ncol / (ms / 1e3), "unit": "columns/s", "ms_per_step": ms, "fp64_ms_per_step": ms64,
+            "speedup_vs_fp64": ms64 / ms, "bar_K_per_sol": bar, "threshold": sg.threshold,
+            "fallback_fraction": float(res["bad"].double().mean()),
+            "max_err_K_per_sol": float(err.max()), "rms_err_K_per_sol": float(err.pow(2).mean().sqrt()),
+            "surrogate_only": {"median_err_K_per_sol": float(perr.median()),
+                               "p90_err_K_per_sol": float(perr.quantile(0.9)),
+                               "max_err_K_per_sol": float(perr.max()),
+                               "median_rel_err": float((perr / exact.abs().clamp(min=1.0)).median()),
+                               "median_abs_heating_K_per_sol": float(exact.abs().median()),
+                               "ms_one_network": ms_net,
+                               "mfma_TFLOPs_one_network": flops * ncol / (ms_net / 1e3) / 1e12},
+            "ablation_mse_standardised": {"trained_network": mse_net, "without_network_mean_predictor": mse_mean},
+            "train_s": t_train, "final_train_loss": [hh[-1] for hh in hists]}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -257,7 +347,7 @@ def main():
     forms = {}
     if not args.no_forms:
         del do
-        for name, fn in (("mixed", form_mixed), ("batched", form_batched)):
+        for name, fn in (("mixed", form_mixed), ("batched", form_batched), ("surrogate", form_surrogate)):
             try:
                 forms[name] = fn(args, rt, stream)
             except Exception as e:  # noqa: BLE001

@@ -46,8 +46,13 @@ enum {
 };
 
 enum {
-  MGCM_RT_FP64 = 0,  /* plain path: every operation in FP64, reference evaluation order */
-  MGCM_RT_MIXED = 1  /* FP32 k-interpolation, optical depths and transmissions; FP64 solve */
+  MGCM_RT_FP64 = 0,       /* plain path: every operation in FP64, reference evaluation order */
+  MGCM_RT_MIXED = 1,      /* k-tables read in FP32, all arithmetic FP64 */
+  /* OR-ed into the precision argument: compute one part only.  The other
+   * part's fluxes are zero, its DETAU/CUMTAUV untouched, and the heating holds
+   * the computed part alone (the IR surrogate's exact VIS and fallback IR). */
+  MGCM_RT_SKIP_VIS = 0x10,
+  MGCM_RT_SKIP_IR = 0x20
 };
 
 typedef struct mgcm_rt_ctx mgcm_rt_ctx;

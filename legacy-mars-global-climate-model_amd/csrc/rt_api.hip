@@ -126,6 +126,16 @@ extern "C" int mgcm_rt_destroy(mgcm_rt_ctx *c) {
 }
 
 namespace mgcm {
+int decode_mode(int precision, int *path, int *bands) {
+  const int p = precision & 0xf, skip = precision & ~0xf;
+  if ((p != MGCM_RT_FP64 && p != MGCM_RT_MIXED) || (skip & ~(MGCM_RT_SKIP_VIS | MGCM_RT_SKIP_IR)) ||
+      skip == (MGCM_RT_SKIP_VIS | MGCM_RT_SKIP_IR))
+    return MGCM_RT_ERR_PRECISION;
+  *path = p;
+  *bands = ((skip & MGCM_RT_SKIP_VIS) ? 0 : 1) | ((skip & MGCM_RT_SKIP_IR) ? 0 : 2);
+  return MGCM_RT_OK;
+}
+
 int check_columns(const mgcm_rt_columns *in, const mgcm_rt_fluxes *out) {
   if (!in || !out) return MGCM_RT_ERR_ARG;
   if (in->ncol < 0) return MGCM_RT_ERR_ARG;
@@ -152,6 +162,7 @@ void fill_args(RunArgs &a, const mgcm_rt_columns *in, const mgcm_rt_fluxes *out,
   a.nfluxtopv = out->nfluxtopv; a.nfluxtopi = out->nfluxtopi; a.diffvt = out->diffvt;
   a.detau = out->detau; a.cumtauv = out->cumtauv; a.hr = out->hr;
   a.flag = flag;
+  a.bands = 3;
 }
 }  // namespace mgcm
 
@@ -160,12 +171,15 @@ extern "C" int mgcm_rt_run(mgcm_rt_ctx *c, const mgcm_rt_columns *in, const mgcm
   if (!c) return MGCM_RT_ERR_ARG;
   int s = check_columns(in, out);
   if (s != MGCM_RT_OK) return s;
-  if (precision != MGCM_RT_FP64 && precision != MGCM_RT_MIXED) return MGCM_RT_ERR_PRECISION;
+  int path, bands;
+  s = decode_mode(precision, &path, &bands);
+  if (s != MGCM_RT_OK) return s;
   if (in->ncol == 0) return MGCM_RT_OK;
   RunArgs a;
   fill_args(a, in, out, c->d_flag);
-  const hipError_t e = (precision == MGCM_RT_FP64) ? launch_plain(c->dt, a, (hipStream_t)stream)
-                                                   : launch_mixed(c->dt, a, (hipStream_t)stream);
+  a.bands = bands;
+  const hipError_t e = (path == MGCM_RT_FP64) ? launch_plain(c->dt, a, (hipStream_t)stream)
+                                              : launch_mixed(c->dt, a, (hipStream_t)stream);
   return e == hipSuccess ? MGCM_RT_OK : MGCM_RT_ERR_DEVICE;
 }
 
@@ -183,7 +197,9 @@ extern "C" int mgcm_rt_run_host(mgcm_rt_ctx *c, const mgcm_rt_columns *in, const
   if (!c) return MGCM_RT_ERR_ARG;
   int s = check_columns(in, out);
   if (s != MGCM_RT_OK) return s;
-  if (precision != MGCM_RT_FP64 && precision != MGCM_RT_MIXED) return MGCM_RT_ERR_PRECISION;
+  int path, bands;
+  s = decode_mode(precision, &path, &bands);
+  if (s != MGCM_RT_OK) return s;
   if (in->ncol == 0) return MGCM_RT_OK;
   const size_t ncol = in->ncol, nlev = 2 * (size_t)in->nlay + 3, nl = in->nlay + 1;
   const size_t nin = ncol * (5 * nlev + 2 * (nlev + 1) + 3);
@@ -258,7 +274,9 @@ extern "C" int mgcm_rt_run_host(mgcm_rt_ctx *c, const mgcm_rt_columns *in, const
 extern "C" int mgcm_rt_run_batched(mgcm_rt_ctx *c, int ngrids, const mgcm_rt_columns *grids,
                                    const mgcm_rt_fluxes *outs, int precision, void *stream) {
   if (!c || ngrids < 0 || (ngrids > 0 && (!grids || !outs))) return MGCM_RT_ERR_ARG;
-  if (precision != MGCM_RT_FP64 && precision != MGCM_RT_MIXED) return MGCM_RT_ERR_PRECISION;
+  int path, bands;
+  const int sm = decode_mode(precision, &path, &bands);
+  if (sm != MGCM_RT_OK) return sm;
   for (int i = 0; i < ngrids; i++) {
     const int s = check_columns(&grids[i], &outs[i]);
     if (s != MGCM_RT_OK) return s;

@@ -57,7 +57,7 @@ hipError_t launch_class(int precision, int k, const DevTables &dt, const BatchSt
                         hipStream_t st) {
   const ClassPlan &p = b->cls[k];
   const uint2 *work = b->d_work + p.work_off;
-  return precision == MGCM_RT_FP64 ? launch_batch_plain(k, dt, b->d_grids, work, nwork, nblocks, p.lds, st)
+  return (precision & 0xf) == MGCM_RT_FP64 ? launch_batch_plain(k, dt, b->d_grids, work, nwork, nblocks, p.lds, st)
                                    : launch_batch_mixed(k, dt, b->d_grids, work, nwork, nblocks, p.lds, st);
 }
 
@@ -150,10 +150,13 @@ int plan(BatchState *b, int ngrids, const mgcm_rt_columns *grids) {
 }
 
 int upload_args(BatchState *b, mgcm_rt_ctx *c, int ngrids, const mgcm_rt_columns *grids,
-                const mgcm_rt_fluxes *outs, hipStream_t st) {
+                const mgcm_rt_fluxes *outs, int bands, hipStream_t st) {
   // the previous call's copy out of the pinned staging must have finished
   if (hipEventSynchronize(b->uploaded) != hipSuccess) return MGCM_RT_ERR_DEVICE;
-  for (int i = 0; i < ngrids; i++) fill_args(b->h_grids[i], &grids[i], &outs[i], c->d_flag);
+  for (int i = 0; i < ngrids; i++) {
+    fill_args(b->h_grids[i], &grids[i], &outs[i], c->d_flag);
+    b->h_grids[i].bands = bands;
+  }
   if (hipMemcpyAsync(b->d_grids, b->h_grids, ngrids * sizeof(RunArgs), hipMemcpyHostToDevice, st) != hipSuccess)
     return MGCM_RT_ERR_DEVICE;
   if (hipEventRecord(b->uploaded, st) != hipSuccess) return MGCM_RT_ERR_DEVICE;
@@ -177,7 +180,9 @@ void batch_free(void *p) {
 int run_batched_impl(mgcm_rt_ctx *c, int ngrids, const mgcm_rt_columns *grids, const mgcm_rt_fluxes *outs,
                      int precision, hipStream_t st) {
   if (ngrids < 0 || (ngrids > 0 && (!grids || !outs))) return MGCM_RT_ERR_ARG;
-  if (precision != MGCM_RT_FP64 && precision != MGCM_RT_MIXED) return MGCM_RT_ERR_PRECISION;
+  int path, bands;
+  const int sm = decode_mode(precision, &path, &bands);
+  if (sm != MGCM_RT_OK) return sm;
   long long total = 0;
   for (int i = 0; i < ngrids; i++) {
     const int s = check_columns(&grids[i], &outs[i]);
@@ -203,7 +208,7 @@ int run_batched_impl(mgcm_rt_ctx *c, int ngrids, const mgcm_rt_columns *grids, c
     const int s = plan(b, ngrids, grids);
     if (s != MGCM_RT_OK) return s;
   }
-  int s = upload_args(b, c, ngrids, grids, outs, st);
+  int s = upload_args(b, c, ngrids, grids, outs, bands, st);
   if (s != MGCM_RT_OK) return s;
   if (fresh) {
     int ncu = 0;

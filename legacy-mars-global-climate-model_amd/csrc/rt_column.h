@@ -315,7 +315,8 @@ __device__ __forceinline__ void column_block(const DevTables &dt, const RunArgs 
     const bool VISB = b < NV;
     const double fz = rc->fzero[b];
     __syncthreads();
-    if (fz >= F32LIT(0.99)) {  // code/sfluxv.f90:64-65, code/sfluxi.f90:67-68
+    // code/sfluxv.f90:64-65, code/sfluxi.f90:67-68; a part the caller skipped contributes nothing
+    if (fz >= F32LIT(0.99) || !(a.bands & (VISB ? 1 : 2))) {
       if (g0 == 0) S.skip[c * NB + b] = 0u;
       continue;
     }
@@ -427,6 +428,7 @@ __device__ __forceinline__ void column_block(const DevTables &dt, const RunArgs 
 
   // 3. continuum point g = 17: pass 0 = VIS (lanes 0..27), pass 1 = IR (lanes 0..19)
   for (int pass = 0; pass < 2; pass++) {
+    if (!(a.bands & (pass == 0 ? 1 : 2))) continue;  // wave-uniform
     const int nbp = pass == 0 ? NV : NI;
     const bool act = lane < 4 * nbp;
     const int cc = act ? lane / nbp : 0;

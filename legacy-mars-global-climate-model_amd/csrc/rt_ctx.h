@@ -24,6 +24,9 @@ hipError_t launch_batch_plain(int cls, const DevTables &dt, const RunArgs *grids
                               int nblocks, size_t lds, hipStream_t st);
 hipError_t launch_batch_mixed(int cls, const DevTables &dt, const RunArgs *grids, const uint2 *work, int nwork,
                               int nblocks, size_t lds, hipStream_t st);
+// Splits the precision argument into the path (MGCM_RT_FP64/MIXED) and the
+// band mask; MGCM_RT_ERR_PRECISION for anything else.
+int decode_mode(int precision, int *path, int *bands);
 int check_columns(const mgcm_rt_columns *in, const mgcm_rt_fluxes *out);
 void fill_args(RunArgs &a, const mgcm_rt_columns *in, const mgcm_rt_fluxes *out, int *flag);
 int run_batched_impl(mgcm_rt_ctx *ctx, int ngrids, const mgcm_rt_columns *grids, const mgcm_rt_fluxes *outs,

@@ -53,7 +53,8 @@ struct RunArgs {
   double sol[NV];
   double *fmnetv, *fluxupv, *fluxdnv, *fmneti, *fluxupi, *fluxdni;
   double *nfluxtopv, *nfluxtopi, *diffvt, *detau, *cumtauv, *hr;
-  int *flag;  // range flag
+  int *flag;   // range flag
+  int bands;   // bit 0: VIS bands, bit 1: IR bands
 };
 
 struct DevTables {

@@ -30,6 +30,7 @@ LIB_PATH = os.path.join(HERE, "libmgcm_rt.so")
 
 NV, NI, NG = 7, 5, 17
 FP64, MIXED = 0, 1
+SKIP_VIS, SKIP_IR = 0x10, 0x20  # OR-ed into the precision argument: one part only
 DP = C.POINTER(C.c_double)
 
 STATUS = {0: "ok", -1: "invalid argument", -2: "layer count outside 1..64", -3: "HIP runtime error",
