@@ -184,14 +184,18 @@ class Surrogate:
 
 
 def calibrate(disagreement, err_max, bar: float) -> float:
-    """Largest disagreement threshold under which every validation column meets
-    the bar: just below the smallest disagreement of any column whose max error
-    exceeds it (-1, i.e. trust nothing, when that smallest disagreement is 0)."""
+    """Disagreement threshold of the check.  It is the smaller of (1) just below
+    the smallest disagreement of any validation column whose max error exceeds
+    the bar -- so every trusted validation column meets it -- and (2) half the
+    bar, so that a column is trusted only when the two networks agree closer
+    than the bar itself (validation alone does not bound fresh columns).
+    -1 (trust nothing) when a failing column has zero disagreement."""
     bad = err_max > bar
-    if not bool(bad.any()):
-        return float("inf")
-    d = float(disagreement[bad].min())
-    return d * (1.0 - 1e-6) if d > 0 else -1.0
+    tau = 0.5 * bar
+    if bool(bad.any()):
+        d = float(disagreement[bad].min())
+        tau = min(tau, d * (1.0 - 1e-6)) if d > 0 else -1.0
+    return tau
 
 
 def make_stats(raw: np.ndarray, targets: np.ndarray, device) -> dict:

@@ -141,7 +141,7 @@ def test_fallback_gives_exact_heating_where_flagged():
     rt, t, stats, net, _, _, _ = _trained(ncol=2048, epochs=3)
     net2 = Net(512, seed=12)
     c = synth.make_columns(55, 300, 36)
-    c["tmid"][:20] += 400.0                  # far outside the training range: must fall back
+    c["qh2o"][:20] = 0.3                     # H2O vmr far above the training range (inside the tables): must fall back
     dc = to_device(c)
     exact = exact_ir_heating(rt, dc)
     sg = Surrogate(stats, [net, net2], threshold=float("inf"))

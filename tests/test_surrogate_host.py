@@ -11,8 +11,10 @@ def test_calibrated_threshold_meets_the_bar_on_validation():
     dis = torch.tensor([0.5, 0.1, 2.0, 0.05, 3.0])
     err = torch.tensor([0.01, 0.2, 0.01, 0.0, 5.0])
     tau = calibrate(dis, err, 0.05)
-    assert tau < 0.1 and bool((err[dis <= tau] <= 0.05).all())
-    assert calibrate(dis, torch.zeros(5), 0.05) == float("inf")
+    assert tau <= 0.025 and bool((err[dis <= tau] <= 0.05).all())
+    assert calibrate(dis, torch.zeros(5), 0.05) == 0.025       # capped at half the bar
+    assert calibrate(dis, err, 100.0) == 50.0
+    assert abs(calibrate(dis, err, 1.0) - 3.0 * (1 - 1e-6)) > 0 and calibrate(dis, err, 1.0) <= 0.5
     assert calibrate(torch.zeros(5), err, 0.05) == -1.0
 
 
