@@ -117,7 +117,7 @@ extern "C" int mgcm_rt_init(const mgcm_rt_tables *t, mgcm_rt_ctx **out) {
 
 extern "C" int mgcm_rt_destroy(mgcm_rt_ctx *c) {
   if (!c) return MGCM_RT_ERR_ARG;
-  void *bufs[] = {c->d_const, c->d_kv, c->d_ki, c->d_kv32, c->d_ki32, c->d_planck, c->d_flag, c->d_host};
+  void *bufs[] = {c->d_const, c->d_kv, c->d_ki, c->d_kv32, c->d_ki32, c->d_planck, c->d_flag, c->d_host, c->d_clr};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   if (c->batch) mgcm::batch_free(c->batch);
@@ -163,6 +163,7 @@ void fill_args(RunArgs &a, const mgcm_rt_columns *in, const mgcm_rt_fluxes *out,
   a.detau = out->detau; a.cumtauv = out->cumtauv; a.hr = out->hr;
   a.flag = flag;
   a.bands = 3;
+  a.clr = nullptr;
 }
 }  // namespace mgcm
 
@@ -175,9 +176,21 @@ extern "C" int mgcm_rt_run(mgcm_rt_ctx *c, const mgcm_rt_columns *in, const mgcm
   s = decode_mode(precision, &path, &bands);
   if (s != MGCM_RT_OK) return s;
   if (in->ncol == 0) return MGCM_RT_OK;
+  const size_t need = clr_workspace_bytes(in->ncol, in->nlay);
+  if (need > c->clr_bytes) {  // grow-only continuum-point workspace
+    if (c->d_clr) {
+      if (hipDeviceSynchronize() != hipSuccess) return MGCM_RT_ERR_DEVICE;
+      (void)hipFree(c->d_clr);
+    }
+    c->d_clr = nullptr;
+    c->clr_bytes = 0;
+    if (hipMalloc(&c->d_clr, need) != hipSuccess) return MGCM_RT_ERR_NOMEM;
+    c->clr_bytes = need;
+  }
   RunArgs a;
   fill_args(a, in, out, c->d_flag);
   a.bands = bands;
+  a.clr = c->d_clr;
   const hipError_t e = (path == MGCM_RT_FP64) ? launch_plain(c->dt, a, (hipStream_t)stream)
                                               : launch_mixed(c->dt, a, (hipStream_t)stream);
   return e == hipSuccess ? MGCM_RT_OK : MGCM_RT_ERR_DEVICE;

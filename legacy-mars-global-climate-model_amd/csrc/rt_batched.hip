@@ -25,11 +25,17 @@ namespace mgcm {
 
 namespace {
 
+// Per layer class, three work lists: kind 0 = column kernel (4-column blocks),
+// kind 1 = VIS continuum points (9-column blocks), kind 2 = IR continuum
+// points (12-column blocks).
+constexpr int NKIND = 3;
+constexpr int KIND_COLS[NKIND] = {4, 9, 12};
+
 struct ClassPlan {
-  int nwork = 0;        // 4-column blocks
-  size_t work_off = 0;  // offset of the class's list in d_work
-  size_t lds = 0;       // dynamic LDS per block
-  int nblocks = 0;      // persistent grid size chosen by the tuner
+  int nwork[NKIND] = {0, 0, 0};
+  size_t work_off[NKIND] = {0, 0, 0};  // offsets of the lists in d_work
+  size_t lds = 0;                      // dynamic LDS per column-kernel block
+  int nblocks[NKIND] = {0, 0, 0};      // persistent grid sizes (kind 0 from the tuner)
 };
 
 struct BatchState {
@@ -40,6 +46,9 @@ struct BatchState {
   size_t grids_cap = 0;
   uint2 *d_work = nullptr;
   size_t work_cap = 0;
+  double *d_clr = nullptr;            // continuum-point records of every grid
+  size_t clr_cap = 0;
+  std::vector<size_t> clr_off;        // per grid, in doubles
   hipEvent_t uploaded = nullptr;
   hipStream_t cap = nullptr;
   hipGraph_t graph = nullptr;
@@ -53,12 +62,13 @@ void drop_graph(BatchState *b) {
   b->graph = nullptr;
 }
 
-hipError_t launch_class(int precision, int k, const DevTables &dt, const BatchState *b, int nwork, int nblocks,
-                        hipStream_t st) {
+hipError_t launch_class(int precision, int k, int kind, const DevTables &dt, const BatchState *b, int nwork,
+                        int nblocks, hipStream_t st) {
   const ClassPlan &p = b->cls[k];
-  const uint2 *work = b->d_work + p.work_off;
-  return (precision & 0xf) == MGCM_RT_FP64 ? launch_batch_plain(k, dt, b->d_grids, work, nwork, nblocks, p.lds, st)
-                                   : launch_batch_mixed(k, dt, b->d_grids, work, nwork, nblocks, p.lds, st);
+  const uint2 *work = b->d_work + p.work_off[kind];
+  return (precision & 0xf) == MGCM_RT_FP64
+             ? launch_batch_plain(k, kind, dt, b->d_grids, work, nwork, nblocks, p.lds, st)
+             : launch_batch_mixed(k, kind, dt, b->d_grids, work, nwork, nblocks, p.lds, st);
 }
 
 // Resident one-wave blocks per CU for `lds` bytes of LDS at 2 waves per SIMD.
@@ -73,8 +83,8 @@ int resident_per_cu(size_t lds) {
 int tune_class(int precision, int k, const DevTables &dt, const BatchState *b, int ncu, hipStream_t st) {
   const ClassPlan &p = b->cls[k];
   const int cap = ncu * resident_per_cu(p.lds);
-  if (p.nwork <= cap) return p.nwork;
-  const int prefix = std::min(p.nwork, 8 * cap);
+  if (p.nwork[0] <= cap) return p.nwork[0];
+  const int prefix = std::min(p.nwork[0], 8 * cap);
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess) return cap;
   if (hipEventCreate(&e1) != hipSuccess) {
@@ -86,9 +96,9 @@ int tune_class(int precision, int k, const DevTables &dt, const BatchState *b, i
   const int cand[3] = {cap, 2 * cap, 4 * cap};
   for (int c : cand) {
     const int nb = std::min(c, prefix);
-    if (launch_class(precision, k, dt, b, prefix, nb, st) != hipSuccess) break;  // warm-up
+    if (launch_class(precision, k, 0, dt, b, prefix, nb, st) != hipSuccess) break;  // warm-up
     (void)hipEventRecord(e0, st);
-    if (launch_class(precision, k, dt, b, prefix, nb, st) != hipSuccess) break;
+    if (launch_class(precision, k, 0, dt, b, prefix, nb, st) != hipSuccess) break;
     (void)hipEventRecord(e1, st);
     if (hipEventSynchronize(e1) != hipSuccess) break;
     float ms = 0.f;
@@ -100,7 +110,7 @@ int tune_class(int precision, int k, const DevTables &dt, const BatchState *b, i
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  return std::min(best, p.nwork);
+  return std::min(best, p.nwork[0]);
 }
 
 int plan(BatchState *b, int ngrids, const mgcm_rt_columns *grids) {
@@ -113,15 +123,32 @@ int plan(BatchState *b, int ngrids, const mgcm_rt_columns *grids) {
   for (int k = 0; k < NCLASS; k++) {
     ClassPlan &p = b->cls[k];
     p = ClassPlan();
-    p.work_off = work.size();
     int maxlay = 0;
-    for (int i : order) {
-      if (grids[i].ncol == 0 || nl_class(grids[i].nlay + 1) != k) continue;
-      maxlay = std::max(maxlay, grids[i].nlay);
-      for (int col = 0; col < grids[i].ncol; col += 4) work.push_back(make_uint2((unsigned)i, (unsigned)col));
+    for (int kind = 0; kind < NKIND; kind++) {
+      p.work_off[kind] = work.size();
+      for (int i : order) {
+        if (grids[i].ncol == 0 || nl_class(grids[i].nlay + 1) != k) continue;
+        maxlay = std::max(maxlay, grids[i].nlay);
+        for (int col = 0; col < grids[i].ncol; col += KIND_COLS[kind])
+          work.push_back(make_uint2((unsigned)i, (unsigned)col));
+      }
+      p.nwork[kind] = (int)(work.size() - p.work_off[kind]);
     }
-    p.nwork = (int)(work.size() - p.work_off);
     p.lds = maxlay ? stage_bytes(maxlay) : 0;
+  }
+  // continuum-point records, grid after grid
+  b->clr_off.assign(ngrids, 0);
+  size_t clr_total = 0;
+  for (int i = 0; i < ngrids; i++) {
+    b->clr_off[i] = clr_total;
+    clr_total += clr_workspace_bytes(grids[i].ncol, grids[i].nlay) / sizeof(double);
+  }
+  if (clr_total * sizeof(double) > b->clr_cap) {
+    if (b->d_clr) (void)hipFree(b->d_clr);
+    b->d_clr = nullptr;
+    b->clr_cap = 0;
+    if (hipMalloc(&b->d_clr, clr_total * sizeof(double)) != hipSuccess) return MGCM_RT_ERR_NOMEM;
+    b->clr_cap = clr_total * sizeof(double);
   }
   if (work.size() > b->work_cap) {
     if (b->d_work) (void)hipFree(b->d_work);
@@ -156,6 +183,7 @@ int upload_args(BatchState *b, mgcm_rt_ctx *c, int ngrids, const mgcm_rt_columns
   for (int i = 0; i < ngrids; i++) {
     fill_args(b->h_grids[i], &grids[i], &outs[i], c->d_flag);
     b->h_grids[i].bands = bands;
+    b->h_grids[i].clr = b->d_clr + b->clr_off[i];
   }
   if (hipMemcpyAsync(b->d_grids, b->h_grids, ngrids * sizeof(RunArgs), hipMemcpyHostToDevice, st) != hipSuccess)
     return MGCM_RT_ERR_DEVICE;
@@ -172,6 +200,7 @@ void batch_free(void *p) {
   if (b->d_grids) (void)hipFree(b->d_grids);
   if (b->h_grids) (void)hipHostFree(b->h_grids);
   if (b->d_work) (void)hipFree(b->d_work);
+  if (b->d_clr) (void)hipFree(b->d_clr);
   if (b->uploaded) (void)hipEventDestroy(b->uploaded);
   if (b->cap) (void)hipStreamDestroy(b->cap);
   delete b;
@@ -214,15 +243,31 @@ int run_batched_impl(mgcm_rt_ctx *c, int ngrids, const mgcm_rt_columns *grids, c
     int ncu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || ncu <= 0)
       return MGCM_RT_ERR_DEVICE;
+    // continuum points before the column kernels that weight them
     for (int k = 0; k < NCLASS; k++)
-      if (b->cls[k].nwork > 0) b->cls[k].nblocks = tune_class(precision, k, c->dt, b, ncu, st);
+      for (int kind = 1; kind < NKIND; kind++)
+        if ((bands & kind) && b->cls[k].nwork[kind] > 0) {
+          if (launch_class(precision, k, kind, c->dt, b, b->cls[k].nwork[kind],
+                           std::min(b->cls[k].nwork[kind], 16 * ncu), st) != hipSuccess)
+            return MGCM_RT_ERR_DEVICE;
+        }
+    for (int k = 0; k < NCLASS; k++) {
+      ClassPlan &p = b->cls[k];
+      if (p.nwork[0] > 0) p.nblocks[0] = tune_class(precision, k, c->dt, b, ncu, st);
+      for (int kind = 1; kind < NKIND; kind++) p.nblocks[kind] = std::min(p.nwork[kind], 16 * ncu);
+    }
     // capture the class launches; the argument upload stays outside the graph
     if (hipStreamSynchronize(st) != hipSuccess) return MGCM_RT_ERR_DEVICE;
     if (hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal) != hipSuccess) return MGCM_RT_ERR_DEVICE;
     bool ok = true;
+    for (int kind = 1; kind < NKIND && ok; kind++)
+      for (int k = 0; k < NCLASS && ok; k++)
+        if ((bands & kind) && b->cls[k].nwork[kind] > 0)
+          ok = launch_class(precision, k, kind, c->dt, b, b->cls[k].nwork[kind], b->cls[k].nblocks[kind], b->cap) ==
+               hipSuccess;
     for (int k = 0; k < NCLASS && ok; k++)
-      if (b->cls[k].nwork > 0)
-        ok = launch_class(precision, k, c->dt, b, b->cls[k].nwork, b->cls[k].nblocks, b->cap) == hipSuccess;
+      if (b->cls[k].nwork[0] > 0)
+        ok = launch_class(precision, k, 0, c->dt, b, b->cls[k].nwork[0], b->cls[k].nblocks[0], b->cap) == hipSuccess;
     hipGraph_t g = nullptr;
     const bool ended = hipStreamEndCapture(b->cap, &g) == hipSuccess;
     if (!ok || !ended || !g) {

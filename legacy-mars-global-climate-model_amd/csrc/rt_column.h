@@ -31,20 +31,21 @@ struct Stage {
   double *c1, *c2;             // [4][nsub]  VIS: TRAY+TAEROS+TCLOUD; IR: TAEROS, TCLOUD
   double *scat, *cosb, *cbar;  // [4][nl]    layer scattering tau, asymmetry, COSBAR = g/(1+g)
   double *bk;                  // [4][nlev]  IR: PLANCKIR(nw, int(TLEV(K)*10-499))
-  double *xfer;                // [64][3]    continuum-point hand-over
   int *base;                   // [4][nsub]  table cell offset; bit 0 set when NH2O = L_REFH2O
   uint32_t *skip;              // [4][NB]    TLIMIT skip masks
 };
 
-// Dynamic LDS of one wave: the exp table, then the staging.  The continuum
-// pass's hand-over area (64 x 3) reuses c1, which that pass does not read
-// (4 * nsub >= 192 for every nlay >= 23; smaller layer counts get their own).
+// Dynamic LDS of one wave: the exp table, then the staging.
 inline size_t stage_bytes(int nlay) {
   const size_t nl = nlay + 1, nlev = 2 * (size_t)nlay + 3, nsub = nlev - 1;
-  const size_t xfer = 4 * nsub >= 64 * 3 ? 0 : 64 * 3;
-  return mgcm_libm::LDS_EXPTAB_BYTES + sizeof(double) * (4 * nsub * 6 + 4 * nl * 3 + 4 * nlev + xfer) +
+  return mgcm_libm::LDS_EXPTAB_BYTES + sizeof(double) * (4 * nsub * 6 + 4 * nl * 3 + 4 * nlev) +
          sizeof(int) * (4 * nsub + 4 * NB);
 }
+
+// Continuum-point record of one (column, band) in RunArgs::clr: FMIDP(1..nl),
+// FMIDM(1..nl), then two top terms.  Records of a column are band-contiguous.
+__host__ __device__ inline int clr_rec(int nl) { return 2 * nl + 2; }
+inline size_t clr_bytes(int ncol, int nlay) { return (size_t)ncol * NB * clr_rec(nlay + 1) * sizeof(double); }
 
 __device__ inline Stage carve(double *p, int nsub, int nl, int nlev) {
   p += mgcm_libm::LDS_EXPTAB_BYTES / sizeof(double);
@@ -59,12 +60,6 @@ __device__ inline Stage carve(double *p, int nsub, int nl, int nlev) {
   S.cosb = p; p += 4 * nl;
   S.cbar = p; p += 4 * nl;
   S.bk = p; p += 4 * nlev;
-  if (4 * nsub >= 64 * 3) {
-    S.xfer = S.c1;
-  } else {
-    S.xfer = p;
-    p += 64 * 3;
-  }
   S.base = (int *)p;
   S.skip = (uint32_t *)(S.base + 4 * nsub);
   return S;
@@ -426,101 +421,63 @@ __device__ __forceinline__ void column_block(const DevTables &dt, const RunArgs 
   }
   __syncthreads();
 
-  // 3. continuum point g = 17: pass 0 = VIS (lanes 0..27), pass 1 = IR (lanes 0..19)
-  for (int pass = 0; pass < 2; pass++) {
-    if (!(a.bands & (pass == 0 ? 1 : 2))) continue;  // wave-uniform
-    const int nbp = pass == 0 ? NV : NI;
-    const bool act = lane < 4 * nbp;
-    const int cc = act ? lane / nbp : 0;
-    const int b = (pass == 0 ? 0 : NV) + (act ? lane % nbp : 0);
-    const int cl = col0 + cc;
-    const bool ok = act && cl < a.ncol;
-    const int clc = ok ? cl : colc;
-    const double *pl = a.plev + (size_t)clc * nlev;
-    const double *tr = a.tauref + (size_t)clc * (nlev + 1);
-    const double *tc = a.taurefcld + (size_t)clc * (nlev + 1);
-    // FZERO plus the weight of the skipped points, in g order
-    double fzero = rc->fzero[b];
-    if (!(fzero >= F32LIT(0.99))) {
-      const uint32_t mk = S.skip[cc * NB + b];
-      for (int g = 0; g < NGAS; g++)
-        if (mk & (1u << g)) fzero = fzero + (1.0 - rc->fzero[b]) * rc->gw[b][g];
+  // 3. continuum point g = 17 of every (column, band): solved by k_rt_clear
+  //    into a.clr, weighted here with FZERO plus the weights of the band's
+  //    TLIMIT-skipped points, in g order, and added band by band
+  //    (code/sfluxv.f90:72-75,121-162, code/sfluxi.f90:72-75,114-143)
+  {
+    double wz[NB];
+    for (int b = 0; b < NB; b++) {
+      double fzero = rc->fzero[b];
+      if (!(fzero >= F32LIT(0.99))) {
+        const uint32_t mk = S.skip[c * NB + b];
+        for (int g = 0; g < NGAS; g++)
+          if (mk & (1u << g)) fzero = fzero + (1.0 - rc->fzero[b]) * rc->gw[b][g];
+      }
+      wz[b] = fzero;
     }
-    double scat[NLM], cosb[NLM], cbar[NLM];
-    for (int L = 1; L <= nl; L++) {
-      if (pass == 0) layer_aerosol<true>(rc, b, pl, tr, tc, nl, L, scat[L - 1], cosb[L - 1]);
-      else layer_aerosol<false>(rc, b, pl, tr, tc, nl, L, scat[L - 1], cosb[L - 1]);
-      cbar[L - 1] = cosb[L - 1] / (1.0 + cosb[L - 1]);
-    }
-    // hand the weighted contributions to the owner lanes of the column, level by level
-    auto xfer_add = [&](int n, double net, double up, double dn, double *an, double *au, double *ad) {
-      __syncthreads();
-      S.xfer[lane * 3 + 0] = net;
-      S.xfer[lane * 3 + 1] = up;
-      S.xfer[lane * 3 + 2] = dn;
-      __syncthreads();
-      if (((n - 1) & 15) == g0) {
+    const int rec = clr_rec(nl);
+    const double *cr = a.clr + (size_t)colc * NB * rec;
+    if (a.bands & 1) {
+      for (int n = 1; n <= nl; n++) {
+        if (((n - 1) & 15) != g0) continue;
         double sn = 0.0, su = 0.0, sd = 0.0;
-        for (int bb = 0; bb < nbp; bb++) {
-          const int src = c * nbp + bb;
-          sn = sn + S.xfer[src * 3 + 0];
-          su = su + S.xfer[src * 3 + 1];
-          sd = sd + S.xfer[src * 3 + 2];
+        for (int b = 0; b < NV; b++) {
+          const double fp = cr[b * rec + n - 1], fm = cr[b * rec + nl + n - 1];
+          sn = sn + (sun ? (fp - fm) * wz[b] : 0.0);
+          su = su + (sun ? fp * wz[b] : 0.0);
+          sd = sd + (sun ? fm * wz[b] : 0.0);
         }
-        acc_add<SL>(an, n, g0, sn);
-        acc_add<SL>(au, n, g0, su);
-        acc_add<SL>(ad, n, g0, sd);
+        acc_add<SL>(acc.vnet, n, g0, sn);
+        acc_add<SL>(acc.vup, n, g0, su);
+        acc_add<SL>(acc.vdn, n, g0, sd);
       }
-    };
-    auto xfer_scalar = [&](double v) {  // every lane receives its column's sum over bands
-      __syncthreads();
-      S.xfer[lane * 3] = v;
-      __syncthreads();
-      double sum = 0.0;
-      for (int bb = 0; bb < nbp; bb++) sum = sum + S.xfer[(c * nbp + bb) * 3];
-      return sum;
-    };
-    if (pass == 0) {
-      const double mu = a.acosz[clc];
-      const bool lsun = ok && mu >= F32LIT(1.0e-5);
-      ClearSrc<true> src{rc, b, pl, tr, tc, 0.0};
-      VisState<NLM> s;
-      double cumtau = 0.0, detau = 0.0;
-      if (lsun) {
-        detau = vis_optics<NLM>(nl, src, scat, cosb, s, cumtau);
-        a.detau[(size_t)cl * NG * NV + 16 * NV + b] = detau;
-        a.cumtauv[(size_t)cl * NG * NV + 16 * NV + b] = cumtau;
+      double st = 0.0, sdf = 0.0;
+      for (int b = 0; b < NV; b++) {
+        st = st + (sun ? cr[b * rec + 2 * nl] * wz[b] : 0.0);
+        sdf = sdf + (sun ? cr[b * rec + 2 * nl + 1] * wz[b] : 0.0);
       }
-      if (__any(lsun)) {
-        double fluxup = 0.0, fluxdn = 0.0, diffv = 0.0;
-        auto accf = [&](int n, double fp, double fm) {
-          xfer_add(n, lsun ? (fp - fm) * fzero : 0.0, lsun ? fp * fzero : 0.0, lsun ? fm * fzero : 0.0,
-                   acc.vnet, acc.vup, acc.vdn);
-        };
-        const double bsurf = a.als[clc] * mu * a.sol[b] * rt_exp(-dmin(detau / mu, MAXEXP));
-        vis_solve<NLM>(nl, s, cbar, mu, a.sol[b], a.als[clc], 0.0, bsurf, accf, fluxup, fluxdn, diffv);
-        topv = topv + xfer_scalar(lsun ? (fluxup - fluxdn) * fzero : 0.0);
-        difv = difv + xfer_scalar(lsun ? diffv * fzero : 0.0);
+      topv = topv + st;
+      difv = difv + sdf;
+    }
+    if (a.bands & 2) {
+      for (int n = 1; n <= nl; n++) {
+        if (((n - 1) & 15) != g0) continue;
+        double sn = 0.0, su = 0.0, sd = 0.0;
+        for (int b = NV; b < NB; b++) {
+          const double dw = rc->dwni[b - NV];
+          const double fp = cr[b * rec + n - 1], fm = cr[b * rec + nl + n - 1];
+          sn = sn + (valid ? (fp - fm) * dw * wz[b] : 0.0);
+          su = su + (valid ? fp * dw * wz[b] : 0.0);
+          sd = sd + (valid ? fm * dw * wz[b] : 0.0);
+        }
+        acc_add<SL>(acc.inet, n, g0, sn);
+        acc_add<SL>(acc.iup, n, g0, su);
+        acc_add<SL>(acc.idn, n, g0, sd);
       }
-    } else {
-      const int nw = b - NV;
-      const double dw = rc->dwni[nw];
-      ClearSrc<false> src{rc, b, pl, tr, tc, 0.0};
-      IrState<NLM> s;
-      ir_optics<NLM>(nl, src, scat, s);
-      double bk[2 * NLM + 1];
-      for (int K = 1; K <= nlev; K++)
-        bk[K - 1] = dt.planck[(size_t)(planck_index(a.tlev[(size_t)clc * nlev + K - 1], a.flag) - 1) * NI + nw];
-      const double rsf = a.albi[clc];
-      const double bsurf = (1. - rsf) * bk[nlev - 1];
-      const double tautop = s.dtau[1] * pl[1] / (pl[3] - pl[1]);
-      const double btop = (1.0 - rt_exp(-tautop * 2.0)) * bk[1];
-      auto accf = [&](int n, double fp, double fm) {
-        xfer_add(n, ok ? (fp - fm) * dw * fzero : 0.0, ok ? fp * dw * fzero : 0.0, ok ? fm * dw * fzero : 0.0,
-                 acc.inet, acc.iup, acc.idn);
-      };
-      const double ftopup = ir_solve<NLM>(nl, s, cosb, bk, rsf, btop, bsurf, accf);
-      topi = topi + xfer_scalar(ok ? ftopup * dw * fzero : 0.0);
+      double st = 0.0;
+      for (int b = NV; b < NB; b++) st = st + (valid ? cr[b * rec + 2 * nl] * rc->dwni[b - NV] * wz[b] : 0.0);
+      topi = topi + st;
     }
   }
 
@@ -562,6 +519,87 @@ __device__ __forceinline__ void column_block(const DevTables &dt, const RunArgs 
   }
 }
 
+// The continuum-only point g = 17 of (column, band) items, one per lane: a
+// VIS wave takes 9 columns x 7 bands, an IR wave 12 columns x 5 bands.  Each
+// item writes its unweighted mid-layer fluxes FMIDP/FMIDM and its top terms
+// (VIS: FLUXUP-FLUXDN and DIFFV, IR: FTOPUP) to its record in a.clr; the
+// column kernel weights and adds them.  The exp table must be in LDS.
+template <bool VIS>
+struct ClearShape {
+  static constexpr int nb = VIS ? NV : NI, cols = VIS ? 9 : 12;
+};
+
+template <int NLM, bool VIS>
+__device__ __forceinline__ void clear_block(const DevTables &dt, const RunArgs &a, int col0) {
+  constexpr int nbp = ClearShape<VIS>::nb, per = ClearShape<VIS>::cols;
+  const int lane = threadIdx.x, cc = lane / nbp, bb = lane % nbp;
+  const int col = col0 + cc;
+  if (cc >= per || col >= a.ncol) return;  // no cross-lane work below
+  const int nl = a.nlay + 1, nlev = 2 * a.nlay + 3;
+  const RtConst *rc = dt.c;
+  const int b = (VIS ? 0 : NV) + bb;
+  const double *pl = a.plev + (size_t)col * nlev;
+  const double *tr = a.tauref + (size_t)col * (nlev + 1);
+  const double *tc = a.taurefcld + (size_t)col * (nlev + 1);
+  double *rec = a.clr + ((size_t)col * NB + b) * clr_rec(nl);
+  double scat[NLM], cosb[NLM], cbar[NLM];
+  for (int L = 1; L <= nl; L++) {
+    layer_aerosol<VIS>(rc, b, pl, tr, tc, nl, L, scat[L - 1], cosb[L - 1]);
+    cbar[L - 1] = cosb[L - 1] / (1.0 + cosb[L - 1]);
+  }
+  auto accf = [&](int n, double fp, double fm) {
+    rec[n - 1] = fp;
+    rec[nl + n - 1] = fm;
+  };
+  if (VIS) {
+    const double mu = a.acosz[col];
+    if (!(mu >= F32LIT(1.0e-5))) return;  // code/comp3.f:637
+    ClearSrc<true> src{rc, b, pl, tr, tc, 0.0};
+    VisState<NLM> s;
+    double cumtau = 0.0;
+    const double detau = vis_optics<NLM>(nl, src, scat, cosb, s, cumtau);
+    a.detau[(size_t)col * NG * NV + 16 * NV + b] = detau;
+    a.cumtauv[(size_t)col * NG * NV + 16 * NV + b] = cumtau;
+    double fluxup = 0.0, fluxdn = 0.0, diffv = 0.0;
+    const double bsurf = a.als[col] * mu * a.sol[b] * rt_exp(-dmin(detau / mu, MAXEXP));
+    vis_solve<NLM>(nl, s, cbar, mu, a.sol[b], a.als[col], 0.0, bsurf, accf, fluxup, fluxdn, diffv);
+    rec[2 * nl] = fluxup - fluxdn;
+    rec[2 * nl + 1] = diffv;
+  } else {
+    const int nw = b - NV;
+    ClearSrc<false> src{rc, b, pl, tr, tc, 0.0};
+    IrState<NLM> s;
+    ir_optics<NLM>(nl, src, scat, s);
+    double bk[2 * NLM + 1];
+    for (int K = 1; K <= nlev; K++)
+      bk[K - 1] = dt.planck[(size_t)(planck_index(a.tlev[(size_t)col * nlev + K - 1], a.flag) - 1) * NI + nw];
+    const double rsf = a.albi[col];
+    const double bsurf = (1. - rsf) * bk[nlev - 1];
+    const double tautop = s.dtau[1] * pl[1] / (pl[3] - pl[1]);
+    const double btop = (1.0 - rt_exp(-tautop * 2.0)) * bk[1];
+    rec[2 * nl] = ir_solve<NLM>(nl, s, cosb, bk, rsf, btop, bsurf, accf);
+  }
+}
+
+template <int NLM, bool VIS>
+__global__ __launch_bounds__(64) void k_rt_clear(DevTables dt, RunArgs a) {
+  mgcm_libm::load_lds_exptab(threadIdx.x, 64);
+  __syncthreads();
+  clear_block<NLM, VIS>(dt, a, blockIdx.x * ClearShape<VIS>::cols);
+}
+
+template <int NLM, bool VIS>
+__global__ __launch_bounds__(64) void k_rt_clear_batch(DevTables dt, const RunArgs *__restrict__ grids,
+                                                       const uint2 *__restrict__ work, int nwork) {
+  mgcm_libm::load_lds_exptab(threadIdx.x, 64);
+  __syncthreads();
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const uint2 wk = work[w];
+    const RunArgs a = grids[wk.x];
+    clear_block<NLM, VIS>(dt, a, (int)wk.y);
+  }
+}
+
 // One grid: block b owns columns 4b..4b+3.
 template <int NLM, bool MIXED>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MGCM_FP64_WAVES)))
@@ -590,14 +628,22 @@ void k_rt_batch(DevTables dt, const RunArgs *__restrict__ grids, const uint2 *__
 
 template <int NLM, bool MIXED>
 static hipError_t launch_nlm(const DevTables &dt, const RunArgs &a, hipStream_t st) {
+  // continuum points first (their records feed the column kernel's sums)
+  if (a.bands & 1)
+    hipLaunchKernelGGL((k_rt_clear<NLM, true>), dim3((a.ncol + 8) / 9), dim3(64), mgcm_libm::LDS_EXPTAB_BYTES, st,
+                       dt, a);
+  if (a.bands & 2)
+    hipLaunchKernelGGL((k_rt_clear<NLM, false>), dim3((a.ncol + 11) / 12), dim3(64), mgcm_libm::LDS_EXPTAB_BYTES,
+                       st, dt, a);
   const size_t lds = stage_bytes(a.nlay);
   const unsigned gx = (unsigned)((a.ncol + 3) / 4);
   hipLaunchKernelGGL((k_rt<NLM, MIXED>), dim3(gx), dim3(64), lds, st, dt, a);
   return hipGetLastError();
 }
 
-// One launch over a grid: 4 columns per wave, the kernel instance sized for
-// the grid's layer count.
+// One grid: the continuum-point kernels, then the column kernel (4 columns per
+// wave), each instance sized for the grid's layer count.  a.clr must hold
+// clr_bytes(a.ncol, a.nlay).
 template <bool MIXED>
 hipError_t launch_column(const DevTables &dt, const RunArgs &a, hipStream_t st) {
   const int nl = a.nlay + 1;
@@ -618,22 +664,30 @@ inline int nl_class(int nl) {
   return NCLASS - 1;
 }
 
+// kind 0: column kernel; 1: VIS continuum points; 2: IR continuum points
 template <int NLM, bool MIXED>
-static hipError_t launch_batch_nlm(const DevTables &dt, const RunArgs *grids, const uint2 *work, int nwork,
+static hipError_t launch_batch_nlm(int kind, const DevTables &dt, const RunArgs *grids, const uint2 *work, int nwork,
                                    int nblocks, size_t lds, hipStream_t st) {
-  hipLaunchKernelGGL((k_rt_batch<NLM, MIXED>), dim3(nblocks), dim3(64), lds, st, dt, grids, work, nwork);
+  if (kind == 1)
+    hipLaunchKernelGGL((k_rt_clear_batch<NLM, true>), dim3(nblocks), dim3(64), mgcm_libm::LDS_EXPTAB_BYTES, st, dt,
+                       grids, work, nwork);
+  else if (kind == 2)
+    hipLaunchKernelGGL((k_rt_clear_batch<NLM, false>), dim3(nblocks), dim3(64), mgcm_libm::LDS_EXPTAB_BYTES, st, dt,
+                       grids, work, nwork);
+  else
+    hipLaunchKernelGGL((k_rt_batch<NLM, MIXED>), dim3(nblocks), dim3(64), lds, st, dt, grids, work, nwork);
   return hipGetLastError();
 }
 
 template <bool MIXED>
-hipError_t launch_batch_class(int cls, const DevTables &dt, const RunArgs *grids, const uint2 *work, int nwork,
-                              int nblocks, size_t lds, hipStream_t st) {
+hipError_t launch_batch_class(int cls, int kind, const DevTables &dt, const RunArgs *grids, const uint2 *work,
+                              int nwork, int nblocks, size_t lds, hipStream_t st) {
   switch (cls) {
-    case 0: return launch_batch_nlm<17, MIXED>(dt, grids, work, nwork, nblocks, lds, st);
-    case 1: return launch_batch_nlm<25, MIXED>(dt, grids, work, nwork, nblocks, lds, st);
-    case 2: return launch_batch_nlm<37, MIXED>(dt, grids, work, nwork, nblocks, lds, st);
-    case 3: return launch_batch_nlm<49, MIXED>(dt, grids, work, nwork, nblocks, lds, st);
-    default: return launch_batch_nlm<MAX_NLAY + 1, MIXED>(dt, grids, work, nwork, nblocks, lds, st);
+    case 0: return launch_batch_nlm<17, MIXED>(kind, dt, grids, work, nwork, nblocks, lds, st);
+    case 1: return launch_batch_nlm<25, MIXED>(kind, dt, grids, work, nwork, nblocks, lds, st);
+    case 2: return launch_batch_nlm<37, MIXED>(kind, dt, grids, work, nwork, nblocks, lds, st);
+    case 3: return launch_batch_nlm<49, MIXED>(kind, dt, grids, work, nwork, nblocks, lds, st);
+    default: return launch_batch_nlm<MAX_NLAY + 1, MIXED>(kind, dt, grids, work, nwork, nblocks, lds, st);
   }
 }
 

@@ -10,6 +10,9 @@ struct mgcm_rt_ctx {
   float *d_kv32, *d_ki32;
   mgcm::DevTables dt;
   int *d_flag;
+  // continuum-point records of mgcm_rt_run (grow-only)
+  double *d_clr;
+  size_t clr_bytes;
   // device staging of the host-pointer path (grow-only)
   size_t host_bytes;
   double *d_host;
@@ -20,10 +23,11 @@ struct mgcm_rt_ctx {
 namespace mgcm {
 hipError_t launch_plain(const DevTables &dt, const RunArgs &a, hipStream_t st);
 hipError_t launch_mixed(const DevTables &dt, const RunArgs &a, hipStream_t st);
-hipError_t launch_batch_plain(int cls, const DevTables &dt, const RunArgs *grids, const uint2 *work, int nwork,
-                              int nblocks, size_t lds, hipStream_t st);
-hipError_t launch_batch_mixed(int cls, const DevTables &dt, const RunArgs *grids, const uint2 *work, int nwork,
-                              int nblocks, size_t lds, hipStream_t st);
+hipError_t launch_batch_plain(int cls, int kind, const DevTables &dt, const RunArgs *grids, const uint2 *work,
+                              int nwork, int nblocks, size_t lds, hipStream_t st);
+hipError_t launch_batch_mixed(int cls, int kind, const DevTables &dt, const RunArgs *grids, const uint2 *work,
+                              int nwork, int nblocks, size_t lds, hipStream_t st);
+size_t clr_workspace_bytes(int ncol, int nlay);
 // Splits the precision argument into the path (MGCM_RT_FP64/MIXED) and the
 // band mask; MGCM_RT_ERR_PRECISION for anything else.
 int decode_mode(int precision, int *path, int *bands);

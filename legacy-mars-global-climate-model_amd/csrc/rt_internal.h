@@ -55,6 +55,7 @@ struct RunArgs {
   double *nfluxtopv, *nfluxtopi, *diffvt, *detau, *cumtauv, *hr;
   int *flag;   // range flag
   int bands;   // bit 0: VIS bands, bit 1: IR bands
+  double *clr; // continuum-point records, [ncol][NB][2*nl+2] (workspace)
 };
 
 struct DevTables {
