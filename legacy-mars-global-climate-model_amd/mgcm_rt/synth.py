@@ -229,7 +229,8 @@ def make_columns(seed: int, ncol: int, nlay: int, clouds: bool = False) -> dict:
     # layer mid-point temperatures (K = 2l+2, 0-based index 2l+1... for l=1..nlay -> 2l+1)
     kmid = np.arange(1, nlay + 1) * 2 + 1                     # 0-based index of K=2l+2
     lp = np.log(pl[:, kmid])
-    frac = (lp - lp[:, :1]) / (lp[:, -1:] - lp[:, :1])
+    den = lp[:, -1:] - lp[:, :1]                              # 0 for a single layer
+    frac = np.divide(lp - lp[:, :1], den, out=np.full_like(lp, 0.5), where=den > 0)
     tl = 140.0 + 110.0 * frac + rng.normal(0.0, 5.0, (ncol, nlay))
     tstrat = 140.0 + rng.normal(0.0, 5.0, ncol)
     tg = tl[:, -1] + rng.uniform(-5.0, 15.0, ncol)

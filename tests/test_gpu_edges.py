@@ -1,0 +1,100 @@
+"""Edge cases of the plain FP64 path on the GPU, through the C ABI:
+the reference's own edge fixture (sun down at 0 and 9.99999e-6, grazing sun at
+1e-5, H2O above the water grid, T at the ends of the k-table and Planck
+ranges, a TLIMIT-skipped band, thick dust, cloud); the smallest and largest
+layer counts; one-column, empty and non-multiple-of-4 grids; the range flag;
+determinism; status codes."""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, rel_err
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("fmnetv", "fluxupv", "fluxdnv", "fmneti", "fluxupi", "fluxdni", "nfluxtopv", "nfluxtopi",
+        "diffvt", "detau", "cumtauv", "hr")
+
+
+def _fixture(name):
+    if GOLDEN not in sys.path:
+        sys.path.insert(0, GOLDEN)
+    from test_oracle import load_fixture
+    return load_fixture(name)
+
+
+def test_reference_edge_fixture():
+    from mgcm_rt import Radiation
+    t, c, want = _fixture("edge24.npz")
+    got = Radiation(t).run_host(c)
+    for k in KEYS:
+        assert rel_err(got[k], want[k]) <= 1e-10, (k, rel_err(got[k], want[k]))
+    down = c["acosz"] < np.float32(1e-5)
+    assert down[0] and down[1] and not down[5]
+    assert not np.any(got["detau"][down]) and not np.any(got["fmnetv"][down])
+
+
+@pytest.mark.parametrize("nlay,ncol", [(1, 37), (16, 5), (64, 9), (24, 1), (36, 2)])
+def test_layer_and_column_extremes(oracle, nlay, ncol):
+    from mgcm_rt import Radiation, synth
+    t, c = synth.make_workload(60 + nlay, ncol, nlay, clouds=(nlay == 36))
+    got = Radiation(t).run_host(c)
+    want = oracle.run(t, c)
+    for k in KEYS:
+        assert rel_err(got[k], want[k]) <= 1e-10, (nlay, ncol, k)
+
+
+def test_empty_grid():
+    from mgcm_rt import Radiation, synth
+    t, c = synth.make_workload(70, 3, 24)
+    empty = {k: (v[:0] if isinstance(v, np.ndarray) else v) for k, v in c.items()}
+    empty["ncol"] = 0
+    out = Radiation(t).run_host(empty)
+    assert out["hr"].shape == (0, 25)
+
+
+def test_range_flag_reported():
+    from mgcm_rt import Radiation, RadiationError, synth
+    t, c = synth.make_workload(71, 8, 24)
+    rt = Radiation(t)
+    c["tlev"][3, 10] = 950.0          # above the Planck table (50..900 K)
+    with pytest.raises(RadiationError) as e:
+        rt.run_host(c)
+    assert e.value.status == -5
+    t2, c2 = synth.make_workload(72, 8, 24)
+    rt.run_host(c2)                    # the flag was cleared by the check
+    c2["tlev"][5, 4] = np.nan
+    with pytest.raises(RadiationError) as e:
+        rt.run_host(c2)
+    assert e.value.status == -5
+
+
+def test_deterministic():
+    from mgcm_rt import Radiation, synth
+    t, c = synth.make_workload(73, 101, 24)
+    rt = Radiation(t)
+    a, b = rt.run_host(c), rt.run_host(c)
+    for k in KEYS:
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_status_codes():
+    from mgcm_rt import Radiation, RadiationError, synth
+    t, c = synth.make_workload(74, 4, 24)
+    rt = Radiation(t)
+    for prec in (7, 0x30, 0x40):
+        with pytest.raises(RadiationError) as e:
+            rt.run_host(c, precision=prec)
+        assert e.value.status == -6
+    for nl in (0, 65):
+        bad = dict(c)
+        bad["nlay"] = nl
+        with pytest.raises(RadiationError) as e:
+            rt.run_host(bad)
+        assert e.value.status == -2
+    assert rt.lib.mgcm_rt_run(rt.ctx, None, None, 0, None) == -1
+    assert b"Planck" in rt.lib.mgcm_rt_strerror(-5)
