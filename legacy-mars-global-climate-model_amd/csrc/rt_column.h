@@ -240,14 +240,13 @@ struct Acc {
   double vnet[SL], vup[SL], vdn[SL], inet[SL], iup[SL], idn[SL];
 };
 
+// Branch-free: the owner lane's slot takes the sum, every other value is kept.
 template <int SL>
 __device__ __forceinline__ void acc_add(double *arr, int n, int g0, double v) {
   const int slot = (n - 1) >> 4;
-  if (((n - 1) & 15) == g0) {
+  const bool own = ((n - 1) & 15) == g0;
 #pragma unroll
-    for (int j = 0; j < SL; j++)
-      if (j == slot) arr[j] = arr[j] + v;
-  }
+  for (int j = 0; j < SL; j++) arr[j] = (own && j == slot) ? arr[j] + v : arr[j];
 }
 
 template <int SL>
@@ -384,7 +383,7 @@ __device__ __forceinline__ void column_block(const DevTables &dt, const RunArgs 
           acc_add<SL>(acc.vdn, n, g0, group16_sum(dn));
         };
         double fluxup = 0.0, fluxdn = 0.0, diffv = 0.0;
-        const double bsurf = a.als[colc] * u0 * a.sol[nw] * rt_exp(-dmin(detau / u0, MAXEXP));
+        const double bsurf = a.als[colc] * u0 * a.sol[nw] * rt_exp35(-dmin(detau / u0, MAXEXP));
         vis_solve<NLM>(nl, s, cbar, u0, a.sol[nw], a.als[colc], 0.0, bsurf, accf, fluxup, fluxdn, diffv);
         topv = topv + group16_sum(skip ? 0.0 : (fluxup - fluxdn) * gw * onemf);
         difv = difv + group16_sum(skip ? 0.0 : diffv * gw * onemf);
@@ -561,7 +560,7 @@ __device__ __forceinline__ void clear_block(const DevTables &dt, const RunArgs &
     a.detau[(size_t)col * NG * NV + 16 * NV + b] = detau;
     a.cumtauv[(size_t)col * NG * NV + 16 * NV + b] = cumtau;
     double fluxup = 0.0, fluxdn = 0.0, diffv = 0.0;
-    const double bsurf = a.als[col] * mu * a.sol[b] * rt_exp(-dmin(detau / mu, MAXEXP));
+    const double bsurf = a.als[col] * mu * a.sol[b] * rt_exp35(-dmin(detau / mu, MAXEXP));
     vis_solve<NLM>(nl, s, cbar, mu, a.sol[b], a.als[col], 0.0, bsurf, accf, fluxup, fluxdn, diffv);
     rec[2 * nl] = fluxup - fluxdn;
     rec[2 * nl + 1] = diffv;

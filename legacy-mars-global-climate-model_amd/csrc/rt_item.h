@@ -10,6 +10,8 @@
 namespace mgcm {
 // exp of the column kernels: glibc's bits, table in LDS (see rt_libm.h)
 __device__ __forceinline__ double rt_exp(double x) { return mgcm_libm::mgcm_exp_lds(x); }
+// exp of an argument clamped to [-35, 35] (MAXEXP): branch-free, same bits
+__device__ __forceinline__ double rt_exp35(double x) { return mgcm_libm::mgcm_exp_lds_mid(x); }
 }  // namespace mgcm
 
 namespace mgcm {
@@ -192,7 +194,7 @@ __device__ __forceinline__ void vis_solve(int nl, VisState<NLM> &s, const double
   // the row below and the transmission at the layer bottom (the top of the
   // layer below, the same argument) are carried in registers.
   double gama_n1 = 0.0, cpm1_n1 = 0.0, cmm1_n1 = 0.0, as1 = 0.0, ds1 = 0.0;
-  double xbot = rt_exp(-dmin(s.tau[nl + 1] / u0, MAXEXP));
+  double xbot = rt_exp35(-dmin(s.tau[nl + 1] / u0, MAXEXP));
   // the next layer's state is loaded one iteration ahead
   double w0_n = s.w0[nl], cb_n = cbar[nl - 1], tau_n = s.tau[nl], dtau_n = s.dtau[nl];
   for (int n = nl; n >= 1; n--) {
@@ -205,10 +207,10 @@ __device__ __forceinline__ void vis_solve(int nl, VisState<NLM> &s, const double
     }
     const VisLayer v = vis_layer(w0c, cbc, u0, f0pi);
     const double lam = v.lam, gama = v.gama, am = v.am, ap = v.ap;
-    const double xtop = rt_exp(-dmin(tauc / u0, MAXEXP));
+    const double xtop = rt_exp35(-dmin(tauc / u0, MAXEXP));
     const double cpm1 = ap * xtop, cmm1 = am * xtop;
     const double cp = ap * xbot, cm = am * xbot;
-    const double ep = rt_exp(dmin(MAXEXP, lam * dtauc));
+    const double ep = rt_exp35(dmin(MAXEXP, lam * dtauc));
     const double em = 1.0 / ep;
     const double e1 = ep + gama * em, e2 = ep - gama * em;
     const double e3 = gama * ep + em, e4 = gama * ep - em;
@@ -284,24 +286,24 @@ __device__ __forceinline__ void vis_solve(int nl, VisState<NLM> &s, const double
     xkprev = xb;
     const double xk1 = xa + xb;
     double xk2 = xa - xb;
-    if (xk2 != 0.0 && fabs(xk2 / (xa + F32LIT(1.e-20))) < F32LIT(1.e-30)) xk2 = 0.0;
+    xk2 = (xk2 != 0.0 && fabs(xk2 / (xa + F32LIT(1.e-20))) < F32LIT(1.e-30)) ? 0.0 : xk2;  // dsolver.f90:101
     const VisLayer v = vis_layer(w0c, cbc, u0, f0pi);
     const double lam = v.lam, gama = v.gama, am = v.am, ap = v.ap;
     double fmidp, fmidm;
     if (n < nl) {  // gfluxv.f90:170-206
-      const double ep = rt_exp(dmin(MAXEXP, lam * (hmc - ttc)));
+      const double ep = rt_exp35(dmin(MAXEXP, lam * (hmc - ttc)));
       const double em = 1.0 / ep;
       const double taumid = hmc;
-      const double xm = rt_exp(-dmin(taumid / u0, MAXEXP));
+      const double xm = rt_exp35(-dmin(taumid / u0, MAXEXP));
       const double cpmid = ap * xm, cmmid = am * xm;
       fmidp = xk1 * ep + gama * xk2 * em + cpmid;
       fmidm = xk1 * ep * gama + xk2 * em + cmmid;
       fmidm = fmidm + u0 * f0pi * xm;
     } else {  // bottom layer, full sub-layer (gfluxv.f90:243-280)
-      const double ep = rt_exp(dmin(MAXEXP, lam * (tau_bot - tau_lastl)));
+      const double ep = rt_exp35(dmin(MAXEXP, lam * (tau_bot - tau_lastl)));
       const double em = 1.0 / ep;
       const double taumid = dmin(tau_bot, MAXEXP);
-      const double xm = rt_exp(-dmin(taumid / u0, MAXEXP));
+      const double xm = rt_exp35(-dmin(taumid / u0, MAXEXP));
       const double cpmid = ap * xm, cmmid = am * xm;
       fmidp = xk1 * ep + gama * xk2 * em + cpmid;
       fmidm = xk1 * ep * gama + xk2 * em + cmmid;
@@ -315,7 +317,7 @@ __device__ __forceinline__ void vis_solve(int nl, VisState<NLM> &s, const double
     const double ep = 1.0, em = 1.0;
     fluxup = xk1_1 * ep + gama_1 * xk2_1 * em + ap_1;
     fluxdn = xk1_1 * ep * gama_1 + xk2_1 * em + am_1;
-    fluxdn = fluxdn + u0 * f0pi * rt_exp(-dmin(0.0 / u0, MAXEXP));
+    fluxdn = fluxdn + u0 * f0pi * rt_exp35(-dmin(0.0 / u0, MAXEXP));
   }
 }
 
@@ -397,7 +399,7 @@ __device__ __forceinline__ double ir_solve(int nl, IrState<NLM> &s, const double
     const double cm = b0 + b1 * dtau - b1 * term;
     const double cpm1 = b0 + b1 * term;
     const double cmm1 = b0 - b1 * term;
-    const double ep = rt_exp(dmin(lam * dtau, MAXEXP));
+    const double ep = rt_exp35(dmin(lam * dtau, MAXEXP));
     const double em = 1.0 / ep;
     const double e1 = ep + gama * em, e2 = ep - gama * em;
     const double e3 = gama * ep + em, e4 = gama * ep - em;
@@ -473,11 +475,11 @@ __device__ __forceinline__ double ir_solve(int nl, IrState<NLM> &s, const double
     xkprev = xb;
     const double xk1 = xa + xb;
     double xk2 = xa - xb;
-    if (xk2 != 0.0 && fabs(xk2 / (xa + F32LIT(1.e-20))) < F32LIT(1.e-30)) xk2 = 0.0;
+    xk2 = (xk2 != 0.0 && fabs(xk2 / (xa + F32LIT(1.e-20))) < F32LIT(1.e-30)) ? 0.0 : xk2;  // dsolver.f90:101
     const IrLayer v = ir_layer(w0c, cbc, dtc, btc, bbc);
     const double lam = v.lam, gama = v.gama, b0 = v.b0, b1 = v.b1, term = v.term;
     const double dk = (n < nl) ? dmc : dtc;
-    const double ep = rt_exp(dmin(lam * dk, MAXEXP));
+    const double ep = rt_exp35(dmin(lam * dk, MAXEXP));
     const double em = 1.0 / ep;
     const double cpmid = b0 + b1 * dk + b1 * term;
     const double cmmid = b0 + b1 * dk - b1 * term;

@@ -125,6 +125,28 @@ struct TabLds {
   __device__ __forceinline__ uint64_t operator()(int i) const { return mgcm_lds_exptab[i]; }
 };
 __device__ __forceinline__ double mgcm_exp_lds(double x) { return exp_core(x, TabLds{}); }
+// The same bits for |x| < 512 without the special-case branch: there the
+// general path is exact, including |x| < 2^-54, where glibc returns 1 + x and
+// the general path rounds fma(1, x, 1) = 1 + x.  Callers guarantee the range.
+__device__ __forceinline__ double mgcm_exp_lds_mid(double x) {
+  const double InvLn2N = 0x1.71547652b82fep7, NegLn2hiN = -0x1.62e42fefa0000p-8,
+               NegLn2loN = -0x1.cf79abc9e3b3ap-47, Shift = 0x1.8p52;
+  const double C2 = 0x1.ffffffffffdbdp-2, C3 = 0x1.555555555543cp-3, C4 = 0x1.55555cf172b91p-5,
+               C5 = 0x1.1111167a4d017p-7;
+  double kd = MGCM_FMA(InvLn2N, x, Shift);
+  const uint64_t ki = asu(kd);
+  kd -= Shift;
+  const double r = MGCM_FMA(kd, NegLn2loN, MGCM_FMA(kd, NegLn2hiN, x));
+  const int idx = (int)(2 * (ki % 128));
+  const uint64_t top = ki << 45;
+  const double tail = asd(mgcm_lds_exptab[idx]);
+  const uint64_t sbits = mgcm_lds_exptab[idx + 1] + top;
+  const double r2 = r * r;
+  const double p1 = MGCM_FMA(r, C3, C2), p2 = MGCM_FMA(r, C5, C4);
+  const double tmp = MGCM_FMA(r2 * r2, p2, MGCM_FMA(r2, p1, tail + r));
+  const double scale = asd(sbits);
+  return MGCM_FMA(scale, tmp, scale);
+}
 // Fills the table; every thread of the block calls it, then the block synchronises.
 __device__ __forceinline__ void load_lds_exptab(int tid, int nthreads) {
   for (int i = tid; i < 256; i += nthreads) mgcm_lds_exptab[i] = exp_tab(i);
