@@ -409,7 +409,7 @@ __device__ __forceinline__ void column_block(const DevTables &dt, const RunArgs 
         // boundary conditions (code/sfluxi.f90:61-62,81-82)
         const double rsf = a.albi[colc];
         const double bsurf = (1. - rsf) * bk[nlev - 1];
-        const double tautop = s.dtau[1] * plev[1] / (plev[3] - plev[1]);
+        const double tautop = s.wd[1].y * plev[1] / (plev[3] - plev[1]);
         const double btop = (1.0 - rt_exp(-tautop * 2.0)) * bk[1];  // /UBARI, UBARI = 0.5
         const double ftopup = ir_solve<NLM>(nl, s, cosb, bk, rsf, btop, bsurf, accf);
         topi = topi + group16_sum(skip ? 0.0 : ftopup * dw * gw * onemf);
@@ -574,7 +574,7 @@ __device__ __forceinline__ void clear_block(const DevTables &dt, const RunArgs &
       bk[K - 1] = dt.planck[(size_t)(planck_index(a.tlev[(size_t)col * nlev + K - 1], a.flag) - 1) * NI + nw];
     const double rsf = a.albi[col];
     const double bsurf = (1. - rsf) * bk[nlev - 1];
-    const double tautop = s.dtau[1] * pl[1] / (pl[3] - pl[1]);
+    const double tautop = s.wd[1].y * pl[1] / (pl[3] - pl[1]);
     const double btop = (1.0 - rt_exp(-tautop * 2.0)) * bk[1];
     rec[2 * nl] = ir_solve<NLM>(nl, s, cosb, bk, rsf, btop, bsurf, accf);
   }

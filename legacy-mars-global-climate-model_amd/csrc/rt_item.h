@@ -90,17 +90,24 @@ __device__ __forceinline__ void layer_aerosol(const RtConst *__restrict__ rc, in
 // 1-based).  Inputs arrive through the Src functor: finish(K, fetch(K)) gives the sub-level
 // optical depth DTAUK(K) and adds the gas part to taugsurf.
 
-// Per-item state of a VIS item (1-based layer arrays): scaled single-scattering
-// albedo W0, scaled layer optical depth DTAU, the scaled optical depth TAU at
-// each layer top, the scaled optical depth HM at each layer mid-point
-// (TAUCUM(2L+1) of code/gfluxv.f90:76-103) and the reverse-Thomas
-// coefficients AS/DS of the 2*NL rows.  TAUCUM(2L) = TAU(L) for L >= 2; TM1
-// holds TAUCUM(2).
+// Two FP64 values that both passes of a solve read together: one 16-byte
+// private-memory access instead of two 8-byte ones.
+struct alignas(16) Pair {
+  double x, y;
+};
+
+// Per-item state of a VIS item (1-based layer arrays): wt[L] = {scaled
+// single-scattering albedo W0, scaled optical depth TAU at the layer top},
+// dh[L] = {scaled layer optical depth DTAU, scaled optical depth HM at the
+// layer mid-point (TAUCUM(2L+1) of code/gfluxv.f90:76-103)}, and row[i] =
+// {AS, DS}, the reverse-Thomas coefficients of the 2*NL rows.  TAUCUM(2L) =
+// TAU(L) for L >= 2; TM1 holds TAUCUM(2), TBOT the optical depth TAU(NL+1) of
+// the surface.
 template <int NLM>
 struct VisState {
-  double w0[NLM + 1], dtau[NLM + 1], tau[NLM + 2], hm[NLM + 1];
-  double tm1;
-  double as[2 * NLM + 1], ds[2 * NLM + 1];
+  Pair wt[NLM + 1], dh[NLM + 1];
+  double tm1, tbot;
+  Pair row[2 * NLM + 1];
 };
 
 // Two-stream constants of one VIS layer (code/gfluxv.f90:108-135): the
@@ -139,7 +146,6 @@ __device__ __forceinline__ double vis_optics(int nl, Src &src, const double *__r
   const double d2 = src.finish(2, r2);
   double tcv = d2;  // TAUCUMV(2)
   double fprev = 0.0, tcv_prev_level = 0.0, tauL = 0.0;  // tauL = TAU(L), carried in a register
-  s.tau[1] = 0.0;
   for (int L = 1; L <= nl - 1; L++) {
     const double a = src.finish(2 * L + 1, ra), b = src.finish(2 * L + 2, rb);
     ra = src.fetch(2 * L + 3);                   // L = nl-1: the bottom sub-level L_LEVELS
@@ -149,20 +155,19 @@ __device__ __forceinline__ double vis_optics(int nl, Src &src, const double *__r
     const double tcB = tcA + b;  // TAUCUMV(2L+2)
     const double W = scat[L - 1] / dtauv, C = cosb[L - 1];
     const double F = 1.0 - W * (C * C);
+    double hm;
     if (L == 1) {
       tauL = 0.0 * F;      // TAU(1) = TDEL(1)*FACTOR with TDEL(1) = 0
-      s.tau[1] = tauL;
       s.tm1 = tcv * F;     // TAUCUM(2)
-      s.hm[1] = s.tm1 + (tcA - tcv) * F;
+      hm = s.tm1 + (tcA - tcv) * F;
     } else {
       // TAUCUM(2L) = TAU(L); TAUCUM(2L+1) uses the factor of layer L-1
-      s.hm[L] = tauL + (tcA - tcv_prev_level) * fprev;
+      hm = tauL + (tcA - tcv_prev_level) * fprev;
     }
-    s.w0[L] = W * (1.0 - C * C) / F;
     const double dtau = dtauv * F;
-    s.dtau[L] = dtau;
+    s.wt[L] = Pair{W * (1.0 - C * C) / F, tauL};
+    s.dh[L] = Pair{dtau, hm};
     tauL = tauL + dtau;
-    s.tau[L + 1] = tauL;
     fprev = F;
     tcv_prev_level = tcB;  // TAUCUMV(2L+2), the base of the next layer's mid level
     tcv = tcB;
@@ -174,11 +179,11 @@ __device__ __forceinline__ double vis_optics(int nl, Src &src, const double *__r
     cumtau = tcv + a;  // TAUCUMV(L_LEVELS)
     const double W = scat[L - 1] / dtauv, C = cosb[L - 1];
     const double F = 1.0 - W * (C * C);
-    s.w0[L] = W * (1.0 - C * C) / F;
     const double dtau = dtauv * F;
-    s.dtau[L] = dtau;
+    s.wt[L] = Pair{W * (1.0 - C * C) / F, tauL};
+    s.dh[L] = Pair{dtau, 0.0};  // the bottom layer has no HM
     tauL = tauL + dtau;
-    s.tau[L + 1] = tauL;
+    s.tbot = tauL;
   }
   return tauL;
 }
@@ -194,16 +199,16 @@ __device__ __forceinline__ void vis_solve(int nl, VisState<NLM> &s, const double
   // the row below and the transmission at the layer bottom (the top of the
   // layer below, the same argument) are carried in registers.
   double gama_n1 = 0.0, cpm1_n1 = 0.0, cmm1_n1 = 0.0, as1 = 0.0, ds1 = 0.0;
-  double xbot = rt_exp35(-dmin(s.tau[nl + 1] / u0, MAXEXP));
+  double xbot = rt_exp35(-dmin(s.tbot / u0, MAXEXP));
   // the next layer's state is loaded one iteration ahead
-  double w0_n = s.w0[nl], cb_n = cbar[nl - 1], tau_n = s.tau[nl], dtau_n = s.dtau[nl];
+  Pair wt_n = s.wt[nl];
+  double cb_n = cbar[nl - 1], dtau_n = s.dh[nl].x;
   for (int n = nl; n >= 1; n--) {
-    const double w0c = w0_n, cbc = cb_n, tauc = tau_n, dtauc = dtau_n;
+    const double w0c = wt_n.x, cbc = cb_n, tauc = wt_n.y, dtauc = dtau_n;
     if (n > 1) {
-      w0_n = s.w0[n - 1];
+      wt_n = s.wt[n - 1];
       cb_n = cbar[n - 2];
-      tau_n = s.tau[n - 1];
-      dtau_n = s.dtau[n - 1];
+      dtau_n = s.dh[n - 1].x;
     }
     const VisLayer v = vis_layer(w0c, cbc, u0, f0pi);
     const double lam = v.lam, gama = v.gama, am = v.am, ap = v.ap;
@@ -220,8 +225,7 @@ __device__ __forceinline__ void vis_solve(int nl, VisState<NLM> &s, const double
       const double df = bsurf - cp + rsf * cm;
       as1 = af / bf;
       ds1 = df / bf;
-      s.as[i] = as1;
-      s.ds[i] = ds1;
+      s.row[i] = Pair{as1, ds1};
     } else {
       {  // odd row 2n+1 (dsolver.f90:74-80)
         const int i = 2 * n + 1;
@@ -232,8 +236,7 @@ __device__ __forceinline__ void vis_solve(int nl, VisState<NLM> &s, const double
         const double x = 1.0 / (bf - cf * as1);
         as1 = af * x;
         ds1 = (df - cf * ds1) * x;
-        s.as[i] = as1;
-        s.ds[i] = ds1;
+        s.row[i] = Pair{as1, ds1};
       }
       {  // even row 2n (dsolver.f90:63-70)
         const int i = 2 * n;
@@ -244,15 +247,13 @@ __device__ __forceinline__ void vis_solve(int nl, VisState<NLM> &s, const double
         const double x = 1.0 / (bf - cf * as1);
         as1 = af * x;
         ds1 = (df - cf * ds1) * x;
-        s.as[i] = as1;
-        s.ds[i] = ds1;
+        s.row[i] = Pair{as1, ds1};
       }
     }
     if (n == 1) {  // top row (dsolver.f90:54-57)
       const double af = 0.0, bf = gama + 1.0, cf = gama - 1.0, df = btop - cmm1;
       const double x = 1.0 / (bf - cf * as1);
-      s.as[1] = af * x;
-      s.ds[1] = (df - cf * ds1) * x;
+      s.row[1] = Pair{af * x, (df - cf * ds1) * x};
     }
     gama_n1 = gama;
     cpm1_n1 = cpm1;
@@ -260,26 +261,25 @@ __device__ __forceinline__ void vis_solve(int nl, VisState<NLM> &s, const double
     xbot = xtop;
   }
   // top-down: substitution, unmixing and the layer fluxes
-  double xkprev = s.ds[1];
+  Pair r1_n = s.row[1], r2_n = s.row[2];
+  double xkprev = r1_n.y;
   double xk1_1 = 0.0, xk2_1 = 0.0, gama_1 = 0.0, am_1 = 0.0, ap_1 = 0.0;
   // row pair, layer state and sub-level depth of the next layer, loaded one iteration ahead
-  double as1_n = s.as[1], ds1_n = s.ds[1], as2_n = s.as[2], ds2_n = s.ds[2];
-  w0_n = s.w0[1];
+  double w0_n = s.wt[1].x, tt_n = s.tm1;
   cb_n = cbar[0];
-  double hm_n = s.hm[1], tt_n = s.tm1;
-  const double tau_bot = s.tau[nl + 1], tau_lastl = s.tau[nl];
+  double hm_n = s.dh[1].y;
+  const double tau_bot = s.tbot, tau_lastl = s.wt[nl].y;
   for (int n = 1; n <= nl; n++) {
-    const double as1c = as1_n, ds1c = ds1_n, as2c = as2_n, ds2c = ds2_n;
+    const double as1c = r1_n.x, ds1c = r1_n.y, as2c = r2_n.x, ds2c = r2_n.y;
     const double w0c = w0_n, cbc = cb_n, hmc = hm_n, ttc = tt_n;
     if (n < nl) {
-      as1_n = s.as[2 * n + 1];
-      ds1_n = s.ds[2 * n + 1];
-      as2_n = s.as[2 * n + 2];
-      ds2_n = s.ds[2 * n + 2];
-      w0_n = s.w0[n + 1];
+      r1_n = s.row[2 * n + 1];
+      r2_n = s.row[2 * n + 2];
+      const Pair wt = s.wt[n + 1];
+      w0_n = wt.x;
+      tt_n = wt.y;
       cb_n = cbar[n];
-      hm_n = s.hm[n + 1];
-      tt_n = s.tau[n + 1];
+      hm_n = s.dh[n + 1].y;
     }
     const double xa = (n == 1) ? ds1c : ds1c - as1c * xkprev;
     const double xb = ds2c - as2c * xa;
@@ -321,12 +321,14 @@ __device__ __forceinline__ void vis_solve(int nl, VisState<NLM> &s, const double
   }
 }
 
-// Per-item state of an IR item: W0 (= WBARI), DTAU (= DTAUI), the half-layer
-// optical depth TAUCUMI(2L+1)-TAUCUMI(2L) and the reverse-Thomas AS/DS.
+// Per-item state of an IR item: wd[L] = {W0 (= WBARI), DTAU (= DTAUI)}, the
+// half-layer optical depth DMID(L) = TAUCUMI(2L+1)-TAUCUMI(2L) and row[i] =
+// {AS, DS}, the reverse-Thomas coefficients.
 template <int NLM>
 struct IrState {
-  double w0[NLM + 1], dtau[NLM + 1], dmid[NLM + 1];
-  double as[2 * NLM + 1], ds[2 * NLM + 1];
+  Pair wd[NLM + 1];
+  double dmid[NLM + 1];
+  Pair row[2 * NLM + 1];
 };
 
 // Hemispheric-mean constants of one IR layer (code/gfluxi.f90:69-93): LAMDA,
@@ -369,8 +371,7 @@ __device__ __forceinline__ void ir_optics(int nl, Src &src, const double *__rest
     const double tcA = tcv + a;  // TAUCUMI(2L+1)
     s.dmid[L] = tcA - tcv;       // TAUCUMI(2L+1) - TAUCUMI(2L)
     tcv = (L < nl) ? tcA + b : tcA;
-    s.w0[L] = w;
-    s.dtau[L] = dt;
+    s.wd[L] = Pair{w, dt};
   }
 }
 
@@ -382,12 +383,12 @@ __device__ __forceinline__ double ir_solve(int nl, IrState<NLM> &s, const double
                            double btop, double bsurf, Acc &acc) {
   double gama_n1 = 0.0, cpm1_n1 = 0.0, cmm1_n1 = 0.0, as1 = 0.0, ds1 = 0.0;
   // the next layer's state is loaded one iteration ahead
-  double dtau_n = s.dtau[nl], w0_n = s.w0[nl], cb_n = cosb[nl - 1], bt_n = bk[2 * nl - 1], bb_n = bk[2 * nl];
+  Pair wd_n = s.wd[nl];
+  double cb_n = cosb[nl - 1], bt_n = bk[2 * nl - 1], bb_n = bk[2 * nl];
   for (int n = nl; n >= 1; n--) {
-    const double dtau = dtau_n, w0c = w0_n, cbc = cb_n, btc = bt_n, bbc = bb_n;
+    const double dtau = wd_n.y, w0c = wd_n.x, cbc = cb_n, btc = bt_n, bbc = bb_n;
     if (n > 1) {
-      dtau_n = s.dtau[n - 1];
-      w0_n = s.w0[n - 1];
+      wd_n = s.wd[n - 1];
       cb_n = cosb[n - 2];
       bt_n = bk[2 * n - 3];
       bb_n = bk[2 * n - 1];
@@ -409,8 +410,7 @@ __device__ __forceinline__ double ir_solve(int nl, IrState<NLM> &s, const double
       const double df = bsurf - cp + rsf * cm;
       as1 = af / bf;
       ds1 = df / bf;
-      s.as[i] = as1;
-      s.ds[i] = ds1;
+      s.row[i] = Pair{as1, ds1};
     } else {
       {
         const int i = 2 * n + 1;
@@ -421,8 +421,7 @@ __device__ __forceinline__ double ir_solve(int nl, IrState<NLM> &s, const double
         const double x = 1.0 / (bf - cf * as1);
         as1 = af * x;
         ds1 = (df - cf * ds1) * x;
-        s.as[i] = as1;
-        s.ds[i] = ds1;
+        s.row[i] = Pair{as1, ds1};
       }
       {
         const int i = 2 * n;
@@ -433,39 +432,34 @@ __device__ __forceinline__ double ir_solve(int nl, IrState<NLM> &s, const double
         const double x = 1.0 / (bf - cf * as1);
         as1 = af * x;
         ds1 = (df - cf * ds1) * x;
-        s.as[i] = as1;
-        s.ds[i] = ds1;
+        s.row[i] = Pair{as1, ds1};
       }
     }
     if (n == 1) {
       const double af = 0.0, bf = gama + 1.0, cf = gama - 1.0, df = btop - cmm1;
       const double x = 1.0 / (bf - cf * as1);
-      s.as[1] = af * x;
-      s.ds[1] = (df - cf * ds1) * x;
+      s.row[1] = Pair{af * x, (df - cf * ds1) * x};
     }
     gama_n1 = gama;
     cpm1_n1 = cpm1;
     cmm1_n1 = cmm1;
   }
-  double xkprev = s.ds[1];
+  Pair r1_n = s.row[1], r2_n = s.row[2];
+  double xkprev = r1_n.y;
   double xk1_1 = 0.0, xk2_1 = 0.0, gama_1 = 0.0, b0_1 = 0.0, b1_1 = 0.0, term_1 = 0.0;
-  double as1_n = s.as[1], ds1_n = s.ds[1], as2_n = s.as[2], ds2_n = s.ds[2];
-  w0_n = s.w0[1];
+  wd_n = s.wd[1];
   cb_n = cosb[0];
   bt_n = bk[1];
   bb_n = (1 < nl) ? bk[3] : bk[2];
-  double dt_n = s.dtau[1], dm_n = s.dmid[1];
+  double dm_n = s.dmid[1];
   for (int n = 1; n <= nl; n++) {
-    const double as1c = as1_n, ds1c = ds1_n, as2c = as2_n, ds2c = ds2_n;
-    const double w0c = w0_n, cbc = cb_n, dtc = dt_n, dmc = dm_n, btc = bt_n, bbc = bb_n;
+    const double as1c = r1_n.x, ds1c = r1_n.y, as2c = r2_n.x, ds2c = r2_n.y;
+    const double w0c = wd_n.x, cbc = cb_n, dtc = wd_n.y, dmc = dm_n, btc = bt_n, bbc = bb_n;
     if (n < nl) {
-      as1_n = s.as[2 * n + 1];
-      ds1_n = s.ds[2 * n + 1];
-      as2_n = s.as[2 * n + 2];
-      ds2_n = s.ds[2 * n + 2];
-      w0_n = s.w0[n + 1];
+      r1_n = s.row[2 * n + 1];
+      r2_n = s.row[2 * n + 2];
+      wd_n = s.wd[n + 1];
       cb_n = cosb[n];
-      dt_n = s.dtau[n + 1];
       dm_n = s.dmid[n + 1];
       bt_n = bk[2 * n + 1];
       bb_n = (n + 1 < nl) ? bk[2 * n + 3] : bk[2 * n + 2];
