@@ -42,8 +42,10 @@ inline size_t stage_bytes(int nlay) {
          sizeof(int) * (4 * nsub + 4 * NB);
 }
 
-// Continuum-point record of one (column, band) in RunArgs::clr: FMIDP(1..nl),
-// FMIDM(1..nl), then two top terms.  Records of a column are band-contiguous.
+// Continuum-point records in RunArgs::clr: per column clr_rec(nl) slots of NB
+// doubles, slot k holding value k of every band (band index fastest):
+// FMIDP(1..nl), FMIDM(1..nl), then two top terms.  The lanes of a column write
+// and read adjacent doubles.
 __host__ __device__ inline int clr_rec(int nl) { return 2 * nl + 2; }
 inline size_t clr_bytes(int ncol, int nlay) { return (size_t)ncol * NB * clr_rec(nlay + 1) * sizeof(double); }
 
@@ -442,7 +444,7 @@ __device__ __forceinline__ void column_block(const DevTables &dt, const RunArgs 
         if (((n - 1) & 15) != g0) continue;
         double sn = 0.0, su = 0.0, sd = 0.0;
         for (int b = 0; b < NV; b++) {
-          const double fp = cr[b * rec + n - 1], fm = cr[b * rec + nl + n - 1];
+          const double fp = cr[(n - 1) * NB + b], fm = cr[(nl + n - 1) * NB + b];
           sn = sn + (sun ? (fp - fm) * wz[b] : 0.0);
           su = su + (sun ? fp * wz[b] : 0.0);
           sd = sd + (sun ? fm * wz[b] : 0.0);
@@ -453,8 +455,8 @@ __device__ __forceinline__ void column_block(const DevTables &dt, const RunArgs 
       }
       double st = 0.0, sdf = 0.0;
       for (int b = 0; b < NV; b++) {
-        st = st + (sun ? cr[b * rec + 2 * nl] * wz[b] : 0.0);
-        sdf = sdf + (sun ? cr[b * rec + 2 * nl + 1] * wz[b] : 0.0);
+        st = st + (sun ? cr[2 * nl * NB + b] * wz[b] : 0.0);
+        sdf = sdf + (sun ? cr[(2 * nl + 1) * NB + b] * wz[b] : 0.0);
       }
       topv = topv + st;
       difv = difv + sdf;
@@ -465,7 +467,7 @@ __device__ __forceinline__ void column_block(const DevTables &dt, const RunArgs 
         double sn = 0.0, su = 0.0, sd = 0.0;
         for (int b = NV; b < NB; b++) {
           const double dw = rc->dwni[b - NV];
-          const double fp = cr[b * rec + n - 1], fm = cr[b * rec + nl + n - 1];
+          const double fp = cr[(n - 1) * NB + b], fm = cr[(nl + n - 1) * NB + b];
           sn = sn + (valid ? (fp - fm) * dw * wz[b] : 0.0);
           su = su + (valid ? fp * dw * wz[b] : 0.0);
           sd = sd + (valid ? fm * dw * wz[b] : 0.0);
@@ -475,7 +477,7 @@ __device__ __forceinline__ void column_block(const DevTables &dt, const RunArgs 
         acc_add<SL>(acc.idn, n, g0, sd);
       }
       double st = 0.0;
-      for (int b = NV; b < NB; b++) st = st + (valid ? cr[b * rec + 2 * nl] * rc->dwni[b - NV] * wz[b] : 0.0);
+      for (int b = NV; b < NB; b++) st = st + (valid ? cr[2 * nl * NB + b] * rc->dwni[b - NV] * wz[b] : 0.0);
       topi = topi + st;
     }
   }
@@ -540,15 +542,15 @@ __device__ __forceinline__ void clear_block(const DevTables &dt, const RunArgs &
   const double *pl = a.plev + (size_t)col * nlev;
   const double *tr = a.tauref + (size_t)col * (nlev + 1);
   const double *tc = a.taurefcld + (size_t)col * (nlev + 1);
-  double *rec = a.clr + ((size_t)col * NB + b) * clr_rec(nl);
+  double *rec = a.clr + (size_t)col * NB * clr_rec(nl) + b;  // slot k of this item: rec[k * NB]
   double scat[NLM], cosb[NLM], cbar[NLM];
   for (int L = 1; L <= nl; L++) {
     layer_aerosol<VIS>(rc, b, pl, tr, tc, nl, L, scat[L - 1], cosb[L - 1]);
     cbar[L - 1] = cosb[L - 1] / (1.0 + cosb[L - 1]);
   }
   auto accf = [&](int n, double fp, double fm) {
-    rec[n - 1] = fp;
-    rec[nl + n - 1] = fm;
+    rec[(n - 1) * NB] = fp;
+    rec[(nl + n - 1) * NB] = fm;
   };
   if (VIS) {
     const double mu = a.acosz[col];
@@ -562,8 +564,8 @@ __device__ __forceinline__ void clear_block(const DevTables &dt, const RunArgs &
     double fluxup = 0.0, fluxdn = 0.0, diffv = 0.0;
     const double bsurf = a.als[col] * mu * a.sol[b] * rt_exp35(-dmin(detau / mu, MAXEXP));
     vis_solve<NLM>(nl, s, cbar, mu, a.sol[b], a.als[col], 0.0, bsurf, accf, fluxup, fluxdn, diffv);
-    rec[2 * nl] = fluxup - fluxdn;
-    rec[2 * nl + 1] = diffv;
+    rec[2 * nl * NB] = fluxup - fluxdn;
+    rec[(2 * nl + 1) * NB] = diffv;
   } else {
     const int nw = b - NV;
     ClearSrc<false> src{rc, b, pl, tr, tc, 0.0};
@@ -576,7 +578,7 @@ __device__ __forceinline__ void clear_block(const DevTables &dt, const RunArgs &
     const double bsurf = (1. - rsf) * bk[nlev - 1];
     const double tautop = s.wd[1].y * pl[1] / (pl[3] - pl[1]);
     const double btop = (1.0 - rt_exp(-tautop * 2.0)) * bk[1];
-    rec[2 * nl] = ir_solve<NLM>(nl, s, cosb, bk, rsf, btop, bsurf, accf);
+    rec[2 * nl * NB] = ir_solve<NLM>(nl, s, cosb, bk, rsf, btop, bsurf, accf);
   }
 }
 
