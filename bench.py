@@ -167,6 +167,30 @@ def form_mixed(args, rt_unused, stream):
     return out
 
 
+def form_real(args, rt_unused, stream):
+    """The plain FP64 path on the reference's own k-distribution: the records of
+    run/data/CO2H2O_V_12_95_INTEL and CO2H2O_IR_12_95_INTEL (carried as data by
+    tests/golden/column_real24.npz) written back to files, read by the library,
+    refined by the device LAGINTERP; the same seeded columns as the headline."""
+    import tempfile
+
+    from mgcm_rt import FP64, Radiation, alloc_outputs, realdata, synth, to_device
+    f = np.load(os.path.join(ROOT, "tests", "golden", "column_real24.npz"))
+    with tempfile.TemporaryDirectory() as d:
+        realdata.write_ktable(os.path.join(d, realdata.VIS_FILE), f["raw_v"], f["fzero_v"])
+        realdata.write_ktable(os.path.join(d, realdata.IR_FILE), f["raw_i"], f["fzero_i"])
+        t = realdata.load_tables(d, args.nlay)
+    c = synth.make_columns(args.seed, args.ncol, args.nlay)
+    rt = Radiation(t)
+    dc = to_device(c)
+    do = to_device(alloc_outputs(args.ncol, args.nlay))
+    ms, _ = timed_steps(lambda: rt.run_device(dc, do, FP64, stream.cuda_stream), args.steps, args.warmup, stream)
+    rt.check()
+    return {"workload": f"{args.ncol} columns x {args.nlay} layers, the reference's k-distribution "
+                        "(CO2H2O_*_12_95_INTEL via LAGINTERP)",
+            "value": args.ncol / (ms / 1e3), "unit": "columns/s", "ms_per_step": ms}
+
+
 def form_batched(args, rt, stream):
     """512 grids, U{2000..20000} columns, U{16..64} layers, in one
     mgcm_rt_run_batched call (plan, tuning and graph capture on the first call,
@@ -349,7 +373,8 @@ def main():
     forms = {}
     if not args.no_forms:
         del do
-        for name, fn in (("mixed", form_mixed), ("batched", form_batched), ("surrogate", form_surrogate)):
+        for name, fn in (("real_tables", form_real), ("mixed", form_mixed), ("batched", form_batched),
+                         ("surrogate", form_surrogate)):
             try:
                 forms[name] = fn(args, rt, stream)
             except Exception as e:  # noqa: BLE001

@@ -10,7 +10,8 @@
  *     (code/comp3.f:731-733), with the net-flux differencing of
  *     code/comp3.f:677-681,735-738 turned into layer heating rates;
  *   - the table set-up of RADSETUP (code/main.f:211): SETSPI's Planck table
- *     (code/setspi.f90:91-105) and LAGINTERP's 11 -> 51 pressure refinement
+ *     (code/setspi.f90:91-105), LAGINTERP's reads of the k-coefficient files
+ *     (code/laginterp.f90:59-69) and its 11 -> 51 pressure refinement
  *     (code/laginterp.f90:30-236).
  *
  * Arrays use the reference's Fortran layouts (first index fastest), one column
@@ -42,7 +43,8 @@ enum {
   MGCM_RT_ERR_DEVICE = -3,   /* HIP runtime failure */
   MGCM_RT_ERR_NOMEM = -4,
   MGCM_RT_ERR_RANGE = -5,    /* a temperature outside the Planck table (50.0..900.0 K) or a NaN input */
-  MGCM_RT_ERR_PRECISION = -6 /* unknown precision mode */
+  MGCM_RT_ERR_PRECISION = -6, /* unknown precision mode */
+  MGCM_RT_ERR_IO = -7         /* data file missing, unreadable or not in the expected record layout */
 };
 
 enum {
@@ -138,14 +140,26 @@ int mgcm_rt_check(mgcm_rt_ctx *ctx);
 int mgcm_rt_run_batched(mgcm_rt_ctx *ctx, int ngrids, const mgcm_rt_columns *grids,
                         const mgcm_rt_fluxes *outs, int precision, void *stream);
 
-/* Init-time table builders on the device (host in/out pointers):
- * SETSPI's Planck table PLANCKIR(5,8501) (code/setspi.f90:91-105);
+/* Init-time table builders (host in/out pointers), both equal to the
+ * reference's tables bit for bit:
+ * SETSPI's Planck table PLANCKIR(5,8501) (code/setspi.f90:91-105), on the device;
  * LAGINTERP (code/laginterp.f90): raw tables (7,11,10,nb,17) on the 11
  * reference pressures pgref[11] (mbar) -> (7,51,10,nb,17) on log10 p =
- * -6.0..4.0 step 0.2, returned in pint[51]. */
+ * -6.0..4.0 step 0.2, returned in pint[51].  The interpolations run on the
+ * device; log10 of the raw k and 10**ans use the host C library, as the
+ * reference's do. */
 int mgcm_rt_planck_table(double *planckir, double *dwni, double *wnoi);
 int mgcm_rt_laginterp(const double *pgref, const double *raw, int nbands, double *pint,
                       double *out);
+
+/* Replaces LAGINTERP's reads of the k-coefficient files (code/laginterp.f90:
+ * 59-69, data/CO2H2O_V_12_95_INTEL with nbands = 7, data/CO2H2O_IR_12_95_INTEL
+ * with nbands = 5): Fortran sequential unformatted records with 4-byte
+ * little-endian length markers.  raw receives CO2(7,11,10,nbands,17), the
+ * input of mgcm_rt_laginterp; fzero receives FZERO(nbands).  Host code, no
+ * device needed.  MGCM_RT_ERR_IO if the file is missing or its records are
+ * short, unframed or truncated. */
+int mgcm_rt_read_ktable(const char *path, int nbands, double *raw, double *fzero);
 
 /* Diagnostic: the device's exp, log10 and sqrt on n host arguments (x > 0 for
  * log10, x >= 0 for sqrt; other entries return 0).  The FP64 path reproduces the

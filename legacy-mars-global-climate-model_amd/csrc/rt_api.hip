@@ -24,6 +24,7 @@ extern "C" const char *mgcm_rt_strerror(int s) {
     case MGCM_RT_ERR_NOMEM: return "out of device memory";
     case MGCM_RT_ERR_RANGE: return "temperature outside the Planck table (50.0..900.0 K)";
     case MGCM_RT_ERR_PRECISION: return "unknown precision mode";
+    case MGCM_RT_ERR_IO: return "data file missing, unreadable or not in the expected record layout";
     default: return "unknown status";
   }
 }
@@ -320,7 +321,7 @@ __global__ void k_planck(double *planck, const double *bwni, const double *dwni)
   double ans = 0.0;
   for (int m = 0; m < 12; m++) {
     const double y = bma * x[m] + bpa;
-    const double y5 = y * y * (y * y) * y;
+    const double y5 = y * y * y * y * y;  // Y**5 as the reference build evaluates it: products left to right
     ans = ans + w[m] * c1 / (y5 * (mgcm_libm::mgcm_exp(c2 / (y * tt)) - 1.0));
   }
   planck[(size_t)n * NI + nw] = ans * bma / (PI * dwni[nw]);
@@ -351,8 +352,13 @@ extern "C" int mgcm_rt_planck_table(double *planckir, double *dwni, double *wnoi
   return st;
 }
 
-// LAGINTERP (code/laginterp.f90:74-236) + LAGRANGE (code/lagrange.f90:17-31):
-// one thread per (T, H2O, band, g) table row of 11 pressures -> 51.
+// LAGINTERP (code/laginterp.f90:74-236) + LAGRANGE (code/lagrange.f90:17-31).
+// The 4-point interpolations in log10 p run on the device, one thread per
+// (T, H2O, band, g) table row of 11 pressures -> 51.  The two libm steps
+// around them -- log10 of the raw k (floor -200) and 10**ans -- use the host C
+// library, as the reference does: the refined table then equals the reference's
+// bit for bit.  That matters: on the reference's k-distribution a one-ulp
+// change of a k-coefficient moves the IR fluxes of thin layers by ~1e-9.
 __device__ double lagrange4(double x, const double *xi, const double *yi) {
   const double fm1 = x - xi[0], fm2 = x - xi[1], fm3 = x - xi[2], fm4 = x - xi[3];
   return fm2 * fm3 * fm4 * yi[0] / ((xi[0] - xi[1]) * (xi[0] - xi[2]) * (xi[0] - xi[3])) +
@@ -361,39 +367,39 @@ __device__ double lagrange4(double x, const double *xi, const double *yi) {
          fm1 * fm2 * fm3 * yi[3] / ((xi[3] - xi[0]) * (xi[3] - xi[1]) * (xi[3] - xi[2]));
 }
 
-__global__ void k_laginterp(const double *raw, const double *pref, const double *pint, int nb, double *out) {
+// lg: log10 of the raw table (7,11,10,nb,17); ans: log10 k on (7,51,10,nb,17).
+__global__ void k_laginterp(const double *lg, const double *pref, const double *pint, int nb, double *ans) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;  // (t, h, nw, g) with t fastest
   const int nrow = NT * NH * nb * NG;
   if (i >= nrow) return;
   const int t = i % NT, h = (i / NT) % NH, nw = (i / (NT * NH)) % nb, g = i / (NT * NH * nb);
   const int npref = 11;
-  auto RAW = [&](int p) {  // log10 with floor -200 (code/laginterp.f90:74-98)
-    const double v = raw[(size_t)t + NT * ((size_t)p + npref * ((size_t)h + NH * ((size_t)nw + (size_t)nb * g)))];
-    return (v > 1.0e-200) ? mgcm_libm::mgcm_log10(v) : -200.0;
+  auto RAW = [&](int p) {
+    return lg[(size_t)t + NT * ((size_t)p + npref * ((size_t)h + NH * ((size_t)nw + (size_t)nb * g)))];
   };
   auto OUT = [&](int p) -> double & {
-    return out[(size_t)t + NT * ((size_t)p + NP * ((size_t)h + NH * ((size_t)nw + (size_t)nb * g)))];
+    return ans[(size_t)t + NT * ((size_t)p + NP * ((size_t)h + NH * ((size_t)nw + (size_t)nb * g)))];
   };
   double xi[4], yi[4];
   for (int m = 0; m < 5; m++) {  // first interval, one-sided stencil
     for (int j = 0; j < 4; j++) { xi[j] = pref[j]; yi[j] = RAW(j); }
-    OUT(m) = pow(10.0, lagrange4(pint[m], xi, yi));
+    OUT(m) = lagrange4(pint[m], xi, yi);
   }
   for (int n = 2; n <= npref - 2; n++)  // interior (1-based n)
     for (int m = 1; m <= 5; m++) {
       const int ii = (n - 1) * 5 + m;
       for (int j = 0; j < 4; j++) { xi[j] = pref[n - 2 + j]; yi[j] = RAW(n - 2 + j); }
-      OUT(ii - 1) = pow(10.0, lagrange4(pint[ii - 1], xi, yi));
+      OUT(ii - 1) = lagrange4(pint[ii - 1], xi, yi);
     }
   {
     const int n = npref - 1;  // last interval
     for (int m = 1; m <= 5; m++) {
       const int ii = (n - 1) * 5 + m;
       for (int j = 0; j < 4; j++) { xi[j] = pref[n - 3 + j]; yi[j] = RAW(n - 3 + j); }
-      OUT(ii - 1) = pow(10.0, lagrange4(pint[ii - 1], xi, yi));
+      OUT(ii - 1) = lagrange4(pint[ii - 1], xi, yi);
     }
   }
-  OUT(NP - 1) = pow(10.0, RAW(npref - 1));
+  OUT(NP - 1) = RAW(npref - 1);  // the last point is copied (code/laginterp.f90:162,231)
 }
 
 extern "C" int mgcm_rt_laginterp(const double *pgref, const double *raw, int nbands, double *pint, double *out) {
@@ -408,27 +414,26 @@ extern "C" int mgcm_rt_laginterp(const double *pgref, const double *raw, int nba
   double pref[11];
   for (int n = 0; n < 11; n++) pref[n] = log10(pgref[n]);
   const size_t nraw = (size_t)NT * 11 * NH * nbands * NG, nout = (size_t)NT * NP * NH * nbands * NG;
+  std::vector<double> lg(nraw);
+  for (size_t q = 0; q < nraw; q++) lg[q] = (raw[q] > 1.0e-200) ? log10(raw[q]) : -200.0;  // laginterp.f90:74-98
   double *d = nullptr;
-  HIP_TRY(hipMalloc(&d, (nraw + nout + 64) * sizeof(double)));
-  double *draw = d, *dout = d + nraw, *dpref = d + nraw + nout, *dpint = dpref;
+  HIP_TRY(hipMalloc(&d, (nraw + nout + 11 + NP) * sizeof(double)));
+  double *dlg = d, *dans = d + nraw, *dpref = d + nraw + nout, *dpin = dpref + 11;
   int st = MGCM_RT_OK;
-  double *dpin = nullptr;
-  if (hipMalloc(&dpin, NP * sizeof(double)) != hipSuccess) st = MGCM_RT_ERR_NOMEM;
-  if (st == MGCM_RT_OK &&
-      (hipMemcpy(draw, raw, nraw * 8, hipMemcpyHostToDevice) != hipSuccess ||
-       hipMemcpy(dpref, pref, 11 * 8, hipMemcpyHostToDevice) != hipSuccess ||
-       hipMemcpy(dpin, pin, NP * 8, hipMemcpyHostToDevice) != hipSuccess))
+  if (hipMemcpy(dlg, lg.data(), nraw * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dpref, pref, 11 * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dpin, pin, NP * 8, hipMemcpyHostToDevice) != hipSuccess)
     st = MGCM_RT_ERR_DEVICE;
-  (void)dpint;
   if (st == MGCM_RT_OK) {
     const int nrow = NT * NH * nbands * NG;
-    hipLaunchKernelGGL(k_laginterp, dim3((nrow + 127) / 128), dim3(128), 0, nullptr, draw, dpref, dpin, nbands, dout);
+    hipLaunchKernelGGL(k_laginterp, dim3((nrow + 127) / 128), dim3(128), 0, nullptr, dlg, dpref, dpin, nbands, dans);
     if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
-        hipMemcpy(out, dout, nout * 8, hipMemcpyDeviceToHost) != hipSuccess)
+        hipMemcpy(out, dans, nout * 8, hipMemcpyDeviceToHost) != hipSuccess)
       st = MGCM_RT_ERR_DEVICE;
   }
-  if (dpin) (void)hipFree(dpin);
   (void)hipFree(d);
+  if (st == MGCM_RT_OK)
+    for (size_t q = 0; q < nout; q++) out[q] = pow(10.0, out[q]);  // 10.0**ans
   return st;
 }
 

@@ -34,7 +34,8 @@ SKIP_VIS, SKIP_IR = 0x10, 0x20  # OR-ed into the precision argument: one part on
 DP = C.POINTER(C.c_double)
 
 STATUS = {0: "ok", -1: "invalid argument", -2: "layer count outside 1..64", -3: "HIP runtime error",
-          -4: "out of device memory", -5: "temperature outside the Planck table", -6: "unknown precision"}
+          -4: "out of device memory", -5: "temperature outside the Planck table", -6: "unknown precision",
+          -7: "data file missing, unreadable or not in the expected record layout"}
 
 
 class RadiationError(RuntimeError):
@@ -100,6 +101,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         lib.mgcm_rt_check.argtypes = [C.c_void_p]
         lib.mgcm_rt_planck_table.argtypes = [DP, DP, DP]
         lib.mgcm_rt_laginterp.argtypes = [DP, DP, C.c_int, DP, DP]
+        lib.mgcm_rt_read_ktable.argtypes = [C.c_char_p, C.c_int, DP, DP]
         lib.mgcm_rt_libm_check.argtypes = [DP, C.c_int, DP, DP, DP]
         lib.mgcm_rt_strerror.restype = C.c_char_p
         lib.mgcm_rt_strerror.argtypes = [C.c_int]
@@ -236,6 +238,14 @@ def laginterp(pgref11, raw) -> tuple:
     pint = np.zeros(51)
     _check(lib.mgcm_rt_laginterp(_dp(_f64(pgref11)), _dp(raw), nb, _dp(pint), _dp(out)), "mgcm_rt_laginterp")
     return pint, out
+
+
+def read_ktable(path: str, nbands: int) -> tuple:
+    """A CO2H2O_*_INTEL k-coefficient file (LAGINTERP's reads): raw (17, nb, 10, 11, 7), FZERO[nb]."""
+    lib = load_library()
+    raw, fz = np.zeros((NG, nbands, 10, 11, 7)), np.zeros(nbands)
+    _check(lib.mgcm_rt_read_ktable(os.fsencode(path), int(nbands), _dp(raw), _dp(fz)), "mgcm_rt_read_ktable")
+    return raw, fz
 
 
 def libm_check(x) -> tuple:

@@ -31,6 +31,9 @@ Generator decisions (the workload statement leaves them open):
 """
 from __future__ import annotations
 
+import functools
+import math
+
 import numpy as np
 
 NT, NP, NH, NG, NV, NI = 7, 51, 10, 17, 7, 5
@@ -91,33 +94,46 @@ def setspv():
     return wnov, dwnv, solar, tauray
 
 
-def setspi():
-    """IR band centres/widths and the Planck table PLANCKIR(5,8501) (code/setspi.f90:48-105).
-
-    Returns planckir in Fortran layout as a C array of shape (8501, 5)."""
+@functools.lru_cache(maxsize=1)
+def _planck_table() -> tuple:
+    """SETSPI's band edges and PLANCKIR(5,8501) as the reference computes them
+    (code/setspi.f90:48-105): the C library's exp (math.exp) and Y**5 as left-to-right
+    products, so every entry equals the reference build's bit for bit."""
     bwni = np.array([10.000, 166.667, 416.667, 833.333, 1250.000, 2222.222])
-    x = np.array([-0.981560634246719, -0.904117256370475, -0.769902674194305, -0.587317954286617,
-                  -0.367831498998180, -0.125233408511469, 0.125233408511469, 0.367831498998180,
-                  0.587317954286617, 0.769902674194305, 0.904117256370475, 0.981560634246719])
-    w = np.array([0.047175336386512, 0.106939325995318, 0.160078328543346, 0.203167426723066,
-                  0.233492536538355, 0.249147045813403, 0.249147045813403, 0.233492536538355,
-                  0.203167426723066, 0.160078328543346, 0.106939325995318, 0.047175336386512])
+    x = (-0.981560634246719, -0.904117256370475, -0.769902674194305, -0.587317954286617,
+         -0.367831498998180, -0.125233408511469, 0.125233408511469, 0.367831498998180,
+         0.587317954286617, 0.769902674194305, 0.904117256370475, 0.981560634246719)
+    w = (0.047175336386512, 0.106939325995318, 0.160078328543346, 0.203167426723066,
+         0.233492536538355, 0.249147045813403, 0.249147045813403, 0.233492536538355,
+         0.203167426723066, 0.160078328543346, 0.106939325995318, 0.047175336386512)
     c1, c2 = 3.741832e-16, 1.438786e-2
     wnoi = 0.5 * (bwni[1:] + bwni[:-1])
     dwni = bwni[1:] - bwni[:-1]
     planck = np.empty((NPLANCK, NI))
-    t = np.arange(500, 9001, dtype=np.float64) / 1.0e1
+    exp = math.exp
     for nw in range(NI):
-        a = 1.0e-2 / bwni[nw + 1]
-        b = 1.0e-2 / bwni[nw]
+        a = 1.0e-2 / float(bwni[nw + 1])
+        b = 1.0e-2 / float(bwni[nw])
         bpa, bma = (b + a) / 2.0, (b - a) / 2.0
-        ans = np.zeros_like(t)
-        for m in range(12):
-            y = bma * x[m] + bpa
-            y5 = (y * y) * (y * y) * y
-            ans = ans + w[m] * c1 / (y5 * (np.exp(c2 / (y * t)) - 1.0))
-        planck[:, nw] = ans * bma / (np.pi * dwni[nw])
+        ys = [bma * xm + bpa for xm in x]
+        y5s = [y * y * y * y * y for y in ys]
+        den = math.pi * float(dwni[nw])
+        col = planck[:, nw]
+        for i in range(NPLANCK):
+            t = (i + 500) / 1.0e1
+            ans = 0.0
+            for m in range(12):
+                ans = ans + w[m] * c1 / (y5s[m] * (exp(c2 / (ys[m] * t)) - 1.0))
+            col[i] = ans * bma / den
     return wnoi, dwni, planck
+
+
+def setspi():
+    """IR band centres/widths and the Planck table PLANCKIR(5,8501) (code/setspi.f90:48-105).
+
+    Returns planckir in Fortran layout as a C array of shape (8501, 5)."""
+    wnoi, dwni, planck = _planck_table()
+    return wnoi.copy(), dwni.copy(), planck.copy()
 
 
 def sigma_levels(nlay: int) -> np.ndarray:
@@ -141,25 +157,35 @@ def sigma_levels(nlay: int) -> np.ndarray:
 def make_tables(seed: int, nlay: int, weights: str = "dirichlet") -> dict:
     """k-tables, grids, Gauss weights and aerosol constants for one workload."""
     rng = np.random.Generator(np.random.PCG64(seed))
-    t = {}
-    t["nlay"] = nlay
-    t["tgasref"] = np.arange(50.0, 351.0, 50.0)
-    t["pfgasref"] = np.linspace(-6.0, 5.0, NP)
-    t["wrefh2o"] = np.logspace(-7.0, -2.0, NH)
+    ks = {}
     for name, nb in (("co2v", NV), ("co2i", NI)):
         u = rng.uniform(-8.0, 2.0, size=(NG, nb, NH, NP, NT))
         k = np.power(10.0, u) / CMK
         k[NG - 1] = 0.0
-        t[name] = np.ascontiguousarray(k)
+        ks[name] = np.ascontiguousarray(k)
+    gws = {}
     for name, nb in (("gwv", NV), ("gwi", NI)):
         gw = np.zeros((nb, NG))
         if weights == "reference":
             gw[:] = GWEIGHT_REF
         else:
             gw[:, :16] = rng.dirichlet(np.ones(16), size=nb)
-        t[name] = gw
+        gws[name] = gw
+    t = base_tables(nlay, gws["gwv"], gws["gwi"])
+    t.update(ks)
+    t["tgasref"] = np.arange(50.0, 351.0, 50.0)
+    t["pfgasref"] = np.linspace(-6.0, 5.0, NP)
+    t["wrefh2o"] = np.logspace(-7.0, -2.0, NH)
     t["fzerov"] = np.zeros(NV)
     t["fzeroi"] = np.zeros(NI)
+    return t
+
+
+def base_tables(nlay: int, gwv: np.ndarray, gwi: np.ndarray) -> dict:
+    """The table entries that do not come from a k-distribution: Gauss weights,
+    SETSPV's solar fluxes and Rayleigh tau, SETSPI's DWNI and Planck table, and
+    the per-level aerosol constants (dust: code/setrad.f90:73-154)."""
+    t = {"nlay": nlay, "gwv": np.asarray(gwv, dtype=np.float64), "gwi": np.asarray(gwi, dtype=np.float64)}
     _, _, solarf, tauray = setspv()
     t["solarf"] = solarf
     t["tauray"] = tauray

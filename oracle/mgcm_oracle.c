@@ -954,7 +954,7 @@ void or_planck_table(double *planckir, double *dwni, double *wnoi)
       double ans = 0.0;
       for (m = 0; m < 12; m++) {
         const double y = bma * x[m] + bpa;
-        const double y2 = y * y, y5 = y2 * y2 * y;
+        const double y5 = y * y * y * y * y; /* Y**5 as the amdflang build evaluates it: left to right */
         ans = ans + wq[m] * c1 / (y5 * (exp(c2 / (y * tt)) - 1.0));
       }
       PLANCK(planckir, nw, nt - 499) = ans * bma / (PI * dwni[nw - 1]);

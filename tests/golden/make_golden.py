@@ -19,9 +19,16 @@ Fixtures (all data: inputs and the reference's outputs):
                       (TAURAY, SOLARF), SETSPI (PLANCKIR), SETRAD (TGASREF, PFGASREF) and
                       a sample of LAGINTERP rows with their raw 11-pressure inputs.
   solver_ref.npz      DSOLVER/DTRIDGL and TPINDEX called directly on seeded inputs.
+  column_real24.npz   the reference's own k-distribution: the raw 11-pressure records and
+                      FZERO of run/data/CO2H2O_V_12_95_INTEL and CO2H2O_IR_12_95_INTEL (data
+                      files, read as data), and 32 seeded columns (24 layers) run through the
+                      reference's LAGINTERP/SETSPV/SETSPI/SETRAD tables and its OPTCV/SFLUXV/
+                      OPTCI/SFLUXI with the single GWEIGHT set.
 
-k-tables are not stored: they are regenerated from the seed by mgcm_rt.synth and checked
-against the stored SHA-256 of their bytes.
+Synthetic k-tables are not stored: they are regenerated from the seed by mgcm_rt.synth and
+checked against the stored SHA-256 of their bytes.
+
+    python tests/golden/make_golden.py [fixture names]   # default: all
 """
 from __future__ import annotations
 
@@ -74,9 +81,13 @@ def edge_columns(seed: int, t: dict) -> dict:
     return c
 
 
-def main() -> None:
+def main(only=None) -> None:
     ref = Reference()
     assert ref.nlay == 24
+    if only:
+        if "column_real24" in only:
+            real_columns(ref)
+        return
     out = {}
     for name, seed, weights in (("column_ref24", 101, "reference"), ("column_dir24", 102, "dirichlet")):
         t, c = synth.make_workload(seed, 32, 24, weights=weights)
@@ -143,18 +154,51 @@ def main() -> None:
         tp.append(np.concatenate([[pw, tw, q], lc, [mt, mp, nh, wr]]))
     np.savez_compressed(os.path.join(HERE, "solver_ref.npz"), dsolver=np.array(cases), dsolver_nl=nl,
                         tpindex=np.array(tp))
+    real_columns(ref)
     print("golden fixtures written to", HERE)
 
 
-def _read_ktable(path: str, nb: int) -> np.ndarray:
-    """Sequential unformatted file: [4-byte marker][7*11*10*nb*17 FP64][marker]... (LAGINTERP's reads)."""
+def real_columns(ref) -> None:
+    """column_real24.npz: the reference's k-distribution and its outputs on 32 columns."""
+    tab = ref.tables(REF_RUN)
+    raw_v, fzv = _read_ktable(os.path.join(REF_RUN, "data", "CO2H2O_V_12_95_INTEL"), 7, with_fzero=True)
+    raw_i, fzi = _read_ktable(os.path.join(REF_RUN, "data", "CO2H2O_IR_12_95_INTEL"), 5, with_fzero=True)
+    assert np.array_equal(fzv, tab["fzerov"]) and np.array_equal(fzi, tab["fzeroi"])
+    gw = synth.GWEIGHT_REF
+    t = synth.base_tables(24, np.tile(gw, (7, 1)), np.tile(gw, (5, 1)))
+    for k in ("co2v", "co2i", "pfgasref", "tgasref", "fzerov", "fzeroi", "planckir", "dwni", "solarf", "tauray"):
+        t[k] = tab[k]
+    t["sol"] = tab["solarf"] / synth.RSDIST
+    # WREFH2O is a PARAMETER of radcommon_h (code/modules.f90:353-355) that OPTCV/OPTCI take as an argument
+    t["wrefh2o"] = np.array([1.0e-7, 1.0e-6, 1.0e-5, 1.0e-4, 1.0e-3, 1.0e-2, 1.0e-1, 2.0e-1, 3.0e-1, 4.0e-1])
+    seed = 106
+    c = synth.make_columns(seed, 32, 24)
+    o = ref.run(t, c)
+    d = {"seed": seed, "nlay": 24, "ncol": 32, "weights": "reference", "raw_v": raw_v, "fzero_v": fzv,
+         "raw_i": raw_i, "fzero_i": fzi, "wrefh2o": t["wrefh2o"], "solarf": tab["solarf"],
+         "tauray": tab["tauray"]}
+    for k in IN_KEYS:
+        d["in_" + k] = c[k]
+    for k, v in o.items():
+        d["out_" + k] = v
+    np.savez_compressed(os.path.join(HERE, "column_real24.npz"), **d)
+
+
+def _read_ktable(path: str, nb: int, with_fzero: bool = False):
+    """Sequential unformatted file: [4-byte marker][7*11*10*nb*17 FP64][marker], then
+    [marker][FZERO(nb) FP64][marker] (LAGINTERP's reads)."""
     raw = np.fromfile(path, dtype=np.uint8)
     n = 7 * 11 * 10 * nb * 17
     rec = int(np.frombuffer(raw[:4].tobytes(), dtype="<i4")[0])
     assert rec == 8 * n, (rec, n)
-    vals = np.frombuffer(raw[4:4 + 8 * n].tobytes(), dtype="<f8")
-    return vals.reshape(17, nb, 10, 11, 7)
+    vals = np.frombuffer(raw[4:4 + 8 * n].tobytes(), dtype="<f8").reshape(17, nb, 10, 11, 7)
+    if not with_fzero:
+        return vals
+    o = 4 + 8 * n + 4
+    rec2 = int(np.frombuffer(raw[o:o + 4].tobytes(), dtype="<i4")[0])
+    assert rec2 == 8 * nb, (rec2, nb)
+    return vals, np.frombuffer(raw[o + 4:o + 4 + 8 * nb].tobytes(), dtype="<f8").copy()
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

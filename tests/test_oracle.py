@@ -261,7 +261,9 @@ def test_beer_lambert_direct_beam(oracle):
 def test_planck_table_matches_reference_and_stefan_boltzmann(oracle):
     f = np.load(os.path.join(GOLDEN, "tables_ref.npz"))
     p, dwni, wnoi = oracle.planck_table()
-    assert rel_err(p, f["planckir"]) < 1e-14
+    # bit for bit: the IR path on the reference's k-distribution moves its fluxes by ~1e-8
+    # for a one-ulp change of PLANCKIR (thin layers: B1 = dB/DTAU through TERM)
+    assert np.array_equal(p, f["planckir"]), int(np.sum(p != f["planckir"]))
     assert np.array_equal(dwni, f["dwni"])
     # pi * sum_nw B*DWNI approaches sigma T^4 at temperatures whose emission lies in 10-2222 cm^-1
     T = 250.0
