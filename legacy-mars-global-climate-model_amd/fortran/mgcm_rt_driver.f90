@@ -3,7 +3,12 @@
 ! tables up once, runs the batch through mgcm_rt_radiation and writes the
 ! results to a second stream file.
 !
-!   mgcm_rt_driver <case file> <result file>
+!   mgcm_rt_driver <case file> <result file> [data directory]
+!
+! With a data directory, the k-tables are set up as RADSETUP does on the GCM's
+! start (code/setrad.f90:112-122, code/laginterp.f90): CO2H2O_V_12_95_INTEL and
+! CO2H2O_IR_12_95_INTEL are read and refined to the 51 PINT points, and CO2V,
+! CO2I, PFGASREF, FZEROV and FZEROI of the case file are replaced.
 !
 ! Case file (native-endian stream): int32 ncol, nlay, precision; then FP64
 ! CO2V(7,51,10,7,17), CO2I(7,51,10,5,17), PFGASREF(51), TGASREF(7),
@@ -19,7 +24,10 @@ program mgcm_rt_driver
   use iso_c_binding
   use mgcm_rt_mod
   implicit none
-  character(len=4096) :: fin, fout
+  character(len=4096) :: fin, fout, datadir
+  ! PGASREF: REAL(4) literals, as SETRAD writes them (code/setrad.f90:112-122)
+  real(c_double), parameter :: pgref(11) = [1.0e-6, 1.0e-5, 1.0e-4, 1.0e-3, 1.0e-2, 1.0e-1, 1.0, 1.0e1, &
+                                            1.0e2, 1.0e3, 1.0e4]
   integer :: u, ncol, nlay, nlev, nl, ios
   integer(c_int) :: precision, status, s2
   type(c_ptr) :: ctx
@@ -39,12 +47,14 @@ program mgcm_rt_driver
   real(c_double), allocatable :: fluxdni(:,:), nfluxtopv(:), nfluxtopi(:), diffvt(:)
   real(c_double), allocatable :: detau(:,:,:), cumtauv(:,:,:), hr(:,:)
 
-  if (command_argument_count() /= 2) then
-    write(*, '(a)') 'usage: mgcm_rt_driver <case file> <result file>'
+  if (command_argument_count() /= 2 .and. command_argument_count() /= 3) then
+    write(*, '(a)') 'usage: mgcm_rt_driver <case file> <result file> [data directory]'
     stop 2
   end if
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
+  datadir = ' '
+  if (command_argument_count() == 3) call get_command_argument(3, datadir)
 
   open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old', iostat=ios)
   if (ios /= 0) then
@@ -71,9 +81,12 @@ program mgcm_rt_driver
   detau = 0.0d0
   cumtauv = 0.0d0
 
-  call mgcm_rt_setup(co2v, co2i, pfgasref, tgasref, wrefh2o, gwv, gwi, fzerov, fzeroi, tauray, dwni, &
-                     planckir, qxvdst, qsvdst, gvdst, qxidst, qsidst, gidst, qxvcld, qsvcld, gvcld, &
-                     qxicld, qsicld, gicld, qextrefdst, qextrefcld, ctx, status)
+  status = MGCM_RT_OK
+  if (len_trim(datadir) > 0) call mgcm_rt_laginterp_files(datadir, pgref, pfgasref, co2i, co2v, fzeroi, fzerov, status)
+  if (status == MGCM_RT_OK) &
+    call mgcm_rt_setup(co2v, co2i, pfgasref, tgasref, wrefh2o, gwv, gwi, fzerov, fzeroi, tauray, dwni, &
+                       planckir, qxvdst, qsvdst, gvdst, qxidst, qsidst, gidst, qxvcld, qsvcld, gvcld, &
+                       qxicld, qsicld, gicld, qextrefdst, qextrefcld, ctx, status)
   if (status == MGCM_RT_OK) then
     call mgcm_rt_radiation(ctx, ncol, nlay, plev, tlev, pmid, tmid, qh2o, tauref, taurefcld, als, albi, &
                            acosz, sol, fmnetv, fluxupv, fluxdnv, fmneti, fluxupi, fluxdni, nfluxtopv, &

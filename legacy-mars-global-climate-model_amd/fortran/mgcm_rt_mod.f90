@@ -16,7 +16,7 @@ module mgcm_rt_mod
   integer, parameter, public :: MGCM_RT_NV = 7, MGCM_RT_NI = 5, MGCM_RT_NPLANCK = 8501, MGCM_RT_MAX_NLAY = 64
   integer(c_int), parameter, public :: MGCM_RT_OK = 0, MGCM_RT_ERR_ARG = -1, MGCM_RT_ERR_NLAY = -2
   integer(c_int), parameter, public :: MGCM_RT_ERR_DEVICE = -3, MGCM_RT_ERR_NOMEM = -4
-  integer(c_int), parameter, public :: MGCM_RT_ERR_RANGE = -5, MGCM_RT_ERR_PRECISION = -6
+  integer(c_int), parameter, public :: MGCM_RT_ERR_RANGE = -5, MGCM_RT_ERR_PRECISION = -6, MGCM_RT_ERR_IO = -7
   integer(c_int), parameter, public :: MGCM_RT_FP64 = 0, MGCM_RT_MIXED = 1
 
   ! mirrors mgcm_rt_tables (host pointers, copied by mgcm_rt_init)
@@ -41,8 +41,8 @@ module mgcm_rt_mod
   end type
 
   public :: mgcm_rt_init, mgcm_rt_destroy, mgcm_rt_run, mgcm_rt_run_host, mgcm_rt_check
-  public :: mgcm_rt_run_batched, mgcm_rt_planck_table, mgcm_rt_laginterp, mgcm_rt_strerror
-  public :: mgcm_rt_setup, mgcm_rt_radiation, mgcm_rt_message
+  public :: mgcm_rt_run_batched, mgcm_rt_planck_table, mgcm_rt_laginterp, mgcm_rt_read_ktable, mgcm_rt_strerror
+  public :: mgcm_rt_setup, mgcm_rt_radiation, mgcm_rt_message, mgcm_rt_laginterp_files
 
   interface
     integer(c_int) function mgcm_rt_init(tables, ctx) bind(C, name="mgcm_rt_init")
@@ -93,6 +93,12 @@ module mgcm_rt_mod
       integer(c_int), value :: nbands
       real(c_double), intent(out) :: pint(*), out(*)
     end function
+    integer(c_int) function mgcm_rt_read_ktable(path, nbands, raw, fzero) bind(C, name="mgcm_rt_read_ktable")
+      import :: c_int, c_double, c_char
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: nbands
+      real(c_double), intent(out) :: raw(*), fzero(*)
+    end function
     type(c_ptr) function mgcm_rt_strerror(status) bind(C, name="mgcm_rt_strerror")
       import :: c_int, c_ptr
       integer(c_int), value :: status
@@ -114,6 +120,31 @@ contains
       msg(i:i) = s(i)
     end do
   end function
+
+  ! LAGINTERP (code/laginterp.f90:1-239) with its own arguments, from the data
+  ! directory that holds CO2H2O_V_12_95_INTEL and CO2H2O_IR_12_95_INTEL: the
+  ! library reads the files and refines the tables from the 11 reference
+  ! pressures pgref (mbar) to the 51 PINT points.  status: MGCM_RT_OK or the
+  ! library's error code (MGCM_RT_ERR_IO for a missing or malformed file).
+  subroutine mgcm_rt_laginterp_files(datadir, pgref, pint, co2i, co2v, fzeroi, fzerov, status)
+    character(len=*), intent(in) :: datadir
+    real(c_double), intent(in) :: pgref(11)
+    real(c_double), intent(out) :: pint(MGCM_RT_NP)
+    real(c_double), intent(out) :: co2i(MGCM_RT_NT, MGCM_RT_NP, MGCM_RT_NH, MGCM_RT_NI, MGCM_RT_NG)
+    real(c_double), intent(out) :: co2v(MGCM_RT_NT, MGCM_RT_NP, MGCM_RT_NH, MGCM_RT_NV, MGCM_RT_NG)
+    real(c_double), intent(out) :: fzeroi(MGCM_RT_NI), fzerov(MGCM_RT_NV)
+    integer(c_int), intent(out) :: status
+    real(c_double), allocatable :: raw(:)
+    allocate(raw(MGCM_RT_NT * 11 * MGCM_RT_NH * MGCM_RT_NV * MGCM_RT_NG))
+    status = mgcm_rt_read_ktable(trim(datadir)//'/CO2H2O_V_12_95_INTEL'//c_null_char, int(MGCM_RT_NV, c_int), &
+                                 raw, fzerov)
+    if (status == MGCM_RT_OK) status = mgcm_rt_laginterp(pgref, raw, int(MGCM_RT_NV, c_int), pint, co2v)
+    if (status == MGCM_RT_OK) &
+      status = mgcm_rt_read_ktable(trim(datadir)//'/CO2H2O_IR_12_95_INTEL'//c_null_char, int(MGCM_RT_NI, c_int), &
+                                   raw, fzeroi)
+    if (status == MGCM_RT_OK) status = mgcm_rt_laginterp(pgref, raw, int(MGCM_RT_NI, c_int), pint, co2i)
+    deallocate(raw)
+  end subroutine
 
   ! RADSETUP's tables (code/radsetup.f90:26-46) into a device context.  Arrays
   ! in the reference's layouts; the aerosol constants per band.

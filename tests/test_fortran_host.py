@@ -71,3 +71,47 @@ def test_fortran_driver_reports_bad_layer_count(tmp_path):
     status, _ = casefile.read_result(res, 2, 70)
     assert status == -2
     assert "layer count" in r.stdout
+
+
+def test_fortran_driver_reports_missing_data_files(tmp_path):
+    """RADSETUP from a data directory without the k-coefficient files: a status, no abort
+    (the files are read before any device work, so this runs without a GPU)."""
+    from mgcm_rt import casefile, synth
+    t, c = synth.make_workload(19, 2, 24)
+    case, res = str(tmp_path / "case.bin"), str(tmp_path / "res.bin")
+    casefile.write_case(case, t, c)
+    r = subprocess.run([DRIVER, case, res, str(tmp_path / "no_such_dir")], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 1
+    status, _ = casefile.read_result(res, 2, 24)
+    assert status == -7
+    assert "data file" in r.stdout
+
+
+@pytest.mark.gpu
+def test_fortran_driver_on_reference_data_files(tmp_path):
+    """The driver sets the k-tables up from the reference's data files (written from the
+    fixture's records) and reproduces the reference's outputs on its own k-distribution."""
+    from mgcm_rt import casefile, realdata
+    from conftest import GOLDEN, rel_err
+    real = np.load(os.path.join(GOLDEN, "column_real24.npz"))
+    d = tmp_path / "data"
+    d.mkdir()
+    realdata.write_ktable(str(d / realdata.VIS_FILE), real["raw_v"], real["fzero_v"])
+    realdata.write_ktable(str(d / realdata.IR_FILE), real["raw_i"], real["fzero_i"])
+    t = realdata.load_tables(str(d), 24)
+    t["tauray"], t["sol"] = real["tauray"], real["solarf"] / 2.4277
+    # the case file carries other k-tables; the driver replaces them from the files
+    t["co2v"], t["co2i"] = t["co2v"] * 0.5, t["co2i"] * 2.0
+    t["fzerov"], t["fzeroi"] = np.zeros(7), np.zeros(5)
+    c = {k: real["in_" + k] for k in casefile.COLUMN_KEYS}
+    c["ncol"], c["nlay"] = 32, 24
+    case, res = str(tmp_path / "case.bin"), str(tmp_path / "res.bin")
+    casefile.write_case(case, t, c)
+    r = subprocess.run([DRIVER, case, res, str(d)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    status, got = casefile.read_result(res, 32, 24)
+    assert status == 0
+    for k in casefile.RESULT_KEYS:
+        e = rel_err(got[k], real["out_" + k])
+        assert e <= 1e-10, (k, e)
