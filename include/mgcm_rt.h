@@ -83,8 +83,26 @@ typedef struct {
   double qextref_cld;     /* QEXTREFCLD */
 } mgcm_rt_tables;
 
+/* Per-level aerosol optical properties of each column, as OPT_DST and OPT_CLD
+ * produce them for active dust and water-ice clouds (code/opt_dst.f:114-158,
+ * code/opt_cld.f; COMP3 hands them to OPTCV/OPTCI, code/comp3.f:595-631).  Any
+ * pointer may be NULL: that quantity then takes the tables' level-constant band
+ * value (INI_OPTDST/INI_OPTCLD).  Per column the reference's layout
+ * Q(L_LEVELS+1, nspect): [ncol][nspect][nlev+1].  The VIS dust reference
+ * extinction is QXVDST(K, L_NREFV = 6) (code/optcv.f90:114).  Device pointers for
+ * mgcm_rt_run/mgcm_rt_run_batched, host pointers for mgcm_rt_run_host. */
+typedef struct {
+  const double *qxv_dst, *qsv_dst, *gv_dst; /* [ncol][7][nlev+1] dust Qext, Qscat, g */
+  const double *qxi_dst, *qsi_dst, *gi_dst; /* [ncol][5][nlev+1] */
+  const double *qxv_cld, *qsv_cld, *gv_cld; /* [ncol][7][nlev+1] water ice */
+  const double *qxi_cld, *qsi_cld, *gi_cld; /* [ncol][5][nlev+1] */
+  const double *qextref_dst;                /* [ncol][nlev+1] QEXTREFDST (OPTCI's QREFV) */
+  const double *qextref_cld;                /* [ncol][nlev+1] QEXTREFCLD */
+} mgcm_rt_aerosol;
+
 /* One grid of columns.  Device pointers for mgcm_rt_run, host pointers for
- * mgcm_rt_run_host.  sol[] is SOL(NW) = SOLARF(NW)/RSDIST (code/comp3.f:300-304). */
+ * mgcm_rt_run_host.  sol[] is SOL(NW) = SOLARF(NW)/RSDIST (code/comp3.f:300-304).
+ * aerosol: optional per-level aerosol optics (NULL = the tables' constants). */
 typedef struct {
   int ncol;
   int nlay;
@@ -94,6 +112,7 @@ typedef struct {
   const double *albi;                             /* [ncol] IR albedo 1-emissivity (RSFI) */
   const double *acosz;                            /* [ncol] cos(zenith) (UBAR0); < 1e-5 = sun down */
   double sol[MGCM_RT_NV];
+  const mgcm_rt_aerosol *aerosol;
 } mgcm_rt_columns;
 
 /* Outputs: SFLUXV/SFLUXI results per column and the layer heating rates.

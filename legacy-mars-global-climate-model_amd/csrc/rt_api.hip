@@ -165,6 +165,16 @@ void fill_args(RunArgs &a, const mgcm_rt_columns *in, const mgcm_rt_fluxes *out,
   a.flag = flag;
   a.bands = 3;
   a.clr = nullptr;
+  const mgcm_rt_aerosol *ae = in->aerosol;
+  const double *aq[12] = {nullptr};
+  if (ae) {
+    const double *src[12] = {ae->qxv_dst, ae->qsv_dst, ae->gv_dst, ae->qxi_dst, ae->qsi_dst, ae->gi_dst,
+                             ae->qxv_cld, ae->qsv_cld, ae->gv_cld, ae->qxi_cld, ae->qsi_cld, ae->gi_cld};
+    for (int i = 0; i < 12; i++) aq[i] = src[i];
+  }
+  for (int i = 0; i < 12; i++) a.aq[i] = aq[i];
+  a.qref_dst = ae ? ae->qextref_dst : nullptr;
+  a.qref_cld = ae ? ae->qextref_cld : nullptr;
 }
 }  // namespace mgcm
 
@@ -218,7 +228,21 @@ extern "C" int mgcm_rt_run_host(mgcm_rt_ctx *c, const mgcm_rt_columns *in, const
   const size_t ncol = in->ncol, nlev = 2 * (size_t)in->nlay + 3, nl = in->nlay + 1;
   const size_t nin = ncol * (5 * nlev + 2 * (nlev + 1) + 3);
   const size_t nout = ncol * (6 * nl + 3 + 2 * NG * NV + nl);
-  const size_t bytes = (nin + nout) * sizeof(double);
+  // optional per-level aerosol optics: the non-null arrays in mgcm_rt_aerosol order
+  const mgcm_rt_aerosol *hae = in->aerosol;
+  const double *asrc[14] = {nullptr};
+  size_t alen[14] = {0}, naer = 0;
+  if (hae) {
+    const double *src[14] = {hae->qxv_dst, hae->qsv_dst, hae->gv_dst, hae->qxi_dst, hae->qsi_dst, hae->gi_dst,
+                             hae->qxv_cld, hae->qsv_cld, hae->gv_cld, hae->qxi_cld, hae->qsi_cld, hae->gi_cld,
+                             hae->qextref_dst, hae->qextref_cld};
+    for (int i = 0; i < 14; i++) {
+      asrc[i] = src[i];
+      alen[i] = src[i] ? ncol * (i < 12 ? (((i / 3) % 2 == 0) ? NV : NI) : 1) * (nlev + 1) : 0;
+      naer += alen[i];
+    }
+  }
+  const size_t bytes = (nin + nout + naer) * sizeof(double);
   if (bytes > c->host_bytes) {
     if (c->d_host) (void)hipFree(c->d_host);
     c->d_host = nullptr;
@@ -251,6 +275,19 @@ extern "C" int mgcm_rt_run_host(mgcm_rt_ctx *c, const mgcm_rt_columns *in, const
   HIP_TRY(hipMemcpy(dacosz, in->acosz, ncol * 8, hipMemcpyHostToDevice));
   din.plev = dplev; din.tlev = dtlev; din.pmid = dpmid; din.tmid = dtmid; din.qh2o = dqh2o;
   din.tauref = dtauref; din.taurefcld = dtaurefcld; din.als = dals; din.albi = dalbi; din.acosz = dacosz;
+  mgcm_rt_aerosol dae = {};
+  if (hae) {
+    const double **dst[14] = {&dae.qxv_dst, &dae.qsv_dst, &dae.gv_dst, &dae.qxi_dst, &dae.qsi_dst, &dae.gi_dst,
+                              &dae.qxv_cld, &dae.qsv_cld, &dae.gv_cld, &dae.qxi_cld, &dae.qsi_cld, &dae.gi_cld,
+                              &dae.qextref_dst, &dae.qextref_cld};
+    for (int i = 0; i < 14; i++) {
+      if (!asrc[i]) continue;
+      HIP_TRY(hipMemcpy(p, asrc[i], alen[i] * 8, hipMemcpyHostToDevice));
+      *dst[i] = p;
+      p += alen[i];
+    }
+    din.aerosol = &dae;
+  }
   dout.fmnetv = p; p += ncol * nl;
   dout.fluxupv = p; p += ncol * nl;
   dout.fluxdnv = p; p += ncol * nl;

@@ -192,14 +192,14 @@ __device__ __forceinline__ bool below_tlimit(const Src &src, const double *tab64
 // profiles (code/optcv.f90:172-173, code/optci.f90:173-174).
 template <bool VIS>
 struct ClearSrc {
-  const RtConst *rc;
+  const Aer &ae;
   int b;
   const double *plev, *tauref, *taurefcld;
   double taugsurf;
   struct Raw {};
   __device__ __forceinline__ Raw fetch(int) const { return Raw{}; }
   __device__ __forceinline__ double finish(int K, const Raw &) {
-    const SubAer s = sub_aerosol<VIS>(rc, b, plev, tauref, taurefcld, K);
+    const SubAer s = sub_aerosol<VIS>(ae, b, plev, tauref, taurefcld, K);
     if (VIS) return s.taeros + s.tray + s.tcloud;
     return s.taeros + s.tcloud;
   }
@@ -323,11 +323,12 @@ __device__ __forceinline__ void column_block(const DevTables &dt, const RunArgs 
       const double *pl = a.plev + (size_t)cl * nlev;
       const double *tr = a.tauref + (size_t)cl * (nlev + 1);
       const double *tc = a.taurefcld + (size_t)cl * (nlev + 1);
+      const Aer ae = column_aerosol(a, rc, cl);
       if (VISB) {
-        const SubAer s = sub_aerosol<true>(rc, b, pl, tr, tc, K);
+        const SubAer s = sub_aerosol<true>(ae, b, pl, tr, tc, K);
         S.c1[t] = s.tray + s.taeros + s.tcloud;
       } else {
-        const SubAer s = sub_aerosol<false>(rc, b, pl, tr, tc, K);
+        const SubAer s = sub_aerosol<false>(ae, b, pl, tr, tc, K);
         S.c1[t] = s.taeros;
         S.c2[t] = s.tcloud;
       }
@@ -339,9 +340,10 @@ __device__ __forceinline__ void column_block(const DevTables &dt, const RunArgs 
       const double *pl = a.plev + (size_t)cl * nlev;
       const double *tr = a.tauref + (size_t)cl * (nlev + 1);
       const double *tc = a.taurefcld + (size_t)cl * (nlev + 1);
+      const Aer ae = column_aerosol(a, rc, cl);
       double sc, cb;
-      if (VISB) layer_aerosol<true>(rc, b, pl, tr, tc, nl, L, sc, cb);
-      else layer_aerosol<false>(rc, b, pl, tr, tc, nl, L, sc, cb);
+      if (VISB) layer_aerosol<true>(ae, b, pl, tr, tc, nl, L, sc, cb);
+      else layer_aerosol<false>(ae, b, pl, tr, tc, nl, L, sc, cb);
       S.scat[t] = sc;
       S.cosb[t] = cb;
       S.cbar[t] = cb / (1.0 + cb);  // COSBAR of the delta-Eddington scaling (code/gfluxv.f90:86)
@@ -543,9 +545,10 @@ __device__ __forceinline__ void clear_block(const DevTables &dt, const RunArgs &
   const double *tr = a.tauref + (size_t)col * (nlev + 1);
   const double *tc = a.taurefcld + (size_t)col * (nlev + 1);
   double *rec = a.clr + (size_t)col * NB * clr_rec(nl) + b;  // slot k of this item: rec[k * NB]
+  const Aer ae = column_aerosol(a, rc, col);
   double scat[NLM], cosb[NLM], cbar[NLM];
   for (int L = 1; L <= nl; L++) {
-    layer_aerosol<VIS>(rc, b, pl, tr, tc, nl, L, scat[L - 1], cosb[L - 1]);
+    layer_aerosol<VIS>(ae, b, pl, tr, tc, nl, L, scat[L - 1], cosb[L - 1]);
     cbar[L - 1] = cosb[L - 1] / (1.0 + cosb[L - 1]);
   }
   auto accf = [&](int n, double fp, double fm) {
@@ -555,7 +558,7 @@ __device__ __forceinline__ void clear_block(const DevTables &dt, const RunArgs &
   if (VIS) {
     const double mu = a.acosz[col];
     if (!(mu >= F32LIT(1.0e-5))) return;  // code/comp3.f:637
-    ClearSrc<true> src{rc, b, pl, tr, tc, 0.0};
+    ClearSrc<true> src{ae, b, pl, tr, tc, 0.0};
     VisState<NLM> s;
     double cumtau = 0.0;
     const double detau = vis_optics<NLM>(nl, src, scat, cosb, s, cumtau);
@@ -568,7 +571,7 @@ __device__ __forceinline__ void clear_block(const DevTables &dt, const RunArgs &
     rec[(2 * nl + 1) * NB] = diffv;
   } else {
     const int nw = b - NV;
-    ClearSrc<false> src{rc, b, pl, tr, tc, 0.0};
+    ClearSrc<false> src{ae, b, pl, tr, tc, 0.0};
     IrState<NLM> s;
     ir_optics<NLM>(nl, src, scat, s);
     double bk[2 * NLM + 1];

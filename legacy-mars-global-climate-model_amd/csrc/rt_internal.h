@@ -10,6 +10,7 @@
 namespace mgcm {
 
 constexpr int NT = 7, NP = 51, NH = 10, NG = 17, NGAS = 16, NV = 7, NI = 5, NB = NV + NI;
+constexpr int L_NREFV = 6;  // reference VIS band, 1-based (code/modules.f90:238)
 constexpr int NPLANCK = 8501;
 constexpr int MAX_NLAY = 64;
 
@@ -56,6 +57,11 @@ struct RunArgs {
   int *flag;   // range flag
   int bands;   // bit 0: VIS bands, bit 1: IR bands
   double *clr; // continuum-point records, [ncol][NB][2*nl+2] (workspace)
+  // per-level aerosol optics of each column, or null for the tables' band
+  // constants: QXVDST, QSVDST, GVDST, QXIDST, QSIDST, GIDST, then the same six
+  // of the water-ice cloud, [ncol][nb][nlev+1]; QEXTREFDST, QEXTREFCLD [ncol][nlev+1]
+  const double *aq[12];
+  const double *qref_dst, *qref_cld;
 };
 
 struct DevTables {

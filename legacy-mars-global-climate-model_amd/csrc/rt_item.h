@@ -16,6 +16,48 @@ __device__ __forceinline__ double rt_exp35(double x) { return mgcm_libm::mgcm_ex
 
 namespace mgcm {
 
+// Aerosol optical properties of one column (code/optcv.f90:114-123,
+// code/optci.f90:124-130,182-187): the caller's per-level arrays where given
+// (OPT_DST/OPT_CLD for active dust and water-ice clouds, code/comp3.f:595-631),
+// else the tables' level-constant band values (INI_OPTDST/INI_OPTCLD).  b: band
+// 0..NB-1 (VIS first); K: 1-based level 1..L_LEVELS+1.
+struct Aer {
+  const RtConst *rc;
+  const double *q[12];           // RunArgs::aq of this column ([nb][nlev+1]), or null
+  const double *qref_d, *qref_c; // this column's [nlev+1], or null
+  int ld;                        // nlev + 1
+  // kind: 0 Qext, 1 Qscat, 2 g; cld: 0 dust, 1 water ice
+  __device__ __forceinline__ double get(int kind, int cld, int b, int K, double cst) const {
+    const double *p = q[cld * 6 + (b < NV ? 0 : 3) + kind];
+    return p ? p[(b < NV ? b : b - NV) * ld + K - 1] : cst;
+  }
+  __device__ __forceinline__ double qx_dst(int b, int K) const { return get(0, 0, b, K, rc->qx_dst[b]); }
+  __device__ __forceinline__ double qs_dst(int b, int K) const { return get(1, 0, b, K, rc->qs_dst[b]); }
+  __device__ __forceinline__ double g_dst(int b, int K) const { return get(2, 0, b, K, rc->g_dst[b]); }
+  __device__ __forceinline__ double qx_cld(int b, int K) const { return get(0, 1, b, K, rc->qx_cld[b]); }
+  __device__ __forceinline__ double qs_cld(int b, int K) const { return get(1, 1, b, K, rc->qs_cld[b]); }
+  __device__ __forceinline__ double g_cld(int b, int K) const { return get(2, 1, b, K, rc->g_cld[b]); }
+  // Qextref(K) = QXVDST(K, L_NREFV) (code/optcv.f90:114), QREFV(K) = QEXTREFDST(K), QEXTREFCLD(K)
+  __device__ __forceinline__ double qref_vis(int K) const {
+    return q[0] ? q[0][(L_NREFV - 1) * ld + K - 1] : rc->qextref_vis;
+  }
+  __device__ __forceinline__ double qref_dst(int K) const { return qref_d ? qref_d[K - 1] : rc->qextref_dst; }
+  __device__ __forceinline__ double qref_cld(int K) const { return qref_c ? qref_c[K - 1] : rc->qextref_cld; }
+};
+
+__device__ __forceinline__ Aer column_aerosol(const RunArgs &a, const RtConst *rc, int col) {
+  Aer e;
+  e.rc = rc;
+  e.ld = 2 * a.nlay + 4;
+  for (int i = 0; i < 12; i++) {
+    const int nb = ((i / 3) % 2 == 0) ? NV : NI;
+    e.q[i] = a.aq[i] ? a.aq[i] + (size_t)col * nb * e.ld : nullptr;
+  }
+  e.qref_d = a.qref_dst ? a.qref_dst + (size_t)col * e.ld : nullptr;
+  e.qref_c = a.qref_cld ? a.qref_cld + (size_t)col * e.ld : nullptr;
+  return e;
+}
+
 // Aerosol and Rayleigh optical depths of one sub-level K (1-based, 2..NLEV),
 // OPTCV code/optcv.f90:103-123,184-185 and OPTCI code/optci.f90:113-130,184-185.
 struct SubAer {
@@ -23,47 +65,46 @@ struct SubAer {
 };
 
 template <bool VIS>
-__device__ __forceinline__ SubAer sub_aerosol(const RtConst *__restrict__ rc, int b,
-                                              const double *__restrict__ plev,
+__device__ __forceinline__ SubAer sub_aerosol(const Aer &ae, int b, const double *__restrict__ plev,
                                               const double *__restrict__ tauref,
                                               const double *__restrict__ taurefcld, int K) {
   SubAer s;
   const double dpr = plev[K - 1] - plev[K - 2];
-  const double tr = tauref[K - 1] / (VIS ? rc->qextref_vis : rc->qextref_dst);
-  const double tc = taurefcld[K - 1] / rc->qextref_cld;
-  s.tray = VIS ? rc->tauray[b] * dpr : 0.0;
-  s.taeros = tr * rc->qx_dst[b];
-  s.tcloud = tc * rc->qx_cld[b];
-  s.tlk = tr * rc->qs_dst[b];
-  s.tck = tc * rc->qs_cld[b];
+  const double tr = tauref[K - 1] / (VIS ? ae.qref_vis(K) : ae.qref_dst(K));
+  const double tc = taurefcld[K - 1] / ae.qref_cld(K);
+  s.tray = VIS ? ae.rc->tauray[b] * dpr : 0.0;
+  s.taeros = tr * ae.qx_dst(b, K);
+  s.tcloud = tc * ae.qx_cld(b, K);
+  s.tlk = tr * ae.qs_dst(b, K);
+  s.tck = tc * ae.qs_cld(b, K);
   return s;
 }
 
 // IR padding level L_LEVELS+1 (code/optci.f90:89-95,182-187): TAUREF there is
 // not divided by the reference extinction.
-__device__ __forceinline__ void ir_pad(const RtConst *__restrict__ rc, int b,
-                                       const double *__restrict__ tauref,
+__device__ __forceinline__ void ir_pad(const Aer &ae, int b, const double *__restrict__ tauref,
                                        const double *__restrict__ taurefcld, int nlev,
                                        double &tlk, double &tck) {
-  tlk = tauref[nlev] * rc->qs_dst[b];
-  tck = taurefcld[nlev] * rc->qs_cld[b];
+  tlk = tauref[nlev] * ae.qs_dst(b, nlev + 1);
+  tck = taurefcld[nlev] * ae.qs_cld(b, nlev + 1);
 }
 
 // Layer scattering optical depth and asymmetry of the continuum channel:
 // VIS code/optcv.f90:194-222, IR code/optci.f90:195-219.
 template <bool VIS>
-__device__ __forceinline__ void layer_aerosol(const RtConst *__restrict__ rc, int b, const double *__restrict__ plev,
+__device__ __forceinline__ void layer_aerosol(const Aer &ae, int b, const double *__restrict__ plev,
                               const double *__restrict__ tauref, const double *__restrict__ taurefcld,
                               int nl, int L, double &scat, double &cosb) {
   const int nlev = 2 * nl + 1;
   const int K = 2 * L + 1;
-  const double gd = rc->g_dst[b], gc = rc->g_cld[b];
+  const double gd = ae.g_dst(b, K), gc = ae.g_cld(b, K);
   if (VIS) {
     const double f9999 = F32LIT(0.9999);
-    const SubAer a = sub_aerosol<true>(rc, b, plev, tauref, taurefcld, K);
+    const SubAer a = sub_aerosol<true>(ae, b, plev, tauref, taurefcld, K);
     if (L < nl) {
-      const SubAer a2 = sub_aerosol<true>(rc, b, plev, tauref, taurefcld, K + 1);
-      cosb = (gd * a.tlk + gd * a2.tlk + gc * a.tck + gc * a2.tck) /
+      const SubAer a2 = sub_aerosol<true>(ae, b, plev, tauref, taurefcld, K + 1);
+      const double gd2 = ae.g_dst(b, K + 1), gc2 = ae.g_cld(b, K + 1);
+      cosb = (gd * a.tlk + gd2 * a2.tlk + gc * a.tck + gc2 * a2.tck) /
              (a.tray + a2.tray + a.tlk + a2.tlk + a.tck + a2.tck);
       scat = a.tlk + a2.tlk + a.tck + a2.tck + (a.tray + a2.tray) * f9999;
     } else {
@@ -71,17 +112,18 @@ __device__ __forceinline__ void layer_aerosol(const RtConst *__restrict__ rc, in
       scat = a.tlk + a.tck + a.tray * f9999;
     }
   } else {
-    const SubAer a = sub_aerosol<false>(rc, b, plev, tauref, taurefcld, K);
+    const SubAer a = sub_aerosol<false>(ae, b, plev, tauref, taurefcld, K);
     double tlk2, tck2;
     if (K + 1 <= nlev) {
-      const SubAer a2 = sub_aerosol<false>(rc, b, plev, tauref, taurefcld, K + 1);
+      const SubAer a2 = sub_aerosol<false>(ae, b, plev, tauref, taurefcld, K + 1);
       tlk2 = a2.tlk;
       tck2 = a2.tck;
     } else {
-      ir_pad(rc, b, tauref, taurefcld, nlev, tlk2, tck2);
+      ir_pad(ae, b, tauref, taurefcld, nlev, tlk2, tck2);
     }
+    const double gd2 = ae.g_dst(b, K + 1), gc2 = ae.g_cld(b, K + 1);
     scat = a.tlk + tlk2 + a.tck + tck2;
-    cosb = (scat > 0.0) ? (gd * a.tlk + gd * tlk2 + gc * a.tck + gc * tck2) / scat : 0.0;
+    cosb = (scat > 0.0) ? (gd * a.tlk + gd2 * tlk2 + gc * a.tck + gc2 * tck2) / scat : 0.0;
   }
 }
 

@@ -27,11 +27,20 @@ module mgcm_rt_mod
     real(c_double) :: qextref_dst, qextref_cld
   end type
 
+  ! mirrors mgcm_rt_aerosol: per-level aerosol optics of each column (OPT_DST/OPT_CLD
+  ! outputs, Q(L_LEVELS+1, nspect) per column); c_null_ptr = the tables' constants
+  type, bind(C), public :: mgcm_rt_aerosol
+    type(c_ptr) :: qxv_dst, qsv_dst, gv_dst, qxi_dst, qsi_dst, gi_dst
+    type(c_ptr) :: qxv_cld, qsv_cld, gv_cld, qxi_cld, qsi_cld, gi_cld
+    type(c_ptr) :: qextref_dst, qextref_cld
+  end type
+
   ! mirrors mgcm_rt_columns
   type, bind(C), public :: mgcm_rt_columns
     integer(c_int) :: ncol, nlay
     type(c_ptr) :: plev, tlev, pmid, tmid, qh2o, tauref, taurefcld, als, albi, acosz
     real(c_double) :: sol(7)
+    type(c_ptr) :: aerosol
   end type
 
   ! mirrors mgcm_rt_fluxes
@@ -201,7 +210,8 @@ contains
   ! leaves its module arrays.
   subroutine mgcm_rt_radiation(ctx, ncol, nlay, plev, tlev, pmid, tmid, qh2o, tauref, taurefcld, &
                                als, albi, acosz, sol, fmnetv, fluxupv, fluxdnv, fmneti, fluxupi, &
-                               fluxdni, nfluxtopv, nfluxtopi, diffvt, detau, cumtauv, hr, precision, status)
+                               fluxdni, nfluxtopv, nfluxtopi, diffvt, detau, cumtauv, hr, precision, status, &
+                               aerosol)
     type(c_ptr), intent(in) :: ctx
     integer, intent(in) :: ncol, nlay
     real(c_double), intent(in), target :: plev(2*nlay+3, ncol), tlev(2*nlay+3, ncol)
@@ -217,6 +227,8 @@ contains
     real(c_double), intent(out), target :: hr(nlay+1, ncol)
     integer(c_int), intent(in) :: precision
     integer(c_int), intent(out) :: status
+    ! per-level aerosol optics (active dust/clouds); absent = the tables' constants
+    type(mgcm_rt_aerosol), intent(in), target, optional :: aerosol
     type(mgcm_rt_columns) :: c
     type(mgcm_rt_fluxes) :: o
     c%ncol = ncol
@@ -232,6 +244,8 @@ contains
     c%albi = c_loc(albi)
     c%acosz = c_loc(acosz)
     c%sol = sol
+    c%aerosol = c_null_ptr
+    if (present(aerosol)) c%aerosol = c_loc(aerosol)
     o%fmnetv = c_loc(fmnetv)
     o%fluxupv = c_loc(fluxupv)
     o%fluxdnv = c_loc(fluxdnv)

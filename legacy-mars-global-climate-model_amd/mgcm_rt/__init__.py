@@ -53,10 +53,20 @@ class Tables(C.Structure):
         ("qextref_dst", C.c_double), ("qextref_cld", C.c_double)]
 
 
+# per-level aerosol optics of each column (mgcm_rt_aerosol): optional column-dict keys,
+# [ncol][nb][nlev+1] for the band quantities, [ncol][nlev+1] for the two references
+AER_KEYS = ("qxv_dst", "qsv_dst", "gv_dst", "qxi_dst", "qsi_dst", "gi_dst",
+            "qxv_cld", "qsv_cld", "gv_cld", "qxi_cld", "qsi_cld", "gi_cld", "qextref_dst", "qextref_cld")
+
+
+class Aerosol(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in AER_KEYS]
+
+
 class Columns(C.Structure):
     _fields_ = [("ncol", C.c_int), ("nlay", C.c_int)] + [(n, C.c_void_p) for n in (
         "plev", "tlev", "pmid", "tmid", "qh2o", "tauref", "taurefcld", "als", "albi", "acosz")] + [
-        ("sol", C.c_double * NV)]
+        ("sol", C.c_double * NV), ("aerosol", C.POINTER(Aerosol))]
 
 
 class Fluxes(C.Structure):
@@ -134,6 +144,8 @@ def alloc_outputs(ncol: int, nlay: int) -> dict:
 
 
 def column_struct(cols: dict, ptr=lambda a: a.ctypes.data, sol=None) -> Columns:
+    """The mgcm_rt_columns of a column dict; its aerosol block (if the dict has any of
+    AER_KEYS) is kept alive on the returned struct."""
     s = Columns()
     s.ncol = int(cols["ncol"])
     s.nlay = int(cols["nlay"])
@@ -141,6 +153,13 @@ def column_struct(cols: dict, ptr=lambda a: a.ctypes.data, sol=None) -> Columns:
         setattr(s, k, ptr(cols[k]))
     for i in range(NV):
         s.sol[i] = float(sol[i])
+    if any(k in cols for k in AER_KEYS):
+        a = Aerosol()
+        for k in AER_KEYS:
+            if k in cols:
+                setattr(a, k, ptr(cols[k]))
+        s._aerosol = a
+        s.aerosol = C.pointer(a)
     return s
 
 
@@ -193,7 +212,7 @@ class Radiation:
 
     def run_host(self, cols: dict, precision: int = FP64, out: dict | None = None) -> dict:
         """Host arrays in, host arrays out (copies, runs, synchronises)."""
-        c = {k: _f64(cols[k]) for k in IN_KEYS}
+        c = {k: _f64(cols[k]) for k in IN_KEYS + AER_KEYS if k in cols}
         c["ncol"], c["nlay"] = int(cols["ncol"]), int(cols["nlay"])
         o = out if out is not None else alloc_outputs(c["ncol"], c["nlay"])
         cs = column_struct(c, sol=self.sol)
@@ -212,10 +231,12 @@ class Radiation:
     def run_batched(self, grids: list, outs: list, precision: int = FP64, stream: int = 0) -> None:
         ptr = lambda t: t.data_ptr()  # noqa: E731
         n = len(grids)
-        cs = (Columns * n)(*[column_struct(g, ptr=ptr, sol=self.sol) for g in grids])
+        structs = [column_struct(g, ptr=ptr, sol=self.sol) for g in grids]  # keeps the aerosol blocks alive
+        cs = (Columns * n)(*structs)
         fs = (Fluxes * n)(*[flux_struct(o, ptr=ptr) for o in outs])
         _check(self.lib.mgcm_rt_run_batched(self.ctx, n, cs, fs, int(precision), C.c_void_p(stream)),
                "mgcm_rt_run_batched")
+        del structs
 
     def check(self) -> None:
         _check(self.lib.mgcm_rt_check(self.ctx), "mgcm_rt_check")

@@ -329,6 +329,35 @@ def make_columns(seed: int, ncol: int, nlay: int, clouds: bool = False) -> dict:
     return c
 
 
+def make_aerosol(seed: int, ncol: int, nlay: int, dust: bool = True, cloud: bool = True) -> dict:
+    """Per-column, per-level aerosol optics in the shape OPT_DST/OPT_CLD hand to the
+    radiation (code/comp3.f:595-631), as column-dict keys of mgcm_rt.AER_KEYS.
+
+    Generator decision: each (column, band, level) value is the reference constant times
+    U(0.6, 1.4) (g times U(0.8, 1.2), capped at 0.95); Qscat is capped at 0.99999 Qext as
+    OPT_DST does (code/opt_dst.f:136,148); the references follow OPT_DST: QEXTREFDST(K) =
+    QXVDST(K, L_NREFV) (code/opt_dst.f:151), and likewise for the cloud."""
+    rng = np.random.Generator(np.random.PCG64(seed + 7919))
+    ld = 2 * nlay + 4
+    out = {}
+
+    def group(qe, qsc, gg, nb, suffix):
+        qx = qe[None, :, None] * rng.uniform(0.6, 1.4, (ncol, nb, ld))
+        qs = np.minimum(qsc[None, :, None] * rng.uniform(0.6, 1.4, (ncol, nb, ld)), _f32(0.99999) * qx)
+        g = np.minimum(gg[None, :, None] * rng.uniform(0.8, 1.2, (ncol, nb, ld)), 0.95)
+        return {f"qx{suffix}": qx, f"qs{suffix}": qs, f"g{suffix}": g}
+
+    if dust:
+        out.update(group(QEV1, QSV1, GV1, NV, "v_dst"))
+        out.update(group(QEI1, QSI1, GI1, NI, "i_dst"))
+        out["qextref_dst"] = np.ascontiguousarray(out["qxv_dst"][:, L_NREFV - 1, :])
+    if cloud:
+        out.update(group(QEV_CLD, QSV_CLD, GV_CLD, NV, "v_cld"))
+        out.update(group(QEI_CLD, QSI_CLD, GI_CLD, NI, "i_cld"))
+        out["qextref_cld"] = np.ascontiguousarray(out["qxv_cld"][:, L_NREFV - 1, :])
+    return {k: np.ascontiguousarray(v) for k, v in out.items()}
+
+
 def make_workload(seed: int, ncol: int, nlay: int, clouds: bool = False, weights: str = "dirichlet"):
     return make_tables(seed, nlay, weights), make_columns(seed, ncol, nlay, clouds)
 

@@ -751,6 +751,19 @@ static void batch_column(const or_batch_in *in, const or_batch_out *out, int nla
   o->hr = out->hr + (size_t)i * (nlay + 1);
 }
 
+/* The tables of column i: its own aerosol optics where the batch carries them
+ * (OPT_DST/OPT_CLD, code/comp3.f:595-631), else the shared ones. */
+static void column_tables(const or_tables *t, const or_batch_in *in, int i, or_tables *tc)
+{
+  const size_t ld = (size_t)(2 * t->nlay + 4);
+  *tc = *t;
+#define AERO(f, nb) if (in->f) tc->f = in->f + (size_t)i * (nb) * ld
+  AERO(qxvdst, NV); AERO(qsvdst, NV); AERO(gvdst, NV); AERO(qxvcld, NV); AERO(qsvcld, NV); AERO(gvcld, NV);
+  AERO(qxidst, NI); AERO(qsidst, NI); AERO(gidst, NI); AERO(qxicld, NI); AERO(qsicld, NI); AERO(gicld, NI);
+  AERO(qextrefdst, 1); AERO(qextrefcld, 1);
+#undef AERO
+}
+
 void or_batch_run(const or_tables *t, const double *sol, const or_batch_in *in,
                   or_batch_out *out, int nthreads)
 {
@@ -767,8 +780,10 @@ void or_batch_run(const or_tables *t, const double *sol, const or_batch_in *in,
     for (i = 0; i < in->ncol; i++) {
       or_column c;
       or_out o;
+      or_tables tc;
       batch_column(in, out, t->nlay, i, &c, &o);
-      column_run_work(t, sol, &c, &o, w);
+      column_tables(t, in, i, &tc);
+      column_run_work(&tc, sol, &c, &o, w);
     }
     free(w);
   }
@@ -847,8 +862,10 @@ void or_batch_count(const or_tables *t, const double *sol, const or_batch_in *in
     o.fmneti = dummy.fmneti; o.fluxupi = dummy.fluxupi; o.fluxdni = dummy.fluxdni;
     o.nfluxtopv = dummy.nfluxtopv; o.nfluxtopi = dummy.nfluxtopi; o.diffvt = dummy.diffvt;
     o.detau = dummy.detau; o.cumtauv = dummy.cumtauv; o.hr = dummy.hr;
-    column_run_work(t, sol, &c, &o, w);
-    count_column(t, &c, w, count);
+    or_tables tc;
+    column_tables(t, in, i, &tc);
+    column_run_work(&tc, sol, &c, &o, w);
+    count_column(&tc, &c, w, count);
   }
   free(buf);
   free(w);

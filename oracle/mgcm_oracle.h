@@ -76,6 +76,11 @@ typedef struct {
   const double *plev, *tlev, *pmid, *tmid, *qh2o; /* [ncol][nlev] */
   const double *tauref, *taurefcld;               /* [ncol][nlev+1] */
   const double *als, *albi, *acosz;               /* [ncol] */
+  /* optional per-column aerosol optics (NULL = the tables' arrays); per column
+   * the tables' layout (nlev+1, nb): [ncol][nb][nlev+1], references [ncol][nlev+1] */
+  const double *qxvdst, *qsvdst, *gvdst, *qxvcld, *qsvcld, *gvcld;
+  const double *qxidst, *qsidst, *gidst, *qxicld, *qsicld, *gicld;
+  const double *qextrefdst, *qextrefcld;
 } or_batch_in;
 
 typedef struct {

@@ -37,9 +37,18 @@ class _Tables(C.Structure):
         "qextrefdst", "qextrefcld")]
 
 
+# per-column aerosol optics: column-dict key (the library's name) -> oracle field
+AER_FIELDS = (("qxv_dst", "qxvdst"), ("qsv_dst", "qsvdst"), ("gv_dst", "gvdst"),
+              ("qxv_cld", "qxvcld"), ("qsv_cld", "qsvcld"), ("gv_cld", "gvcld"),
+              ("qxi_dst", "qxidst"), ("qsi_dst", "qsidst"), ("gi_dst", "gidst"),
+              ("qxi_cld", "qxicld"), ("qsi_cld", "qsicld"), ("gi_cld", "gicld"),
+              ("qextref_dst", "qextrefdst"), ("qextref_cld", "qextrefcld"))
+
+
 class _BatchIn(C.Structure):
     _fields_ = [("ncol", C.c_int)] + [(n, DP) for n in (
-        "plev", "tlev", "pmid", "tmid", "qh2o", "tauref", "taurefcld", "als", "albi", "acosz")]
+        "plev", "tlev", "pmid", "tmid", "qh2o", "tauref", "taurefcld", "als", "albi", "acosz")] + [
+        (f, DP) for _, f in AER_FIELDS]
 
 
 class _BatchOut(C.Structure):
@@ -103,10 +112,15 @@ class Oracle:
     def _inputs(c: dict, keep: list) -> _BatchIn:
         s = _BatchIn()
         s.ncol = int(c["ncol"])
-        for name, _ in _BatchIn._fields_[1:]:
+        for name in ("plev", "tlev", "pmid", "tmid", "qh2o", "tauref", "taurefcld", "als", "albi", "acosz"):
             a = _c(c[name])
             keep.append(a)
             setattr(s, name, _p(a))
+        for key, field in AER_FIELDS:
+            if key in c:
+                a = _c(c[key])
+                keep.append(a)
+                setattr(s, field, _p(a))
         return s
 
     @staticmethod
@@ -213,11 +227,17 @@ class Reference:
                                    "qxvdst", "qsvdst", "gvdst", "qxvcld", "qsvcld", "gvcld",
                                    "qxidst", "qsidst", "gidst", "qxicld", "qsicld", "gicld",
                                    "qextrefdst", "qextrefcld")]
+        tab_names = ("co2v", "co2i", "pfgasref", "tgasref", "wrefh2o", "gwv", "gwi", "fzerov", "fzeroi", "tauray",
+                     "dwni", "sol", "qxvdst", "qsvdst", "gvdst", "qxvcld", "qsvcld", "gvcld",
+                     "qxidst", "qsidst", "gidst", "qxicld", "qsicld", "gicld", "qextrefdst", "qextrefcld")
+        per_col = {tab_names.index(f): key for key, f in AER_FIELDS if key in c}
         for i in cols:
             prof = [_c(c[k][i]) for k in ("plev", "tlev", "pmid", "tmid", "qh2o", "tauref", "taurefcld")]
             outs = [o[k][i] for k in ("fmnetv", "fluxupv", "fluxdnv", "fmneti", "fluxupi", "fluxdni")]
             sc = [o[k][i:i + 1] for k in ("nfluxtopv", "nfluxtopi", "diffvt")]
-            self.lib.ref_column(*[_p(a) for a in tabs], *[_p(a) for a in prof],
+            # the column's own aerosol optics (OPT_DST/OPT_CLD outputs) where given
+            tabs_i = [(_c(c[per_col[j]][i]) if j in per_col else a) for j, a in enumerate(tabs)]
+            self.lib.ref_column(*[_p(a) for a in tabs_i], *[_p(a) for a in prof],
                                 float(c["als"][i]), float(c["albi"][i]), float(c["acosz"][i]),
                                 *[_p(a) for a in outs], *[_p(a) for a in sc],
                                 _p(o["detau"][i]), _p(o["cumtauv"][i]))
@@ -235,6 +255,7 @@ class Reference:
                                    "qxvdst", "qsvdst", "gvdst", "qxvcld", "qsvcld", "gvcld",
                                    "qxidst", "qsidst", "gidst", "qxicld", "qsicld", "gicld",
                                    "qextrefdst", "qextrefcld")]
+        assert not any(k in c for k, _ in AER_FIELDS), "run_grid takes the shared aerosol tables; use run"
         prof = [_c(c[k][:ncol]) for k in ("plev", "tlev", "pmid", "tmid", "qh2o", "tauref", "taurefcld",
                                           "als", "albi", "acosz")]
         outs = [o[k] for k in ("fmnetv", "fluxupv", "fluxdnv", "fmneti", "fluxupi", "fluxdni",

@@ -19,6 +19,9 @@ Fixtures (all data: inputs and the reference's outputs):
                       (TAURAY, SOLARF), SETSPI (PLANCKIR), SETRAD (TGASREF, PFGASREF) and
                       a sample of LAGINTERP rows with their raw 11-pressure inputs.
   solver_ref.npz      DSOLVER/DTRIDGL and TPINDEX called directly on seeded inputs.
+  column_aer24.npz    32 seeded columns (24 layers, clouds) with per-column, per-level aerosol
+                      optics (mgcm_rt.synth.make_aerosol, regenerated from the seed and checked
+                      against the stored SHA-256), as OPT_DST/OPT_CLD hand them to OPTCV/OPTCI.
   column_real24.npz   the reference's own k-distribution: the raw 11-pressure records and
                       FZERO of run/data/CO2H2O_V_12_95_INTEL and CO2H2O_IR_12_95_INTEL (data
                       files, read as data), and 32 seeded columns (24 layers) run through the
@@ -87,6 +90,8 @@ def main(only=None) -> None:
     if only:
         if "column_real24" in only:
             real_columns(ref)
+        if "column_aer24" in only:
+            aerosol_columns(ref)
         return
     out = {}
     for name, seed, weights in (("column_ref24", 101, "reference"), ("column_dir24", 102, "dirichlet")):
@@ -155,7 +160,31 @@ def main(only=None) -> None:
     np.savez_compressed(os.path.join(HERE, "solver_ref.npz"), dsolver=np.array(cases), dsolver_nl=nl,
                         tpindex=np.array(tp))
     real_columns(ref)
+    aerosol_columns(ref)
     print("golden fixtures written to", HERE)
+
+
+def aerosol_digest(a: dict) -> str:
+    h = hashlib.sha256()
+    for k in sorted(a):
+        h.update(k.encode())
+        h.update(np.ascontiguousarray(a[k]).tobytes())
+    return h.hexdigest()
+
+
+def aerosol_columns(ref) -> None:
+    """column_aer24.npz: per-column, per-level aerosol optics through the reference."""
+    seed = 107
+    t, c = synth.make_workload(seed, 32, 24, clouds=True, weights="reference")
+    aer = synth.make_aerosol(seed, 32, 24)
+    o = ref.run(t, c | aer)
+    d = {"seed": seed, "nlay": 24, "ncol": 32, "weights": "reference", "table_sha256": table_digest(t),
+         "aerosol_sha256": aerosol_digest(aer)}
+    for k in IN_KEYS:
+        d["in_" + k] = c[k]
+    for k, v in o.items():
+        d["out_" + k] = v
+    np.savez_compressed(os.path.join(HERE, "column_aer24.npz"), **d)
 
 
 def real_columns(ref) -> None:

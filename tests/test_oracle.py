@@ -309,3 +309,15 @@ def test_laginterp_matches_reference_rows(oracle):
         row = got[f["lag_rows"][0][2], :, f["lag_rows"][0][3]]
         pos = r > 1e-200
         np.testing.assert_allclose(row[0::5][pos], r[pos], rtol=1e-12)
+
+
+def test_oracle_bit_exact_vs_reference_per_level_aerosol(oracle):
+    """Per-column, per-level aerosol optics (OPT_DST/OPT_CLD outputs) through OPTCV/OPTCI."""
+    from make_golden import aerosol_digest
+    from mgcm_rt import synth
+    t, c, want = load_fixture("column_aer24.npz")
+    aer = synth.make_aerosol(int(np.load(os.path.join(GOLDEN, "column_aer24.npz"))["seed"]), 32, 24)
+    assert aerosol_digest(aer) == str(np.load(os.path.join(GOLDEN, "column_aer24.npz"))["aerosol_sha256"])
+    got = oracle.run(t, c | aer, nthreads=2)
+    for k in OUT_KEYS:
+        assert np.array_equal(got[k], want[k]), k
