@@ -180,6 +180,47 @@ int mgcm_rt_laginterp(const double *pgref, const double *raw, int nbands, double
  * short, unframed or truncated. */
 int mgcm_rt_read_ktable(const char *path, int nbands, double *raw, double *fzero);
 
+/* ---- Aerosol producers of the radiation's inputs (active dust and clouds) ----
+ *
+ * Replace INITCLD's set-up of the radiative size grid and optical tables
+ * (code/initcld.f:131-184).  mgcm_rt_read_aerosol_tables reads
+ * datadir/Dust_vis_wolff2010_JD_12bands.dat, Dust_ir_wolff2010_JD_12bands.dat,
+ * waterCoated_vis_JD_12bands.dat and waterCoated_ir_JD_12bands.dat (host code,
+ * no device needed) into tables[MGCM_RT_AER_TABLE_DOUBLES]: radb_rt(21) at 0;
+ * dust Qext/Qscat/g(20,7,3) at 32 and (20,5,3) at 452; ice-coated dust
+ * (15,20,7,3) at 752 and (15,20,5,3) at 7052 (REAL*4 values, first index
+ * fastest); MGCM_RT_ERR_IO for a missing or malformed file.
+ * mgcm_rt_set_aerosol_tables uploads such a block to the context's device. */
+#define MGCM_RT_AER_TABLE_DOUBLES 11552
+int mgcm_rt_read_aerosol_tables(const char *datadir, double *tables);
+int mgcm_rt_set_aerosol_tables(mgcm_rt_ctx *ctx, const double *tables);
+
+/* Tracers of a grid of columns (device pointers): pl[ncol][nlev] is PL(L_LEVELS)
+ * in mbar; qtrace[ncol][6][nlay] is QTRACE(L_LAYERS, NTRACE) of each column:
+ * 1 dust mass, 2 dust number, 3 ice mass, 4 ice number, 5 dust-core mass,
+ * 6 vapour mixing ratio (code/initcld.f:37-48). */
+typedef struct {
+  int ncol, nlay;
+  const double *pl;
+  const double *qtrace;
+} mgcm_rt_tracers;
+
+/* Outputs of OPT_DST / OPT_CLD (device pointers), the layouts of mgcm_rt_aerosol:
+ * Qext/Qscat/g [ncol][7|5][nlev+1]; qextref and tauref [ncol][nlev+1] (the
+ * reference extinction, = Qext of VIS band 6, and TAUREF / TAUREFCLD); tau2
+ * [ncol][2] the diagnostics taudust/taucloud (REAL*4 sums), optional. */
+typedef struct {
+  double *qxv, *qsv, *gv;
+  double *qxi, *qsi, *gi;
+  double *qextref, *tauref;
+  double *tau2;
+} mgcm_rt_aerosol_out;
+
+/* OPT_DST (code/opt_dst.f) and OPT_CLD (code/opt_cld.f) for every column and
+ * layer, asynchronous on stream.  Needs mgcm_rt_set_aerosol_tables first. */
+int mgcm_rt_opt_dst(mgcm_rt_ctx *ctx, const mgcm_rt_tracers *in, const mgcm_rt_aerosol_out *out, void *stream);
+int mgcm_rt_opt_cld(mgcm_rt_ctx *ctx, const mgcm_rt_tracers *in, const mgcm_rt_aerosol_out *out, void *stream);
+
 /* Diagnostic: the device's exp, log10 and sqrt on n host arguments (x > 0 for
  * log10, x >= 0 for sqrt; other entries return 0).  The FP64 path reproduces the
  * reference's libm (exp and log10 of glibc 2.35, correctly rounded sqrt); this

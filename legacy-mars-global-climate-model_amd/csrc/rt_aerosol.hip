@@ -1,0 +1,302 @@
+// rt_aerosol.hip -- the aerosol producers of the radiation's inputs:
+//   * INITCLD's radiative size grid and optical tables (code/initcld.f:131-184):
+//     the 20-bin grid rad_rt/radb_rt, and Qext/Qscat/g per bin and band read
+//     from data/Dust_{vis,ir}_wolff2010_JD_12bands.dat and
+//     data/waterCoated_{vis,ir}_JD_12bands.dat (host code, once);
+//   * OPT_DST (code/opt_dst.f) and OPT_CLD (code/opt_cld.f): per column and
+//     layer, the log-normal size distribution of the dust (or ice-coated dust)
+//     tracers, its cross-section weights over the 20 bins (erf), and the
+//     weighted Qext/Qscat/g per band, the reference extinction and TAUREF (on the
+//     device, one 64-lane wave per column, lane l-1 = layer l).
+// The outputs are the per-level arrays of mgcm_rt_aerosol plus TAUREF/TAUREFCLD.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "rt_ctx.h"
+#include "rt_libm.h"
+
+namespace mgcm {
+namespace aer {
+
+constexpr int NBIN = 20, NRATIO = 15, NTRACE = 6;
+// offsets (doubles) into the device table block
+constexpr int OFF_RADB = 0;                                  // radb_rt(21)
+constexpr int OFF_DV = 32;                                   // dust VIS (3, 7, 20)
+constexpr int OFF_DI = OFF_DV + 3 * NV * NBIN;               // dust IR (3, 5, 20)
+constexpr int OFF_CV = OFF_DI + 3 * NI * NBIN;               // cloud VIS (3, 7, 20, 15)
+constexpr int OFF_CI = OFF_CV + 3 * NV * NBIN * NRATIO;      // cloud IR (3, 5, 20, 15)
+constexpr int TABLE_DOUBLES = OFF_CI + 3 * NI * NBIN * NRATIO;
+
+// code/modules.f90:1112-1114 (REAL*4)
+__constant__ float cor_ratio[NRATIO] = {0.1f, 0.2f, 0.25f, 0.3f, 0.35f, 0.4f, 0.45f, 0.5f,
+                                        0.55f, 0.6f, 0.65f, 0.7f, 0.8f, 0.9f, 0.99f};
+
+struct AerOut {
+  double *qxv, *qsv, *gv, *qxi, *qsi, *gi, *qextref, *tauref, *tau2;
+};
+
+// OPT_DST (CLOUD = false) / OPT_CLD (CLOUD = true) for one column per block.
+// qtr: [ncol][6][nlay]; pl: [ncol][nlev].
+template <bool CLOUD>
+__global__ __launch_bounds__(64) void k_opt_aer(const double *__restrict__ tab, int ncol, int nlay,
+                                                const double *__restrict__ qtr, const double *__restrict__ pl_all,
+                                                AerOut o) {
+  __shared__ double s_tau[64][4];
+  __shared__ int s_on[64];
+  const int col = blockIdx.x, lane = threadIdx.x;
+  if (col >= ncol) return;
+  const int nlevels = 2 * nlay + 3, ld = nlevels + 1;
+  const double athird = (double)(1.0f / 3.0f);        // code/initcld.f:57 (REAL(4) 1./3.)
+  const double dpden_dt = 2.5e3, dpden_ice = 917.;    // code/initdt.f:38, code/initcld.f:61
+  const double dev = CLOUD ? (double)0.3087f : (double)0.63676f;  // code/initcld.f:93-95
+  const double dev2 = 1. / ((double)sqrtf(2.f) * dev);
+  const double qs_init = (double)0.99f;
+  const double *q = qtr + (size_t)col * NTRACE * nlay;
+  const double *pl = pl_all + (size_t)col * nlevels;
+  double *qxv = o.qxv + (size_t)col * NV * ld, *qsv = o.qsv + (size_t)col * NV * ld;
+  double *gv = o.gv + (size_t)col * NV * ld, *qxi = o.qxi + (size_t)col * NI * ld;
+  double *qsi = o.qsi + (size_t)col * NI * ld, *gi = o.gi + (size_t)col * NI * ld;
+  double *qextref = o.qextref + (size_t)col * ld, *tauref = o.tauref + (size_t)col * ld;
+  auto defaults = [&](int k) {  // code/opt_dst.f:52-69: levels without aerosol
+    qextref[k - 1] = 1.;
+    tauref[k - 1] = 0.;
+    for (int w = 0; w < NV; w++) { qxv[w * ld + k - 1] = 1.; qsv[w * ld + k - 1] = qs_init; gv[w * ld + k - 1] = 0.; }
+    for (int w = 0; w < NI; w++) { qxi[w * ld + k - 1] = 1.; qsi[w * ld + k - 1] = qs_init; gi[w * ld + k - 1] = 0.; }
+  };
+  if (lane == 0) {
+    for (int k = 1; k <= 3; k++) defaults(k);
+    defaults(ld);
+  }
+  const int l = lane + 1;
+  bool on = false;
+  if (l <= nlay) {
+    auto QT = [&](int m) { return q[(m - 1) * nlay + l - 1]; };
+    double mo, no, cst;
+    int irap = 0;
+    if (!CLOUD) {  // iMa_dt = 1, iNb_dt = 2 (code/initcld.f:37-38)
+      on = QT(1) > (double)1.e-8f && QT(2) > 1.;
+      mo = QT(1);
+      no = QT(2) + 1.;
+      cst = 0.75 / (PI * dpden_dt);
+    } else {       // iMa_cld = 3, iNb_cld = 4, iMa_cor = 5
+      on = QT(3) + QT(5) > (double)1.e-7f && QT(4) > 1.;
+      mo = QT(3) + QT(5);
+      no = QT(4);
+      cst = 0.;
+      if (on) {
+        float mtc = (float)(QT(5) / dpden_dt / (QT(3) / dpden_ice + QT(5) / dpden_dt));
+        mtc = (float)pow((double)mtc, athird);
+        irap = NRATIO;
+        for (int i = 1; i <= NRATIO; i++) {  // code/opt_cld.f:104-116
+          if (mtc < cor_ratio[i - 1] && i != 1) { irap = i - 1; break; }
+          else if (mtc == cor_ratio[i - 1]) { irap = i; break; }
+          else if (mtc < cor_ratio[i - 1] && i == 1) { irap = 1; break; }
+        }
+        const double dens = QT(3) / mo * dpden_ice + QT(5) / mo * dpden_dt;
+        cst = 0.75 / (PI * dens);
+      }
+    }
+    if (!on) {
+      defaults(2 * l + 2);
+      defaults(2 * l + 3);
+    } else {
+      double rs = pow(mo / no * cst, athird) * mgcm_libm::mgcm_exp(-0.5 * (dev * dev));
+      const double ao = no * PI * (rs * rs);
+      rs = rs * mgcm_libm::mgcm_exp(1.5 * (dev * dev));
+      rs = fmin(fmax(rs, (double)1.e-7f), CLOUD ? (double)100.e-6f : (double)50.e-6f);
+      rs = 1. / rs;
+      // surf(i) = (erf(bin top) - erf(bin bottom))/2; adjacent bins share an edge value
+      double surf[NBIN];
+      double e0 = erf(mgcm_libm::mgcm_log(tab[OFF_RADB] * rs) * dev2);
+#pragma unroll
+      for (int i = 0; i < NBIN; i++) {
+        const double e1 = erf(mgcm_libm::mgcm_log(tab[OFF_RADB + i + 1] * rs) * dev2);
+        surf[i] = 0.5 * (e1 - e0);
+        e0 = e1;
+      }
+      const double *tv = tab + (CLOUD ? OFF_CV : OFF_DV), *ti = tab + (CLOUD ? OFF_CI : OFF_DI);
+      const int sv = CLOUD ? NV * NBIN * NRATIO : NV * NBIN, si = CLOUD ? NI * NBIN * NRATIO : NI * NBIN;
+      auto idx = [&](int w, int i) { return CLOUD ? (irap - 1) + NRATIO * (i + NBIN * w) : i + NBIN * w; };
+      double qx6 = 0.0, qxi4 = 0.0, qsi4 = 0.0;
+      for (int w = 0; w < NV; w++) {
+        double x = 0., sc = 0., g = 0.;
+        for (int i = 0; i < NBIN; i++) {
+          const int oo = idx(w, i);
+          x = x + surf[i] * tv[oo];
+          sc = sc + surf[i] * tv[oo + sv];
+          g = g + surf[i] * tv[oo + 2 * sv];
+        }
+        sc = fmin(sc, (double)0.99999f * x);  // code/opt_dst.f:136
+        for (int k = 2 * l + 2; k <= 2 * l + 3; k++) {
+          qxv[w * ld + k - 1] = x;
+          qsv[w * ld + k - 1] = sc;
+          gv[w * ld + k - 1] = g;
+        }
+        if (w == L_NREFV - 1) qx6 = x;
+      }
+      for (int w = 0; w < NI; w++) {
+        double x = 0., sc = 0., g = 0.;
+        for (int i = 0; i < NBIN; i++) {
+          const int oo = idx(w, i);
+          x = x + surf[i] * ti[oo];
+          sc = sc + surf[i] * ti[oo + si];
+          g = g + surf[i] * ti[oo + 2 * si];
+        }
+        sc = fmin(sc, (double)0.99999f * x);
+        for (int k = 2 * l + 2; k <= 2 * l + 3; k++) {
+          qxi[w * ld + k - 1] = x;
+          qsi[w * ld + k - 1] = sc;
+          gi[w * ld + k - 1] = g;
+        }
+        if (w == 3) { qxi4 = x; qsi4 = sc; }
+      }
+      // code/opt_dst.f:151-160: reference extinction, TAUREF, diagnostic terms
+      for (int nn = 1; nn <= 2; nn++) {
+        const int k = 2 * l + 1 + nn;
+        qextref[k - 1] = qx6;
+        const double mass = 100. * (pl[k - 1] - pl[k - 2]) / GRAV;
+        const double t = ao * qx6 * mass;
+        tauref[k - 1] = t;
+        s_tau[lane][2 * (nn - 1)] = t;
+        s_tau[lane][2 * (nn - 1) + 1] = t / qx6 * (qxi4 - qsi4);
+      }
+    }
+  }
+  s_on[lane] = on ? 1 : 0;
+  __syncthreads();
+  if (lane == 0 && o.tau2) {  // taudust/taucloud: REAL*4 sums in layer order
+    float t1 = 0.f, t2 = 0.f;
+    for (int j = 0; j < nlay; j++) {
+      if (!s_on[j]) continue;
+      for (int nn = 0; nn < 2; nn++) {
+        t1 = (float)((double)t1 + s_tau[j][2 * nn]);
+        t2 = (float)((double)t2 + s_tau[j][2 * nn + 1]);
+      }
+    }
+    o.tau2[2 * col] = t1;
+    o.tau2[2 * col + 1] = t2;
+  }
+}
+
+// One Fortran formatted record of n fields of width w separated by one blank
+// ("(n(Ew.d,x))"); a short line reads as blanks (zero).
+static bool read_fields(FILE *f, int n, int w, float *out) {
+  char line[512];
+  if (!fgets(line, sizeof line, f)) return false;
+  const size_t len = strcspn(line, "\r\n");
+  for (int j = 0; j < n; j++) {
+    char field[64] = {0};
+    const size_t a = (size_t)j * (w + 1);
+    if (a < len) memcpy(field, line + a, std::min((size_t)w, len - a));
+    char *end = nullptr;
+    out[j] = strtof(field, &end);  // decimal -> REAL(4), correctly rounded
+    for (const char *p = end; *p; p++)
+      if (*p != ' ') return false;
+  }
+  return true;
+}
+
+}  // namespace aer
+}  // namespace mgcm
+
+using namespace mgcm;
+using namespace mgcm::aer;
+
+// INITCLD's radiative size grid (code/initcld.f:131-146): host C library, as
+// the reference computes it.
+static void aer_grid(double *rad_rt, double *radb_rt) {
+  const double athird = (double)(1.0f / 3.0f);
+  rad_rt[0] = (double)1.e-7f;
+  rad_rt[NBIN - 1] = (double)50.e-6f;
+  radb_rt[0] = (double)0.0001e-6f;
+  radb_rt[NBIN] = (double)1.e-2f;
+  double vrat_rt = log(rad_rt[NBIN - 1] / rad_rt[0]) / (double)(float)(NBIN - 1) * 3.;
+  vrat_rt = exp(vrat_rt);
+  for (int i = 1; i <= NBIN - 1; i++) {
+    rad_rt[i] = rad_rt[i - 1] * pow(vrat_rt, athird);
+    radb_rt[i] = pow((2. * vrat_rt) / (vrat_rt + 1.), athird) * rad_rt[i - 1];
+  }
+}
+
+extern "C" int mgcm_rt_read_aerosol_tables(const char *datadir, double *tables) {
+  if (!datadir || !tables) return MGCM_RT_ERR_ARG;
+  std::vector<double> t(TABLE_DOUBLES, 0.0);
+  double rad[NBIN];
+  aer_grid(rad, &t[OFF_RADB]);
+  const std::string dir(datadir);
+  FILE *f60 = fopen((dir + "/waterCoated_vis_JD_12bands.dat").c_str(), "r");
+  FILE *f61 = fopen((dir + "/waterCoated_ir_JD_12bands.dat").c_str(), "r");
+  FILE *f62 = fopen((dir + "/Dust_vis_wolff2010_JD_12bands.dat").c_str(), "r");
+  FILE *f63 = fopen((dir + "/Dust_ir_wolff2010_JD_12bands.dat").c_str(), "r");
+  bool ok = f60 && f61 && f62 && f63;
+  float v[7];
+  // code/initcld.f:158-167: per core ratio j and bin i, three records per file
+  for (int j = 0; ok && j < NRATIO; j++)
+    for (int i = 0; ok && i < NBIN; i++)
+      for (int kind = 0; ok && kind < 3; kind++) {
+        ok = read_fields(f60, NV, 12, v);
+        for (int w = 0; ok && w < NV; w++) t[OFF_CV + j + NRATIO * (i + NBIN * (w + NV * kind))] = v[w];
+        ok = ok && read_fields(f61, NI, 12, v);
+        for (int w = 0; ok && w < NI; w++) t[OFF_CI + j + NRATIO * (i + NBIN * (w + NI * kind))] = v[w];
+      }
+  // code/initcld.f:169-182 (the IR scaling factor there is 1.0)
+  for (int i = 0; ok && i < NBIN; i++)
+    for (int kind = 0; ok && kind < 3; kind++) {
+      ok = read_fields(f62, NV, 11, v);
+      for (int w = 0; ok && w < NV; w++) t[OFF_DV + i + NBIN * (w + NV * kind)] = v[w];
+      ok = ok && read_fields(f63, NI, 11, v);
+      for (int w = 0; ok && w < NI; w++) t[OFF_DI + i + NBIN * (w + NI * kind)] = v[w];
+    }
+  for (FILE *f : {f60, f61, f62, f63})
+    if (f) fclose(f);
+  if (!ok) return MGCM_RT_ERR_IO;
+  memcpy(tables, t.data(), sizeof(double) * TABLE_DOUBLES);
+  return MGCM_RT_OK;
+}
+
+extern "C" int mgcm_rt_set_aerosol_tables(mgcm_rt_ctx *c, const double *tables) {
+  if (!c || !tables) return MGCM_RT_ERR_ARG;
+  if (!c->d_aer && hipMalloc(&c->d_aer, sizeof(double) * TABLE_DOUBLES) != hipSuccess) {
+    c->d_aer = nullptr;
+    return MGCM_RT_ERR_NOMEM;
+  }
+  if (hipMemcpy(c->d_aer, tables, sizeof(double) * TABLE_DOUBLES, hipMemcpyHostToDevice) != hipSuccess)
+    return MGCM_RT_ERR_DEVICE;
+  return MGCM_RT_OK;
+}
+
+static int opt_aer(bool cloud, mgcm_rt_ctx *c, const mgcm_rt_tracers *in, const mgcm_rt_aerosol_out *out,
+                   void *stream) {
+  if (!c || !in || !out) return MGCM_RT_ERR_ARG;
+  if (!c->d_aer) return MGCM_RT_ERR_ARG;  // mgcm_rt_set_aerosol_tables first
+  if (in->ncol < 0) return MGCM_RT_ERR_ARG;
+  if (in->nlay < 1 || in->nlay > MAX_NLAY) return MGCM_RT_ERR_NLAY;
+  if (in->ncol == 0) return MGCM_RT_OK;
+  const void *req[] = {in->pl, in->qtrace, out->qxv, out->qsv, out->gv, out->qxi, out->qsi, out->gi,
+                       out->qextref, out->tauref};
+  for (const void *p : req)
+    if (!p) return MGCM_RT_ERR_ARG;
+  AerOut o{out->qxv, out->qsv, out->gv, out->qxi, out->qsi, out->gi, out->qextref, out->tauref, out->tau2};
+  hipStream_t st = (hipStream_t)stream;
+  if (cloud)
+    hipLaunchKernelGGL(k_opt_aer<true>, dim3(in->ncol), dim3(64), 0, st, c->d_aer, in->ncol, in->nlay,
+                       in->qtrace, in->pl, o);
+  else
+    hipLaunchKernelGGL(k_opt_aer<false>, dim3(in->ncol), dim3(64), 0, st, c->d_aer, in->ncol, in->nlay,
+                       in->qtrace, in->pl, o);
+  return hipGetLastError() == hipSuccess ? MGCM_RT_OK : MGCM_RT_ERR_DEVICE;
+}
+
+extern "C" int mgcm_rt_opt_dst(mgcm_rt_ctx *c, const mgcm_rt_tracers *in, const mgcm_rt_aerosol_out *out,
+                               void *stream) {
+  return opt_aer(false, c, in, out, stream);
+}
+
+extern "C" int mgcm_rt_opt_cld(mgcm_rt_ctx *c, const mgcm_rt_tracers *in, const mgcm_rt_aerosol_out *out,
+                               void *stream) {
+  return opt_aer(true, c, in, out, stream);
+}

@@ -118,7 +118,8 @@ extern "C" int mgcm_rt_init(const mgcm_rt_tables *t, mgcm_rt_ctx **out) {
 
 extern "C" int mgcm_rt_destroy(mgcm_rt_ctx *c) {
   if (!c) return MGCM_RT_ERR_ARG;
-  void *bufs[] = {c->d_const, c->d_kv, c->d_ki, c->d_kv32, c->d_ki32, c->d_planck, c->d_flag, c->d_host, c->d_clr};
+  void *bufs[] = {c->d_const, c->d_kv, c->d_ki,   c->d_kv32, c->d_ki32,
+                  c->d_planck, c->d_flag, c->d_host, c->d_clr, c->d_aer};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   if (c->batch) mgcm::batch_free(c->batch);

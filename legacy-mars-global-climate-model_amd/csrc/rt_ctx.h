@@ -18,6 +18,8 @@ struct mgcm_rt_ctx {
   double *d_host;
   // batched-path state (rt_batched.hip)
   void *batch;
+  // aerosol producers' size grid and optical tables (rt_aerosol.hip)
+  double *d_aer;
 };
 
 namespace mgcm {

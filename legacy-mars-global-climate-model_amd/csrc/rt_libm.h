@@ -185,11 +185,9 @@ MGCM_HD DD dd_div(DD a, DD b) {
   return dd_fast(q1, r.h / b.h);
 }
 
-// Correctly rounded natural log of a positive normal double in [0.5, 2).
-MGCM_HD double cr_log_unit(double m) {
-  int k = 0;
-  if (m < 0.70710678118654752) { m *= 2.0; k = -1; }
-  else if (m >= 1.4142135623730951) { m *= 0.5; k = 1; }
+// k*ln2 + log(m) for m in [sqrt(1/2), sqrt(2)), rounded once from double-double
+// (atanh series, 2^-100 relative): correctly rounded but for ties closer than that.
+MGCM_HD double log_dd_round(double m, int k) {
   const DD s = dd_div({m - 1.0, 0.0}, dd_two_sum(m, 1.0));
   const DD t = dd_mul(s, s);
   DD p = {0.0, 0.0};
@@ -201,6 +199,23 @@ MGCM_HD double cr_log_unit(double m) {
   const DD ln2 = {0x1.62e42fefa39efp-1, 0x1.abc9e3b39803fp-56};
   const DD r = dd_add(dd_mul({(double)k, 0.0}, ln2), L);
   return r.h + r.l;
+}
+
+// Correctly rounded natural log of a positive normal double in [0.5, 2).
+MGCM_HD double cr_log_unit(double m) {
+  int k = 0;
+  if (m < 0.70710678118654752) { m *= 2.0; k = -1; }
+  else if (m >= 1.4142135623730951) { m *= 0.5; k = 1; }
+  return log_dd_round(m, k);
+}
+
+// Correctly rounded natural log of a positive normal double.
+MGCM_HD double mgcm_log(double x) {
+  const uint64_t u = asu(x);
+  int k = (int)((u >> 52) & 0x7ff) - 1023;
+  double m = asd((u & 0x000fffffffffffffull) | 0x3ff0000000000000ull);  // [1, 2)
+  if (m >= 1.4142135623730951) { m *= 0.5; k += 1; }
+  return log_dd_round(m, k);
 }
 
 MGCM_HD double mgcm_log10(double x) {

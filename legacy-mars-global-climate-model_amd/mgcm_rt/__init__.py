@@ -69,6 +69,18 @@ class Columns(C.Structure):
         ("sol", C.c_double * NV), ("aerosol", C.POINTER(Aerosol))]
 
 
+class Tracers(C.Structure):
+    _fields_ = [("ncol", C.c_int), ("nlay", C.c_int), ("pl", C.c_void_p), ("qtrace", C.c_void_p)]
+
+
+AER_OUT_KEYS = ("qxv", "qsv", "gv", "qxi", "qsi", "gi", "qextref", "tauref", "tau2")
+AER_TABLE_DOUBLES = 11552
+
+
+class AerosolOut(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in AER_OUT_KEYS]
+
+
 class Fluxes(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in (
         "fmnetv", "fluxupv", "fluxdnv", "fmneti", "fluxupi", "fluxdni",
@@ -112,6 +124,10 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         lib.mgcm_rt_planck_table.argtypes = [DP, DP, DP]
         lib.mgcm_rt_laginterp.argtypes = [DP, DP, C.c_int, DP, DP]
         lib.mgcm_rt_read_ktable.argtypes = [C.c_char_p, C.c_int, DP, DP]
+        lib.mgcm_rt_read_aerosol_tables.argtypes = [C.c_char_p, DP]
+        lib.mgcm_rt_set_aerosol_tables.argtypes = [C.c_void_p, DP]
+        lib.mgcm_rt_opt_dst.argtypes = [C.c_void_p, C.POINTER(Tracers), C.POINTER(AerosolOut), C.c_void_p]
+        lib.mgcm_rt_opt_cld.argtypes = [C.c_void_p, C.POINTER(Tracers), C.POINTER(AerosolOut), C.c_void_p]
         lib.mgcm_rt_libm_check.argtypes = [DP, C.c_int, DP, DP, DP]
         lib.mgcm_rt_strerror.restype = C.c_char_p
         lib.mgcm_rt_strerror.argtypes = [C.c_int]
@@ -241,6 +257,35 @@ class Radiation:
     def check(self) -> None:
         _check(self.lib.mgcm_rt_check(self.ctx), "mgcm_rt_check")
 
+    def set_aerosol_tables(self, tables: np.ndarray) -> None:
+        """Upload INITCLD's size grid and optical tables (read_aerosol_tables' block)."""
+        t = _f64(tables)
+        assert t.size == AER_TABLE_DOUBLES
+        _check(self.lib.mgcm_rt_set_aerosol_tables(self.ctx, _dp(t)), "mgcm_rt_set_aerosol_tables")
+
+    def opt_aerosol(self, cloud: bool, pl, qtrace, out: dict, stream: int = 0) -> None:
+        """OPT_DST (cloud False) / OPT_CLD on device tensors: pl [ncol][nlev],
+        qtrace [ncol][6][nlay]; out holds AER_OUT_KEYS tensors (tau2 optional)."""
+        ncol, nlay = int(qtrace.shape[0]), int(qtrace.shape[2])
+        tr = Tracers(ncol, nlay, pl.data_ptr(), qtrace.data_ptr())
+        ao = AerosolOut(*[(out[k].data_ptr() if out.get(k) is not None else None) for k in AER_OUT_KEYS])
+        fn = self.lib.mgcm_rt_opt_cld if cloud else self.lib.mgcm_rt_opt_dst
+        _check(fn(self.ctx, C.byref(tr), C.byref(ao), C.c_void_p(stream)), "mgcm_rt_opt_cld" if cloud else "mgcm_rt_opt_dst")
+
+    def opt_aerosol_host(self, cloud: bool, pl: np.ndarray, qtrace: np.ndarray) -> dict:
+        """opt_aerosol on host arrays (copies to the device and back; synchronises)."""
+        import torch
+        ncol, nlay = qtrace.shape[0], qtrace.shape[2]
+        ld = 2 * nlay + 4
+        dev = {"pl": torch.from_numpy(_f64(pl)).cuda(), "qtrace": torch.from_numpy(_f64(qtrace)).cuda()}
+        shapes = {"qxv": (ncol, NV, ld), "qsv": (ncol, NV, ld), "gv": (ncol, NV, ld), "qxi": (ncol, NI, ld),
+                  "qsi": (ncol, NI, ld), "gi": (ncol, NI, ld), "qextref": (ncol, ld), "tauref": (ncol, ld),
+                  "tau2": (ncol, 2)}
+        out = {k: torch.empty(v, dtype=torch.float64, device="cuda") for k, v in shapes.items()}
+        self.opt_aerosol(cloud, dev["pl"], dev["qtrace"], out, torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        return {k: v.cpu().numpy() for k, v in out.items()}
+
 
 def planck_table():
     """PLANCKIR(5,8501) as C (8501, 5), DWNI, WNOI -- built on the device."""
@@ -259,6 +304,20 @@ def laginterp(pgref11, raw) -> tuple:
     pint = np.zeros(51)
     _check(lib.mgcm_rt_laginterp(_dp(_f64(pgref11)), _dp(raw), nb, _dp(pint), _dp(out)), "mgcm_rt_laginterp")
     return pint, out
+
+
+def read_aerosol_tables(datadir: str) -> np.ndarray:
+    """INITCLD's size grid and the four aerosol optical-property files of datadir (host)."""
+    lib = load_library()
+    t = np.zeros(AER_TABLE_DOUBLES)
+    _check(lib.mgcm_rt_read_aerosol_tables(os.fsencode(datadir), _dp(t)), "mgcm_rt_read_aerosol_tables")
+    return t
+
+
+def aerosol_table_views(t: np.ndarray) -> dict:
+    """Named views of a read_aerosol_tables block (C shapes of the Fortran arrays)."""
+    return {"radb_rt": t[0:21], "q_v_dst": t[32:452].reshape(3, NV, 20), "q_i_dst": t[452:752].reshape(3, NI, 20),
+            "q_v_cld": t[752:7052].reshape(3, NV, 20, 15), "q_i_cld": t[7052:11552].reshape(3, NI, 20, 15)}
 
 
 def read_ktable(path: str, nbands: int) -> tuple:

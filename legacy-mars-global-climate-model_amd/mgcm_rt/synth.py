@@ -358,6 +358,22 @@ def make_aerosol(seed: int, ncol: int, nlay: int, dust: bool = True, cloud: bool
     return {k: np.ascontiguousarray(v) for k, v in out.items()}
 
 
+def make_tracers(seed: int, ncol: int, nlay: int) -> np.ndarray:
+    """QTRACE(L_LAYERS, NTRACE) per column, [ncol][6][nlay]: dust mass and number,
+    ice mass and number, dust-core mass, vapour (code/initcld.f:37-48).
+
+    Generator decision: log-uniform mixing ratios spanning both sides of OPT_DST's and
+    OPT_CLD's thresholds (mass 1e-8 / 1e-7, number 1), dust mass 1e-10..1e-4 kg/kg and
+    number 1e-1..1e10 /kg, ice mass 1e-10..1e-4 and number 1e-1..1e8, core mass
+    1e-12..1e-6, vapour 1e-7..1e-3; one layer in eight of each species is empty."""
+    rng = np.random.Generator(np.random.PCG64(seed + 15485863))
+    q = np.empty((ncol, 6, nlay))
+    for m, (lo, hi) in enumerate(((-10, -4), (-1, 10), (-10, -4), (-1, 8), (-12, -6), (-7, -3))):
+        q[:, m] = 10.0 ** rng.uniform(lo, hi, (ncol, nlay))
+        q[:, m][rng.random((ncol, nlay)) < 0.125] = 0.0
+    return q
+
+
 def make_workload(seed: int, ncol: int, nlay: int, clouds: bool = False, weights: str = "dirichlet"):
     return make_tables(seed, nlay, weights), make_columns(seed, ncol, nlay, clouds)
 

@@ -19,11 +19,11 @@ trap 'rm -rf "$TMP"' EXIT
 mkdir -p "$OUT"
 SRCS="modules.f90 tpindex.f90 optcv.f90 optci.f90 getdetau.f90 gfluxv.f90 gfluxi.f90
       dsolver.f90 dtridgl.f90 sfluxv.f90 sfluxi.f90 setspv.f90 setspi.f90 setrad.f90
-      laginterp.f90 lagrange.f90"
+      laginterp.f90 lagrange.f90 initcld.f opt_dst.f opt_cld.f"
 cd "$TMP"
 # -O2 without fast-math; x86-64 baseline (no FMA), as the reference's own gfortran -O3 build.
 for f in $SRCS; do
-  "$FC" -O2 -fPIC -w -c "$REF/$f" -o "${f%.f90}.o"
+  "$FC" -O2 -fPIC -w -c "$REF/$f" -o "${f%.*}.o"
 done
 "$FC" -O2 -fPIC -c "$HERE/ref_harness.f90" -o ref_harness.o
 "$FC" -shared -fPIC -o "$OUT/libmgcm_ref.so.tmp" ./*.o

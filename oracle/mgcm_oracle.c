@@ -996,3 +996,143 @@ void or_setspv(double *wnov, double *dwnv, double *solarf, double *tauray)
                 ((double)1.527f * (1.0 + (double)0.013f / (wl * wl)) / ((wl * wl) * (wl * wl))) * SCALEP / p0;
   }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Aerosol producers. */
+
+/* INITCLD: code/initcld.f:131-146 (REAL(4) literals promoted; ATHIRD = 1./3.
+ * evaluated in REAL(4), code/initcld.f:57). */
+void or_aer_grid(double *rad_rt, double *radb_rt)
+{
+  const int nbin_rt = 20;
+  const double athird = (double)(1.0f / 3.0f);
+  double vrat_rt;
+  int i;
+  rad_rt[0] = (double)1.e-7f;
+  rad_rt[nbin_rt - 1] = (double)50.e-6f;
+  radb_rt[0] = (double)0.0001e-6f;
+  radb_rt[nbin_rt] = (double)1.e-2f;
+  vrat_rt = log(rad_rt[nbin_rt - 1] / rad_rt[0]) / (double)(float)(nbin_rt - 1) * 3.;
+  vrat_rt = exp(vrat_rt);
+  for (i = 1; i <= nbin_rt - 1; i++) {
+    rad_rt[i] = rad_rt[i - 1] * pow(vrat_rt, athird);
+    radb_rt[i] = pow((2. * vrat_rt) / (vrat_rt + 1.), athird) * rad_rt[i - 1];
+  }
+}
+
+/* OPT_DST code/opt_dst.f:45-163 and OPT_CLD code/opt_cld.f:45-189. */
+void or_opt_aer(int cloud, const or_aer_tables *tb, int ncol, int nlay, const double *qtr,
+                const double *pl_all, double *qxv_all, double *qsv_all, double *gv_all, double *qxi_all,
+                double *qsi_all, double *gi_all, double *qextref_all, double *tauref_all, double *tau2)
+{
+  enum { NBIN = 20, NRATIO = 15 };
+  static const float cor_ratio[NRATIO] = {0.1f, 0.2f, 0.25f, 0.3f, 0.35f, 0.4f, 0.45f, 0.5f,
+                                          0.55f, 0.6f, 0.65f, 0.7f, 0.8f, 0.9f, 0.99f};
+  const int nlevels = 2 * nlay + 3, ld = nlevels + 1;
+  const double athird = (double)(1.0f / 3.0f);
+  const double dpden_dt = 2.5e3, dpden_ice = 917.;
+  const double dev = cloud ? (double)0.3087f : (double)0.63676f;
+  const double dev2 = 1. / ((double)sqrtf(2.f) * dev);
+  const double pi = 3.1415926535897932, grav = 3.72;
+  int col;
+  for (col = 0; col < ncol; col++) {
+    const double *q = qtr + (size_t)col * 6 * nlay; /* QTRACE(l, m) = q[(m-1)*nlay + l-1] */
+    const double *pl = pl_all + (size_t)col * nlevels;
+    double *qxv = qxv_all + (size_t)col * NV * ld, *qsv = qsv_all + (size_t)col * NV * ld;
+    double *gv = gv_all + (size_t)col * NV * ld, *qxi = qxi_all + (size_t)col * NI * ld;
+    double *qsi = qsi_all + (size_t)col * NI * ld, *gi = gi_all + (size_t)col * NI * ld;
+    double *qextref = qextref_all + (size_t)col * ld, *tauref = tauref_all + (size_t)col * ld;
+    float t1 = 0.f, t2 = 0.f;
+    int k, l, i, w;
+#define QT(l_, m_) q[((m_)-1) * nlay + (l_)-1]
+    for (k = 1; k <= ld; k++) {
+      qextref[k - 1] = 1.;
+      tauref[k - 1] = 0.;
+    }
+    for (w = 0; w < NV; w++)
+      for (k = 1; k <= ld; k++) {
+        qxv[w * ld + k - 1] = qextref[k - 1];
+        qsv[w * ld + k - 1] = qextref[k - 1] * (double)0.99f;
+        gv[w * ld + k - 1] = 0.;
+      }
+    for (w = 0; w < NI; w++)
+      for (k = 1; k <= ld; k++) {
+        qxi[w * ld + k - 1] = qextref[k - 1];
+        qsi[w * ld + k - 1] = qextref[k - 1] * (double)0.99f;
+        gi[w * ld + k - 1] = 0.;
+      }
+    for (l = 1; l <= nlay; l++) {
+      double rs, ao, mo, no, surf[NBIN], e[NBIN + 1], cst;
+      int irap = 0, nn;
+      if (!cloud) {
+        /* iMa_dt = 1, iNb_dt = 2 (code/initcld.f:37-38) */
+        if (!(QT(l, 1) > (double)1.e-8f && QT(l, 2) > 1.)) continue;
+        mo = QT(l, 1);
+        no = QT(l, 2) + 1.;
+        cst = 0.75 / (pi * dpden_dt);
+      } else {
+        /* iMa_cld = 3, iNb_cld = 4, iMa_cor = 5 */
+        float mtc;
+        double dens;
+        if (!(QT(l, 3) + QT(l, 5) > (double)1.e-7f && QT(l, 4) > 1.)) continue;
+        mtc = (float)(QT(l, 5) / dpden_dt / (QT(l, 3) / dpden_ice + QT(l, 5) / dpden_dt));
+        mtc = (float)pow((double)mtc, athird);
+        irap = NRATIO;
+        for (i = 1; i <= NRATIO; i++) {
+          if (mtc < cor_ratio[i - 1] && i != 1) { irap = i - 1; break; }
+          else if (mtc == cor_ratio[i - 1]) { irap = i; break; }
+          else if (mtc < cor_ratio[i - 1] && i == 1) { irap = 1; break; }
+        }
+        mo = QT(l, 3) + QT(l, 5);
+        no = QT(l, 4);
+        dens = QT(l, 3) / mo * dpden_ice + QT(l, 5) / mo * dpden_dt;
+        cst = 0.75 / (pi * dens);
+      }
+      rs = pow(mo / no * cst, athird) * exp(-0.5 * (dev * dev));
+      ao = no * pi * (rs * rs);
+      rs = rs * exp(1.5 * (dev * dev));
+      rs = fmin(fmax(rs, (double)1.e-7f), cloud ? (double)100.e-6f : (double)50.e-6f);
+      rs = 1. / rs;
+      for (i = 0; i <= NBIN; i++) e[i] = erf(log(tb->radb_rt[i] * rs) * dev2);
+      for (i = 0; i < NBIN; i++) surf[i] = 0.5 * (e[i + 1] - e[i]);
+      for (nn = 1; nn <= 2; nn++) {
+        const int kk = 2 * l + 1 + nn;
+        double mass;
+        for (w = 0; w < NV; w++) {
+          double x = 0., sc = 0., g = 0.;
+          for (i = 0; i < NBIN; i++) {
+            const size_t o = cloud ? (size_t)(irap - 1) + NRATIO * (i + NBIN * (size_t)w) : (size_t)i + NBIN * (size_t)w;
+            const size_t st = cloud ? (size_t)NRATIO * NBIN * NV : (size_t)NBIN * NV;
+            x = x + surf[i] * tb->q_v[o];
+            sc = sc + surf[i] * tb->q_v[o + st];
+            g = g + surf[i] * tb->q_v[o + 2 * st];
+          }
+          qxv[w * ld + kk - 1] = x;
+          qsv[w * ld + kk - 1] = fmin(sc, (double)0.99999f * x);
+          gv[w * ld + kk - 1] = g;
+        }
+        for (w = 0; w < NI; w++) {
+          double x = 0., sc = 0., g = 0.;
+          for (i = 0; i < NBIN; i++) {
+            const size_t o = cloud ? (size_t)(irap - 1) + NRATIO * (i + NBIN * (size_t)w) : (size_t)i + NBIN * (size_t)w;
+            const size_t st = cloud ? (size_t)NRATIO * NBIN * NI : (size_t)NBIN * NI;
+            x = x + surf[i] * tb->q_i[o];
+            sc = sc + surf[i] * tb->q_i[o + st];
+            g = g + surf[i] * tb->q_i[o + 2 * st];
+          }
+          qxi[w * ld + kk - 1] = x;
+          qsi[w * ld + kk - 1] = fmin(sc, (double)0.99999f * x);
+          gi[w * ld + kk - 1] = g;
+        }
+        qextref[kk - 1] = qxv[5 * ld + kk - 1]; /* L_NREFV = 6 */
+        mass = 100. * (pl[kk - 1] - pl[kk - 2]) / grav;
+        tauref[kk - 1] = ao * qextref[kk - 1] * mass;
+        t1 = (float)((double)t1 + tauref[kk - 1]);
+        t2 = (float)((double)t2 + tauref[kk - 1] / qextref[kk - 1] * (qxi[3 * ld + kk - 1] - qsi[3 * ld + kk - 1]));
+      }
+    }
+#undef QT
+    tau2[2 * col] = t1;
+    tau2[2 * col + 1] = t2;
+  }
+}

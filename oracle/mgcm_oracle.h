@@ -114,6 +114,24 @@ void or_laginterp(const double *pgref11, const double *co2v8, const double *co2i
 void or_planck_table(double *planckir, double *dwni, double *wnoi);
 void or_setspv(double *wnov, double *dwnv, double *solarf, double *tauray);
 
+/* Aerosol producers.  INITCLD's radiative size grid (code/initcld.f:131-146):
+ * rad_rt(20), radb_rt(21). */
+void or_aer_grid(double *rad_rt, double *radb_rt);
+
+/* OPT_DST (cloud = 0, code/opt_dst.f) or OPT_CLD (cloud = 1, code/opt_cld.f) on
+ * ncol columns of nlay layers.  Tables as INITCLD reads them (REAL*4 values):
+ * dust q_v (20,7,3), q_i (20,5,3); cloud q_v (15,20,7,3), q_i (15,20,5,3); the
+ * last index is Qext, Qscat, g.  qtr: [ncol][6][nlay] QTRACE(L_LAYERS, NTRACE);
+ * pl: [ncol][nlev].  Outputs [ncol][nb][nlev+1], [ncol][nlev+1], tau2 [ncol][2]
+ * (taudust / taucloud, REAL*4 sums). */
+typedef struct {
+  const double *radb_rt;
+  const double *q_v, *q_i;
+} or_aer_tables;
+void or_opt_aer(int cloud, const or_aer_tables *tb, int ncol, int nlay, const double *qtr,
+                const double *pl, double *qxv, double *qsv, double *gv, double *qxi, double *qsi,
+                double *gi, double *qextref, double *tauref, double *tau2);
+
 #ifdef __cplusplus
 }
 #endif

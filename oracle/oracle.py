@@ -82,6 +82,10 @@ def _c(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
+class _AerTables(C.Structure):
+    _fields_ = [("radb_rt", DP), ("q_v", DP), ("q_i", DP)]
+
+
 class Oracle:
     def __init__(self, path: str = ORACLE_SO):
         if not os.path.exists(path):
@@ -97,6 +101,8 @@ class Oracle:
         self.lib.or_laginterp.argtypes = [DP] * 6
         self.lib.or_planck_table.argtypes = [DP, DP, DP]
         self.lib.or_setspv.argtypes = [DP] * 4
+        self.lib.or_aer_grid.argtypes = [DP, DP]
+        self.lib.or_opt_aer.argtypes = [C.c_int, C.POINTER(_AerTables), C.c_int, C.c_int] + [DP] * 11
 
     @staticmethod
     def _tables(t: dict, keep: list) -> _Tables:
@@ -137,6 +143,27 @@ class Oracle:
         os_ = _BatchOut(*[_p(o[k]) for k in OUT_KEYS])
         sol = _c(t["sol"])
         self.lib.or_batch_run(C.byref(ts), _p(sol), C.byref(ins), C.byref(os_), int(nthreads))
+        return o
+
+    def aer_grid(self) -> tuple:
+        """INITCLD's radiative size grid: rad_rt(20), radb_rt(21)."""
+        rad, radb = np.zeros(20), np.zeros(21)
+        self.lib.or_aer_grid(_p(rad), _p(radb))
+        return rad, radb
+
+    def opt_aer(self, cloud: bool, radb: np.ndarray, q_v: np.ndarray, q_i: np.ndarray, qtr: np.ndarray,
+                pl: np.ndarray) -> dict:
+        """OPT_DST / OPT_CLD over columns: qtr [ncol][6][nlay], pl [ncol][nlev]; tables as
+        Reference.initcld returns them (dust (3,nb,20), cloud (3,nb,20,15))."""
+        qtr, pl = _c(qtr), _c(pl)
+        ncol, nlay = qtr.shape[0], qtr.shape[2]
+        ld = 2 * nlay + 4
+        keep = [_c(radb), _c(q_v), _c(q_i)]
+        tb = _AerTables(_p(keep[0]), _p(keep[1]), _p(keep[2]))
+        o = {k: np.zeros((ncol, NV if k[-1] == "v" else NI, ld)) for k in ("qxv", "qsv", "gv", "qxi", "qsi", "gi")}
+        o.update(qextref=np.zeros((ncol, ld)), tauref=np.zeros((ncol, ld)), tau2=np.zeros((ncol, 2)))
+        self.lib.or_opt_aer(int(cloud), C.byref(tb), ncol, nlay, _p(qtr), _p(pl),
+                            *[_p(o[k]) for k in ("qxv", "qsv", "gv", "qxi", "qsi", "gi", "qextref", "tauref", "tau2")])
         return o
 
     def count(self, t: dict, c: dict) -> dict:
@@ -215,6 +242,8 @@ class Reference:
         self.lib.ref_setrad.argtypes = [DP] * 14
         self.lib.ref_dsolver.argtypes = [C.c_int] + [DP] * 9 + [C.c_double] * 3 + [DP, DP]
         self.lib.ref_tpindex.argtypes = [C.c_double] * 3 + [DP] * 4 + [C.POINTER(C.c_int)] * 3 + [DP]
+        self.lib.ref_aer_tables.argtypes = [DP] * 7
+        self.lib.ref_opt_aer.argtypes = [C.c_int] + [DP] * 11
 
     def run(self, t: dict, c: dict, cols=None) -> dict:
         assert int(t["nlay"]) == self.nlay, "reference build is compiled for %d layers" % self.nlay
@@ -262,6 +291,32 @@ class Reference:
                                "nfluxtopv", "nfluxtopi", "diffvt", "detau", "cumtauv")]
         self.lib.ref_grid(ncol, *[_p(a) for a in tabs], *[_p(a) for a in prof], *[_p(a) for a in outs])
         _heating(t, c, o)
+        return o
+
+    def initcld(self, rundir: str) -> dict:
+        """INITCLD: the aerosol size grid and the optical tables from rundir/data."""
+        cwd = os.getcwd()
+        os.chdir(rundir)
+        try:
+            self.lib.ref_initcld()
+        finally:
+            os.chdir(cwd)
+        rad, radb = np.zeros(20), np.zeros(21)
+        qv_d, qi_d = np.zeros((3, NV, 20)), np.zeros((3, NI, 20))
+        qv_c, qi_c = np.zeros((3, NV, 20, 15)), np.zeros((3, NI, 20, 15))
+        consts = np.zeros(6)
+        self.lib.ref_aer_tables(_p(rad), _p(radb), _p(qv_d), _p(qi_d), _p(qv_c), _p(qi_c), _p(consts))
+        return dict(rad_rt=rad, radb_rt=radb, qv_dst=qv_d, qi_dst=qi_d, qv_cld=qv_c, qi_cld=qi_c,
+                    athird=consts[0], dev_dt=consts[1], dev_ice=consts[2], dpden_dt=consts[3], dpden_ice=consts[4])
+
+    def opt_aer(self, cloud: bool, qtr: np.ndarray, pl: np.ndarray) -> dict:
+        """OPT_DST (cloud False) or OPT_CLD on one column: qtr (NTRACE=6, nlay), pl (nlev)."""
+        nlev = 2 * self.nlay + 3
+        o = {k: np.zeros((NV if k[-1] == "v" else NI, nlev + 1)) for k in ("qxv", "qsv", "gv", "qxi", "qsi", "gi")}
+        qextref, tauref, tau2 = np.zeros(nlev + 1), np.zeros(nlev + 1), np.zeros(2)
+        self.lib.ref_opt_aer(int(cloud), _p(_c(qtr)), _p(_c(pl)), _p(o["qxv"]), _p(o["qxi"]), _p(o["qsv"]),
+                             _p(o["qsi"]), _p(o["gv"]), _p(o["gi"]), _p(qextref), _p(tauref), _p(tau2))
+        o.update(qextref=qextref, tauref=tauref, tau2=tau2)
         return o
 
     def tables(self, rundir: str):

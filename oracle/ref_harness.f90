@@ -6,6 +6,9 @@
 !  for one column exactly as COMP3 does (code/comp3.f:637-738), and exposes
 !  the init routines (SETSPV/SETSPI/SETRAD/LAGINTERP) for table checks.
 !
+!  The aerosol producers (INITCLD's size grid and optical tables, OPT_DST,
+!  OPT_CLD) are exposed the same way: one column at grid point (1,1).
+!
 !  Per-band Gauss weights: the reference takes one GWEIGHT(17) for all bands.
 !  When the caller's weights differ by band, SFLUXV is called once per band
 !  with SOL zeroed in the other bands (every VIS term is proportional to
@@ -19,7 +22,59 @@
       use radcommon_h, only: planckir, tlimits, tlimiti, ubari
       implicit none
       real*8, save :: detau(L_J,L_I,L_NSPECTV,L_NGAUSS)
+      real*8, save, allocatable :: qtrace_h(:,:,:,:)
       contains
+
+!     INITCLD (reads data/*_12bands.dat relative to the working directory)
+!     with the dust density INITDT sets (code/initdt.f:38).
+      subroutine ref_initcld() bind(C, name="ref_initcld")
+      use dtcommon_h, only: dpden_dt
+      dpden_dt = 2.5E+3
+      call initcld
+      end subroutine ref_initcld
+
+!     The size grid and optical tables INITCLD set up (REAL*4 tables promoted).
+      subroutine ref_aer_tables(rad, radb, qv_dst, qi_dst, qv_cld, qi_cld,  &
+           consts) bind(C, name="ref_aer_tables")
+      use cldcommon_h
+      use dtcommon_h, only: dpden_dt
+      real(c_double), intent(out) :: rad(nbin_rt), radb(nbin_rt+1)
+      real(c_double), intent(out) :: qv_dst(nbin_rt, nlonv, 3), qi_dst(nbin_rt, nloni, 3)
+      real(c_double), intent(out) :: qv_cld(nratio, nbin_rt, nlonv, 3)
+      real(c_double), intent(out) :: qi_cld(nratio, nbin_rt, nloni, 3)
+      real(c_double), intent(out) :: consts(6)
+      rad = rad_rt
+      radb = radb_rt
+      qv_dst(:,:,1) = qextv_dst; qv_dst(:,:,2) = qscatv_dst; qv_dst(:,:,3) = gv_dst
+      qi_dst(:,:,1) = qexti_dst; qi_dst(:,:,2) = qscati_dst; qi_dst(:,:,3) = gi_dst
+      qv_cld(:,:,:,1) = qextv_cld; qv_cld(:,:,:,2) = qscatv_cld; qv_cld(:,:,:,3) = gv_cld
+      qi_cld(:,:,:,1) = qexti_cld; qi_cld(:,:,:,2) = qscati_cld; qi_cld(:,:,:,3) = gi_cld
+      consts = [athird, dev_dt, dev_ice, dpden_dt, dpden_ice, 0.0d0]
+      end subroutine ref_aer_tables
+
+!     OPT_DST (cloud = 0) or OPT_CLD (cloud = 1) on one column: tracers
+!     QTRACE(L_LAYERS, NTRACE) of the column, PL(L_LEVELS).
+      subroutine ref_opt_aer(cloud, qtr, pl, qxv, qxi, qsv, qsi, gv, gi,  &
+           qextref, tauref, tau2) bind(C, name="ref_opt_aer")
+      use cldcommon_h, only: taudust, taucloud
+      integer(c_int), value :: cloud
+      real(c_double), intent(in) :: qtr(L_LAYERS, NTRACE), pl(L_LEVELS)
+      real(c_double), intent(out) :: qxv(L_LEVELS+1, L_NSPECTV), qsv(L_LEVELS+1, L_NSPECTV)
+      real(c_double), intent(out) :: gv(L_LEVELS+1, L_NSPECTV)
+      real(c_double), intent(out) :: qxi(L_LEVELS+1, L_NSPECTI), qsi(L_LEVELS+1, L_NSPECTI)
+      real(c_double), intent(out) :: gi(L_LEVELS+1, L_NSPECTI)
+      real(c_double), intent(out) :: qextref(L_LEVELS+1), tauref(L_LEVELS+1), tau2(2)
+      if (.not. allocated(qtrace_h)) allocate(qtrace_h(L_JSIZE, L_ISIZE, L_LAYERS, NTRACE))
+      qtrace_h = 0.0d0
+      qtrace_h(1, 1, :, :) = qtr
+      if (cloud == 0) then
+        call opt_dst(1, 1, qtrace_h, pl, qxv, qxi, qsv, qsi, gv, gi, qextref, tauref)
+        tau2 = taudust(1, 1, :)
+      else
+        call opt_cld(1, 1, qtrace_h, pl, qxv, qxi, qsv, qsi, gv, gi, qextref, tauref)
+        tau2 = taucloud(1, 1, :)
+      end if
+      end subroutine ref_opt_aer
 
       subroutine ref_set_planck(p) bind(C, name="ref_set_planck")
       real(c_double), intent(in) :: p(L_NSPECTI, 8501)

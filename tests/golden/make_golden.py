@@ -22,6 +22,9 @@ Fixtures (all data: inputs and the reference's outputs):
   column_aer24.npz    32 seeded columns (24 layers, clouds) with per-column, per-level aerosol
                       optics (mgcm_rt.synth.make_aerosol, regenerated from the seed and checked
                       against the stored SHA-256), as OPT_DST/OPT_CLD hand them to OPTCV/OPTCI.
+  aerosol_ref.npz     the aerosol producers: the four optical-property files of run/data
+                      (Dust_*_wolff2010_JD_12bands.dat, waterCoated_*_JD_12bands.dat, as bytes),
+                      and the reference's OPT_DST and OPT_CLD on 4 seeded columns of tracers.
   column_real24.npz   the reference's own k-distribution: the raw 11-pressure records and
                       FZERO of run/data/CO2H2O_V_12_95_INTEL and CO2H2O_IR_12_95_INTEL (data
                       files, read as data), and 32 seeded columns (24 layers) run through the
@@ -92,6 +95,8 @@ def main(only=None) -> None:
             real_columns(ref)
         if "column_aer24" in only:
             aerosol_columns(ref)
+        if "aerosol_ref" in only:
+            aerosol_producers(ref)
         return
     out = {}
     for name, seed, weights in (("column_ref24", 101, "reference"), ("column_dir24", 102, "dirichlet")):
@@ -161,7 +166,28 @@ def main(only=None) -> None:
                         tpindex=np.array(tp))
     real_columns(ref)
     aerosol_columns(ref)
+    aerosol_producers(ref)
     print("golden fixtures written to", HERE)
+
+
+AEROSOL_FILES = ("Dust_vis_wolff2010_JD_12bands.dat", "Dust_ir_wolff2010_JD_12bands.dat",
+                 "waterCoated_vis_JD_12bands.dat", "waterCoated_ir_JD_12bands.dat")
+
+
+def aerosol_producers(ref) -> None:
+    """aerosol_ref.npz: INITCLD's data files and the reference's OPT_DST/OPT_CLD outputs."""
+    ref.initcld(REF_RUN)
+    seed, ncol, nlay = 108, 4, 24
+    qtr = synth.make_tracers(seed, ncol, nlay)
+    pl = synth.make_columns(seed, ncol, nlay)["plev"]
+    d = {"seed": seed, "ncol": ncol, "nlay": nlay, "qtrace": qtr, "pl": pl}
+    for f in AEROSOL_FILES:
+        d["file_" + f] = np.frombuffer(open(os.path.join(REF_RUN, "data", f), "rb").read(), dtype=np.uint8)
+    for name, cloud in (("dst", False), ("cld", True)):
+        outs = [ref.opt_aer(cloud, qtr[i], pl[i]) for i in range(ncol)]
+        for k in outs[0]:
+            d[f"{name}_{k}"] = np.stack([o[k] for o in outs])
+    np.savez_compressed(os.path.join(HERE, "aerosol_ref.npz"), **d)
 
 
 def aerosol_digest(a: dict) -> str:
