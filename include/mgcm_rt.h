@@ -221,6 +221,28 @@ typedef struct {
 int mgcm_rt_opt_dst(mgcm_rt_ctx *ctx, const mgcm_rt_tracers *in, const mgcm_rt_aerosol_out *out, void *stream);
 int mgcm_rt_opt_cld(mgcm_rt_ctx *ctx, const mgcm_rt_tracers *in, const mgcm_rt_aerosol_out *out, void *stream);
 
+/* Prescribed (Conrath) dust, the TAUREF of runs without active dust.
+ * mgcm_rt_dust_grid: DUSTPROFILE's pressure grid PRDST(npdst) from PSF and PTROP
+ * (code/dustprofile.f:30-37; host code, npdst <= 128).
+ * mgcm_rt_dust_tauref: DUSTPROFILE's Conrath profile of each column
+ * (code/dustprofile.f:56-113) and FILLTAUCUM's TAUREF on the sub-levels
+ * (code/filltaucum.f:33-52), device pointers, asynchronous on stream: tauref
+ * [ncol][nlev+1]; taudst [ncol][npdst] (TAUDST, optional).  The Conrath
+ * parameter is the caller's per column (DUSTPROFILE: CONRNU, or its latitude and
+ * season formula when VARY_CONR is set). */
+typedef struct {
+  int ncol, nlay;
+  const double *tautot; /* [ncol] TAUTOTJI: dust opacity at RPTAU */
+  const double *conr;   /* [ncol] Conrath parameter */
+  const double *p;      /* [ncol] P(J,I): surface pressure - PTROP, mbar */
+  const double *sigma;  /* [nlev] SIGMA(L_LEVELS) */
+  const double *prdst;  /* [npdst] PRDST */
+  int npdst;
+  double rptau, ptrop;  /* RPTAU, PTROP (mbar) */
+} mgcm_rt_dust_columns;
+int mgcm_rt_dust_grid(double psf, double ptrop, int npdst, double *prdst);
+int mgcm_rt_dust_tauref(const mgcm_rt_dust_columns *in, double *tauref, double *taudst, void *stream);
+
 /* Diagnostic: the device's exp, log10 and sqrt on n host arguments (x > 0 for
  * log10, x >= 0 for sqrt; other entries return 0).  The FP64 path reproduces the
  * reference's libm (exp and log10 of glibc 2.35, correctly rounded sqrt); this

@@ -300,3 +300,98 @@ extern "C" int mgcm_rt_opt_cld(mgcm_rt_ctx *c, const mgcm_rt_tracers *in, const 
                                void *stream) {
   return opt_aer(true, c, in, out, stream);
 }
+
+// ---------------------------------------------------------------------------
+// Prescribed (Conrath) dust: DUSTPROFILE's column profile (code/dustprofile.f:
+// 56-113) and FILLTAUCUM (code/filltaucum.f:33-52), one thread per column.
+// Every operation is IEEE or the glibc-exact exp, so TAUREF equals the
+// reference's bit for bit.
+namespace mgcm {
+namespace aer {
+constexpr int NPDST_MAX = 128;
+
+__device__ int jsrchgt(int n, const double *sx, double target) {  // code/jsrchgt.f90:28-66
+  if (n < 1) return 0;
+  if (target > sx[n - 1]) return n + 1;
+  if (target < sx[0]) return 1;
+  int jl = 1, jh = n;
+  while (jh - jl > 1) {
+    const int jm = (jl + jh) / 2;
+    if (target > sx[jm - 1]) jl = jm;
+    else jh = jm;
+  }
+  if (target == sx[jh - 1]) jh = jh + 1;
+  return jh;
+}
+
+__global__ __launch_bounds__(64) void k_dust_tauref(mgcm_rt_dust_columns in, double *tauref_all, double *taudst_out) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= in.ncol) return;
+  const int npdst = in.npdst, nlevels = 2 * in.nlay + 3;
+  const double *prdst = in.prdst, rptau = in.rptau, conr = in.conr[col];
+  double qrdst[NPDST_MAX], taudst[NPDST_MAX];
+  double sum = 0., pave;
+  int n;
+  for (n = 2; n <= npdst; n++) {
+    if (prdst[n - 1] < rptau) {
+      pave = 0.5 * (prdst[n - 1] + prdst[n - 2]);
+      sum = sum + mgcm_libm::mgcm_exp(conr * (1. - (rptau / pave))) * (prdst[n - 1] - prdst[n - 2]);
+    }
+    if (prdst[n - 1] >= rptau) break;
+  }
+  pave = 0.5 * (rptau + prdst[n - 2]);
+  sum = sum + mgcm_libm::mgcm_exp(conr * (1. - (rptau / pave))) * (rptau - prdst[n - 2]);
+  const double qrdst0 = in.tautot[col] / sum;
+  for (n = 1; n <= npdst - 1; n++) {
+    if (rptau > prdst[n]) {  // region 1
+      pave = 0.5 * (prdst[n] + prdst[n - 1]);
+      qrdst[n - 1] = qrdst0 * mgcm_libm::mgcm_exp(conr * (1.0 - (rptau / pave)));
+    }
+    if (rptau <= prdst[n] && rptau >= prdst[n - 1]) {  // region 2
+      const double pdif1 = rptau - prdst[n - 1], pdif2 = prdst[n] - rptau;
+      pave = 0.5 * (prdst[n - 1] + rptau);
+      qrdst[n - 1] = qrdst0 * (mgcm_libm::mgcm_exp(conr * (1.0 - (rptau / pave))) * pdif1 + pdif2) /
+                     (prdst[n] - prdst[n - 1]);
+    }
+    if (rptau < prdst[n - 1]) qrdst[n - 1] = qrdst0;  // region 3
+  }
+  taudst[0] = 0.0;
+  for (n = 2; n <= npdst; n++) taudst[n - 1] = taudst[n - 2] + qrdst[n - 2] * (prdst[n - 1] - prdst[n - 2]);
+  if (taudst_out)
+    for (n = 0; n < npdst; n++) taudst_out[(size_t)col * npdst + n] = taudst[n];
+  double *tauref = tauref_all + (size_t)col * (nlevels + 1);
+  double prev = 0.0;  // TAUCUM(K-1); TAUCUM(1..3) = 0
+  for (int k = 1; k <= 3; k++) tauref[k - 1] = 0.0;
+  for (int k = 4; k <= nlevels; k++) {
+    const double pstar = in.p[col] * in.sigma[k - 1] + in.ptrop;
+    const double pstar1 = fmax(pstar, prdst[0]);
+    int nstar = jsrchgt(npdst - 1, prdst, pstar1) - 1;
+    if (nstar > npdst - 1) nstar = npdst - 1;
+    const double tc = taudst[nstar - 1] + (pstar1 - prdst[nstar - 1]) * (taudst[nstar] - taudst[nstar - 1]) /
+                                              (prdst[nstar] - prdst[nstar - 1]);
+    tauref[k - 1] = tc - prev;
+    prev = tc;
+  }
+  tauref[nlevels] = 0.0;
+}
+}  // namespace aer
+}  // namespace mgcm
+
+extern "C" int mgcm_rt_dust_grid(double psf, double ptrop, int npdst, double *prdst) {
+  if (!prdst || npdst < 2 || npdst > NPDST_MAX || !(psf > 0.0) || !(ptrop > 0.0)) return MGCM_RT_ERR_ARG;
+  const double refpr = pow(5.0 * psf / ptrop, (double)(1.0f / ((float)npdst - 1.0f)));  // code/dustprofile.f:32
+  prdst[0] = ptrop;
+  for (int n = 2; n <= npdst; n++) prdst[n - 1] = refpr * prdst[n - 2];
+  return MGCM_RT_OK;
+}
+
+extern "C" int mgcm_rt_dust_tauref(const mgcm_rt_dust_columns *in, double *tauref, double *taudst, void *stream) {
+  if (!in || !tauref) return MGCM_RT_ERR_ARG;
+  if (in->ncol < 0 || in->npdst < 2 || in->npdst > NPDST_MAX) return MGCM_RT_ERR_ARG;
+  if (in->nlay < 1 || in->nlay > MAX_NLAY) return MGCM_RT_ERR_NLAY;
+  if (in->ncol == 0) return MGCM_RT_OK;
+  if (!in->tautot || !in->conr || !in->p || !in->sigma || !in->prdst) return MGCM_RT_ERR_ARG;
+  hipLaunchKernelGGL(k_dust_tauref, dim3((in->ncol + 63) / 64), dim3(64), 0, (hipStream_t)stream, *in, tauref,
+                     taudst);
+  return hipGetLastError() == hipSuccess ? MGCM_RT_OK : MGCM_RT_ERR_DEVICE;
+}

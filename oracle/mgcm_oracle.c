@@ -1136,3 +1136,88 @@ void or_opt_aer(int cloud, const or_aer_tables *tb, int ncol, int nlay, const do
     tau2[2 * col + 1] = t2;
   }
 }
+
+/* DUSTPROFILE: code/dustprofile.f:30-37 (1.0/(FLOAT(NPDST)-1.0) evaluated in REAL(4)). */
+void or_dust_grid(double psf, double ptrop, int npdst, double *prdst)
+{
+  const double refpr = pow(5.0 * psf / ptrop, (double)(1.0f / ((float)npdst - 1.0f)));
+  int n;
+  prdst[0] = ptrop;
+  for (n = 2; n <= npdst; n++) prdst[n - 1] = refpr * prdst[n - 2];
+}
+
+/* JSRCHGT: code/jsrchgt.f90:28-66, 1-based result. */
+static int jsrchgt(int n, const double *sx, double target)
+{
+  int jl, jh, jm;
+  if (n < 1) return 0;
+  if (target > sx[n - 1]) return n + 1;
+  if (target < sx[0]) return 1;
+  jl = 1;
+  jh = n;
+  while (jh - jl > 1) {
+    jm = (jl + jh) / 2;
+    if (target > sx[jm - 1]) jl = jm;
+    else jh = jm;
+  }
+  if (target == sx[jh - 1]) jh = jh + 1;
+  return jh;
+}
+
+void or_dust_columns(int ncol, int nlay, int npdst, const double *prdst, double rptau, double ptrop,
+                     const double *tautot, const double *conr_all, const double *p, const double *sigma,
+                     double *taudst_all, double *tauref_all, double *taucum_all)
+{
+  const int nlevels = 2 * nlay + 3;
+  int col;
+  for (col = 0; col < ncol; col++) {
+    const double conr = conr_all[col];
+    double *taudst = taudst_all + (size_t)col * npdst;
+    double *tauref = tauref_all + (size_t)col * (nlevels + 1);
+    double *taucum = taucum_all + (size_t)col * nlevels;
+    double qrdst[256];
+    double sum = 0., pave, qrdst0;
+    int n, k;
+    /* code/dustprofile.f:56-71 */
+    for (n = 2; n <= npdst; n++) {
+      if (prdst[n - 1] < rptau) {
+        pave = 0.5 * (prdst[n - 1] + prdst[n - 2]);
+        sum = sum + exp(conr * (1. - (rptau / pave))) * (prdst[n - 1] - prdst[n - 2]);
+      }
+      if (prdst[n - 1] >= rptau) break;
+    }
+    pave = 0.5 * (rptau + prdst[n - 2]);
+    sum = sum + exp(conr * (1. - (rptau / pave))) * (rptau - prdst[n - 2]);
+    qrdst0 = tautot[col] / sum;
+    /* code/dustprofile.f:75-104 */
+    for (n = 1; n <= npdst - 1; n++) {
+      if (rptau > prdst[n]) {
+        pave = 0.5 * (prdst[n] + prdst[n - 1]);
+        qrdst[n - 1] = qrdst0 * exp(conr * (1.0 - (rptau / pave)));
+      }
+      if (rptau <= prdst[n] && rptau >= prdst[n - 1]) {
+        const double pdif1 = rptau - prdst[n - 1], pdif2 = prdst[n] - rptau;
+        pave = 0.5 * (prdst[n - 1] + rptau);
+        qrdst[n - 1] = qrdst0 * (exp(conr * (1.0 - (rptau / pave))) * pdif1 + pdif2) / (prdst[n] - prdst[n - 1]);
+      }
+      if (rptau < prdst[n - 1]) qrdst[n - 1] = qrdst0;
+    }
+    taudst[0] = 0.0;
+    for (n = 2; n <= npdst; n++) taudst[n - 1] = taudst[n - 2] + qrdst[n - 2] * (prdst[n - 1] - prdst[n - 2]);
+    /* FILLTAUCUM: code/filltaucum.f:33-52 */
+    for (n = 1; n <= 3; n++) {
+      taucum[n - 1] = 0.0;
+      tauref[n - 1] = 0.0;
+    }
+    for (k = 4; k <= nlevels; k++) {
+      const double pstar = p[col] * sigma[k - 1] + ptrop;
+      const double pstar1 = fmax(pstar, prdst[0]);
+      int nstar = jsrchgt(npdst - 1, prdst, pstar1) - 1;
+      if (nstar > npdst - 1) nstar = npdst - 1;
+      taucum[k - 1] = taudst[nstar - 1] + (pstar1 - prdst[nstar - 1]) * (taudst[nstar] - taudst[nstar - 1]) /
+                                              (prdst[nstar] - prdst[nstar - 1]);
+      tauref[k - 1] = taucum[k - 1] - taucum[k - 2];
+    }
+    tauref[nlevels] = 0.0;
+  }
+}

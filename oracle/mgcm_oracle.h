@@ -132,6 +132,17 @@ void or_opt_aer(int cloud, const or_aer_tables *tb, int ncol, int nlay, const do
                 const double *pl, double *qxv, double *qsv, double *gv, double *qxi, double *qsi,
                 double *gi, double *qextref, double *tauref, double *tau2);
 
+/* DUSTPROFILE's reference pressure grid PRDST(npdst) (code/dustprofile.f:30-37). */
+void or_dust_grid(double psf, double ptrop, int npdst, double *prdst);
+
+/* DUSTPROFILE's column profile (code/dustprofile.f:56-113) and FILLTAUCUM
+ * (code/filltaucum.f:33-52) for ncol columns: tautot[ncol] the opacity at
+ * RPTAU, conr[ncol] the Conrath parameter, p[ncol] = P(J,I), sigma[nlev].
+ * Outputs taudst[ncol][npdst], tauref[ncol][nlev+1], taucum[ncol][nlev]. */
+void or_dust_columns(int ncol, int nlay, int npdst, const double *prdst, double rptau, double ptrop,
+                     const double *tautot, const double *conr, const double *p, const double *sigma,
+                     double *taudst, double *tauref, double *taucum);
+
 #ifdef __cplusplus
 }
 #endif

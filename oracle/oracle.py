@@ -102,6 +102,8 @@ class Oracle:
         self.lib.or_planck_table.argtypes = [DP, DP, DP]
         self.lib.or_setspv.argtypes = [DP] * 4
         self.lib.or_aer_grid.argtypes = [DP, DP]
+        self.lib.or_dust_grid.argtypes = [C.c_double, C.c_double, C.c_int, DP]
+        self.lib.or_dust_columns.argtypes = [C.c_int, C.c_int, C.c_int, DP, C.c_double, C.c_double] + [DP] * 7
         self.lib.or_opt_aer.argtypes = [C.c_int, C.POINTER(_AerTables), C.c_int, C.c_int] + [DP] * 11
 
     @staticmethod
@@ -164,6 +166,21 @@ class Oracle:
         o.update(qextref=np.zeros((ncol, ld)), tauref=np.zeros((ncol, ld)), tau2=np.zeros((ncol, 2)))
         self.lib.or_opt_aer(int(cloud), C.byref(tb), ncol, nlay, _p(qtr), _p(pl),
                             *[_p(o[k]) for k in ("qxv", "qsv", "gv", "qxi", "qsi", "gi", "qextref", "tauref", "tau2")])
+        return o
+
+    def dust_grid(self, psf: float, ptrop: float, npdst: int = 100) -> np.ndarray:
+        prdst = np.zeros(npdst)
+        self.lib.or_dust_grid(psf, ptrop, npdst, _p(prdst))
+        return prdst
+
+    def dust_columns(self, prdst, rptau, ptrop, tautot, conr, p, sigma, nlay: int) -> dict:
+        """DUSTPROFILE's column profile and FILLTAUCUM: TAUDST, TAUREF, TAUCUM per column."""
+        tautot, conr, p, sigma, prdst = _c(tautot), _c(conr), _c(p), _c(sigma), _c(prdst)
+        ncol, nlev = tautot.size, 2 * nlay + 3
+        o = {"taudst": np.zeros((ncol, prdst.size)), "tauref": np.zeros((ncol, nlev + 1)),
+             "taucum": np.zeros((ncol, nlev))}
+        self.lib.or_dust_columns(ncol, nlay, prdst.size, _p(prdst), rptau, ptrop, _p(tautot), _p(conr), _p(p),
+                                 _p(sigma), _p(o["taudst"]), _p(o["tauref"]), _p(o["taucum"]))
         return o
 
     def count(self, t: dict, c: dict) -> dict:
@@ -244,6 +261,8 @@ class Reference:
         self.lib.ref_tpindex.argtypes = [C.c_double] * 3 + [DP] * 4 + [C.POINTER(C.c_int)] * 3 + [DP]
         self.lib.ref_aer_tables.argtypes = [DP] * 7
         self.lib.ref_opt_aer.argtypes = [C.c_int] + [DP] * 11
+        self.lib.ref_dust_setup.argtypes = [C.c_double] * 4 + [C.c_int, C.c_double, DP]
+        self.lib.ref_filltaucum.argtypes = [C.c_int, C.c_int, C.c_double] + [DP] * 5
 
     def run(self, t: dict, c: dict, cols=None) -> dict:
         assert int(t["nlay"]) == self.nlay, "reference build is compiled for %d layers" % self.nlay
@@ -318,6 +337,20 @@ class Reference:
                              _p(o["qsi"]), _p(o["gv"]), _p(o["gi"]), _p(qextref), _p(tauref), _p(tau2))
         o.update(qextref=qextref, tauref=tauref, tau2=tau2)
         return o
+
+    def dust(self, psf, ptrop, rptau, conrnu, vary, vpout, tautot, p, sigma, cols) -> dict:
+        """DUSTPROFILE over the grid (tautot (L_J, L_I) C-shaped (L_I, L_J)), then FILLTAUCUM
+        for each (j, i) in cols (1-based) with P(j, i) = p[(j, i)]."""
+        self.lib.ref_dust_setup(psf, ptrop, rptau, conrnu, int(vary), vpout, _p(_c(tautot)))
+        nlev = 2 * self.nlay + 3
+        out = {"prdst": np.zeros(100), "taudst": [], "tauref": [], "taucum": []}
+        for (j, i) in cols:
+            td, tr, tc = np.zeros(100), np.zeros(nlev + 1), np.zeros(nlev)
+            self.lib.ref_filltaucum(j, i, float(p[(j, i)]), _p(_c(sigma)), _p(out["prdst"]), _p(td), _p(tr), _p(tc))
+            out["taudst"].append(td)
+            out["tauref"].append(tr)
+            out["taucum"].append(tc)
+        return {k: np.array(v) for k, v in out.items()}
 
     def tables(self, rundir: str):
         """SETSPV/SETSPI/SETRAD (+LAGINTERP) as the reference computes them at start-up."""

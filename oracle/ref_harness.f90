@@ -52,6 +52,45 @@
       consts = [athird, dev_dt, dev_ice, dpden_dt, dpden_ice, 0.0d0]
       end subroutine ref_aer_tables
 
+!     DUSTPROFILE over the whole L_J x L_I grid (code/dustprofile.f) with the
+!     given namelist values and column opacities TAUTOTJI(L_J, L_I).
+      subroutine ref_dust_setup(psf_, ptrop_, rptau_, conrnu_, vary, vpout_, tautot) &
+           bind(C, name="ref_dust_setup")
+      use standard_h, only: PSF, PTROP, RPTAU, CONRNU, VARY_CONR, VPOUT, TAUTOTJI
+      real(c_double), value :: psf_, ptrop_, rptau_, conrnu_, vpout_
+      integer(c_int), value :: vary
+      real(c_double), intent(in) :: tautot(L_J, L_I)
+      PSF = psf_
+      PTROP = ptrop_
+      RPTAU = rptau_
+      CONRNU = conrnu_
+      VARY_CONR = vary /= 0
+      VPOUT = vpout_
+      TAUTOTJI = tautot
+      call dustprofile
+      end subroutine ref_dust_setup
+
+!     PRDST and TAUDST(j, i, :) of the last ref_dust_setup; then FILLTAUCUM
+!     (code/filltaucum.f) for column (j, i) with P(j, i) = p_ji and SIGMA.
+      subroutine ref_filltaucum(j, i, p_ji, sig, prdst_out, taudst_out, tauref_out, taucum_out) &
+           bind(C, name="ref_filltaucum")
+      use standard_h, only: P, SIGMA, TAUCUM, PRDST, TAUDST
+      use defines_h, only: L_NPDST
+      use radcommon_h, only: TAUREF
+      integer(c_int), value :: j, i
+      real(c_double), value :: p_ji
+      real(c_double), intent(in) :: sig(L_LEVELS)
+      real(c_double), intent(out) :: prdst_out(L_NPDST), taudst_out(L_NPDST)
+      real(c_double), intent(out) :: tauref_out(L_LEVELS+1), taucum_out(L_LEVELS)
+      P(j, i) = p_ji
+      SIGMA = sig
+      call filltaucum(j, i)
+      prdst_out = PRDST
+      taudst_out = TAUDST(j, i, :)
+      tauref_out = TAUREF
+      taucum_out = TAUCUM
+      end subroutine ref_filltaucum
+
 !     OPT_DST (cloud = 0) or OPT_CLD (cloud = 1) on one column: tracers
 !     QTRACE(L_LAYERS, NTRACE) of the column, PL(L_LEVELS).
       subroutine ref_opt_aer(cloud, qtr, pl, qxv, qxi, qsv, qsi, gv, gi,  &

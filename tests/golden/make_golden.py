@@ -25,6 +25,9 @@ Fixtures (all data: inputs and the reference's outputs):
   aerosol_ref.npz     the aerosol producers: the four optical-property files of run/data
                       (Dust_*_wolff2010_JD_12bands.dat, waterCoated_*_JD_12bands.dat, as bytes),
                       and the reference's OPT_DST and OPT_CLD on 4 seeded columns of tracers.
+  dust_ref.npz        prescribed dust: the reference's DUSTPROFILE + FILLTAUCUM (PRDST, TAUDST,
+                      TAUREF, TAUCUM) for 8 columns of two namelist settings (CONRNU and the
+                      latitude/season Conrath parameter).
   column_real24.npz   the reference's own k-distribution: the raw 11-pressure records and
                       FZERO of run/data/CO2H2O_V_12_95_INTEL and CO2H2O_IR_12_95_INTEL (data
                       files, read as data), and 32 seeded columns (24 layers) run through the
@@ -97,6 +100,8 @@ def main(only=None) -> None:
             aerosol_columns(ref)
         if "aerosol_ref" in only:
             aerosol_producers(ref)
+        if "dust_ref" in only:
+            dust_profiles(ref)
         return
     out = {}
     for name, seed, weights in (("column_ref24", 101, "reference"), ("column_dir24", 102, "dirichlet")):
@@ -167,7 +172,32 @@ def main(only=None) -> None:
     real_columns(ref)
     aerosol_columns(ref)
     aerosol_producers(ref)
+    dust_profiles(ref)
     print("golden fixtures written to", HERE)
+
+
+DUST_CASES = ((6.1, 0.0005, False, 0.0), (7.01, 0.0008, True, 300.0))  # PSF, PTROP, VARY_CONR, VPOUT
+
+
+def dust_profiles(ref) -> None:
+    """dust_ref.npz: DUSTPROFILE + FILLTAUCUM of the reference on 8 columns per case."""
+    from mgcm_rt import dust
+    rng = np.random.Generator(np.random.PCG64(110))
+    sigma = synth.sigma_levels(24)
+    d = {"sigma": sigma, "rptau": 6.1, "conrnu": 0.03}
+    for ci, (psf, ptrop, vary, vpout) in enumerate(DUST_CASES):
+        tautot = rng.uniform(0.1, 1.0, (60, 36))
+        cols = [(int(rng.integers(1, 37)), int(rng.integers(1, 61))) for _ in range(8)]
+        p = {c: float(rng.uniform(4.0, 10.0)) for c in cols}
+        r = ref.dust(psf, ptrop, 6.1, 0.03, vary, vpout, tautot, p, sigma, cols)
+        d[f"case{ci}"] = np.array([psf, ptrop, float(vary), vpout])
+        d[f"case{ci}_cols"] = np.array(cols)
+        d[f"case{ci}_tautot"] = np.array([tautot[i - 1, j - 1] for j, i in cols])
+        d[f"case{ci}_conr"] = np.array([dust.conrath_nu(vpout, j, 36, vary) for j, _ in cols])
+        d[f"case{ci}_p"] = np.array([p[c] for c in cols])
+        for k, v in r.items():
+            d[f"case{ci}_{k}"] = v
+    np.savez_compressed(os.path.join(HERE, "dust_ref.npz"), **d)
 
 
 AEROSOL_FILES = ("Dust_vis_wolff2010_JD_12bands.dat", "Dust_ir_wolff2010_JD_12bands.dat",
