@@ -120,7 +120,10 @@ typedef struct {
  * for sun-down columns, as the reference leaves its module arrays.  hr may be
  * NULL.  hr[0] is the stratosphere (mass PTROP*100/g), hr[l] GCM layer l, in
  * K/sol: (SUNTOT+IRTOTAL)*g/(Cp*100*dp)*88775 (code/comp3.f:765-771 without
- * the non-LTE factor and the theta conversion, which stay with the caller). */
+ * the non-LTE factor and the theta conversion, which stay with the caller).
+ * directsol/dnvflux: the surface fluxes COMP3 derives for TEMPGR
+ * (code/comp3.f:336-348, code/dsolflux.f), computed after the radiation when
+ * given (0 for sun-down columns). */
 typedef struct {
   double *fmnetv, *fluxupv, *fluxdnv; /* [ncol][nlayrad] */
   double *fmneti, *fluxupi, *fluxdni; /* [ncol][nlayrad] */
@@ -129,6 +132,8 @@ typedef struct {
   double *detau;                      /* [ncol][17][7] DETAU(nw,ng) */
   double *cumtauv;                    /* [ncol][17][7] TAUCUMV(L_LEVELS,nw,ng) */
   double *hr;                         /* [ncol][nlay+1] K/sol, optional */
+  double *directsol;                  /* [ncol] DSOLFLUX's direct solar flux at the surface, optional */
+  double *dnvflux;                    /* [ncol] DIFFVT + directsol (COMP3's DNVFLUX), optional */
 } mgcm_rt_fluxes;
 
 /* Replaces RADSETUP's table state (code/radsetup.f90:26-46, code/modules.f90:
@@ -242,6 +247,15 @@ typedef struct {
 } mgcm_rt_dust_columns;
 int mgcm_rt_dust_grid(double psf, double ptrop, int npdst, double *prdst);
 int mgcm_rt_dust_tauref(const mgcm_rt_dust_columns *in, double *tauref, double *taudst, void *stream);
+
+/* DSOLFLUX (code/dsolflux.f:18-45) and COMP3's surface VIS flux for TEMPGR
+ * (code/comp3.f:336-348) from the DETAU and DIFFVT outputs of mgcm_rt_run:
+ * directsol[ncol], the direct solar flux at the surface (0 where the sun is
+ * down), and dnvflux[ncol] = DIFFVT + directsol (optional; needs diffvt).  sol:
+ * host SOL(7); acosz, detau [ncol][17][7], diffvt: device.  The Gauss weights
+ * and FZEROV are the context's.  Asynchronous on stream. */
+int mgcm_rt_dsolflux(mgcm_rt_ctx *ctx, int ncol, const double *sol, const double *acosz, const double *detau,
+                     const double *diffvt, double *directsol, double *dnvflux, void *stream);
 
 /* Diagnostic: the device's exp, log10 and sqrt on n host arguments (x > 0 for
  * log10, x >= 0 for sqrt; other entries return 0).  The FP64 path reproduces the

@@ -203,8 +203,12 @@ extern "C" int mgcm_rt_run(mgcm_rt_ctx *c, const mgcm_rt_columns *in, const mgcm
   fill_args(a, in, out, c->d_flag);
   a.bands = bands;
   a.clr = c->d_clr;
-  const hipError_t e = (path == MGCM_RT_FP64) ? launch_plain(c->dt, a, (hipStream_t)stream)
-                                              : launch_mixed(c->dt, a, (hipStream_t)stream);
+  hipError_t e = (path == MGCM_RT_FP64) ? launch_plain(c->dt, a, (hipStream_t)stream)
+                                        : launch_mixed(c->dt, a, (hipStream_t)stream);
+  // surface fluxes for TEMPGR from this run's DETAU/DIFFVT (code/comp3.f:336-348)
+  if (e == hipSuccess && out->directsol)
+    e = launch_dsolflux(c->d_const, in->ncol, in->sol, in->acosz, out->detau, out->diffvt, out->directsol,
+                        out->dnvflux, (hipStream_t)stream);
   return e == hipSuccess ? MGCM_RT_OK : MGCM_RT_ERR_DEVICE;
 }
 
@@ -228,7 +232,7 @@ extern "C" int mgcm_rt_run_host(mgcm_rt_ctx *c, const mgcm_rt_columns *in, const
   if (in->ncol == 0) return MGCM_RT_OK;
   const size_t ncol = in->ncol, nlev = 2 * (size_t)in->nlay + 3, nl = in->nlay + 1;
   const size_t nin = ncol * (5 * nlev + 2 * (nlev + 1) + 3);
-  const size_t nout = ncol * (6 * nl + 3 + 2 * NG * NV + nl);
+  const size_t nout = ncol * (6 * nl + 3 + 2 * NG * NV + nl + 2);
   // optional per-level aerosol optics: the non-null arrays in mgcm_rt_aerosol order
   const mgcm_rt_aerosol *hae = in->aerosol;
   const double *asrc[14] = {nullptr};
@@ -301,6 +305,11 @@ extern "C" int mgcm_rt_run_host(mgcm_rt_ctx *c, const mgcm_rt_columns *in, const
   dout.detau = p; p += ncol * NG * NV;
   dout.cumtauv = p; p += ncol * NG * NV;
   dout.hr = out->hr ? p : nullptr;
+  p += ncol * nl;
+  dout.directsol = out->directsol ? p : nullptr;
+  p += ncol;
+  dout.dnvflux = (out->directsol && out->dnvflux) ? p : nullptr;
+  p += ncol;
   // detau/cumtauv are in/out: sun-down columns keep the caller's values
   HIP_TRY(hipMemcpy(dout.detau, out->detau, ncol * NG * NV * 8, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(dout.cumtauv, out->cumtauv, ncol * NG * NV * 8, hipMemcpyHostToDevice));
@@ -320,6 +329,8 @@ extern "C" int mgcm_rt_run_host(mgcm_rt_ctx *c, const mgcm_rt_columns *in, const
   HIP_TRY(hipMemcpy(out->detau, dout.detau, ncol * NG * NV * 8, hipMemcpyDeviceToHost));
   HIP_TRY(hipMemcpy(out->cumtauv, dout.cumtauv, ncol * NG * NV * 8, hipMemcpyDeviceToHost));
   if (out->hr) HIP_TRY(hipMemcpy(out->hr, dout.hr, ncol * nl * 8, hipMemcpyDeviceToHost));
+  if (dout.directsol) HIP_TRY(hipMemcpy(out->directsol, dout.directsol, ncol * 8, hipMemcpyDeviceToHost));
+  if (dout.dnvflux) HIP_TRY(hipMemcpy(out->dnvflux, dout.dnvflux, ncol * 8, hipMemcpyDeviceToHost));
   return mgcm_rt_check(c);
 }
 

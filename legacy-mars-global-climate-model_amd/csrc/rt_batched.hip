@@ -281,7 +281,14 @@ int run_batched_impl(mgcm_rt_ctx *c, int ngrids, const mgcm_rt_columns *grids, c
     }
     b->sig = sig;
   }
-  return hipGraphLaunch(b->exec, st) == hipSuccess ? MGCM_RT_OK : MGCM_RT_ERR_DEVICE;
+  if (hipGraphLaunch(b->exec, st) != hipSuccess) return MGCM_RT_ERR_DEVICE;
+  // surface fluxes for TEMPGR of the grids that ask for them (code/comp3.f:336-348)
+  for (int i = 0; i < ngrids; i++)
+    if (outs[i].directsol && grids[i].ncol > 0 &&
+        launch_dsolflux(c->d_const, grids[i].ncol, grids[i].sol, grids[i].acosz, outs[i].detau, outs[i].diffvt,
+                        outs[i].directsol, outs[i].dnvflux, st) != hipSuccess)
+      return MGCM_RT_ERR_DEVICE;
+  return MGCM_RT_OK;
 }
 
 }  // namespace mgcm

@@ -38,4 +38,8 @@ void fill_args(RunArgs &a, const mgcm_rt_columns *in, const mgcm_rt_fluxes *out,
 int run_batched_impl(mgcm_rt_ctx *ctx, int ngrids, const mgcm_rt_columns *grids, const mgcm_rt_fluxes *outs,
                      int precision, hipStream_t st);
 void batch_free(void *state);
+// DSOLFLUX + DNVFLUX of a grid (rt_surface.hip); dnvflux may be null.
+hipError_t launch_dsolflux(const RtConst *rc, int ncol, const double *sol, const double *acosz,
+                           const double *detau, const double *diffvt, double *directsol, double *dnvflux,
+                           hipStream_t st);
 }  // namespace mgcm

@@ -47,6 +47,7 @@ module mgcm_rt_mod
   type, bind(C), public :: mgcm_rt_fluxes
     type(c_ptr) :: fmnetv, fluxupv, fluxdnv, fmneti, fluxupi, fluxdni
     type(c_ptr) :: nfluxtopv, nfluxtopi, diffvt, detau, cumtauv, hr
+    type(c_ptr) :: directsol, dnvflux
   end type
 
   public :: mgcm_rt_init, mgcm_rt_destroy, mgcm_rt_run, mgcm_rt_run_host, mgcm_rt_check
@@ -211,7 +212,7 @@ contains
   subroutine mgcm_rt_radiation(ctx, ncol, nlay, plev, tlev, pmid, tmid, qh2o, tauref, taurefcld, &
                                als, albi, acosz, sol, fmnetv, fluxupv, fluxdnv, fmneti, fluxupi, &
                                fluxdni, nfluxtopv, nfluxtopi, diffvt, detau, cumtauv, hr, precision, status, &
-                               aerosol)
+                               aerosol, directsol, dnvflux)
     type(c_ptr), intent(in) :: ctx
     integer, intent(in) :: ncol, nlay
     real(c_double), intent(in), target :: plev(2*nlay+3, ncol), tlev(2*nlay+3, ncol)
@@ -229,6 +230,8 @@ contains
     integer(c_int), intent(out) :: status
     ! per-level aerosol optics (active dust/clouds); absent = the tables' constants
     type(mgcm_rt_aerosol), intent(in), target, optional :: aerosol
+    ! DSOLFLUX's direct surface solar flux and DNVFLUX (code/comp3.f:336-348), optional
+    real(c_double), intent(out), target, optional :: directsol(ncol), dnvflux(ncol)
     type(mgcm_rt_columns) :: c
     type(mgcm_rt_fluxes) :: o
     c%ncol = ncol
@@ -258,6 +261,10 @@ contains
     o%detau = c_loc(detau)
     o%cumtauv = c_loc(cumtauv)
     o%hr = c_loc(hr)
+    o%directsol = c_null_ptr
+    o%dnvflux = c_null_ptr
+    if (present(directsol)) o%directsol = c_loc(directsol)
+    if (present(directsol) .and. present(dnvflux)) o%dnvflux = c_loc(dnvflux)
     status = mgcm_rt_run_host(ctx, c, o, precision)
   end subroutine
 

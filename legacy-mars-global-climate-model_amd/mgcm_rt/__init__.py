@@ -84,7 +84,7 @@ class AerosolOut(C.Structure):
 class Fluxes(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in (
         "fmnetv", "fluxupv", "fluxdnv", "fmneti", "fluxupi", "fluxdni",
-        "nfluxtopv", "nfluxtopi", "diffvt", "detau", "cumtauv", "hr")]
+        "nfluxtopv", "nfluxtopi", "diffvt", "detau", "cumtauv", "hr", "directsol", "dnvflux")]
 
 
 IN_KEYS = ("plev", "tlev", "pmid", "tmid", "qh2o", "tauref", "taurefcld", "als", "albi", "acosz")
@@ -125,6 +125,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         lib.mgcm_rt_laginterp.argtypes = [DP, DP, C.c_int, DP, DP]
         lib.mgcm_rt_read_ktable.argtypes = [C.c_char_p, C.c_int, DP, DP]
         lib.mgcm_rt_read_aerosol_tables.argtypes = [C.c_char_p, DP]
+        lib.mgcm_rt_dsolflux.argtypes = [C.c_void_p, C.c_int, DP] + [C.c_void_p] * 6
         lib.mgcm_rt_set_aerosol_tables.argtypes = [C.c_void_p, DP]
         lib.mgcm_rt_opt_dst.argtypes = [C.c_void_p, C.POINTER(Tracers), C.POINTER(AerosolOut), C.c_void_p]
         lib.mgcm_rt_opt_cld.argtypes = [C.c_void_p, C.POINTER(Tracers), C.POINTER(AerosolOut), C.c_void_p]
@@ -179,9 +180,12 @@ def column_struct(cols: dict, ptr=lambda a: a.ctypes.data, sol=None) -> Columns:
     return s
 
 
+SURFACE_KEYS = ("directsol", "dnvflux")  # optional outputs: DSOLFLUX and COMP3's DNVFLUX, [ncol]
+
+
 def flux_struct(out: dict, ptr=lambda a: a.ctypes.data) -> Fluxes:
     s = Fluxes()
-    for k in OUT_KEYS:
+    for k in OUT_KEYS + SURFACE_KEYS:
         v = out.get(k)
         setattr(s, k, None if v is None else ptr(v))
     return s
@@ -256,6 +260,19 @@ class Radiation:
 
     def check(self) -> None:
         _check(self.lib.mgcm_rt_check(self.ctx), "mgcm_rt_check")
+
+    def dsolflux(self, cols: dict, out: dict, stream: int = 0) -> dict:
+        """DSOLFLUX and COMP3's DNVFLUX from a run's outputs (device tensors): returns
+        device tensors directsol[ncol] and dnvflux[ncol]."""
+        import torch
+        n = int(cols["ncol"])
+        ds = torch.empty(n, dtype=torch.float64, device=out["detau"].device)
+        dn = torch.empty_like(ds)
+        sol = _f64(self.sol)
+        _check(self.lib.mgcm_rt_dsolflux(self.ctx, n, _dp(sol), cols["acosz"].data_ptr(), out["detau"].data_ptr(),
+                                         out["diffvt"].data_ptr(), ds.data_ptr(), dn.data_ptr(), C.c_void_p(stream)),
+               "mgcm_rt_dsolflux")
+        return {"directsol": ds, "dnvflux": dn}
 
     def set_aerosol_tables(self, tables: np.ndarray) -> None:
         """Upload INITCLD's size grid and optical tables (read_aerosol_tables' block)."""

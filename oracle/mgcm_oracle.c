@@ -1221,3 +1221,22 @@ void or_dust_columns(int ncol, int nlay, int npdst, const double *prdst, double 
     tauref[nlevels] = 0.0;
   }
 }
+
+/* DSOLFLUX: code/dsolflux.f:18-45. */
+double or_dsolflux(const double *sol, double ubar0, const double *gw, const double *fzerov,
+                   const double *detau)
+{
+  double directsol = 0.0, factor;
+  int nw, ng;
+  for (nw = 1; nw <= NV; nw++) {
+    factor = ubar0 * sol[nw - 1];
+    for (ng = 1; ng <= NG - 1; ng++)
+      if (detau[(nw - 1) + NV * (ng - 1)] <= 5.0)
+        directsol = directsol + factor * exp(-detau[(nw - 1) + NV * (ng - 1)] / ubar0) * GW(gw, ng, nw) *
+                                    (1.0 - fzerov[nw - 1]);
+    ng = NG;
+    if (detau[(nw - 1) + NV * (ng - 1)] <= 5.0)
+      directsol = directsol + factor * exp(-detau[(nw - 1) + NV * (ng - 1)] / ubar0) * fzerov[nw - 1];
+  }
+  return directsol;
+}

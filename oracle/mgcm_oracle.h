@@ -143,6 +143,11 @@ void or_dust_columns(int ncol, int nlay, int npdst, const double *prdst, double 
                      const double *tautot, const double *conr, const double *p, const double *sigma,
                      double *taudst, double *tauref, double *taucum);
 
+/* DSOLFLUX (code/dsolflux.f:18-45) of one column: gw (17,7) per-band Gauss
+ * weights (the reference's single GWEIGHT repeated), detau (7,17). */
+double or_dsolflux(const double *sol, double ubar0, const double *gw, const double *fzerov,
+                   const double *detau);
+
 #ifdef __cplusplus
 }
 #endif

@@ -103,6 +103,8 @@ class Oracle:
         self.lib.or_setspv.argtypes = [DP] * 4
         self.lib.or_aer_grid.argtypes = [DP, DP]
         self.lib.or_dust_grid.argtypes = [C.c_double, C.c_double, C.c_int, DP]
+        self.lib.or_dsolflux.argtypes = [DP, C.c_double, DP, DP, DP]
+        self.lib.or_dsolflux.restype = C.c_double
         self.lib.or_dust_columns.argtypes = [C.c_int, C.c_int, C.c_int, DP, C.c_double, C.c_double] + [DP] * 7
         self.lib.or_opt_aer.argtypes = [C.c_int, C.POINTER(_AerTables), C.c_int, C.c_int] + [DP] * 11
 
@@ -167,6 +169,15 @@ class Oracle:
         self.lib.or_opt_aer(int(cloud), C.byref(tb), ncol, nlay, _p(qtr), _p(pl),
                             *[_p(o[k]) for k in ("qxv", "qsv", "gv", "qxi", "qsi", "gi", "qextref", "tauref", "tau2")])
         return o
+
+    def dsolflux(self, t: dict, acosz: np.ndarray, detau: np.ndarray) -> np.ndarray:
+        """DSOLFLUX per column (0 where ACOSZ < REAL(4) 1e-5, as COMP3 sets it)."""
+        sol, gw, fz = _c(t["sol"]), _c(t["gwv"]), _c(t["fzerov"])
+        out = np.zeros(len(acosz))
+        for i, u in enumerate(acosz):
+            if u >= float(np.float32(1e-5)):
+                out[i] = self.lib.or_dsolflux(_p(sol), float(u), _p(gw), _p(fz), _p(_c(detau[i])))
+        return out
 
     def dust_grid(self, psf: float, ptrop: float, npdst: int = 100) -> np.ndarray:
         prdst = np.zeros(npdst)
@@ -263,6 +274,7 @@ class Reference:
         self.lib.ref_opt_aer.argtypes = [C.c_int] + [DP] * 11
         self.lib.ref_dust_setup.argtypes = [C.c_double] * 4 + [C.c_int, C.c_double, DP]
         self.lib.ref_filltaucum.argtypes = [C.c_int, C.c_int, C.c_double] + [DP] * 5
+        self.lib.ref_dsolflux.argtypes = [DP, C.c_double, DP, DP, DP, DP]
 
     def run(self, t: dict, c: dict, cols=None) -> dict:
         assert int(t["nlay"]) == self.nlay, "reference build is compiled for %d layers" % self.nlay
@@ -337,6 +349,11 @@ class Reference:
                              _p(o["qsi"]), _p(o["gv"]), _p(o["gi"]), _p(qextref), _p(tauref), _p(tau2))
         o.update(qextref=qextref, tauref=tauref, tau2=tau2)
         return o
+
+    def dsolflux(self, sol, ubar0: float, gweight, fzerov, detau_col) -> float:
+        out = np.zeros(1)
+        self.lib.ref_dsolflux(_p(_c(sol)), float(ubar0), _p(_c(gweight)), _p(_c(fzerov)), _p(_c(detau_col)), _p(out))
+        return float(out[0])
 
     def dust(self, psf, ptrop, rptau, conrnu, vary, vpout, tautot, p, sigma, cols) -> dict:
         """DUSTPROFILE over the grid (tautot (L_J, L_I) C-shaped (L_I, L_J)), then FILLTAUCUM

@@ -91,6 +91,25 @@
       taucum_out = TAUCUM
       end subroutine ref_filltaucum
 
+!     DSOLFLUX (code/dsolflux.f) of one column's DETAU(7,17).
+      subroutine ref_dsolflux(sol, ubar0, gweight, fzerov, detau_col, directsol) &
+           bind(C, name="ref_dsolflux")
+      real(c_double), intent(in) :: sol(L_NSPECTV), gweight(L_NGAUSS), fzerov(L_NSPECTV)
+      real(c_double), value :: ubar0
+      real(c_double), intent(in) :: detau_col(L_NSPECTV, L_NGAUSS)
+      real(c_double), intent(out) :: directsol
+      real(c_double), save, allocatable :: d(:,:,:,:)
+      real*8 :: sol8(L_NSPECTV), gw8(L_NGAUSS), fz8(L_NSPECTV), u8
+      if (.not. allocated(d)) allocate(d(L_J, L_I, L_NSPECTV, L_NGAUSS))
+      d = 0.0d0
+      d(1, 1, :, :) = detau_col
+      sol8 = sol
+      gw8 = gweight
+      fz8 = fzerov
+      u8 = ubar0
+      call dsolflux(sol8, u8, gw8, fz8, d, 1, 1, directsol)
+      end subroutine ref_dsolflux
+
 !     OPT_DST (cloud = 0) or OPT_CLD (cloud = 1) on one column: tracers
 !     QTRACE(L_LAYERS, NTRACE) of the column, PL(L_LEVELS).
       subroutine ref_opt_aer(cloud, qtr, pl, qxv, qxi, qsv, qsi, gv, gi,  &

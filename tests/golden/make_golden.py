@@ -25,6 +25,7 @@ Fixtures (all data: inputs and the reference's outputs):
   aerosol_ref.npz     the aerosol producers: the four optical-property files of run/data
                       (Dust_*_wolff2010_JD_12bands.dat, waterCoated_*_JD_12bands.dat, as bytes),
                       and the reference's OPT_DST and OPT_CLD on 4 seeded columns of tracers.
+  dsolflux_ref.npz    the reference's DSOLFLUX on the DETAU/ACOSZ of column_ref24.npz.
   dust_ref.npz        prescribed dust: the reference's DUSTPROFILE + FILLTAUCUM (PRDST, TAUDST,
                       TAUREF, TAUCUM) for 8 columns of two namelist settings (CONRNU and the
                       latitude/season Conrath parameter).
@@ -102,6 +103,8 @@ def main(only=None) -> None:
             aerosol_producers(ref)
         if "dust_ref" in only:
             dust_profiles(ref)
+        if "dsolflux_ref" in only:
+            dsolflux_columns(ref)
         return
     out = {}
     for name, seed, weights in (("column_ref24", 101, "reference"), ("column_dir24", 102, "dirichlet")):
@@ -173,7 +176,19 @@ def main(only=None) -> None:
     aerosol_columns(ref)
     aerosol_producers(ref)
     dust_profiles(ref)
+    dsolflux_columns(ref)
     print("golden fixtures written to", HERE)
+
+
+def dsolflux_columns(ref) -> None:
+    """dsolflux_ref.npz: DSOLFLUX of the reference on column_ref24.npz's outputs (sun-up columns)."""
+    f = np.load(os.path.join(HERE, "column_ref24.npz"))
+    t = synth.make_tables(int(f["seed"]), 24, weights="reference")
+    acosz, detau = f["in_acosz"], f["out_detau"]
+    sun = acosz >= float(np.float32(1e-5))
+    ds = np.array([ref.dsolflux(t["sol"], acosz[i], synth.GWEIGHT_REF, t["fzerov"], detau[i]) if sun[i] else 0.0
+                   for i in range(len(acosz))])
+    np.savez_compressed(os.path.join(HERE, "dsolflux_ref.npz"), directsol=ds)
 
 
 DUST_CASES = ((6.1, 0.0005, False, 0.0), (7.01, 0.0008, True, 300.0))  # PSF, PTROP, VARY_CONR, VPOUT

@@ -321,3 +321,10 @@ def test_oracle_bit_exact_vs_reference_per_level_aerosol(oracle):
     got = oracle.run(t, c | aer, nthreads=2)
     for k in OUT_KEYS:
         assert np.array_equal(got[k], want[k]), k
+
+
+def test_oracle_dsolflux_bit_exact_vs_reference(oracle):
+    """DSOLFLUX on the reference's own DETAU outputs (column_ref24.npz)."""
+    t, c, want = load_fixture("column_ref24.npz")
+    ds = oracle.dsolflux(t, c["acosz"], want["detau"])
+    assert np.array_equal(ds, np.load(os.path.join(GOLDEN, "dsolflux_ref.npz"))["directsol"])
