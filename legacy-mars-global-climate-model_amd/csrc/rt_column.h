@@ -369,7 +369,7 @@ __device__ __forceinline__ void column_block(const DevTables &dt, const RunArgs 
       GasSrc<true, MIXED> src(MIXED ? (const kt *)(dt.kv32 + toff) : (const kt *)(dt.kv + toff), S, c, nsub, g0);
       VisState<NLM> s;
       double cumtau = 0.0, detau = 0.0;
-      if (valid && sun) detau = vis_optics<NLM>(nl, src, scat, cosb, s, cumtau);
+      if (valid && sun) detau = vis_optics<NLM>(nl, src, StagedLayerAer{scat, cosb}, s, cumtau);
       skip = !(valid && sun) || below_tlimit<MIXED>(src, dt.kv + toff + g0, nlev, TLIMITS, valid && sun);
       if (valid && sun) {  // GETDETAU output and TAUCUMV(L_LEVELS) (code/comp3.f:664-668)
         a.detau[(size_t)col * NG * NV + g0 * NV + nw] = detau;
@@ -398,7 +398,7 @@ __device__ __forceinline__ void column_block(const DevTables &dt, const RunArgs 
       const size_t toff = (size_t)nw * NP * NH * NT * NGAS;
       GasSrc<false, MIXED> src(MIXED ? (const kt *)(dt.ki32 + toff) : (const kt *)(dt.ki + toff), S, c, nsub, g0);
       IrState<NLM> s;
-      if (valid) ir_optics<NLM>(nl, src, scat, s);
+      if (valid) ir_optics<NLM>(nl, src, StagedLayerAer{scat, cosb}, s);
       skip = !valid || below_tlimit<MIXED>(src, dt.ki + toff + g0, nlev, TLIMITI, valid);
       if (__any(!skip)) {
         const double *bk = S.bk + c * nlev;
@@ -546,11 +546,10 @@ __device__ __forceinline__ void clear_block(const DevTables &dt, const RunArgs &
   const double *tc = a.taurefcld + (size_t)col * (nlev + 1);
   double *rec = a.clr + (size_t)col * NB * clr_rec(nl) + b;  // slot k of this item: rec[k * NB]
   const Aer ae = column_aerosol(a, rc, col);
-  double scat[NLM], cosb[NLM], cbar[NLM];
-  for (int L = 1; L <= nl; L++) {
-    layer_aerosol<VIS>(ae, b, pl, tr, tc, nl, L, scat[L - 1], cosb[L - 1]);
-    cbar[L - 1] = cosb[L - 1] / (1.0 + cosb[L - 1]);
-  }
+  // layer aerosol optics computed as the optics pass reaches each layer; the
+  // solve's COSBAR (VIS) or COSB (IR) kept per layer
+  double keep[NLM];
+  const LiveLayerAer<VIS> la{ae, b, nl, pl, tr, tc, keep};
   auto accf = [&](int n, double fp, double fm) {
     rec[(n - 1) * NB] = fp;
     rec[(nl + n - 1) * NB] = fm;
@@ -561,19 +560,19 @@ __device__ __forceinline__ void clear_block(const DevTables &dt, const RunArgs &
     ClearSrc<true> src{ae, b, pl, tr, tc, 0.0};
     VisState<NLM> s;
     double cumtau = 0.0;
-    const double detau = vis_optics<NLM>(nl, src, scat, cosb, s, cumtau);
+    const double detau = vis_optics<NLM>(nl, src, la, s, cumtau);
     a.detau[(size_t)col * NG * NV + 16 * NV + b] = detau;
     a.cumtauv[(size_t)col * NG * NV + 16 * NV + b] = cumtau;
     double fluxup = 0.0, fluxdn = 0.0, diffv = 0.0;
     const double bsurf = a.als[col] * mu * a.sol[b] * rt_exp35(-dmin(detau / mu, MAXEXP));
-    vis_solve<NLM>(nl, s, cbar, mu, a.sol[b], a.als[col], 0.0, bsurf, accf, fluxup, fluxdn, diffv);
+    vis_solve<NLM>(nl, s, keep, mu, a.sol[b], a.als[col], 0.0, bsurf, accf, fluxup, fluxdn, diffv);
     rec[2 * nl * NB] = fluxup - fluxdn;
     rec[(2 * nl + 1) * NB] = diffv;
   } else {
     const int nw = b - NV;
     ClearSrc<false> src{ae, b, pl, tr, tc, 0.0};
     IrState<NLM> s;
-    ir_optics<NLM>(nl, src, scat, s);
+    ir_optics<NLM>(nl, src, la, s);
     double bk[2 * NLM + 1];
     for (int K = 1; K <= nlev; K++)
       bk[K - 1] = dt.planck[(size_t)(planck_index(a.tlev[(size_t)col * nlev + K - 1], a.flag) - 1) * NI + nw];
@@ -581,7 +580,7 @@ __device__ __forceinline__ void clear_block(const DevTables &dt, const RunArgs &
     const double bsurf = (1. - rsf) * bk[nlev - 1];
     const double tautop = s.wd[1].y * pl[1] / (pl[3] - pl[1]);
     const double btop = (1.0 - rt_exp(-tautop * 2.0)) * bk[1];
-    rec[2 * nl * NB] = ir_solve<NLM>(nl, s, cosb, bk, rsf, btop, bsurf, accf);
+    rec[2 * nl * NB] = ir_solve<NLM>(nl, s, keep, bk, rsf, btop, bsurf, accf);
   }
 }
 

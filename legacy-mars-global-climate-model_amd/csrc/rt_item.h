@@ -127,6 +127,36 @@ __device__ __forceinline__ void layer_aerosol(const Aer &ae, int b, const double
   }
 }
 
+// Layer scattering optical depth and asymmetry (SCAT, COSB of layer L) as the
+// optics pass reads them: from arrays staged per column ...
+struct StagedLayerAer {
+  const double *sc, *cb;
+  __device__ __forceinline__ void get(int L, double &scat, double &cosb) const {
+    scat = sc[L - 1];
+    cosb = cb[L - 1];
+  }
+  __device__ __forceinline__ double scat(int L) const { return sc[L - 1]; }
+};
+
+// ... or computed when the pass reaches the layer, keeping what the solve
+// needs later: COSBAR (VIS) or COSB (IR) of each layer in keep[L-1].
+template <bool VIS>
+struct LiveLayerAer {
+  const Aer &ae;
+  int b, nl;
+  const double *plev, *tauref, *taurefcld;
+  double *keep;
+  __device__ __forceinline__ void get(int L, double &scat, double &cosb) const {
+    layer_aerosol<VIS>(ae, b, plev, tauref, taurefcld, nl, L, scat, cosb);
+    keep[L - 1] = VIS ? cosb / (1.0 + cosb) : cosb;  // COSBAR of code/gfluxv.f90:86
+  }
+  __device__ __forceinline__ double scat(int L) const {
+    double sc, cb;
+    get(L, sc, cb);
+    return sc;
+  }
+};
+
 // ---------------------------------------------------------------------------
 // One two-stream item.  NLM bounds the radiation layer count (arrays are
 // 1-based).  Inputs arrive through the Src functor: finish(K, fetch(K)) gives the sub-level
@@ -179,9 +209,8 @@ __device__ __forceinline__ VisLayer vis_layer(double w0, double cb, double u0, d
 // VIS optics of one item + delta-Eddington scaling (OPTCV layer assembly
 // code/optcv.f90:189-274 fused with GETDETAU/GFLUXV code/gfluxv.f90:76-103).
 // Returns DETAU; cumtau receives TAUCUMV(L_LEVELS).
-template <int NLM, typename Src>
-__device__ __forceinline__ double vis_optics(int nl, Src &src, const double *__restrict__ scat,
-                             const double *__restrict__ cosb, VisState<NLM> &s, double &cumtau) {
+template <int NLM, typename Src, typename LA>
+__device__ __forceinline__ double vis_optics(int nl, Src &src, const LA &la, VisState<NLM> &s, double &cumtau) {
   // sub-level inputs are fetched one layer ahead of their use
   typename Src::Raw r2 = src.fetch(2), ra = src.fetch(3), rb = src.fetch(4);
   // sub-level 2 (top half of the cap)
@@ -195,7 +224,9 @@ __device__ __forceinline__ double vis_optics(int nl, Src &src, const double *__r
     const double dtauv = a + b;
     const double tcA = tcv + a;  // TAUCUMV(2L+1)
     const double tcB = tcA + b;  // TAUCUMV(2L+2)
-    const double W = scat[L - 1] / dtauv, C = cosb[L - 1];
+    double Sc, C;
+    la.get(L, Sc, C);
+    const double W = Sc / dtauv;
     const double F = 1.0 - W * (C * C);
     double hm;
     if (L == 1) {
@@ -219,7 +250,9 @@ __device__ __forceinline__ double vis_optics(int nl, Src &src, const double *__r
     const double a = src.finish(2 * L + 1, ra);
     const double dtauv = a;
     cumtau = tcv + a;  // TAUCUMV(L_LEVELS)
-    const double W = scat[L - 1] / dtauv, C = cosb[L - 1];
+    double Sc, C;
+    la.get(L, Sc, C);
+    const double W = Sc / dtauv;
     const double F = 1.0 - W * (C * C);
     const double dtau = dtauv * F;
     s.wt[L] = Pair{W * (1.0 - C * C) / F, tauL};
@@ -391,8 +424,8 @@ __device__ __forceinline__ IrLayer ir_layer(double w0, double cb, double dtau, d
 }
 
 // IR optics of one item (OPTCI code/optci.f90:189-269).
-template <int NLM, typename Src>
-__device__ __forceinline__ void ir_optics(int nl, Src &src, const double *__restrict__ scat, IrState<NLM> &s) {
+template <int NLM, typename Src, typename LA>
+__device__ __forceinline__ void ir_optics(int nl, Src &src, const LA &la, IrState<NLM> &s) {
   const double f1em9 = F32LIT(1.0e-9);
   // sub-level inputs are fetched one layer ahead of their use
   typename Src::Raw r2 = src.fetch(2), ra = src.fetch(3), rb = src.fetch(4);
@@ -403,9 +436,10 @@ __device__ __forceinline__ void ir_optics(int nl, Src &src, const double *__rest
     if (L < nl) ra = src.fetch(2 * L + 3);
     if (L < nl - 1) rb = src.fetch(2 * L + 4);
     double dt = a + b + 1.e-50;
+    const double sc = la.scat(L);  // every layer: a live provider keeps its COSB here
     double w;
     if (dt > f1em9) {
-      w = scat[L - 1] / dt;
+      w = sc / dt;
     } else {
       w = 0.0;
       dt = f1em9;
