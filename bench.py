@@ -349,10 +349,10 @@ def main():
         "data": "synthetic (seeded PCG64 generator, mgcm_rt/synth.py)",
         "config": {"workload": f"plain FP64 radiation step, {args.ncol} columns x {args.nlay} layers, "
                                "7 VIS + 5 IR bands x 17 Gauss points",
-                   "ncol": args.ncol, "nlay": args.nlay, "seed": args.seed, "launches_per_step": 3,
-                   "launches": "k_rt_clear (VIS g=17), k_rt_clear (IR g=17), k_rt (column kernel)"},
+                   "ncol": args.ncol, "nlay": args.nlay, "seed": args.seed, "launches_per_step": 2,
+                   "launches": "k_rt_clear2 (continuum point g=17, VIS and IR bands), k_rt (column kernel)"},
         "roofline": {"bound": "mfma", "pipe": "FP64 vector ALU (MI355X FP64 vector peak = matrix peak)",
-                     "scope": "the whole step (its 3 launches); flops and traffic summed over them",
+                     "scope": "the whole step (its 2 launches); flops and traffic summed over them",
                      "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
                      "hbm_GBps": None if traffic is None else traffic / (ms / 1e3) / 1e9,

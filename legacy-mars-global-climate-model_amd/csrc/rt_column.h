@@ -591,6 +591,16 @@ __global__ __launch_bounds__(64) void k_rt_clear(DevTables dt, RunArgs a) {
   clear_block<NLM, VIS>(dt, a, blockIdx.x * ClearShape<VIS>::cols);
 }
 
+// Both parts in one launch: blocks below nvis take VIS items, the rest IR
+// items, so the two halves share one dispatch and fill each other's tail.
+template <int NLM>
+__global__ __launch_bounds__(64) void k_rt_clear2(DevTables dt, RunArgs a, int nvis) {
+  mgcm_libm::load_lds_exptab(threadIdx.x, 64);
+  __syncthreads();
+  if ((int)blockIdx.x < nvis) clear_block<NLM, true>(dt, a, blockIdx.x * ClearShape<true>::cols);
+  else clear_block<NLM, false>(dt, a, (blockIdx.x - nvis) * ClearShape<false>::cols);
+}
+
 template <int NLM, bool VIS>
 __global__ __launch_bounds__(64) void k_rt_clear_batch(DevTables dt, const RunArgs *__restrict__ grids,
                                                        const uint2 *__restrict__ work, int nwork) {
@@ -632,12 +642,13 @@ void k_rt_batch(DevTables dt, const RunArgs *__restrict__ grids, const uint2 *__
 template <int NLM, bool MIXED>
 static hipError_t launch_nlm(const DevTables &dt, const RunArgs &a, hipStream_t st) {
   // continuum points first (their records feed the column kernel's sums)
-  if (a.bands & 1)
-    hipLaunchKernelGGL((k_rt_clear<NLM, true>), dim3((a.ncol + 8) / 9), dim3(64), mgcm_libm::LDS_EXPTAB_BYTES, st,
-                       dt, a);
-  if (a.bands & 2)
-    hipLaunchKernelGGL((k_rt_clear<NLM, false>), dim3((a.ncol + 11) / 12), dim3(64), mgcm_libm::LDS_EXPTAB_BYTES,
-                       st, dt, a);
+  const int nvis = (a.ncol + 8) / 9, nir = (a.ncol + 11) / 12;
+  if (a.bands == 3)
+    hipLaunchKernelGGL((k_rt_clear2<NLM>), dim3(nvis + nir), dim3(64), mgcm_libm::LDS_EXPTAB_BYTES, st, dt, a, nvis);
+  else if (a.bands & 1)
+    hipLaunchKernelGGL((k_rt_clear<NLM, true>), dim3(nvis), dim3(64), mgcm_libm::LDS_EXPTAB_BYTES, st, dt, a);
+  else if (a.bands & 2)
+    hipLaunchKernelGGL((k_rt_clear<NLM, false>), dim3(nir), dim3(64), mgcm_libm::LDS_EXPTAB_BYTES, st, dt, a);
   const size_t lds = stage_bytes(a.nlay);
   const unsigned gx = (unsigned)((a.ncol + 3) / 4);
   hipLaunchKernelGGL((k_rt<NLM, MIXED>), dim3(gx), dim3(64), lds, st, dt, a);
