@@ -54,3 +54,34 @@ def test_device_libm_matches_host_libm():
     nn = x >= 0
     assert np.array_equal(q[nn], np.sqrt(x[nn]))
     print("log10 differences", nbad, "of", int(pos.sum()), flush=True)
+
+
+def test_run_captured_in_a_hip_graph():
+    """mgcm_rt_run needs no host synchronisation or allocation after its first call at a
+    size, so a caller can capture it (here in a torch CUDA graph, i.e. a hipGraph) and
+    replay it; every replay, with new inputs copied in between, equals the eager run."""
+    import torch
+    from mgcm_rt import FP64, Radiation, alloc_outputs, synth, to_device
+    t, c = synth.make_workload(81, 900, 24)
+    rt = Radiation(t)
+    dc = to_device(c)
+    do = to_device(alloc_outputs(900, 24))
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        rt.run_device(dc, do, FP64, s.cuda_stream)  # first call at this size: workspace
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        rt.run_device(dc, do, FP64, s.cuda_stream)
+    for k in range(3):
+        ck = synth.make_columns(82 + k, 900, 24)
+        want = rt.run_host(ck)
+        for key in dc:
+            if hasattr(dc[key], "copy_"):
+                dc[key].copy_(torch.from_numpy(np.ascontiguousarray(ck[key], dtype=np.float64)))
+        for key, v in do.items():  # DETAU/CUMTAUV of sun-down columns keep what they held
+            v.fill_(0.0 if key in ("detau", "cumtauv") else -1.0)
+        g.replay()
+        torch.cuda.synchronize()
+        for key in want:
+            assert np.array_equal(do[key].cpu().numpy(), want[key]), (k, key)
