@@ -257,6 +257,33 @@ int mgcm_rt_dust_tauref(const mgcm_rt_dust_columns *in, double *tauref, double *
 int mgcm_rt_dsolflux(mgcm_rt_ctx *ctx, int ncol, const double *sol, const double *acosz, const double *detau,
                      const double *diffvt, double *directsol, double *dnvflux, void *stream);
 
+/* Radiation diagnostics of the history file, from the outputs of mgcm_rt_run.
+ * The REAL*4 fields COMP3 stores per column (code/comp3.f:654,703-706,745-747;
+ * declared REAL*4 in code/modules.f90:510-513,984-986) and MHISTV writes as two
+ * records, "fuptopv, fdntopv, fupsurfv, fdnsurfv" and "fuptopir, fupsurfir,
+ * fdnsurfir", then "surfalb" (code/mhistv.f:200-202; read back by
+ * tutorial/historymod.f90:93-95).  Each field is [ncol], rounded from FP64 to
+ * nearest as the REAL*4 assignment does; with columns ordered J fastest
+ * (col = (I-1)*L_J + J-1) a field is the history's (L_J, L_I) array.  Sun-down
+ * columns get zero VIS fields and keep their surfalb entry, as in COMP3.
+ * hr_nlte [ncol][nlay+1] (optional; needs p, sigma): the heating rate of hr with
+ * COMP3's non-LTE factor on the solar part, SUNTOT(K+1)*2.2E4*PNLTE/(1+2.2E4*PNLTE),
+ * PNLTE = SIGMA(K)*(P(J,I)-PTROP)+PTROP, K = 2l+2 (code/comp3.f:765-780): the
+ * radiative temperature tendency COMP3 applies (QRAD times the theta factor), in
+ * K/sol, the unit of the history's DHEAT (code/newstep.f:132).  All pointers
+ * device (sigma: [nlev] device), asynchronous on stream. */
+typedef struct {
+  float *fuptopv, *fdntopv, *fupsurfv, *fdnsurfv; /* [ncol] */
+  float *fuptopir, *fupsurfir, *fdnsurfir;        /* [ncol] */
+  float *surfalb;                                 /* [ncol] ALS, optional */
+  double *hr_nlte;                                /* [ncol][nlay+1] K/sol, optional */
+  const double *p;                                /* [ncol] P(J,I) = surface pressure - PTROP, mbar */
+  const double *sigma;                            /* [nlev] SIGMA(L_LEVELS) */
+  double ptrop;                                   /* PTROP, mbar */
+} mgcm_rt_history;
+int mgcm_rt_history_fields(const mgcm_rt_columns *in, const mgcm_rt_fluxes *out, const mgcm_rt_history *hist,
+                           void *stream);
+
 /* Diagnostic: the device's exp, log10 and sqrt on n host arguments (x > 0 for
  * log10, x >= 0 for sqrt; other entries return 0).  The FP64 path reproduces the
  * reference's libm (exp and log10 of glibc 2.35, correctly rounded sqrt); this

@@ -2,6 +2,8 @@
 // TEMPGR (code/comp3.f:336-348) over a grid, from the DETAU/DIFFVT outputs of the
 // radiation: one thread per column, the glibc-exact exp, the reference's
 // summation order, so the result equals the reference's bit for bit.
+// Also the radiation's history diagnostics (k_history): COMP3's REAL*4 flux
+// fields and the heating rate with its non-LTE factor.
 #include "rt_ctx.h"
 #include "rt_libm.h"
 
@@ -63,4 +65,86 @@ extern "C" int mgcm_rt_dsolflux(mgcm_rt_ctx *c, int ncol, const double *sol, con
                                (hipStream_t)stream) == hipSuccess
              ? MGCM_RT_OK
              : MGCM_RT_ERR_DEVICE;
+}
+
+namespace mgcm {
+
+struct HistArgs {
+  int ncol, nlay;
+  const double *plev, *acosz, *als;
+  const double *fmnetv, *fluxupv, *fluxdnv, *fmneti, *fluxupi, *fluxdni, *nfluxtopv, *nfluxtopi;
+  float *fuptopv, *fdntopv, *fupsurfv, *fdnsurfv, *fuptopir, *fupsurfir, *fdnsurfir, *surfalb;
+  double *hr_nlte;
+  const double *p, *sigma;
+  double ptrop;
+};
+
+// One thread per (column, heating slot l = 0..nlay): coalesced over the
+// [ncol][nlay+1] arrays; the l = 0 thread of a column also writes its REAL*4
+// history fields.
+__global__ __launch_bounds__(256) void k_history(HistArgs a) {
+  const int nl = a.nlay + 1, nlev = 2 * a.nlay + 3;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)a.ncol * nl) return;
+  const int col = (int)(i / nl), l = (int)(i - (long long)col * nl);
+  const size_t row = (size_t)col * nl;
+  const bool sun = a.acosz[col] >= F32LIT(1.0e-5);  // code/comp3.f:637
+  if (l == 0) {
+    // code/comp3.f:654,703-706 (VIS, zero when the sun is down: :690-695) and :745-747 (IR)
+    a.fuptopv[col] = sun ? (float)a.fluxupv[row] : 0.0f;
+    a.fdntopv[col] = sun ? (float)a.fluxdnv[row] : 0.0f;
+    a.fupsurfv[col] = sun ? (float)a.fluxupv[row + nl - 1] : 0.0f;
+    a.fdnsurfv[col] = sun ? (float)a.fluxdnv[row + nl - 1] : 0.0f;
+    a.fuptopir[col] = (float)a.fluxupi[row];
+    a.fupsurfir[col] = (float)a.fluxupi[row + nl - 1];
+    a.fdnsurfir[col] = (float)a.fluxdni[row + nl - 1];
+    if (a.surfalb && sun) a.surfalb[col] = (float)a.als[col];
+  }
+  if (a.hr_nlte) {
+    // SUNTOT(K+1), IRTOTAL(K+1) with K = 2l+2 (code/comp3.f:677-681,735-738)
+    const double nv = a.fmnetv[row + l], ni = a.fmneti[row + l];
+    const double sunt = sun ? (l == 0 ? nv - a.nfluxtopv[col] : nv - a.fmnetv[row + l - 1]) : 0.0;
+    const double irt = (l == 0) ? ni - a.nfluxtopi[col] : ni - a.fmneti[row + l - 1];
+    // code/comp3.f:765-771: the non-LTE factor on the solar part
+    const double pn = a.sigma[2 * l + 1] * (a.p[col] - a.ptrop) + a.ptrop;
+    const double sunlte = ((sunt * F32LIT(2.2e4)) * pn) / (F32LIT(1.0) + F32LIT(2.2e4) * pn);
+    const double *plev = a.plev + (size_t)col * nlev;
+    const double dp = (l == 0) ? (plev[0] + plev[1]) : (plev[2 * l + 2] - plev[2 * l]);
+    a.hr_nlte[row + l] = (((sunlte + irt) * GRAV) / ((CPAIR * SCALEP) * dp)) * SECONDS_PER_SOL;
+  }
+}
+
+}  // namespace mgcm
+
+extern "C" int mgcm_rt_history_fields(const mgcm_rt_columns *in, const mgcm_rt_fluxes *out,
+                                      const mgcm_rt_history *h, void *stream) {
+  if (!in || !out || !h || in->ncol < 0) return MGCM_RT_ERR_ARG;
+  if (in->nlay < 1 || in->nlay > MGCM_RT_MAX_NLAY) return MGCM_RT_ERR_NLAY;
+  if (in->ncol == 0) return MGCM_RT_OK;
+  if (!in->acosz || !out->fluxupv || !out->fluxdnv || !out->fluxupi || !out->fluxdni || !h->fuptopv ||
+      !h->fdntopv || !h->fupsurfv || !h->fdnsurfv || !h->fuptopir || !h->fupsurfir || !h->fdnsurfir)
+    return MGCM_RT_ERR_ARG;
+  if (h->surfalb && !in->als) return MGCM_RT_ERR_ARG;
+  if (h->hr_nlte && (!h->p || !h->sigma || !in->plev || !out->fmnetv || !out->fmneti || !out->nfluxtopv ||
+                     !out->nfluxtopi))
+    return MGCM_RT_ERR_ARG;
+  mgcm::HistArgs a;
+  a.ncol = in->ncol;
+  a.nlay = in->nlay;
+  a.plev = in->plev;
+  a.acosz = in->acosz;
+  a.als = in->als;
+  a.fmnetv = out->fmnetv; a.fluxupv = out->fluxupv; a.fluxdnv = out->fluxdnv;
+  a.fmneti = out->fmneti; a.fluxupi = out->fluxupi; a.fluxdni = out->fluxdni;
+  a.nfluxtopv = out->nfluxtopv; a.nfluxtopi = out->nfluxtopi;
+  a.fuptopv = h->fuptopv; a.fdntopv = h->fdntopv; a.fupsurfv = h->fupsurfv; a.fdnsurfv = h->fdnsurfv;
+  a.fuptopir = h->fuptopir; a.fupsurfir = h->fupsurfir; a.fdnsurfir = h->fdnsurfir;
+  a.surfalb = h->surfalb;
+  a.hr_nlte = h->hr_nlte;
+  a.p = h->p;
+  a.sigma = h->sigma;
+  a.ptrop = h->ptrop;
+  const long long n = (long long)in->ncol * (in->nlay + 1);
+  hipLaunchKernelGGL(mgcm::k_history, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, a);
+  return hipGetLastError() == hipSuccess ? MGCM_RT_OK : MGCM_RT_ERR_DEVICE;
 }

@@ -23,7 +23,7 @@ import os
 
 import numpy as np
 
-from . import synth  # noqa: F401
+from . import history, synth  # noqa: F401
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmgcm_rt.so")
@@ -81,6 +81,13 @@ class AerosolOut(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in AER_OUT_KEYS]
 
 
+HISTORY_KEYS = ("fuptopv", "fdntopv", "fupsurfv", "fdnsurfv", "fuptopir", "fupsurfir", "fdnsurfir", "surfalb")
+
+
+class History(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in HISTORY_KEYS + ("hr_nlte", "p", "sigma")] + [("ptrop", C.c_double)]
+
+
 class Fluxes(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in (
         "fmnetv", "fluxupv", "fluxdnv", "fmneti", "fluxupi", "fluxdni",
@@ -126,6 +133,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         lib.mgcm_rt_read_ktable.argtypes = [C.c_char_p, C.c_int, DP, DP]
         lib.mgcm_rt_read_aerosol_tables.argtypes = [C.c_char_p, DP]
         lib.mgcm_rt_dsolflux.argtypes = [C.c_void_p, C.c_int, DP] + [C.c_void_p] * 6
+        lib.mgcm_rt_history_fields.argtypes = [C.POINTER(Columns), C.POINTER(Fluxes), C.POINTER(History),
+                                               C.c_void_p]
         lib.mgcm_rt_set_aerosol_tables.argtypes = [C.c_void_p, DP]
         lib.mgcm_rt_opt_dst.argtypes = [C.c_void_p, C.POINTER(Tracers), C.POINTER(AerosolOut), C.c_void_p]
         lib.mgcm_rt_opt_cld.argtypes = [C.c_void_p, C.POINTER(Tracers), C.POINTER(AerosolOut), C.c_void_p]
@@ -273,6 +282,31 @@ class Radiation:
                                          out["diffvt"].data_ptr(), ds.data_ptr(), dn.data_ptr(), C.c_void_p(stream)),
                "mgcm_rt_dsolflux")
         return {"directsol": ds, "dnvflux": dn}
+
+    def history_fields(self, cols: dict, out: dict, hist: dict | None = None, p=None, sigma=None,
+                       ptrop: float = 0.0, stream: int = 0) -> dict:
+        """COMP3's history diagnostics of a run (device tensors): the REAL*4 fields
+        HISTORY_KEYS [ncol] and, given p [ncol] and sigma [nlev], hr_nlte [ncol][nlay+1]
+        (K/sol with the non-LTE factor).  hist may hold preallocated tensors (its
+        surfalb keeps its entries for sun-down columns); the filled dict is returned."""
+        import torch
+        n, nl = int(cols["ncol"]), int(cols["nlay"]) + 1
+        dev = out["fluxupv"].device
+        h = dict(hist or {})
+        for k in HISTORY_KEYS:
+            if h.get(k) is None:
+                h[k] = torch.zeros(n, dtype=torch.float32, device=dev)
+        if p is not None and h.get("hr_nlte") is None:
+            h["hr_nlte"] = torch.empty((n, nl), dtype=torch.float64, device=dev)
+        hs = History(*[h[k].data_ptr() for k in HISTORY_KEYS],
+                     h["hr_nlte"].data_ptr() if p is not None else None,
+                     p.data_ptr() if p is not None else None,
+                     sigma.data_ptr() if sigma is not None else None, float(ptrop))
+        cs = column_struct(cols, ptr=lambda t: t.data_ptr(), sol=self.sol)
+        fs = flux_struct(out, ptr=lambda t: t.data_ptr())
+        _check(self.lib.mgcm_rt_history_fields(C.byref(cs), C.byref(fs), C.byref(hs), C.c_void_p(stream)),
+               "mgcm_rt_history_fields")
+        return h
 
     def set_aerosol_tables(self, tables: np.ndarray) -> None:
         """Upload INITCLD's size grid and optical tables (read_aerosol_tables' block)."""

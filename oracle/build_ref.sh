@@ -1,9 +1,10 @@
 #!/bin/bash
 # Builds oracle/_ref/libmgcm_ref.so: the reference Fortran radiation routines,
 # compiled from their own sources where they lie under /root/reference/code,
-# linked with oracle/ref_harness.f90 (bind(C) driver).  Test infrastructure
-# only.  Objects and module files go to a temporary directory; only the shared
-# library lands in oracle/_ref/.  Exits 0 without building when the reference
+# linked with oracle/ref_harness.f90 (bind(C) driver), and oracle/_ref/htest,
+# the reference's history-file inspection tool.  Test infrastructure only.
+# Objects and module files go to a temporary directory; only the shared library
+# and htest land in oracle/_ref/.  Exits 0 without building when the reference
 # tree is absent.
 set -euo pipefail
 HERE="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
@@ -30,3 +31,14 @@ done
 "$FC" -shared -fPIC -o "$OUT/libmgcm_ref.so.tmp" ./*.o
 mv "$OUT/libmgcm_ref.so.tmp" "$OUT/libmgcm_ref.so"
 echo "build_ref: $OUT/libmgcm_ref.so"
+# The reference's history-file reader and its point-inspection tool, as shipped
+# (tutorial/historymod.f90, tutorial/htest.f90): the check that history files
+# written by the library open in the reference's analysis tools.
+TUT="$REF/../tutorial"
+mkdir -p "$TMP/htest"
+cd "$TMP/htest"
+"$FC" -O2 -w -c "$TUT/historymod.f90" -o historymod.o
+"$FC" -O2 -w -c "$TUT/htest.f90" -o htest.o
+"$FC" -o "$OUT/htest.tmp" htest.o historymod.o
+mv "$OUT/htest.tmp" "$OUT/htest"
+echo "build_ref: $OUT/htest"

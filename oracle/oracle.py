@@ -425,3 +425,48 @@ def _heating(t: dict, c: dict, o: dict) -> None:
         dp = (plev[:, 0] + plev[:, 1]) if l == 0 else (plev[:, 2 * l + 2] - plev[:, 2 * l])
         df = sun[:, l] + ir[:, l]
         o["hr"][:, l] = ((df * grav) / ((cpair * scalep) * dp)) * sps
+
+
+def history_fields(c: dict, o: dict, p=None, sigma=None, ptrop: float = 0.0, surfalb=None) -> dict:
+    """COMP3's history diagnostics of one grid from its radiation outputs.
+
+    REAL*4 fields (code/comp3.f:654,703-706,745-747; REAL*4 declarations
+    code/modules.f90:513,984-986): the TOA/surface flux components rounded to
+    nearest, VIS ones zero where the sun is down (code/comp3.f:690-695), surfalb
+    = ALS only where the sun is up (the previous value, `surfalb`, elsewhere).
+    With p, sigma: hr_nlte, the heating rate in K/sol with COMP3's non-LTE factor
+    on SUNTOT (code/comp3.f:765-771), PNLTE = SIGMA(K)*(P-PTROP)+PTROP, K = 2l+2.
+    """
+    up = np.asarray(c["acosz"]) >= np.float64(np.float32(1.0e-5))
+    f32 = lambda a: np.asarray(a, dtype=np.float64).astype(np.float32)  # noqa: E731
+    z = np.float32(0.0)
+    h = {"fuptopv": np.where(up, f32(o["fluxupv"][:, 0]), z),
+         "fdntopv": np.where(up, f32(o["fluxdnv"][:, 0]), z),
+         "fupsurfv": np.where(up, f32(o["fluxupv"][:, -1]), z),
+         "fdnsurfv": np.where(up, f32(o["fluxdnv"][:, -1]), z),
+         "fuptopir": f32(o["fluxupi"][:, 0]),
+         "fupsurfir": f32(o["fluxupi"][:, -1]),
+         "fdnsurfir": f32(o["fluxdni"][:, -1])}
+    prev = np.zeros(up.size, np.float32) if surfalb is None else np.asarray(surfalb, np.float32)
+    h["surfalb"] = np.where(up, f32(c["als"]), prev)
+    if p is not None:
+        nlay = o["fmnetv"].shape[1] - 1
+        grav, cpair, scalep, sps = 3.72, 7.3594e2, 1.0e2, 88775.0
+        k22, one = float(np.float32(2.2e4)), float(np.float32(1.0))
+        fv, fi = o["fmnetv"], o["fmneti"]
+        sun = np.empty_like(fv)
+        sun[:, 0] = fv[:, 0] - o["nfluxtopv"]
+        sun[:, 1:] = fv[:, 1:] - fv[:, :-1]
+        sun[~up] = 0.0
+        ir = np.empty_like(fi)
+        ir[:, 0] = fi[:, 0] - o["nfluxtopi"]
+        ir[:, 1:] = fi[:, 1:] - fi[:, :-1]
+        plev, p, sigma = np.asarray(c["plev"]), np.asarray(p, np.float64), np.asarray(sigma, np.float64)
+        hr = np.empty_like(fv)
+        for l in range(nlay + 1):
+            pn = sigma[2 * l + 1] * (p - ptrop) + ptrop
+            sunlte = ((sun[:, l] * k22) * pn) / (one + k22 * pn)
+            dp = (plev[:, 0] + plev[:, 1]) if l == 0 else (plev[:, 2 * l + 2] - plev[:, 2 * l])
+            hr[:, l] = (((sunlte + ir[:, l]) * grav) / ((cpair * scalep) * dp)) * sps
+        h["hr_nlte"] = hr
+    return h
