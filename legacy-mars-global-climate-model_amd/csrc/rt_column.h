@@ -593,8 +593,11 @@ __global__ __launch_bounds__(64) void k_rt_clear(DevTables dt, RunArgs a) {
 
 // Both parts in one launch: blocks below nvis take VIS items, the rest IR
 // items, so the two halves share one dispatch and fill each other's tail.
+// Two waves per SIMD: the register budget of 256 schedules the item's loads
+// further ahead than the compiler's default occupancy target (4) does.
 template <int NLM>
-__global__ __launch_bounds__(64) void k_rt_clear2(DevTables dt, RunArgs a, int nvis) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void k_rt_clear2(DevTables dt, RunArgs a, int nvis) {
   mgcm_libm::load_lds_exptab(threadIdx.x, 64);
   __syncthreads();
   if ((int)blockIdx.x < nvis) clear_block<NLM, true>(dt, a, blockIdx.x * ClearShape<true>::cols);

@@ -162,6 +162,19 @@ struct LiveLayerAer {
 // 1-based).  Inputs arrive through the Src functor: finish(K, fetch(K)) gives the sub-level
 // optical depth DTAUK(K) and adds the gas part to taugsurf.
 
+// DSOLVER's clip of XK2 (code/dsolver.f90:101): 0 where XK2 /= 0 and
+// |XK2/(XA+1E-20)| < 1E-30.  Where |XK2| > |XA+1E-20| * 2^-99 (a power of two:
+// exact while the product is normal) the quotient's magnitude exceeds
+// 2^-99 > 1E-30 for certain and XK2 stays; only when some lane of the wave is not
+// in that case is the reference's division evaluated, for the whole wave.
+__device__ __forceinline__ double xk2_clip(double xa, double xk2) {
+  const double d = xa + F32LIT(1.e-20);
+  const double t = fabs(d) * 0x1p-99;
+  const bool keep = fabs(xk2) > t && t >= 0x1p-1022;
+  if (__all(keep || xk2 == 0.0)) return xk2;
+  return (xk2 != 0.0 && fabs(xk2 / d) < F32LIT(1.e-30)) ? 0.0 : xk2;
+}
+
 // Two FP64 values that both passes of a solve read together: one 16-byte
 // private-memory access instead of two 8-byte ones.
 struct alignas(16) Pair {
@@ -361,7 +374,7 @@ __device__ __forceinline__ void vis_solve(int nl, VisState<NLM> &s, const double
     xkprev = xb;
     const double xk1 = xa + xb;
     double xk2 = xa - xb;
-    xk2 = (xk2 != 0.0 && fabs(xk2 / (xa + F32LIT(1.e-20))) < F32LIT(1.e-30)) ? 0.0 : xk2;  // dsolver.f90:101
+    xk2 = xk2_clip(xa, xk2);  // dsolver.f90:101
     const VisLayer v = vis_layer(w0c, cbc, u0, f0pi);
     const double lam = v.lam, gama = v.gama, am = v.am, ap = v.ap;
     double fmidp, fmidm;
@@ -545,7 +558,7 @@ __device__ __forceinline__ double ir_solve(int nl, IrState<NLM> &s, const double
     xkprev = xb;
     const double xk1 = xa + xb;
     double xk2 = xa - xb;
-    xk2 = (xk2 != 0.0 && fabs(xk2 / (xa + F32LIT(1.e-20))) < F32LIT(1.e-30)) ? 0.0 : xk2;  // dsolver.f90:101
+    xk2 = xk2_clip(xa, xk2);  // dsolver.f90:101
     const IrLayer v = ir_layer(w0c, cbc, dtc, btc, bbc);
     const double lam = v.lam, gama = v.gama, b0 = v.b0, b1 = v.b1, term = v.term;
     const double dk = (n < nl) ? dmc : dtc;
