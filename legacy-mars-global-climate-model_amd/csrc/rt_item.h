@@ -23,13 +23,14 @@ namespace mgcm {
 // 0..NB-1 (VIS first); K: 1-based level 1..L_LEVELS+1.
 struct Aer {
   const RtConst *rc;
-  const double *q[12];           // RunArgs::aq of this column ([nb][nlev+1]), or null
-  const double *qref_d, *qref_c; // this column's [nlev+1], or null
-  int ld;                        // nlev + 1
+  const double *q[12];            // RunArgs::aq ([ncol][nb][nlev+1]), or null: wave-uniform
+  const double *qref_d, *qref_c;  // RunArgs::qref_dst/_cld ([ncol][nlev+1]), or null
+  size_t offv, offi, offr;        // this column's offsets: col*NV*ld, col*NI*ld, col*ld
+  int ld;                         // nlev + 1
   // kind: 0 Qext, 1 Qscat, 2 g; cld: 0 dust, 1 water ice
   __device__ __forceinline__ double get(int kind, int cld, int b, int K, double cst) const {
     const double *p = q[cld * 6 + (b < NV ? 0 : 3) + kind];
-    return p ? p[(b < NV ? b : b - NV) * ld + K - 1] : cst;
+    return p ? p[(b < NV ? offv + (size_t)b * ld : offi + (size_t)(b - NV) * ld) + K - 1] : cst;
   }
   __device__ __forceinline__ double qx_dst(int b, int K) const { return get(0, 0, b, K, rc->qx_dst[b]); }
   __device__ __forceinline__ double qs_dst(int b, int K) const { return get(1, 0, b, K, rc->qs_dst[b]); }
@@ -39,22 +40,24 @@ struct Aer {
   __device__ __forceinline__ double g_cld(int b, int K) const { return get(2, 1, b, K, rc->g_cld[b]); }
   // Qextref(K) = QXVDST(K, L_NREFV) (code/optcv.f90:114), QREFV(K) = QEXTREFDST(K), QEXTREFCLD(K)
   __device__ __forceinline__ double qref_vis(int K) const {
-    return q[0] ? q[0][(L_NREFV - 1) * ld + K - 1] : rc->qextref_vis;
+    return q[0] ? q[0][offv + (size_t)(L_NREFV - 1) * ld + K - 1] : rc->qextref_vis;
   }
-  __device__ __forceinline__ double qref_dst(int K) const { return qref_d ? qref_d[K - 1] : rc->qextref_dst; }
-  __device__ __forceinline__ double qref_cld(int K) const { return qref_c ? qref_c[K - 1] : rc->qextref_cld; }
+  __device__ __forceinline__ double qref_dst(int K) const { return qref_d ? qref_d[offr + K - 1] : rc->qextref_dst; }
+  __device__ __forceinline__ double qref_cld(int K) const { return qref_c ? qref_c[offr + K - 1] : rc->qextref_cld; }
 };
 
+// The base pointers stay the run's (uniform across the wave); only the
+// column's offsets are per lane.
 __device__ __forceinline__ Aer column_aerosol(const RunArgs &a, const RtConst *rc, int col) {
   Aer e;
   e.rc = rc;
   e.ld = 2 * a.nlay + 4;
-  for (int i = 0; i < 12; i++) {
-    const int nb = ((i / 3) % 2 == 0) ? NV : NI;
-    e.q[i] = a.aq[i] ? a.aq[i] + (size_t)col * nb * e.ld : nullptr;
-  }
-  e.qref_d = a.qref_dst ? a.qref_dst + (size_t)col * e.ld : nullptr;
-  e.qref_c = a.qref_cld ? a.qref_cld + (size_t)col * e.ld : nullptr;
+  for (int i = 0; i < 12; i++) e.q[i] = a.aq[i];
+  e.qref_d = a.qref_dst;
+  e.qref_c = a.qref_cld;
+  e.offr = (size_t)col * e.ld;
+  e.offv = e.offr * NV;
+  e.offi = e.offr * NI;
   return e;
 }
 
