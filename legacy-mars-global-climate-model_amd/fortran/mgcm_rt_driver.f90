@@ -3,7 +3,7 @@
 ! tables up once, runs the batch through mgcm_rt_radiation and writes the
 ! results to a second stream file.
 !
-!   mgcm_rt_driver <case file> <result file> [data directory]
+!   mgcm_rt_driver <case file> <result file> [data directory | -] [history file]
 !
 ! With a data directory, the k-tables are set up as RADSETUP does on the GCM's
 ! start (code/setrad.f90:112-122, code/laginterp.f90): CO2H2O_V_12_95_INTEL and
@@ -17,6 +17,10 @@
 ! VIS(7) and IR(5), QEXTREFDST, QEXTREFCLD, SOL(7); then per column set
 ! PLEV, TLEV, PMID, TMID, QH2O (L_LEVELS, ncol), TAUREF, TAUREFCLD
 ! (L_LEVELS+1, ncol), ALS, ALBI, ACOSZ (ncol).
+! With a history file, the radiation's two history records are written to it
+! the way COMP3 fills them (REAL*4 assignments, code/comp3.f:703-706,745-747)
+! and MHISTV writes them (Fortran sequential unformatted, code/mhistv.f:200-201),
+! the columns taken as (L_J, L_I) = (ncol, 1).
 ! Result file: int32 status; FMNETV, FLUXUPV, FLUXDNV, FMNETI, FLUXUPI,
 ! FLUXDNI (L_NLAYRAD, ncol), NFLUXTOPV, NFLUXTOPI, DIFFVT (ncol), DETAU,
 ! CUMTAUV (7, 17, ncol), heating (L_NLAYRAD, ncol).
@@ -24,7 +28,7 @@ program mgcm_rt_driver
   use iso_c_binding
   use mgcm_rt_mod
   implicit none
-  character(len=4096) :: fin, fout, datadir
+  character(len=4096) :: fin, fout, datadir, fhist
   ! PGASREF: REAL(4) literals, as SETRAD writes them (code/setrad.f90:112-122)
   real(c_double), parameter :: pgref(11) = [1.0e-6, 1.0e-5, 1.0e-4, 1.0e-3, 1.0e-2, 1.0e-1, 1.0, 1.0e1, &
                                             1.0e2, 1.0e3, 1.0e4]
@@ -46,15 +50,21 @@ program mgcm_rt_driver
   real(c_double), allocatable :: fmnetv(:,:), fluxupv(:,:), fluxdnv(:,:), fmneti(:,:), fluxupi(:,:)
   real(c_double), allocatable :: fluxdni(:,:), nfluxtopv(:), nfluxtopi(:), diffvt(:)
   real(c_double), allocatable :: detau(:,:,:), cumtauv(:,:,:), hr(:,:)
+  real(4), allocatable :: fuptopv(:,:), fdntopv(:,:), fupsurfv(:,:), fdnsurfv(:,:)
+  real(4), allocatable :: fuptopir(:,:), fupsurfir(:,:), fdnsurfir(:,:)
+  integer :: n
 
-  if (command_argument_count() /= 2 .and. command_argument_count() /= 3) then
-    write(*, '(a)') 'usage: mgcm_rt_driver <case file> <result file> [data directory]'
+  if (command_argument_count() < 2 .or. command_argument_count() > 4) then
+    write(*, '(a)') 'usage: mgcm_rt_driver <case file> <result file> [data directory | -] [history file]'
     stop 2
   end if
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
   datadir = ' '
-  if (command_argument_count() == 3) call get_command_argument(3, datadir)
+  fhist = ' '
+  if (command_argument_count() >= 3) call get_command_argument(3, datadir)
+  if (trim(datadir) == '-') datadir = ' '
+  if (command_argument_count() == 4) call get_command_argument(4, fhist)
 
   open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old', iostat=ios)
   if (ios /= 0) then
@@ -94,6 +104,31 @@ program mgcm_rt_driver
     s2 = mgcm_rt_destroy(ctx)
   end if
   if (status /= MGCM_RT_OK) write(*, '(a,i0,a)') 'mgcm_rt_driver: status ', status, ': '//trim(mgcm_rt_message(status))
+
+  if (status == MGCM_RT_OK .and. len_trim(fhist) > 0) then
+    allocate(fuptopv(ncol, 1), fdntopv(ncol, 1), fupsurfv(ncol, 1), fdnsurfv(ncol, 1))
+    allocate(fuptopir(ncol, 1), fupsurfir(ncol, 1), fdnsurfir(ncol, 1))
+    do n = 1, ncol
+      if (acosz(n) >= 1.0e-5) then
+        fuptopv(n, 1) = fluxupv(1, n)
+        fdntopv(n, 1) = fluxdnv(1, n)
+        fupsurfv(n, 1) = fluxupv(nl, n)
+        fdnsurfv(n, 1) = fluxdnv(nl, n)
+      else
+        fuptopv(n, 1) = 0.0
+        fdntopv(n, 1) = 0.0
+        fupsurfv(n, 1) = 0.0
+        fdnsurfv(n, 1) = 0.0
+      end if
+      fuptopir(n, 1) = fluxupi(1, n)
+      fupsurfir(n, 1) = fluxupi(nl, n)
+      fdnsurfir(n, 1) = fluxdni(nl, n)
+    end do
+    open(newunit=u, file=trim(fhist), form='unformatted', status='replace')
+    write(u) fuptopv, fdntopv, fupsurfv, fdnsurfv
+    write(u) fuptopir, fupsurfir, fdnsurfir
+    close(u)
+  end if
 
   open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
   write(u) status

@@ -50,7 +50,15 @@ module mgcm_rt_mod
     type(c_ptr) :: directsol, dnvflux
   end type
 
-  public :: mgcm_rt_init, mgcm_rt_destroy, mgcm_rt_run, mgcm_rt_run_host, mgcm_rt_check
+  ! mirrors mgcm_rt_history: COMP3's REAL*4 history fields and the non-LTE
+  ! heating of a device-resident run (mgcm_rt_history_fields)
+  type, bind(C), public :: mgcm_rt_history
+    type(c_ptr) :: fuptopv, fdntopv, fupsurfv, fdnsurfv, fuptopir, fupsurfir, fdnsurfir, surfalb
+    type(c_ptr) :: hr_nlte, p, sigma
+    real(c_double) :: ptrop
+  end type
+
+  public :: mgcm_rt_init, mgcm_rt_destroy, mgcm_rt_run, mgcm_rt_run_host, mgcm_rt_check, mgcm_rt_history_fields
   public :: mgcm_rt_run_batched, mgcm_rt_planck_table, mgcm_rt_laginterp, mgcm_rt_read_ktable, mgcm_rt_strerror
   public :: mgcm_rt_setup, mgcm_rt_radiation, mgcm_rt_message, mgcm_rt_laginterp_files
 
@@ -108,6 +116,13 @@ module mgcm_rt_mod
       character(kind=c_char), intent(in) :: path(*)
       integer(c_int), value :: nbands
       real(c_double), intent(out) :: raw(*), fzero(*)
+    end function
+    integer(c_int) function mgcm_rt_history_fields(cols, outs, hist, stream) bind(C, name="mgcm_rt_history_fields")
+      import :: c_int, c_ptr, mgcm_rt_columns, mgcm_rt_fluxes, mgcm_rt_history
+      type(mgcm_rt_columns), intent(in) :: cols
+      type(mgcm_rt_fluxes), intent(in) :: outs
+      type(mgcm_rt_history), intent(in) :: hist
+      type(c_ptr), value :: stream
     end function
     type(c_ptr) function mgcm_rt_strerror(status) bind(C, name="mgcm_rt_strerror")
       import :: c_int, c_ptr

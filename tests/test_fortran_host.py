@@ -115,3 +115,33 @@ def test_fortran_driver_on_reference_data_files(tmp_path):
     for k in casefile.RESULT_KEYS:
         e = rel_err(got[k], real["out_" + k])
         assert e <= 1e-10, (k, e)
+
+
+@pytest.mark.gpu
+def test_fortran_driver_history_records(tmp_path):
+    """The driver fills the history's REAL*4 flux fields as COMP3 does and writes
+    MHISTV's two radiation records with Fortran unformatted I/O; the records read
+    back with mgcm_rt.history equal the device's k_history fields bit for bit."""
+    import torch
+    from mgcm_rt import HISTORY_KEYS, Radiation, alloc_outputs, casefile, synth, to_device
+    from mgcm_rt import history as H
+    t, c = synth.make_workload(19, 250, 24)
+    c["acosz"][::6] = 0.0
+    case, res, hist = str(tmp_path / "case.bin"), str(tmp_path / "res.bin"), str(tmp_path / "fort.11")
+    casefile.write_case(case, t, c, 0)
+    r = subprocess.run([DRIVER, case, res, "-", hist], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    with open(hist, "rb") as f:
+        rec1, rec2 = H.read_record(f), H.read_record(f)
+        assert H.read_record(f) is None
+    f1 = np.frombuffer(rec1, "<f4").reshape(4, 250)
+    f2 = np.frombuffer(rec2, "<f4").reshape(3, 250)
+    rt = Radiation(t)
+    dc, do = to_device(c), to_device(alloc_outputs(250, 24))
+    rt.run_device(dc, do)
+    h = rt.history_fields(dc, do)
+    torch.cuda.synchronize()
+    got = dict(zip(HISTORY_KEYS[:7], list(f1) + list(f2)))
+    for k in HISTORY_KEYS[:7]:
+        assert np.array_equal(got[k], h[k].cpu().numpy()), k
+    assert not np.any(got["fuptopv"][::6]) and np.all(got["fuptopir"] > 0)
