@@ -3,7 +3,12 @@
 ! tables up once, runs the batch through mgcm_rt_radiation and writes the
 ! results to a second stream file.
 !
-!   mgcm_rt_driver <case file> <result file> [data directory | -] [history file]
+!   mgcm_rt_driver [--per-column] <case file> <result file> [data directory | -] [history file]
+!
+! --per-column runs COMP3's own loop shape instead of one batch: mgcm_rt_column
+! once per column, with DETAU/CUMTAUV as the module arrays (L_J, L_I, 7, 17),
+! here (ncol, 1, 7, 17), in FP64; the heating is not formed (written as zeros).
+! The fluxes, DETAU and CUMTAUV are the batch's, bit for bit.
 !
 ! With a data directory, the k-tables are set up as RADSETUP does on the GCM's
 ! start (code/setrad.f90:112-122, code/laginterp.f90): CO2H2O_V_12_95_INTEL and
@@ -52,19 +57,31 @@ program mgcm_rt_driver
   real(c_double), allocatable :: detau(:,:,:), cumtauv(:,:,:), hr(:,:)
   real(4), allocatable :: fuptopv(:,:), fdntopv(:,:), fupsurfv(:,:), fdnsurfv(:,:)
   real(4), allocatable :: fuptopir(:,:), fupsurfir(:,:), fdnsurfir(:,:)
-  integer :: n
+  integer :: n, a0, nargs
+  logical :: percol
+  real(c_double), allocatable :: detau4(:,:,:,:), cumtauv4(:,:,:,:)
 
-  if (command_argument_count() < 2 .or. command_argument_count() > 4) then
-    write(*, '(a)') 'usage: mgcm_rt_driver <case file> <result file> [data directory | -] [history file]'
+  nargs = command_argument_count()
+  a0 = 0
+  percol = .false.
+  if (nargs >= 1) then
+    call get_command_argument(1, fin)
+    if (trim(fin) == '--per-column') then
+      percol = .true.
+      a0 = 1
+    end if
+  end if
+  if (nargs - a0 < 2 .or. nargs - a0 > 4) then
+    write(*, '(a)') 'usage: mgcm_rt_driver [--per-column] <case file> <result file> [data directory | -] [history file]'
     stop 2
   end if
-  call get_command_argument(1, fin)
-  call get_command_argument(2, fout)
+  call get_command_argument(a0 + 1, fin)
+  call get_command_argument(a0 + 2, fout)
   datadir = ' '
   fhist = ' '
-  if (command_argument_count() >= 3) call get_command_argument(3, datadir)
+  if (nargs - a0 >= 3) call get_command_argument(a0 + 3, datadir)
   if (trim(datadir) == '-') datadir = ' '
-  if (command_argument_count() == 4) call get_command_argument(4, fhist)
+  if (nargs - a0 == 4) call get_command_argument(a0 + 4, fhist)
 
   open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old', iostat=ios)
   if (ios /= 0) then
@@ -97,7 +114,25 @@ program mgcm_rt_driver
     call mgcm_rt_setup(co2v, co2i, pfgasref, tgasref, wrefh2o, gwv, gwi, fzerov, fzeroi, tauray, dwni, &
                        planckir, qxvdst, qsvdst, gvdst, qxidst, qsidst, gidst, qxvcld, qsvcld, gvcld, &
                        qxicld, qsicld, gicld, qextrefdst, qextrefcld, ctx, status)
-  if (status == MGCM_RT_OK) then
+  if (status == MGCM_RT_OK .and. percol) then
+    ! COMP3's loop over (J, I), here J = column, I = 1 (code/comp3.f:411-415,637-738)
+    allocate(detau4(ncol, 1, MGCM_RT_NV, MGCM_RT_NG), cumtauv4(ncol, 1, MGCM_RT_NV, MGCM_RT_NG))
+    detau4 = 0.0d0
+    cumtauv4 = 0.0d0
+    hr = 0.0d0
+    do n = 1, ncol
+      call mgcm_rt_column(ctx, nlay, plev(:, n), tlev(:, n), pmid(:, n), tmid(:, n), qh2o(:, n), tauref(:, n), &
+                          taurefcld(:, n), als(n), albi(n), acosz(n), sol, detau4, cumtauv4, n, 1, &
+                          nfluxtopv(n), fmnetv(:, n), fluxupv(:, n), fluxdnv(:, n), diffvt(n), &
+                          nfluxtopi(n), fmneti(:, n), fluxupi(:, n), fluxdni(:, n), status)
+      if (status /= MGCM_RT_OK) exit
+    end do
+    do n = 1, ncol
+      detau(:, :, n) = detau4(n, 1, :, :)
+      cumtauv(:, :, n) = cumtauv4(n, 1, :, :)
+    end do
+    s2 = mgcm_rt_destroy(ctx)
+  else if (status == MGCM_RT_OK) then
     call mgcm_rt_radiation(ctx, ncol, nlay, plev, tlev, pmid, tmid, qh2o, tauref, taurefcld, als, albi, &
                            acosz, sol, fmnetv, fluxupv, fluxdnv, fmneti, fluxupi, fluxdni, nfluxtopv, &
                            nfluxtopi, diffvt, detau, cumtauv, hr, precision, status)

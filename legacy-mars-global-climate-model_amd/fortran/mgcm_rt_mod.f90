@@ -60,7 +60,7 @@ module mgcm_rt_mod
 
   public :: mgcm_rt_init, mgcm_rt_destroy, mgcm_rt_run, mgcm_rt_run_host, mgcm_rt_check, mgcm_rt_history_fields
   public :: mgcm_rt_run_batched, mgcm_rt_planck_table, mgcm_rt_laginterp, mgcm_rt_read_ktable, mgcm_rt_strerror
-  public :: mgcm_rt_setup, mgcm_rt_radiation, mgcm_rt_message, mgcm_rt_laginterp_files
+  public :: mgcm_rt_setup, mgcm_rt_radiation, mgcm_rt_message, mgcm_rt_laginterp_files, mgcm_rt_column
 
   interface
     integer(c_int) function mgcm_rt_init(tables, ctx) bind(C, name="mgcm_rt_init")
@@ -281,6 +281,78 @@ contains
     if (present(directsol)) o%directsol = c_loc(directsol)
     if (present(directsol) .and. present(dnvflux)) o%dnvflux = c_loc(dnvflux)
     status = mgcm_rt_run_host(ctx, c, o, precision)
+  end subroutine
+
+  ! Per-column drop-in for COMP3's radiation block (code/comp3.f:637-738): one
+  ! call in place of
+  !     if (ACOSZ >= 1.0e-5) then; call optcv(...); call sfluxv(...); end if
+  !     call optci(...); call sfluxi(...)
+  ! for column (J, I), with COMP3's per-column arrays in the reference's shapes:
+  ! PLEV, TLEV, PMID, TMID, QH2O(L_LEVELS), TAUREF, TAUREFCLD(L_LEVELS+1), the
+  ! fluxes (L_NLAYRAD).  DETAU and CUMTAUV are the module arrays
+  ! (L_J, L_I, L_NSPECTV, L_NGAUSS) (code/modules.f90:319,410), written at (J, I)
+  ! as SFLUXV writes detau(j1d,i1d,nw,ng) (code/sfluxv.f90:30,68-70) and COMP3
+  ! CUMTAUV (code/comp3.f:664-668), and left as they are when the sun is down;
+  ! the VIS outputs of a sun-down column are zero.  One device run per column:
+  ! for porting COMP3 a step at a time and for drop-in tests; mgcm_rt_radiation
+  ! on a whole batch is the fast path.
+  subroutine mgcm_rt_column(ctx, nlay, plev, tlev, pmid, tmid, qh2o, tauref, taurefcld, als, albi, acosz, &
+                            sol, detau, cumtauv, j, i, nfluxtopv, fmnetv, fluxupv, fluxdnv, diffvt, &
+                            nfluxtopi, fmneti, fluxupi, fluxdni, status, aerosol)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: nlay, j, i
+    real(c_double), intent(in) :: plev(2*nlay+3), tlev(2*nlay+3), pmid(2*nlay+3), tmid(2*nlay+3)
+    real(c_double), intent(in) :: qh2o(2*nlay+3), tauref(2*nlay+4), taurefcld(2*nlay+4)
+    real(c_double), intent(in) :: als, albi, acosz, sol(MGCM_RT_NV)
+    real(c_double), intent(inout) :: detau(:,:,:,:), cumtauv(:,:,:,:)
+    real(c_double), intent(out) :: nfluxtopv, fmnetv(nlay+1), fluxupv(nlay+1), fluxdnv(nlay+1), diffvt
+    real(c_double), intent(out) :: nfluxtopi, fmneti(nlay+1), fluxupi(nlay+1), fluxdni(nlay+1)
+    integer(c_int), intent(out) :: status
+    type(mgcm_rt_aerosol), intent(in), optional :: aerosol
+    real(c_double) :: pl(2*nlay+3, 1), tl(2*nlay+3, 1), pm(2*nlay+3, 1), tm(2*nlay+3, 1), qh(2*nlay+3, 1)
+    real(c_double) :: tr(2*nlay+4, 1), tc(2*nlay+4, 1), a1(1), a2(1), a3(1)
+    real(c_double) :: fv(nlay+1, 1), uv(nlay+1, 1), dv(nlay+1, 1), fi(nlay+1, 1), ui(nlay+1, 1), di(nlay+1, 1)
+    real(c_double) :: ntv(1), nti(1), dif(1), hr(nlay+1, 1)
+    real(c_double) :: det(MGCM_RT_NV, MGCM_RT_NG, 1), cum(MGCM_RT_NV, MGCM_RT_NG, 1)
+    integer :: nw, ng
+    pl(:, 1) = plev
+    tl(:, 1) = tlev
+    pm(:, 1) = pmid
+    tm(:, 1) = tmid
+    qh(:, 1) = qh2o
+    tr(:, 1) = tauref
+    tc(:, 1) = taurefcld
+    a1(1) = als
+    a2(1) = albi
+    a3(1) = acosz
+    do ng = 1, MGCM_RT_NG
+      do nw = 1, MGCM_RT_NV
+        det(nw, ng, 1) = detau(j, i, nw, ng)
+        cum(nw, ng, 1) = cumtauv(j, i, nw, ng)
+      end do
+    end do
+    if (present(aerosol)) then
+      call mgcm_rt_radiation(ctx, 1, nlay, pl, tl, pm, tm, qh, tr, tc, a1, a2, a3, sol, fv, uv, dv, fi, ui, di, &
+                             ntv, nti, dif, det, cum, hr, MGCM_RT_FP64, status, aerosol=aerosol)
+    else
+      call mgcm_rt_radiation(ctx, 1, nlay, pl, tl, pm, tm, qh, tr, tc, a1, a2, a3, sol, fv, uv, dv, fi, ui, di, &
+                             ntv, nti, dif, det, cum, hr, MGCM_RT_FP64, status)
+    end if
+    fmnetv = fv(:, 1)
+    fluxupv = uv(:, 1)
+    fluxdnv = dv(:, 1)
+    fmneti = fi(:, 1)
+    fluxupi = ui(:, 1)
+    fluxdni = di(:, 1)
+    nfluxtopv = ntv(1)
+    nfluxtopi = nti(1)
+    diffvt = dif(1)
+    do ng = 1, MGCM_RT_NG
+      do nw = 1, MGCM_RT_NV
+        detau(j, i, nw, ng) = det(nw, ng, 1)
+        cumtauv(j, i, nw, ng) = cum(nw, ng, 1)
+      end do
+    end do
   end subroutine
 
 end module mgcm_rt_mod

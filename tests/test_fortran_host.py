@@ -145,3 +145,27 @@ def test_fortran_driver_history_records(tmp_path):
     for k in HISTORY_KEYS[:7]:
         assert np.array_equal(got[k], h[k].cpu().numpy()), k
     assert not np.any(got["fuptopv"][::6]) and np.all(got["fuptopir"] > 0)
+
+
+@pytest.mark.gpu
+def test_fortran_per_column_shim_equals_batch(tmp_path):
+    """COMP3's loop shape: mgcm_rt_column once per column, DETAU/CUMTAUV as the
+    module arrays indexed (J, I), gives the batch call's outputs bit for bit."""
+    from mgcm_rt import casefile, synth
+    t, c = synth.make_workload(23, 40, 24, clouds=True)
+    c["acosz"][::5] = 0.0
+    case = str(tmp_path / "case.bin")
+    casefile.write_case(case, t, c, 0)
+    outs = {}
+    for mode in ("batch", "column"):
+        res = str(tmp_path / f"res_{mode}.bin")
+        args = [DRIVER] + (["--per-column"] if mode == "column" else []) + [case, res]
+        r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stdout + r.stderr
+        status, outs[mode] = casefile.read_result(res, 40, 24)
+        assert status == 0
+    for k in casefile.RESULT_KEYS:
+        if k == "hr":
+            continue
+        assert np.array_equal(outs["column"][k], outs["batch"][k]), k
+    assert not np.any(outs["column"]["fluxdnv"][::5])  # sun-down columns
