@@ -34,7 +34,9 @@ extern "C" {
 #define MGCM_RT_NV 7      /* L_NSPECTV (code/modules.f90:225) */
 #define MGCM_RT_NI 5      /* L_NSPECTI (code/modules.f90:224) */
 #define MGCM_RT_NPLANCK 8501
-#define MGCM_RT_MAX_NLAY 64
+/* 2*NLAYRAD rows within DSOLVER's NMAX = 201 (code/dsolver.f90:12) and NLL <=
+ * 101 of GFLUXI (code/gfluxi.f90:65): at most 99 GCM layers, 100 radiation layers */
+#define MGCM_RT_MAX_NLAY 99
 
 enum {
   MGCM_RT_OK = 0,

@@ -7,7 +7,8 @@
 //     layer, the log-normal size distribution of the dust (or ice-coated dust)
 //     tracers, its cross-section weights over the 20 bins (erf), and the
 //     weighted Qext/Qscat/g per band, the reference extinction and TAUREF (on the
-//     device, one 64-lane wave per column, lane l-1 = layer l).
+//     device, one block per column (one wave up to 64 layers, two above),
+//     thread l-1 = layer l).
 // The outputs are the per-level arrays of mgcm_rt_aerosol plus TAUREF/TAUREFCLD.
 #include <cmath>
 #include <cstdio>
@@ -42,11 +43,11 @@ struct AerOut {
 // OPT_DST (CLOUD = false) / OPT_CLD (CLOUD = true) for one column per block.
 // qtr: [ncol][6][nlay]; pl: [ncol][nlev].
 template <bool CLOUD>
-__global__ __launch_bounds__(64) void k_opt_aer(const double *__restrict__ tab, int ncol, int nlay,
+__global__ __launch_bounds__(128) void k_opt_aer(const double *__restrict__ tab, int ncol, int nlay,
                                                 const double *__restrict__ qtr, const double *__restrict__ pl_all,
                                                 AerOut o) {
-  __shared__ double s_tau[64][4];
-  __shared__ int s_on[64];
+  __shared__ double s_tau[128][4];
+  __shared__ int s_on[128];
   const int col = blockIdx.x, lane = threadIdx.x;
   if (col >= ncol) return;
   const int nlevels = 2 * nlay + 3, ld = nlevels + 1;
@@ -282,11 +283,12 @@ static int opt_aer(bool cloud, mgcm_rt_ctx *c, const mgcm_rt_tracers *in, const 
     if (!p) return MGCM_RT_ERR_ARG;
   AerOut o{out->qxv, out->qsv, out->gv, out->qxi, out->qsi, out->gi, out->qextref, out->tauref, out->tau2};
   hipStream_t st = (hipStream_t)stream;
+  const dim3 blk(in->nlay <= 64 ? 64 : 128);  // thread l-1 = layer l
   if (cloud)
-    hipLaunchKernelGGL(k_opt_aer<true>, dim3(in->ncol), dim3(64), 0, st, c->d_aer, in->ncol, in->nlay,
+    hipLaunchKernelGGL(k_opt_aer<true>, dim3(in->ncol), blk, 0, st, c->d_aer, in->ncol, in->nlay,
                        in->qtrace, in->pl, o);
   else
-    hipLaunchKernelGGL(k_opt_aer<false>, dim3(in->ncol), dim3(64), 0, st, c->d_aer, in->ncol, in->nlay,
+    hipLaunchKernelGGL(k_opt_aer<false>, dim3(in->ncol), blk, 0, st, c->d_aer, in->ncol, in->nlay,
                        in->qtrace, in->pl, o);
   return hipGetLastError() == hipSuccess ? MGCM_RT_OK : MGCM_RT_ERR_DEVICE;
 }

@@ -19,7 +19,7 @@ extern "C" const char *mgcm_rt_strerror(int s) {
   switch (s) {
     case MGCM_RT_OK: return "ok";
     case MGCM_RT_ERR_ARG: return "invalid argument (null pointer or negative size)";
-    case MGCM_RT_ERR_NLAY: return "layer count outside 1..64";
+    case MGCM_RT_ERR_NLAY: return "layer count outside 1..99";
     case MGCM_RT_ERR_DEVICE: return "HIP runtime error";
     case MGCM_RT_ERR_NOMEM: return "out of device memory";
     case MGCM_RT_ERR_RANGE: return "temperature outside the Planck table (50.0..900.0 K)";

@@ -668,13 +668,14 @@ hipError_t launch_column(const DevTables &dt, const RunArgs &a, hipStream_t st) 
   if (nl <= 25) return launch_nlm<25, MIXED>(dt, a, st);
   if (nl <= 37) return launch_nlm<37, MIXED>(dt, a, st);
   if (nl <= 49) return launch_nlm<49, MIXED>(dt, a, st);
+  if (nl <= 65) return launch_nlm<65, MIXED>(dt, a, st);
   return launch_nlm<MAX_NLAY + 1, MIXED>(dt, a, st);
 }
 
 // Layer-count classes of the batched path: kernel instance NLM = KCLASS_NL[k]
 // serves grids with nlay + 1 <= NLM.
-constexpr int NCLASS = 5;
-constexpr int KCLASS_NL[NCLASS] = {17, 25, 37, 49, MAX_NLAY + 1};
+constexpr int NCLASS = 6;
+constexpr int KCLASS_NL[NCLASS] = {17, 25, 37, 49, 65, MAX_NLAY + 1};
 inline int nl_class(int nl) {
   for (int k = 0; k < NCLASS; k++)
     if (nl <= KCLASS_NL[k]) return k;
@@ -704,6 +705,7 @@ hipError_t launch_batch_class(int cls, int kind, const DevTables &dt, const RunA
     case 1: return launch_batch_nlm<25, MIXED>(kind, dt, grids, work, nwork, nblocks, lds, st);
     case 2: return launch_batch_nlm<37, MIXED>(kind, dt, grids, work, nwork, nblocks, lds, st);
     case 3: return launch_batch_nlm<49, MIXED>(kind, dt, grids, work, nwork, nblocks, lds, st);
+    case 4: return launch_batch_nlm<65, MIXED>(kind, dt, grids, work, nwork, nblocks, lds, st);
     default: return launch_batch_nlm<MAX_NLAY + 1, MIXED>(kind, dt, grids, work, nwork, nblocks, lds, st);
   }
 }

@@ -3,6 +3,8 @@
 #include "../../include/mgcm_rt.h"
 #include "rt_internal.h"
 
+static_assert(mgcm::MAX_NLAY == MGCM_RT_MAX_NLAY, "layer limit of the kernels and of the C ABI");
+
 struct mgcm_rt_ctx {
   int device;
   mgcm::RtConst *d_const;

@@ -12,7 +12,7 @@ namespace mgcm {
 constexpr int NT = 7, NP = 51, NH = 10, NG = 17, NGAS = 16, NV = 7, NI = 5, NB = NV + NI;
 constexpr int L_NREFV = 6;  // reference VIS band, 1-based (code/modules.f90:238)
 constexpr int NPLANCK = 8501;
-constexpr int MAX_NLAY = 64;
+constexpr int MAX_NLAY = 99;  // DSOLVER: 2*NLAYRAD <= NMAX = 201 (code/dsolver.f90:12)
 
 // code/modules.f90:152, :230-232, :333, :364-365, :127-143
 constexpr double CMK = 3.51e22;

@@ -13,7 +13,7 @@ module mgcm_rt_mod
   private
 
   integer, parameter, public :: MGCM_RT_NT = 7, MGCM_RT_NP = 51, MGCM_RT_NH = 10, MGCM_RT_NG = 17
-  integer, parameter, public :: MGCM_RT_NV = 7, MGCM_RT_NI = 5, MGCM_RT_NPLANCK = 8501, MGCM_RT_MAX_NLAY = 64
+  integer, parameter, public :: MGCM_RT_NV = 7, MGCM_RT_NI = 5, MGCM_RT_NPLANCK = 8501, MGCM_RT_MAX_NLAY = 99
   integer(c_int), parameter, public :: MGCM_RT_OK = 0, MGCM_RT_ERR_ARG = -1, MGCM_RT_ERR_NLAY = -2
   integer(c_int), parameter, public :: MGCM_RT_ERR_DEVICE = -3, MGCM_RT_ERR_NOMEM = -4
   integer(c_int), parameter, public :: MGCM_RT_ERR_RANGE = -5, MGCM_RT_ERR_PRECISION = -6, MGCM_RT_ERR_IO = -7

@@ -33,7 +33,7 @@ FP64, MIXED = 0, 1
 SKIP_VIS, SKIP_IR = 0x10, 0x20  # OR-ed into the precision argument: one part only
 DP = C.POINTER(C.c_double)
 
-STATUS = {0: "ok", -1: "invalid argument", -2: "layer count outside 1..64", -3: "HIP runtime error",
+STATUS = {0: "ok", -1: "invalid argument", -2: "layer count outside 1..99", -3: "HIP runtime error",
           -4: "out of device memory", -5: "temperature outside the Planck table", -6: "unknown precision",
           -7: "data file missing, unreadable or not in the expected record layout"}
 

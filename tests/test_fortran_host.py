@@ -60,15 +60,15 @@ def test_fortran_driver_matches_python_binding(tmp_path, nlay, precision):
 def test_fortran_driver_reports_bad_layer_count(tmp_path):
     from mgcm_rt import casefile, synth
     t, _ = synth.make_workload(18, 2, 24)
-    nlev = 2 * 70 + 3  # arrays sized for the header; the library rejects nlay > 64 before reading them
-    c = {"ncol": 2, "nlay": 70, "als": np.zeros(2), "albi": np.zeros(2), "acosz": np.zeros(2)}
+    nlev = 2 * 100 + 3  # arrays sized for the header; the library rejects nlay > 99 before reading them
+    c = {"ncol": 2, "nlay": 100, "als": np.zeros(2), "albi": np.zeros(2), "acosz": np.zeros(2)}
     c.update({k: np.zeros((2, nlev)) for k in ("plev", "tlev", "pmid", "tmid", "qh2o")})
     c.update({k: np.zeros((2, nlev + 1)) for k in ("tauref", "taurefcld")})
     case, res = str(tmp_path / "case.bin"), str(tmp_path / "res.bin")
     casefile.write_case(case, t, c)
     r = subprocess.run([DRIVER, case, res], capture_output=True, text=True, timeout=120)
     assert r.returncode == 1
-    status, _ = casefile.read_result(res, 2, 70)
+    status, _ = casefile.read_result(res, 2, 100)
     assert status == -2
     assert "layer count" in r.stdout
 

@@ -40,7 +40,8 @@ def test_producers_match_reference_fixture(tables):
         assert np.array_equal(o["qextref"][off], fx[f"{name}_qextref"][off])
 
 
-@pytest.mark.parametrize("nlay,ncol,seed", [(24, 2048, 51), (36, 1024, 52), (64, 256, 53), (1, 64, 54)])
+@pytest.mark.parametrize("nlay,ncol,seed", [(24, 2048, 51), (36, 1024, 52), (64, 256, 53), (1, 64, 54), (65, 64, 55),
+                                            (99, 128, 56)])
 def test_producers_match_oracle(tables, oracle, nlay, ncol, seed):
     from mgcm_rt import Radiation, aerosol_table_views, synth
     v = aerosol_table_views(tables)

@@ -13,7 +13,7 @@ pytestmark = pytest.mark.gpu
 
 KEYS = ("fmnetv", "fluxupv", "fluxdnv", "fmneti", "fluxupi", "fluxdni", "nfluxtopv", "nfluxtopi",
         "diffvt", "detau", "cumtauv", "hr")
-SHAPES = [(37, 16), (1, 20), (100, 24), (0, 30), (64, 33), (5, 48), (23, 64), (9, 24)]
+SHAPES = [(37, 16), (1, 20), (100, 24), (0, 30), (64, 33), (5, 48), (23, 64), (9, 24), (7, 99), (12, 65), (3, 81)]
 
 
 def _batch(seed, shapes, weights="dirichlet"):
@@ -84,7 +84,7 @@ def test_batched_rejects_bad_layer_count():
     t, cols = _batch(45, [(4, 24)])
     rt = Radiation(t)
     bad = dict(cols[0])
-    bad["nlay"] = 65
+    bad["nlay"] = 100
     with pytest.raises(RadiationError) as e:
         _run(rt, [bad], [(4, 24)])
     assert e.value.status == -2

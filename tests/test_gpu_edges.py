@@ -38,7 +38,7 @@ def test_reference_edge_fixture():
     assert not np.any(got["detau"][down]) and not np.any(got["fmnetv"][down])
 
 
-@pytest.mark.parametrize("nlay,ncol", [(1, 37), (16, 5), (64, 9), (24, 1), (36, 2)])
+@pytest.mark.parametrize("nlay,ncol", [(1, 37), (16, 5), (64, 9), (24, 1), (36, 2), (65, 6), (80, 3), (99, 11)])
 def test_layer_and_column_extremes(oracle, nlay, ncol):
     from mgcm_rt import Radiation, synth
     t, c = synth.make_workload(60 + nlay, ncol, nlay, clouds=(nlay == 36))
@@ -90,7 +90,7 @@ def test_status_codes():
         with pytest.raises(RadiationError) as e:
             rt.run_host(c, precision=prec)
         assert e.value.status == -6
-    for nl in (0, 65):
+    for nl in (0, 100):
         bad = dict(c)
         bad["nlay"] = nl
         with pytest.raises(RadiationError) as e:

@@ -11,7 +11,7 @@ pytestmark = pytest.mark.gpu
 PTROP = 2.0e-4
 
 
-@pytest.mark.parametrize("nlay,ncol,seed", [(24, 3000, 81), (36, 777, 82), (1, 65, 83)])
+@pytest.mark.parametrize("nlay,ncol,seed", [(24, 3000, 81), (36, 777, 82), (1, 65, 83), (99, 70, 84)])
 def test_history_fields_match_oracle(oracle, nlay, ncol, seed):
     import torch
     from mgcm_rt import HISTORY_KEYS, Radiation, alloc_outputs, synth, to_device
