@@ -1,6 +1,6 @@
-// rt_mixed.hip -- the mixed-precision path: FP32 k-interpolation, optical
-// depths and (below the top FP64 layers) layer constants and transmissions;
-// FP64 elimination, substitution, flux sums and heating.
+// rt_mixed.hip -- the mixed-precision path: the k-tables are read in FP32 (half
+// the gather bytes and corner registers); every operation, the interpolation
+// included, is FP64, as on the plain path.
 #include "rt_column.h"
 
 namespace mgcm {
