@@ -98,3 +98,37 @@ def test_status_codes():
         assert e.value.status == -2
     assert rt.lib.mgcm_rt_run(rt.ctx, None, None, 0, None) == -1
     assert b"Planck" in rt.lib.mgcm_rt_strerror(-5)
+
+
+def test_two_million_column_grid_is_a_tiled_copy(oracle):
+    """2,097,152 columns (32 copies of a 65,536-column grid) in one call: every
+    copy's outputs equal the first copy's bit for bit (no index wraps or
+    overlaps at this size), and sampled columns of the first copy equal the
+    oracle.  About 23 GB of device memory."""
+    import torch
+    from mgcm_rt import Radiation, alloc_outputs, synth, to_device
+    base_n, reps, nlay = 65536, 32, 24
+    t, c = synth.make_workload(77, base_n, nlay)
+    c["acosz"][::9] = 0.0
+    rt = Radiation(t)
+    db = to_device(c)
+    big = {k: (v.repeat((reps,) + (1,) * (v.dim() - 1)) if isinstance(v, torch.Tensor) else v) for k, v in db.items()}
+    big["ncol"] = base_n * reps
+    del db
+    proto = alloc_outputs(1, nlay)
+    # -1 marks unwritten outputs; DETAU/CUMTAUV of sun-down columns stay as given (0, the oracle's)
+    out = {k: torch.full((base_n * reps,) + v.shape[1:], 0.0 if k in ("detau", "cumtauv") else -1.0,
+                         dtype=torch.float64, device="cuda") for k, v in proto.items()}
+    rt.run_device(big, out)
+    torch.cuda.synchronize()
+    rt.check()
+    for k, v in out.items():
+        tiles = v.view((reps, base_n) + v.shape[1:])
+        assert bool(torch.all(tiles == tiles[:1])), k
+    idx = np.r_[0:8, base_n - 8:base_n, np.random.default_rng(3).choice(base_n, 48, replace=False)]
+    sub = {k: (v[idx] if isinstance(v, np.ndarray) and v.shape[:1] == (base_n,) else v) for k, v in c.items()}
+    sub["ncol"] = len(idx)
+    want = oracle.run(t, sub)
+    for k in KEYS:
+        got = out[k][(reps - 1) * base_n:][idx].cpu().numpy()  # the last copy
+        assert rel_err(got, want[k]) <= 1e-10, k
