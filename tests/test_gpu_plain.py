@@ -85,3 +85,45 @@ def test_run_captured_in_a_hip_graph():
         torch.cuda.synchronize()
         for key in want:
             assert np.array_equal(do[key].cpu().numpy(), want[key]), (k, key)
+
+
+def test_two_contexts_on_two_streams_and_two_host_threads():
+    """Contexts share no mutable state: two of them (different tables and layer
+    counts) interleaved on two streams, and driven from two host threads at once
+    (ctypes releases the GIL), give each context's serial results bit for bit."""
+    import threading
+
+    import torch
+    from mgcm_rt import FP64, Radiation, alloc_outputs, synth, to_device
+    ta, ca = synth.make_workload(91, 3000, 24)
+    tb, cb = synth.make_workload(92, 2000, 36, clouds=True)
+    ra, rb = Radiation(ta), Radiation(tb)
+    want = {"a": ra.run_host(ca), "b": rb.run_host(cb)}
+    da, db = to_device(ca), to_device(cb)
+    oa, ob = to_device(alloc_outputs(3000, 24)), to_device(alloc_outputs(2000, 36))
+    torch.cuda.synchronize()
+    sa, sb = torch.cuda.Stream(), torch.cuda.Stream()
+    for _ in range(3):
+        ra.run_device(da, oa, FP64, sa.cuda_stream)
+        rb.run_device(db, ob, FP64, sb.cuda_stream)
+    torch.cuda.synchronize()
+    for name, o in (("a", oa), ("b", ob)):
+        for k in KEYS:
+            assert np.array_equal(o[k].cpu().numpy(), want[name][k]), (name, k)
+    got, errors = {}, []
+
+    def work(name, rt, c):
+        try:
+            got[name] = [rt.run_host(c) for _ in range(3)]
+        except Exception as e:  # noqa: BLE001 -- reported below
+            errors.append(e)
+    threads = [threading.Thread(target=work, args=("a", ra, ca)), threading.Thread(target=work, args=("b", rb, cb))]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    assert not errors, errors
+    for name in ("a", "b"):
+        for o in got[name]:
+            for k in KEYS:
+                assert np.array_equal(o[k], want[name][k]), (name, k)
