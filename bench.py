@@ -41,6 +41,7 @@ def parse():
                    help="columns of the same grid the CPU baseline runs (the whole grid, about 14 s on one core)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-forms", action="store_true", help="only the FP64 headline path")
+    p.add_argument("--forms", default="", help="comma-separated subset of the forms (default: all)")
     p.add_argument("--sg-train", type=int, default=2000000, help="surrogate training columns")
     p.add_argument("--sg-epochs", type=int, default=10)
     p.add_argument("--copy-calib", action="store_true",
@@ -224,6 +225,141 @@ def form_batched(args, rt, stream):
     return out
 
 
+def _graph_ms(fn, stream, reps=20, replays=5):
+    """GPU time of one fn() call: reps calls captured in a HIP graph, replayed;
+    a kernel of a few microseconds is not hidden behind Python launch overhead."""
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=stream):
+        for _ in range(reps):
+            fn()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    with torch.cuda.stream(stream):  # replay() launches on the current stream
+        g.replay()
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(replays):
+            g.replay()
+        e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / (replays * reps)
+
+
+def form_next_rows(args, rt_unused, stream_unused):
+    """The kernels next to the radiation call, each alone on the headline grid
+    (64,800 columns x 24 layers): DSOLFLUX (code/dsolflux.f), OPT_DST / OPT_CLD
+    (code/opt_dst.f, code/opt_cld.f), the Conrath profile and FILLTAUCUM
+    (code/dustprofile.f, code/filltaucum.f), COMP3's history diagnostics
+    (code/comp3.f:654,703-747,765-771), and the start-up LAGINTERP and Planck
+    table.  GPU times from HIP graphs of 20 calls; bytes = the inputs and
+    outputs each call must move, against 8 TB/s."""
+    import ctypes as C
+    import tempfile
+
+    import torch
+    from mgcm_rt import (AER_OUT_KEYS, DP, FP64, HISTORY_KEYS, AerosolOut, Radiation, Tracers, alloc_outputs,
+                         dust, laginterp, planck_table, read_aerosol_tables, realdata, synth, to_device)
+    n, nlay = args.ncol, args.nlay
+    nlev, nl = 2 * nlay + 3, nlay + 1
+    t, c = synth.make_workload(args.seed, n, nlay)
+    rt = Radiation(t)
+    dc, do = to_device(c), to_device(alloc_outputs(n, nlay))
+    torch.cuda.synchronize()
+    stream = torch.cuda.Stream()  # graphs are captured on a stream of their own, not the legacy default one
+    sh = stream.cuda_stream
+    rt.run_device(dc, do, FP64, sh)
+    torch.cuda.synchronize()
+    out = {}
+
+    def entry(name, ms, nbytes, what):
+        gbs = nbytes / (ms / 1e3) / 1e9
+        out[name] = {"what": what, "ms": ms, "columns_per_s": n / (ms / 1e3), "bytes": nbytes, "GBps": gbs,
+                     "frac_of_hbm_peak": gbs / HBM_PEAK_GBS}
+
+    # DSOLFLUX + DNVFLUX
+    ds = torch.empty(n, dtype=torch.float64, device="cuda")
+    dn = torch.empty_like(ds)
+    sol = np.ascontiguousarray(rt.sol, dtype=np.float64)
+    solp = sol.ctypes.data_as(DP)
+    args_ds = (rt.ctx, n, solp, dc["acosz"].data_ptr(), do["detau"].data_ptr(), do["diffvt"].data_ptr(),
+               ds.data_ptr(), dn.data_ptr(), C.c_void_p(sh))
+    assert rt.lib.mgcm_rt_dsolflux(*args_ds) == 0
+    entry("dsolflux", _graph_ms(lambda: rt.lib.mgcm_rt_dsolflux(*args_ds), stream),
+          n * 8 * (17 * 7 + 2 + 2), "k_dsolflux: DETAU[17][7], ACOSZ, DIFFVT in; DIRECTSOL, DNVFLUX out")
+
+    # OPT_DST / OPT_CLD on the reference's optical tables (carried as data by the fixture)
+    fx = np.load(os.path.join(ROOT, "tests", "golden", "aerosol_ref.npz"))
+    with tempfile.TemporaryDirectory() as d:
+        for k in fx.files:
+            if k.startswith("file_"):
+                with open(os.path.join(d, k[5:]), "wb") as f:
+                    f.write(fx[k].tobytes())
+        rt.set_aerosol_tables(read_aerosol_tables(d))
+    qtr = torch.from_numpy(synth.make_tracers(args.seed, n, nlay)).cuda()
+    ld = nlev + 1
+    shapes = {"qxv": (n, 7, ld), "qsv": (n, 7, ld), "gv": (n, 7, ld), "qxi": (n, 5, ld), "qsi": (n, 5, ld),
+              "gi": (n, 5, ld), "qextref": (n, ld), "tauref": (n, ld), "tau2": (n, 2)}
+    ao_t = {k: torch.empty(v, dtype=torch.float64, device="cuda") for k, v in shapes.items()}
+    tr = Tracers(n, nlay, dc["plev"].data_ptr(), qtr.data_ptr())
+    ao = AerosolOut(*[ao_t[k].data_ptr() for k in AER_OUT_KEYS])
+    aer_bytes = n * 8 * (6 * nlay + nlev + (12 * 3 + 2) * ld + 2)
+    for name, fn in (("opt_dst", rt.lib.mgcm_rt_opt_dst), ("opt_cld", rt.lib.mgcm_rt_opt_cld)):
+        call = (rt.ctx, C.byref(tr), C.byref(ao), C.c_void_p(sh))
+        assert fn(*call) == 0
+        entry(name, _graph_ms(lambda fn=fn, call=call: fn(*call), stream), aer_bytes,
+              f"k_opt_aer: QTRACE[6][{nlay}], PL in; Qext/Qscat/g per band and level, QEXTREF, TAUREF out")
+
+    # Conrath dust profile + FILLTAUCUM
+    rng = np.random.default_rng(args.seed)
+    prdst = torch.from_numpy(dust.pressure_grid(7.01, 0.0008)).cuda()
+    dd = {"tautot": torch.from_numpy(rng.uniform(0.1, 1.0, n)).cuda(),
+          "conr": torch.full((n,), 0.03, dtype=torch.float64, device="cuda"),
+          "p": torch.from_numpy(c["plev"][:, -1] - 0.0008).cuda(),
+          "sigma": torch.from_numpy(synth.sigma_levels(nlay)).cuda()}
+    trd = torch.empty((n, ld), dtype=torch.float64, device="cuda")
+    tdd = torch.empty((n, prdst.numel()), dtype=torch.float64, device="cuda")
+    lib = rt.lib
+    lib.mgcm_rt_dust_tauref.argtypes = [C.POINTER(dust.DustColumns), C.c_void_p, C.c_void_p, C.c_void_p]
+    dcol = dust.DustColumns(n, nlay, *[dd[k].data_ptr() for k in ("tautot", "conr", "p", "sigma")],
+                            prdst.data_ptr(), prdst.numel(), 6.1, 0.0008)
+    call = (C.byref(dcol), trd.data_ptr(), tdd.data_ptr(), C.c_void_p(sh))
+    assert lib.mgcm_rt_dust_tauref(*call) == 0
+    entry("dust_tauref", _graph_ms(lambda: lib.mgcm_rt_dust_tauref(*call), stream),
+          n * 8 * (3 + ld + prdst.numel()), "k_dust_tauref: TAUTOT, CONR, P in; TAUREF, TAUDST out")
+
+    # history diagnostics with the non-LTE heating
+    from mgcm_rt import History, column_struct, flux_struct
+    hist = {k: torch.zeros(n, dtype=torch.float32, device="cuda") for k in HISTORY_KEYS}
+    hr = torch.empty((n, nl), dtype=torch.float64, device="cuda")
+    hs = History(*[hist[k].data_ptr() for k in HISTORY_KEYS], hr.data_ptr(), dd["p"].data_ptr(),
+                 dd["sigma"].data_ptr(), 0.0008)
+    cs = column_struct(dc, ptr=lambda x: x.data_ptr(), sol=rt.sol)
+    fs = flux_struct(do, ptr=lambda x: x.data_ptr())
+    call = (C.byref(cs), C.byref(fs), C.byref(hs), C.c_void_p(sh))
+    assert lib.mgcm_rt_history_fields(*call) == 0
+    entry("history_fields", _graph_ms(lambda: lib.mgcm_rt_history_fields(*call), stream),
+          n * (8 * (2 * nl + 2 * nl + 2 + 1 + 1 + 6 + nl) + 4 * 8),
+          "k_history: FMNETV/I, PLEV, top/surface fluxes, ACOSZ, ALS, P in; 8 REAL*4 fields and the "
+          "non-LTE heating out")
+
+    # start-up tables (host + device, once per run)
+    f = np.load(os.path.join(ROOT, "tests", "golden", "column_real24.npz"))
+    t0 = time.perf_counter()
+    laginterp(realdata.PGREF11, f["raw_v"])
+    laginterp(realdata.PGREF11, f["raw_i"])
+    t1 = time.perf_counter()
+    planck_table()
+    t2 = time.perf_counter()
+    out["startup"] = {"what": "LAGINTERP of both k-tables (host log10/pow, device interpolation) and SETSPI's "
+                              "Planck table, wall time of each library call", "laginterp_ms": (t1 - t0) * 1e3,
+                      "planck_table_ms": (t2 - t1) * 1e3}
+    out["workload"] = f"{n} columns x {nlay} layers, seed {args.seed}"
+    del dc, do, ao_t, qtr, trd, tdd, hist, hr
+    return out
+
+
 def form_surrogate(args, rt_unused, stream):
     """Learned IR surrogate with exact fallback on 64,800 fresh columns x 36 layers.
 
@@ -373,8 +509,10 @@ def main():
     forms = {}
     if not args.no_forms:
         del do
-        for name, fn in (("real_tables", form_real), ("mixed", form_mixed), ("batched", form_batched),
-                         ("surrogate", form_surrogate)):
+        for name, fn in (("real_tables", form_real), ("next_rows", form_next_rows), ("mixed", form_mixed),
+                         ("batched", form_batched), ("surrogate", form_surrogate)):
+            if args.forms and name not in args.forms.split(","):
+                continue
             try:
                 forms[name] = fn(args, rt, stream)
             except Exception as e:  # noqa: BLE001

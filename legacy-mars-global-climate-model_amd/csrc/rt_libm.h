@@ -191,6 +191,9 @@ MGCM_HD double log_dd_round(double m, int k) {
   const DD s = dd_div({m - 1.0, 0.0}, dd_two_sum(m, 1.0));
   const DD t = dd_mul(s, s);
   DD p = {0.0, 0.0};
+  // unrolled: the coefficients 1/(2j+1) fold to double-double constants at
+  // compile time (the same IEEE operations, evaluated once)
+#pragma unroll
   for (int j = 21; j >= 1; j--) p = dd_add(dd_mul(p, t), dd_div({1.0, 0.0}, {2.0 * j + 1.0, 0.0}));
   p = dd_add(dd_mul(p, t), {1.0, 0.0});
   DD L = dd_mul(s, p);
