@@ -7,8 +7,8 @@
 //     layer, the log-normal size distribution of the dust (or ice-coated dust)
 //     tracers, its cross-section weights over the 20 bins (erf), and the
 //     weighted Qext/Qscat/g per band, the reference extinction and TAUREF (on the
-//     device, one block per column (one wave up to 64 layers, two above),
-//     thread l-1 = layer l).
+//     device, whole columns packed into 256-thread blocks, one thread per
+//     layer).
 // The outputs are the per-level arrays of mgcm_rt_aerosol plus TAUREF/TAUREFCLD.
 #include <cmath>
 #include <cstdio>
@@ -40,41 +40,47 @@ struct AerOut {
   double *qxv, *qsv, *gv, *qxi, *qsi, *gi, *qextref, *tauref, *tau2;
 };
 
-// OPT_DST (CLOUD = false) / OPT_CLD (CLOUD = true) for one column per block.
+// OPT_DST (CLOUD = false) / OPT_CLD (CLOUD = true): a block of AER_BLOCK
+// threads takes cpb = AER_BLOCK / nlay whole columns, thread cc*nlay + l-1 on
+// layer l of its column cc (24 layers: 10 columns, 240 of 256 threads busy).
 // qtr: [ncol][6][nlay]; pl: [ncol][nlev].
+constexpr int AER_BLOCK = 256;
 template <bool CLOUD>
-__global__ __launch_bounds__(128) void k_opt_aer(const double *__restrict__ tab, int ncol, int nlay,
-                                                const double *__restrict__ qtr, const double *__restrict__ pl_all,
-                                                AerOut o) {
-  __shared__ double s_tau[128][4];
-  __shared__ int s_on[128];
-  const int col = blockIdx.x, lane = threadIdx.x;
-  if (col >= ncol) return;
+__global__ __launch_bounds__(AER_BLOCK) void k_opt_aer(const double *__restrict__ tab, int ncol, int nlay,
+                                                      const double *__restrict__ qtr,
+                                                      const double *__restrict__ pl_all, AerOut o) {
+  __shared__ double s_tau[AER_BLOCK][4];
+  __shared__ int s_on[AER_BLOCK];
+  const int cpb = AER_BLOCK / nlay, tid = threadIdx.x;
+  const int cc = tid / nlay, col = blockIdx.x * cpb + cc;
+  const bool active = cc < cpb && col < ncol;  // no early return: the block synchronises below
+  const int lane = active ? tid - cc * nlay : nlay;  // layer - 1; nlay for idle threads
   const int nlevels = 2 * nlay + 3, ld = nlevels + 1;
   const double athird = (double)(1.0f / 3.0f);        // code/initcld.f:57 (REAL(4) 1./3.)
   const double dpden_dt = 2.5e3, dpden_ice = 917.;    // code/initdt.f:38, code/initcld.f:61
   const double dev = CLOUD ? (double)0.3087f : (double)0.63676f;  // code/initcld.f:93-95
   const double dev2 = 1. / ((double)sqrtf(2.f) * dev);
   const double qs_init = (double)0.99f;
-  const double *q = qtr + (size_t)col * NTRACE * nlay;
-  const double *pl = pl_all + (size_t)col * nlevels;
-  double *qxv = o.qxv + (size_t)col * NV * ld, *qsv = o.qsv + (size_t)col * NV * ld;
-  double *gv = o.gv + (size_t)col * NV * ld, *qxi = o.qxi + (size_t)col * NI * ld;
-  double *qsi = o.qsi + (size_t)col * NI * ld, *gi = o.gi + (size_t)col * NI * ld;
-  double *qextref = o.qextref + (size_t)col * ld, *tauref = o.tauref + (size_t)col * ld;
+  const size_t cz = active ? (size_t)col : 0;
+  const double *q = qtr + cz * NTRACE * nlay;
+  const double *pl = pl_all + cz * nlevels;
+  double *qxv = o.qxv + cz * NV * ld, *qsv = o.qsv + cz * NV * ld;
+  double *gv = o.gv + cz * NV * ld, *qxi = o.qxi + cz * NI * ld;
+  double *qsi = o.qsi + cz * NI * ld, *gi = o.gi + cz * NI * ld;
+  double *qextref = o.qextref + cz * ld, *tauref = o.tauref + cz * ld;
   auto defaults = [&](int k) {  // code/opt_dst.f:52-69: levels without aerosol
     qextref[k - 1] = 1.;
     tauref[k - 1] = 0.;
     for (int w = 0; w < NV; w++) { qxv[w * ld + k - 1] = 1.; qsv[w * ld + k - 1] = qs_init; gv[w * ld + k - 1] = 0.; }
     for (int w = 0; w < NI; w++) { qxi[w * ld + k - 1] = 1.; qsi[w * ld + k - 1] = qs_init; gi[w * ld + k - 1] = 0.; }
   };
-  if (lane == 0) {
+  if (active && lane == 0) {
     for (int k = 1; k <= 3; k++) defaults(k);
     defaults(ld);
   }
   const int l = lane + 1;
   bool on = false;
-  if (l <= nlay) {
+  if (active) {
     auto QT = [&](int m) { return q[(m - 1) * nlay + l - 1]; };
     double mo, no, cst;
     int irap = 0;
@@ -162,16 +168,16 @@ __global__ __launch_bounds__(128) void k_opt_aer(const double *__restrict__ tab,
         const double mass = 100. * (pl[k - 1] - pl[k - 2]) / GRAV;
         const double t = ao * qx6 * mass;
         tauref[k - 1] = t;
-        s_tau[lane][2 * (nn - 1)] = t;
-        s_tau[lane][2 * (nn - 1) + 1] = t / qx6 * (qxi4 - qsi4);
+        s_tau[tid][2 * (nn - 1)] = t;
+        s_tau[tid][2 * (nn - 1) + 1] = t / qx6 * (qxi4 - qsi4);
       }
     }
   }
-  s_on[lane] = on ? 1 : 0;
+  s_on[tid] = on ? 1 : 0;
   __syncthreads();
-  if (lane == 0 && o.tau2) {  // taudust/taucloud: REAL*4 sums in layer order
+  if (active && lane == 0 && o.tau2) {  // taudust/taucloud: REAL*4 sums in layer order
     float t1 = 0.f, t2 = 0.f;
-    for (int j = 0; j < nlay; j++) {
+    for (int j = tid; j < tid + nlay; j++) {
       if (!s_on[j]) continue;
       for (int nn = 0; nn < 2; nn++) {
         t1 = (float)((double)t1 + s_tau[j][2 * nn]);
@@ -283,12 +289,13 @@ static int opt_aer(bool cloud, mgcm_rt_ctx *c, const mgcm_rt_tracers *in, const 
     if (!p) return MGCM_RT_ERR_ARG;
   AerOut o{out->qxv, out->qsv, out->gv, out->qxi, out->qsi, out->gi, out->qextref, out->tauref, out->tau2};
   hipStream_t st = (hipStream_t)stream;
-  const dim3 blk(in->nlay <= 64 ? 64 : 128);  // thread l-1 = layer l
+  const int cpb = AER_BLOCK / in->nlay;  // whole columns per block (nlay <= 99 < AER_BLOCK)
+  const dim3 grid((unsigned)((in->ncol + cpb - 1) / cpb));
   if (cloud)
-    hipLaunchKernelGGL(k_opt_aer<true>, dim3(in->ncol), blk, 0, st, c->d_aer, in->ncol, in->nlay,
+    hipLaunchKernelGGL(k_opt_aer<true>, grid, dim3(AER_BLOCK), 0, st, c->d_aer, in->ncol, in->nlay,
                        in->qtrace, in->pl, o);
   else
-    hipLaunchKernelGGL(k_opt_aer<false>, dim3(in->ncol), blk, 0, st, c->d_aer, in->ncol, in->nlay,
+    hipLaunchKernelGGL(k_opt_aer<false>, grid, dim3(AER_BLOCK), 0, st, c->d_aer, in->ncol, in->nlay,
                        in->qtrace, in->pl, o);
   return hipGetLastError() == hipSuccess ? MGCM_RT_OK : MGCM_RT_ERR_DEVICE;
 }
