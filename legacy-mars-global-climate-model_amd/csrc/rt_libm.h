@@ -16,7 +16,8 @@
 //    bit for bit (checked against libm on the host: 2.85M arguments, 0 differences).
 //  * mgcm_log10: glibc's fdlibm wrapper (sysdeps/ieee754/dbl-64/e_log10.c):
 //    x = 2^e m, log10(x) = e*log10_2lo + ivln10*log(m) + e*log10_2hi, over a
-//    correctly rounded log (double-double atanh series).  It equals glibc's
+//    correctly rounded log (a 129-point table and a double-double atanh series,
+//    rt_log_table.inc generated by gen_log_table.py).  It equals glibc's
 //    log10 except where glibc's own log misrounds (about 3 in 10^4 arguments).
 #pragma once
 #include <stdint.h>
@@ -54,6 +55,27 @@ static const uint64_t kExpTabHost[256] = {
 #include "rt_exp_table.inc"
 };
 MGCM_HD uint64_t exp_tab(int i) { return kExpTabHost[i]; }
+#endif
+
+#if defined(__HIPCC__)
+__device__ __constant__ static const uint64_t kLogTab[258] = {
+#include "rt_log_table.inc"
+};
+static const uint64_t kLogTabHost[258] = {
+#include "rt_log_table.inc"
+};
+MGCM_HD uint64_t log_tab(int i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return kLogTab[i];
+#else
+  return kLogTabHost[i];
+#endif
+}
+#else
+static const uint64_t kLogTabHost[258] = {
+#include "rt_log_table.inc"
+};
+MGCM_HD uint64_t log_tab(int i) { return kLogTabHost[i]; }
 #endif
 
 MGCM_HD double exp_special(double tmp, uint64_t sbits, uint64_t ki) {
@@ -185,30 +207,45 @@ MGCM_HD DD dd_div(DD a, DD b) {
   return dd_fast(q1, r.h / b.h);
 }
 
-// k*ln2 + log(m) for m in [sqrt(1/2), sqrt(2)), rounded once from double-double
-// (atanh series, 2^-100 relative): correctly rounded but for ties closer than that.
+// k*ln2 + log(m) for m in [0.70703125, 1.4140625), rounded once from
+// double-double (2^-100 relative): correctly rounded but for ties closer than
+// that.  m = c * (1+s)/(1-s) with c the table point below m (above it under 1,
+// so c = 1 on both sides of 1): log(m) = log(c) + 2 atanh(s), |s| < 1/256, the
+// atanh series to s^13; log(c) is a double-double table (rt_log_table.inc).
 MGCM_HD double log_dd_round(double m, int k) {
-  const DD s = dd_div({m - 1.0, 0.0}, dd_two_sum(m, 1.0));
+  int idx;
+  double c;
+  if (m >= 1.0) {
+    const int j = (int)((m - 1.0) * 128.0);
+    idx = j;
+    c = 1.0 + j * 0x1p-7;
+  } else {
+    const int j = (int)((1.0 - m) * 256.0);
+    idx = 53 + j;
+    c = 1.0 - j * 0x1p-8;
+  }
+  const DD s = dd_div({m - c, 0.0}, dd_two_sum(m, c));  // m - c is exact
   const DD t = dd_mul(s, s);
   DD p = {0.0, 0.0};
   // unrolled: the coefficients 1/(2j+1) fold to double-double constants at
   // compile time (the same IEEE operations, evaluated once)
 #pragma unroll
-  for (int j = 21; j >= 1; j--) p = dd_add(dd_mul(p, t), dd_div({1.0, 0.0}, {2.0 * j + 1.0, 0.0}));
+  for (int j = 6; j >= 1; j--) p = dd_add(dd_mul(p, t), dd_div({1.0, 0.0}, {2.0 * j + 1.0, 0.0}));
   p = dd_add(dd_mul(p, t), {1.0, 0.0});
   DD L = dd_mul(s, p);
   L.h *= 2.0;
   L.l *= 2.0;
+  const DD logc = {asd(log_tab(2 * idx)), asd(log_tab(2 * idx + 1))};
   const DD ln2 = {0x1.62e42fefa39efp-1, 0x1.abc9e3b39803fp-56};
-  const DD r = dd_add(dd_mul({(double)k, 0.0}, ln2), L);
+  const DD r = dd_add(dd_add(dd_mul({(double)k, 0.0}, ln2), logc), L);
   return r.h + r.l;
 }
 
 // Correctly rounded natural log of a positive normal double in [0.5, 2).
 MGCM_HD double cr_log_unit(double m) {
   int k = 0;
-  if (m < 0.70710678118654752) { m *= 2.0; k = -1; }
-  else if (m >= 1.4142135623730951) { m *= 0.5; k = 1; }
+  if (m < 0.70703125) { m *= 2.0; k = -1; }
+  else if (m >= 1.4140625) { m *= 0.5; k = 1; }
   return log_dd_round(m, k);
 }
 
@@ -217,7 +254,7 @@ MGCM_HD double mgcm_log(double x) {
   const uint64_t u = asu(x);
   int k = (int)((u >> 52) & 0x7ff) - 1023;
   double m = asd((u & 0x000fffffffffffffull) | 0x3ff0000000000000ull);  // [1, 2)
-  if (m >= 1.4142135623730951) { m *= 0.5; k += 1; }
+  if (m >= 1.4140625) { m *= 0.5; k += 1; }
   return log_dd_round(m, k);
 }
 
