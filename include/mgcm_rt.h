@@ -227,6 +227,10 @@ typedef struct {
  * layer, asynchronous on stream.  Needs mgcm_rt_set_aerosol_tables first. */
 int mgcm_rt_opt_dst(mgcm_rt_ctx *ctx, const mgcm_rt_tracers *in, const mgcm_rt_aerosol_out *out, void *stream);
 int mgcm_rt_opt_cld(mgcm_rt_ctx *ctx, const mgcm_rt_tracers *in, const mgcm_rt_aerosol_out *out, void *stream);
+/* The same with host pointers (copies in, runs, copies out, synchronises), for a
+ * host that keeps QTRACE and PL in host memory, as COMP3 does. */
+int mgcm_rt_opt_dst_host(mgcm_rt_ctx *ctx, const mgcm_rt_tracers *in, const mgcm_rt_aerosol_out *out);
+int mgcm_rt_opt_cld_host(mgcm_rt_ctx *ctx, const mgcm_rt_tracers *in, const mgcm_rt_aerosol_out *out);
 
 /* Prescribed (Conrath) dust, the TAUREF of runs without active dust.
  * mgcm_rt_dust_grid: DUSTPROFILE's pressure grid PRDST(npdst) from PSF and PTROP
@@ -249,6 +253,8 @@ typedef struct {
 } mgcm_rt_dust_columns;
 int mgcm_rt_dust_grid(double psf, double ptrop, int npdst, double *prdst);
 int mgcm_rt_dust_tauref(const mgcm_rt_dust_columns *in, double *tauref, double *taudst, void *stream);
+/* The same with host pointers (copies in, runs, copies out, synchronises). */
+int mgcm_rt_dust_tauref_host(const mgcm_rt_dust_columns *in, double *tauref, double *taudst);
 
 /* DSOLFLUX (code/dsolflux.f:18-45) and COMP3's surface VIS flux for TEMPGR
  * (code/comp3.f:336-348) from the DETAU and DIFFVT outputs of mgcm_rt_run:

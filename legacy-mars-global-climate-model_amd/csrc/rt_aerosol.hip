@@ -305,6 +305,66 @@ extern "C" int mgcm_rt_opt_dst(mgcm_rt_ctx *c, const mgcm_rt_tracers *in, const 
   return opt_aer(false, c, in, out, stream);
 }
 
+// Host-pointer variants: stage the arrays in the context's grow-only device
+// block (the one mgcm_rt_run_host uses), run on the null stream, copy back,
+// synchronise.
+static double *staging(mgcm_rt_ctx *c, size_t bytes) {
+  if (bytes > c->host_bytes) {
+    if (c->d_host) (void)hipFree(c->d_host);
+    c->d_host = nullptr;
+    c->host_bytes = 0;
+    if (hipMalloc(&c->d_host, bytes) != hipSuccess) return nullptr;
+    c->host_bytes = bytes;
+  }
+  return c->d_host;
+}
+
+static int opt_aer_host(bool cloud, mgcm_rt_ctx *c, const mgcm_rt_tracers *in, const mgcm_rt_aerosol_out *out) {
+  if (!c || !in || !out) return MGCM_RT_ERR_ARG;
+  if (in->ncol < 0) return MGCM_RT_ERR_ARG;
+  if (in->nlay < 1 || in->nlay > MAX_NLAY) return MGCM_RT_ERR_NLAY;
+  if (in->ncol == 0) return MGCM_RT_OK;
+  const size_t ncol = in->ncol, nlev = 2 * (size_t)in->nlay + 3, ld = nlev + 1;
+  const size_t nq = ncol * NTRACE * in->nlay, npl = ncol * nlev;
+  // outputs in mgcm_rt_aerosol_out order: 3 x VIS, 3 x IR, qextref, tauref, tau2
+  double *hout[9] = {out->qxv, out->qsv, out->gv, out->qxi, out->qsi, out->gi, out->qextref, out->tauref,
+                     out->tau2};
+  const size_t nout[9] = {ncol * NV * ld, ncol * NV * ld, ncol * NV * ld, ncol * NI * ld, ncol * NI * ld,
+                          ncol * NI * ld, ncol * ld, ncol * ld, ncol * 2};
+  if (!in->pl || !in->qtrace) return MGCM_RT_ERR_ARG;
+  for (int i = 0; i < 8; i++)
+    if (!hout[i]) return MGCM_RT_ERR_ARG;
+  size_t total = nq + npl;
+  for (int i = 0; i < 9; i++) total += hout[i] ? nout[i] : 0;
+  double *p = staging(c, total * sizeof(double));
+  if (!p) return MGCM_RT_ERR_NOMEM;
+  mgcm_rt_tracers din = *in;
+  din.qtrace = p;
+  if (hipMemcpy(p, in->qtrace, nq * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return MGCM_RT_ERR_DEVICE;
+  p += nq;
+  din.pl = p;
+  if (hipMemcpy(p, in->pl, npl * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return MGCM_RT_ERR_DEVICE;
+  p += npl;
+  double *dout[9] = {nullptr};
+  for (int i = 0; i < 9; i++)
+    if (hout[i]) { dout[i] = p; p += nout[i]; }
+  const mgcm_rt_aerosol_out o{dout[0], dout[1], dout[2], dout[3], dout[4], dout[5], dout[6], dout[7], dout[8]};
+  const int s = opt_aer(cloud, c, &din, &o, nullptr);
+  if (s != MGCM_RT_OK) return s;
+  if (hipStreamSynchronize(nullptr) != hipSuccess) return MGCM_RT_ERR_DEVICE;
+  for (int i = 0; i < 9; i++)
+    if (hout[i] && hipMemcpy(hout[i], dout[i], nout[i] * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+      return MGCM_RT_ERR_DEVICE;
+  return MGCM_RT_OK;
+}
+
+extern "C" int mgcm_rt_opt_dst_host(mgcm_rt_ctx *c, const mgcm_rt_tracers *in, const mgcm_rt_aerosol_out *out) {
+  return opt_aer_host(false, c, in, out);
+}
+extern "C" int mgcm_rt_opt_cld_host(mgcm_rt_ctx *c, const mgcm_rt_tracers *in, const mgcm_rt_aerosol_out *out) {
+  return opt_aer_host(true, c, in, out);
+}
+
 extern "C" int mgcm_rt_opt_cld(mgcm_rt_ctx *c, const mgcm_rt_tracers *in, const mgcm_rt_aerosol_out *out,
                                void *stream) {
   return opt_aer(true, c, in, out, stream);
@@ -403,4 +463,42 @@ extern "C" int mgcm_rt_dust_tauref(const mgcm_rt_dust_columns *in, double *taure
   hipLaunchKernelGGL(k_dust_tauref, dim3((in->ncol + 63) / 64), dim3(64), 0, (hipStream_t)stream, *in, tauref,
                      taudst);
   return hipGetLastError() == hipSuccess ? MGCM_RT_OK : MGCM_RT_ERR_DEVICE;
+}
+
+// Host-pointer variant (no context: a temporary device block per call).
+extern "C" int mgcm_rt_dust_tauref_host(const mgcm_rt_dust_columns *in, double *tauref, double *taudst) {
+  if (!in || !tauref) return MGCM_RT_ERR_ARG;
+  if (in->ncol < 0 || in->npdst < 2 || in->npdst > NPDST_MAX) return MGCM_RT_ERR_ARG;
+  if (in->nlay < 1 || in->nlay > MAX_NLAY) return MGCM_RT_ERR_NLAY;
+  if (in->ncol == 0) return MGCM_RT_OK;
+  if (!in->tautot || !in->conr || !in->p || !in->sigma || !in->prdst) return MGCM_RT_ERR_ARG;
+  const size_t ncol = in->ncol, nlev = 2 * (size_t)in->nlay + 3, np = in->npdst;
+  const size_t total = 3 * ncol + nlev + np + ncol * (nlev + 1) + (taudst ? ncol * np : 0);
+  double *d = nullptr;
+  if (hipMalloc(&d, total * sizeof(double)) != hipSuccess) return MGCM_RT_ERR_NOMEM;
+  mgcm_rt_dust_columns din = *in;
+  double *p = d;
+  bool ok = true;
+  auto up = [&](const double *src, size_t n) {
+    double *q = p;
+    ok = ok && hipMemcpy(q, src, n * sizeof(double), hipMemcpyHostToDevice) == hipSuccess;
+    p += n;
+    return q;
+  };
+  din.tautot = up(in->tautot, ncol);
+  din.conr = up(in->conr, ncol);
+  din.p = up(in->p, ncol);
+  din.sigma = up(in->sigma, nlev);
+  din.prdst = up(in->prdst, np);
+  double *dtr = p;
+  p += ncol * (nlev + 1);
+  double *dtd = taudst ? p : nullptr;
+  int s = ok ? mgcm_rt_dust_tauref(&din, dtr, dtd, nullptr) : MGCM_RT_ERR_DEVICE;
+  if (s == MGCM_RT_OK && hipStreamSynchronize(nullptr) != hipSuccess) s = MGCM_RT_ERR_DEVICE;
+  if (s == MGCM_RT_OK &&
+      (hipMemcpy(tauref, dtr, ncol * (nlev + 1) * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+       (taudst && hipMemcpy(taudst, dtd, ncol * np * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)))
+    s = MGCM_RT_ERR_DEVICE;
+  (void)hipFree(d);
+  return s;
 }

@@ -3,7 +3,14 @@
 ! tables up once, runs the batch through mgcm_rt_radiation and writes the
 ! results to a second stream file.
 !
-!   mgcm_rt_driver [--per-column] <case file> <result file> [data directory | -] [history file]
+!   mgcm_rt_driver [--per-column] [--aerosol <tracer file> <aerosol data dir>]
+!                  <case file> <result file> [data directory | -] [history file]
+!
+! --aerosol runs COMP3's active dust and water-ice clouds (code/comp3.f:595-631):
+! INITCLD's tables from the four optical-property files in <aerosol data dir>,
+! then OPT_DST and OPT_CLD on the tracers QTRACE(nlay, 6, ncol) (FP64 stream
+! file) and the columns' PLEV; their TAUREF/TAUREFCLD replace the case file's
+! and their per-level optics go to the radiation through its aerosol argument.
 !
 ! --per-column runs COMP3's own loop shape instead of one batch: mgcm_rt_column
 ! once per column, with DETAU/CUMTAUV as the module arrays (L_J, L_I, 7, 17),
@@ -58,21 +65,36 @@ program mgcm_rt_driver
   real(4), allocatable :: fuptopv(:,:), fdntopv(:,:), fupsurfv(:,:), fdnsurfv(:,:)
   real(4), allocatable :: fuptopir(:,:), fupsurfir(:,:), fdnsurfir(:,:)
   integer :: n, a0, nargs
-  logical :: percol
+  logical :: percol, aeron
+  character(len=4096) :: ftracer, aerdir
+  real(c_double), allocatable, target :: qtrace(:,:,:)
+  real(c_double), allocatable, target :: dqxv(:,:,:), dqsv(:,:,:), dgv(:,:,:), dqxi(:,:,:), dqsi(:,:,:), dgi(:,:,:)
+  real(c_double), allocatable, target :: cqxv(:,:,:), cqsv(:,:,:), cgv(:,:,:), cqxi(:,:,:), cqsi(:,:,:), cgi(:,:,:)
+  real(c_double), allocatable, target :: dqref(:,:), cqref(:,:)
+  type(mgcm_rt_aerosol), target :: aer
   real(c_double), allocatable :: detau4(:,:,:,:), cumtauv4(:,:,:,:)
 
   nargs = command_argument_count()
   a0 = 0
   percol = .false.
-  if (nargs >= 1) then
-    call get_command_argument(1, fin)
+  aeron = .false.
+  do while (a0 < nargs)
+    call get_command_argument(a0 + 1, fin)
     if (trim(fin) == '--per-column') then
       percol = .true.
-      a0 = 1
+      a0 = a0 + 1
+    else if (trim(fin) == '--aerosol' .and. a0 + 3 <= nargs) then
+      aeron = .true.
+      call get_command_argument(a0 + 2, ftracer)
+      call get_command_argument(a0 + 3, aerdir)
+      a0 = a0 + 3
+    else
+      exit
     end if
-  end if
+  end do
   if (nargs - a0 < 2 .or. nargs - a0 > 4) then
-    write(*, '(a)') 'usage: mgcm_rt_driver [--per-column] <case file> <result file> [data directory | -] [history file]'
+    write(*, '(a)') 'usage: mgcm_rt_driver [--per-column] [--aerosol <tracer file> <aerosol data dir>] '// &
+                    '<case file> <result file> [data directory | -] [history file]'
     stop 2
   end if
   call get_command_argument(a0 + 1, fin)
@@ -114,7 +136,50 @@ program mgcm_rt_driver
     call mgcm_rt_setup(co2v, co2i, pfgasref, tgasref, wrefh2o, gwv, gwi, fzerov, fzeroi, tauray, dwni, &
                        planckir, qxvdst, qsvdst, gvdst, qxidst, qsidst, gidst, qxvcld, qsvcld, gvcld, &
                        qxicld, qsicld, gicld, qextrefdst, qextrefcld, ctx, status)
-  if (status == MGCM_RT_OK .and. percol) then
+  if (status == MGCM_RT_OK .and. aeron) then
+    ! COMP3's active dust and clouds: OPT_DST, OPT_CLD, then their outputs into the radiation
+    allocate(qtrace(nlay, 6, ncol))
+    open(newunit=u, file=trim(ftracer), access='stream', form='unformatted', status='old', iostat=ios)
+    if (ios /= 0) then
+      status = MGCM_RT_ERR_IO
+    else
+      read(u) qtrace
+      close(u)
+      call mgcm_rt_aerosol_setup(ctx, aerdir, status)
+    end if
+    if (status == MGCM_RT_OK) then
+      allocate(dqxv(nlev + 1, MGCM_RT_NV, ncol), dqsv(nlev + 1, MGCM_RT_NV, ncol), dgv(nlev + 1, MGCM_RT_NV, ncol))
+      allocate(dqxi(nlev + 1, MGCM_RT_NI, ncol), dqsi(nlev + 1, MGCM_RT_NI, ncol), dgi(nlev + 1, MGCM_RT_NI, ncol))
+      allocate(cqxv(nlev + 1, MGCM_RT_NV, ncol), cqsv(nlev + 1, MGCM_RT_NV, ncol), cgv(nlev + 1, MGCM_RT_NV, ncol))
+      allocate(cqxi(nlev + 1, MGCM_RT_NI, ncol), cqsi(nlev + 1, MGCM_RT_NI, ncol), cgi(nlev + 1, MGCM_RT_NI, ncol))
+      allocate(dqref(nlev + 1, ncol), cqref(nlev + 1, ncol))
+      call mgcm_rt_opt_aer(ctx, .false., ncol, nlay, plev, qtrace, dqxv, dqsv, dgv, dqxi, dqsi, dgi, dqref, &
+                           tauref, status)
+    end if
+    if (status == MGCM_RT_OK) &
+      call mgcm_rt_opt_aer(ctx, .true., ncol, nlay, plev, qtrace, cqxv, cqsv, cgv, cqxi, cqsi, cgi, cqref, &
+                           taurefcld, status)
+    if (status == MGCM_RT_OK) then
+      aer%qxv_dst = c_loc(dqxv)
+      aer%qsv_dst = c_loc(dqsv)
+      aer%gv_dst = c_loc(dgv)
+      aer%qxi_dst = c_loc(dqxi)
+      aer%qsi_dst = c_loc(dqsi)
+      aer%gi_dst = c_loc(dgi)
+      aer%qxv_cld = c_loc(cqxv)
+      aer%qsv_cld = c_loc(cqsv)
+      aer%gv_cld = c_loc(cgv)
+      aer%qxi_cld = c_loc(cqxi)
+      aer%qsi_cld = c_loc(cqsi)
+      aer%gi_cld = c_loc(cgi)
+      aer%qextref_dst = c_loc(dqref)
+      aer%qextref_cld = c_loc(cqref)
+      call mgcm_rt_radiation(ctx, ncol, nlay, plev, tlev, pmid, tmid, qh2o, tauref, taurefcld, als, albi, &
+                             acosz, sol, fmnetv, fluxupv, fluxdnv, fmneti, fluxupi, fluxdni, nfluxtopv, &
+                             nfluxtopi, diffvt, detau, cumtauv, hr, precision, status, aerosol=aer)
+    end if
+    s2 = mgcm_rt_destroy(ctx)
+  else if (status == MGCM_RT_OK .and. percol) then
     ! COMP3's loop over (J, I), here J = column, I = 1 (code/comp3.f:411-415,637-738)
     allocate(detau4(ncol, 1, MGCM_RT_NV, MGCM_RT_NG), cumtauv4(ncol, 1, MGCM_RT_NV, MGCM_RT_NG))
     detau4 = 0.0d0

@@ -60,7 +60,22 @@ module mgcm_rt_mod
 
   public :: mgcm_rt_init, mgcm_rt_destroy, mgcm_rt_run, mgcm_rt_run_host, mgcm_rt_check, mgcm_rt_history_fields
   public :: mgcm_rt_run_batched, mgcm_rt_planck_table, mgcm_rt_laginterp, mgcm_rt_read_ktable, mgcm_rt_strerror
+  ! mirrors mgcm_rt_tracers: QTRACE(nlay, 6) and PL(L_LEVELS) of each column
+  type, bind(C), public :: mgcm_rt_tracers
+    integer(c_int) :: ncol, nlay
+    type(c_ptr) :: pl, qtrace
+  end type
+
+  ! mirrors mgcm_rt_aerosol_out: OPT_DST / OPT_CLD outputs of each column
+  type, bind(C), public :: mgcm_rt_aerosol_out
+    type(c_ptr) :: qxv, qsv, gv, qxi, qsi, gi, qextref, tauref, tau2
+  end type
+
+  integer, parameter, public :: MGCM_RT_AER_TABLE_DOUBLES = 11552
+
   public :: mgcm_rt_setup, mgcm_rt_radiation, mgcm_rt_message, mgcm_rt_laginterp_files, mgcm_rt_column
+  public :: mgcm_rt_read_aerosol_tables, mgcm_rt_set_aerosol_tables, mgcm_rt_opt_dst_host, mgcm_rt_opt_cld_host
+  public :: mgcm_rt_aerosol_setup, mgcm_rt_opt_aer
 
   interface
     integer(c_int) function mgcm_rt_init(tables, ctx) bind(C, name="mgcm_rt_init")
@@ -123,6 +138,28 @@ module mgcm_rt_mod
       type(mgcm_rt_fluxes), intent(in) :: outs
       type(mgcm_rt_history), intent(in) :: hist
       type(c_ptr), value :: stream
+    end function
+    integer(c_int) function mgcm_rt_read_aerosol_tables(datadir, tables) bind(C, name="mgcm_rt_read_aerosol_tables")
+      import :: c_int, c_double, c_char
+      character(kind=c_char), intent(in) :: datadir(*)
+      real(c_double), intent(out) :: tables(*)
+    end function
+    integer(c_int) function mgcm_rt_set_aerosol_tables(ctx, tables) bind(C, name="mgcm_rt_set_aerosol_tables")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: tables(*)
+    end function
+    integer(c_int) function mgcm_rt_opt_dst_host(ctx, tr, out) bind(C, name="mgcm_rt_opt_dst_host")
+      import :: c_int, c_ptr, mgcm_rt_tracers, mgcm_rt_aerosol_out
+      type(c_ptr), value :: ctx
+      type(mgcm_rt_tracers), intent(in) :: tr
+      type(mgcm_rt_aerosol_out), intent(in) :: out
+    end function
+    integer(c_int) function mgcm_rt_opt_cld_host(ctx, tr, out) bind(C, name="mgcm_rt_opt_cld_host")
+      import :: c_int, c_ptr, mgcm_rt_tracers, mgcm_rt_aerosol_out
+      type(c_ptr), value :: ctx
+      type(mgcm_rt_tracers), intent(in) :: tr
+      type(mgcm_rt_aerosol_out), intent(in) :: out
     end function
     type(c_ptr) function mgcm_rt_strerror(status) bind(C, name="mgcm_rt_strerror")
       import :: c_int, c_ptr
@@ -281,6 +318,60 @@ contains
     if (present(directsol)) o%directsol = c_loc(directsol)
     if (present(directsol) .and. present(dnvflux)) o%dnvflux = c_loc(dnvflux)
     status = mgcm_rt_run_host(ctx, c, o, precision)
+  end subroutine
+
+  ! INITCLD's size grid and optical tables (code/initcld.f:131-184) from the four
+  ! data files in datadir (Dust_*_wolff2010_JD_12bands.dat,
+  ! waterCoated_*_JD_12bands.dat), uploaded to the context.
+  subroutine mgcm_rt_aerosol_setup(ctx, datadir, status)
+    type(c_ptr), intent(in) :: ctx
+    character(len=*), intent(in) :: datadir
+    integer(c_int), intent(out) :: status
+    real(c_double), allocatable :: tables(:)
+    allocate(tables(MGCM_RT_AER_TABLE_DOUBLES))
+    status = mgcm_rt_read_aerosol_tables(trim(datadir)//c_null_char, tables)
+    if (status == MGCM_RT_OK) status = mgcm_rt_set_aerosol_tables(ctx, tables)
+    deallocate(tables)
+  end subroutine
+
+  ! OPT_DST (cloud = .false., code/opt_dst.f) or OPT_CLD (code/opt_cld.f) for a
+  ! batch of columns, host arrays: PL(L_LEVELS, ncol) and QTRACE(nlay, 6, ncol) in;
+  ! Qext/Qscat/g (L_LEVELS+1, nspect, ncol), the reference extinction and
+  ! TAUREF (L_LEVELS+1, ncol) and, optionally, taudust/taucloud (2, ncol) out --
+  ! the per-level optics mgcm_rt_radiation takes through its aerosol argument.
+  subroutine mgcm_rt_opt_aer(ctx, cloud, ncol, nlay, pl, qtrace, qxv, qsv, gv, qxi, qsi, gi, qextref, tauref, &
+                             status, tau2)
+    type(c_ptr), intent(in) :: ctx
+    logical, intent(in) :: cloud
+    integer, intent(in) :: ncol, nlay
+    real(c_double), intent(in), target :: pl(2*nlay+3, ncol), qtrace(nlay, 6, ncol)
+    real(c_double), intent(out), target :: qxv(2*nlay+4, MGCM_RT_NV, ncol), qsv(2*nlay+4, MGCM_RT_NV, ncol)
+    real(c_double), intent(out), target :: gv(2*nlay+4, MGCM_RT_NV, ncol), qxi(2*nlay+4, MGCM_RT_NI, ncol)
+    real(c_double), intent(out), target :: qsi(2*nlay+4, MGCM_RT_NI, ncol), gi(2*nlay+4, MGCM_RT_NI, ncol)
+    real(c_double), intent(out), target :: qextref(2*nlay+4, ncol), tauref(2*nlay+4, ncol)
+    integer(c_int), intent(out) :: status
+    real(c_double), intent(out), target, optional :: tau2(2, ncol)
+    type(mgcm_rt_tracers) :: tr
+    type(mgcm_rt_aerosol_out) :: o
+    tr%ncol = ncol
+    tr%nlay = nlay
+    tr%pl = c_loc(pl)
+    tr%qtrace = c_loc(qtrace)
+    o%qxv = c_loc(qxv)
+    o%qsv = c_loc(qsv)
+    o%gv = c_loc(gv)
+    o%qxi = c_loc(qxi)
+    o%qsi = c_loc(qsi)
+    o%gi = c_loc(gi)
+    o%qextref = c_loc(qextref)
+    o%tauref = c_loc(tauref)
+    o%tau2 = c_null_ptr
+    if (present(tau2)) o%tau2 = c_loc(tau2)
+    if (cloud) then
+      status = mgcm_rt_opt_cld_host(ctx, tr, o)
+    else
+      status = mgcm_rt_opt_dst_host(ctx, tr, o)
+    end if
   end subroutine
 
   ! Per-column drop-in for COMP3's radiation block (code/comp3.f:637-738): one

@@ -138,6 +138,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         lib.mgcm_rt_set_aerosol_tables.argtypes = [C.c_void_p, DP]
         lib.mgcm_rt_opt_dst.argtypes = [C.c_void_p, C.POINTER(Tracers), C.POINTER(AerosolOut), C.c_void_p]
         lib.mgcm_rt_opt_cld.argtypes = [C.c_void_p, C.POINTER(Tracers), C.POINTER(AerosolOut), C.c_void_p]
+        lib.mgcm_rt_opt_dst_host.argtypes = [C.c_void_p, C.POINTER(Tracers), C.POINTER(AerosolOut)]
+        lib.mgcm_rt_opt_cld_host.argtypes = [C.c_void_p, C.POINTER(Tracers), C.POINTER(AerosolOut)]
         lib.mgcm_rt_libm_check.argtypes = [DP, C.c_int, DP, DP, DP]
         lib.mgcm_rt_strerror.restype = C.c_char_p
         lib.mgcm_rt_strerror.argtypes = [C.c_int]
@@ -324,18 +326,20 @@ class Radiation:
         _check(fn(self.ctx, C.byref(tr), C.byref(ao), C.c_void_p(stream)), "mgcm_rt_opt_cld" if cloud else "mgcm_rt_opt_dst")
 
     def opt_aerosol_host(self, cloud: bool, pl: np.ndarray, qtrace: np.ndarray) -> dict:
-        """opt_aerosol on host arrays (copies to the device and back; synchronises)."""
-        import torch
+        """OPT_DST / OPT_CLD on host arrays through mgcm_rt_opt_dst_host / _cld_host
+        (the library copies in and out and synchronises)."""
+        pl, qtrace = _f64(pl), _f64(qtrace)
         ncol, nlay = qtrace.shape[0], qtrace.shape[2]
         ld = 2 * nlay + 4
-        dev = {"pl": torch.from_numpy(_f64(pl)).cuda(), "qtrace": torch.from_numpy(_f64(qtrace)).cuda()}
         shapes = {"qxv": (ncol, NV, ld), "qsv": (ncol, NV, ld), "gv": (ncol, NV, ld), "qxi": (ncol, NI, ld),
                   "qsi": (ncol, NI, ld), "gi": (ncol, NI, ld), "qextref": (ncol, ld), "tauref": (ncol, ld),
                   "tau2": (ncol, 2)}
-        out = {k: torch.empty(v, dtype=torch.float64, device="cuda") for k, v in shapes.items()}
-        self.opt_aerosol(cloud, dev["pl"], dev["qtrace"], out, torch.cuda.current_stream().cuda_stream)
-        torch.cuda.synchronize()
-        return {k: v.cpu().numpy() for k, v in out.items()}
+        out = {k: np.empty(v) for k, v in shapes.items()}
+        tr = Tracers(ncol, nlay, pl.ctypes.data, qtrace.ctypes.data)
+        ao = AerosolOut(*[out[k].ctypes.data for k in AER_OUT_KEYS])
+        fn = self.lib.mgcm_rt_opt_cld_host if cloud else self.lib.mgcm_rt_opt_dst_host
+        _check(fn(self.ctx, C.byref(tr), C.byref(ao)), "mgcm_rt_opt_cld_host" if cloud else "mgcm_rt_opt_dst_host")
+        return out
 
 
 def planck_table():

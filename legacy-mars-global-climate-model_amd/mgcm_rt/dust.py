@@ -52,20 +52,31 @@ class DustColumns(C.Structure):
 
 
 def tauref(prdst, rptau: float, ptrop: float, tautot, conr, p, sigma, nlay: int) -> dict:
-    """mgcm_rt_dust_tauref on host arrays (copies to the device and back): TAUREF
-    [ncol][nlev+1] and TAUDST [ncol][npdst]."""
-    import torch
+    """mgcm_rt_dust_tauref_host on host arrays (the library copies in and out and
+    synchronises): TAUREF [ncol][nlev+1] and TAUDST [ncol][npdst]."""
+    from . import _check, load_library
+    lib = load_library()
+    lib.mgcm_rt_dust_tauref_host.argtypes = [C.POINTER(DustColumns), C.c_void_p, C.c_void_p]
+    a = {k: np.ascontiguousarray(v, dtype=np.float64)
+         for k, v in (("tautot", tautot), ("conr", conr), ("p", p), ("sigma", sigma), ("prdst", prdst))}
+    ncol, npdst = a["tautot"].size, a["prdst"].size
+    tr = np.empty((ncol, 2 * nlay + 4))
+    td = np.empty((ncol, npdst))
+    s = DustColumns(ncol, nlay, *[a[k].ctypes.data for k in ("tautot", "conr", "p", "sigma", "prdst")], npdst,
+                    float(rptau), float(ptrop))
+    _check(lib.mgcm_rt_dust_tauref_host(C.byref(s), tr.ctypes.data, td.ctypes.data), "mgcm_rt_dust_tauref_host")
+    return {"tauref": tr, "taudst": td}
+
+
+def tauref_device(d: dict, rptau: float, ptrop: float, nlay: int, tr, td=None, stream: int = 0) -> None:
+    """mgcm_rt_dust_tauref on device tensors d[tautot, conr, p, sigma, prdst] into
+    tr [ncol][nlev+1] (and td [ncol][npdst]); asynchronous on stream."""
     from . import _check, load_library
     lib = load_library()
     lib.mgcm_rt_dust_tauref.argtypes = [C.POINTER(DustColumns), C.c_void_p, C.c_void_p, C.c_void_p]
-    d = {k: torch.from_numpy(np.ascontiguousarray(v, dtype=np.float64)).cuda()
-         for k, v in (("tautot", tautot), ("conr", conr), ("p", p), ("sigma", sigma), ("prdst", prdst))}
-    ncol, npdst = d["tautot"].numel(), d["prdst"].numel()
-    tr = torch.empty((ncol, 2 * nlay + 4), dtype=torch.float64, device="cuda")
-    td = torch.empty((ncol, npdst), dtype=torch.float64, device="cuda")
-    s = DustColumns(ncol, nlay, *[d[k].data_ptr() for k in ("tautot", "conr", "p", "sigma", "prdst")], npdst,
-                    float(rptau), float(ptrop))
-    _check(lib.mgcm_rt_dust_tauref(C.byref(s), C.c_void_p(tr.data_ptr()), C.c_void_p(td.data_ptr()),
-                                   C.c_void_p(torch.cuda.current_stream().cuda_stream)), "mgcm_rt_dust_tauref")
-    torch.cuda.synchronize()
-    return {"tauref": tr.cpu().numpy(), "taudst": td.cpu().numpy()}
+    s = DustColumns(int(d["tautot"].numel()), nlay, *[d[k].data_ptr() for k in ("tautot", "conr", "p", "sigma",
+                                                                                 "prdst")],
+                    int(d["prdst"].numel()), float(rptau), float(ptrop))
+    _check(lib.mgcm_rt_dust_tauref(C.byref(s), C.c_void_p(tr.data_ptr()),
+                                   C.c_void_p(None if td is None else td.data_ptr()), C.c_void_p(stream)),
+           "mgcm_rt_dust_tauref")

@@ -169,3 +169,42 @@ def test_fortran_per_column_shim_equals_batch(tmp_path):
             continue
         assert np.array_equal(outs["column"][k], outs["batch"][k]), k
     assert not np.any(outs["column"]["fluxdnv"][::5])  # sun-down columns
+
+
+@pytest.mark.gpu
+def test_fortran_driver_active_dust_and_clouds(tmp_path):
+    """COMP3's active dust and clouds from Fortran: INITCLD's tables read from the
+    reference's optical-property files, OPT_DST and OPT_CLD on host arrays
+    (mgcm_rt_opt_aer over mgcm_rt_opt_*_host), their optics through the aerosol
+    argument of mgcm_rt_radiation.  Equal bit for bit to the same chain through
+    the Python binding."""
+    from mgcm_rt import Radiation, casefile, read_aerosol_tables, synth
+    from test_aerosol_producers import write_files
+    from conftest import GOLDEN
+    fx = np.load(os.path.join(GOLDEN, "aerosol_ref.npz"))
+    aerdir = tmp_path / "aer"
+    aerdir.mkdir()
+    write_files(str(aerdir), fx)
+    nlay, ncol = 24, 120
+    t, c = synth.make_workload(29, ncol, nlay)
+    qtr = synth.make_tracers(29, ncol, nlay)                    # [ncol][6][nlay] = Fortran (nlay, 6, ncol)
+    case, res, trf = str(tmp_path / "case.bin"), str(tmp_path / "res.bin"), str(tmp_path / "qtrace.bin")
+    casefile.write_case(case, t, c, 0)
+    np.ascontiguousarray(qtr, dtype=np.float64).tofile(trf)
+    r = subprocess.run([DRIVER, "--aerosol", trf, str(aerdir), case, res], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    status, got = casefile.read_result(res, ncol, nlay)
+    assert status == 0
+    rt = Radiation(t)
+    rt.set_aerosol_tables(read_aerosol_tables(str(aerdir)))
+    d, k = rt.opt_aerosol_host(False, c["plev"], qtr), rt.opt_aerosol_host(True, c["plev"], qtr)
+    cc = dict(c)
+    cc["tauref"], cc["taurefcld"] = d["tauref"], k["tauref"]
+    for q in ("qxv", "qsv", "gv", "qxi", "qsi", "gi"):
+        cc[q + "_dst"], cc[q + "_cld"] = d[q], k[q]
+    cc["qextref_dst"], cc["qextref_cld"] = d["qextref"], k["qextref"]
+    want = rt.run_host(cc)
+    for key in casefile.RESULT_KEYS:
+        assert np.array_equal(got[key], want[key]), key
+    assert np.any(d["tauref"] > 0) and np.any(k["tauref"] > 0)  # dust and cloud are both active

@@ -95,3 +95,26 @@ def test_active_dust_and_cloud_chain_matches_oracle(tables, oracle):
     own = rel_err(oracle.run(t, cg)["hr"], want["hr"])
     print(f"chain heating {rel_err(got['hr'], want['hr']):.1e}; oracle's own response {own:.1e}", flush=True)
     assert rel_err(got["hr"], want["hr"]) <= max(1e-10, 4.0 * own)
+
+
+def test_host_entry_points_equal_device_entry_points(tables):
+    """mgcm_rt_opt_dst_host / _cld_host (host arrays, library-managed copies) give the
+    device entry points' results bit for bit, at 24 and 99 layers."""
+    import torch
+    from mgcm_rt import AER_OUT_KEYS, Radiation, synth
+    for nlay, ncol in ((24, 300), (99, 37)):
+        rt = Radiation(synth.make_tables(1, nlay))
+        rt.set_aerosol_tables(tables)
+        qtr = synth.make_tracers(7 + nlay, ncol, nlay)
+        pl = synth.make_columns(7 + nlay, ncol, nlay)["plev"]
+        ld = 2 * nlay + 4
+        shapes = {"qxv": (ncol, 7, ld), "qsv": (ncol, 7, ld), "gv": (ncol, 7, ld), "qxi": (ncol, 5, ld),
+                  "qsi": (ncol, 5, ld), "gi": (ncol, 5, ld), "qextref": (ncol, ld), "tauref": (ncol, ld),
+                  "tau2": (ncol, 2)}
+        for cloud in (False, True):
+            host = rt.opt_aerosol_host(cloud, pl, qtr)
+            dev = {k: torch.empty(v, dtype=torch.float64, device="cuda") for k, v in shapes.items()}
+            rt.opt_aerosol(cloud, torch.from_numpy(pl).cuda(), torch.from_numpy(qtr).cuda(), dev)
+            torch.cuda.synchronize()
+            for k in AER_OUT_KEYS:
+                assert np.array_equal(host[k], dev[k].cpu().numpy()), (nlay, cloud, k)

@@ -50,3 +50,24 @@ def test_dust_tauref_into_radiation(oracle):
     tr = dust.tauref(prdst, 6.1, 0.0008, tautot, conr, psurf, synth.sigma_levels(24), 24)["tauref"]
     c["tauref"] = tr
     compare(Radiation(t).run_host(c), oracle.run(t, c))
+
+
+def test_dust_host_entry_equals_device_entry():
+    """mgcm_rt_dust_tauref_host equals mgcm_rt_dust_tauref on device tensors bit for bit."""
+    import torch
+    from mgcm_rt import dust, synth
+    rng = np.random.default_rng(65)
+    nlay, ncol = 36, 500
+    sigma = synth.sigma_levels(nlay)
+    prdst = dust.pressure_grid(7.01, 0.0008)
+    tautot, p = rng.uniform(0.05, 3.0, ncol), rng.uniform(3.0, 11.0, ncol)
+    conr = rng.uniform(0.01, 0.05, ncol)
+    host = dust.tauref(prdst, 6.1, 0.0008, tautot, conr, p, sigma, nlay)
+    d = {k: torch.from_numpy(np.ascontiguousarray(v, dtype=np.float64)).cuda()
+         for k, v in (("tautot", tautot), ("conr", conr), ("p", p), ("sigma", sigma), ("prdst", prdst))}
+    tr = torch.empty((ncol, 2 * nlay + 4), dtype=torch.float64, device="cuda")
+    td = torch.empty((ncol, prdst.size), dtype=torch.float64, device="cuda")
+    dust.tauref_device(d, 6.1, 0.0008, nlay, tr, td)
+    torch.cuda.synchronize()
+    assert np.array_equal(host["tauref"], tr.cpu().numpy())
+    assert np.array_equal(host["taudst"], td.cpu().numpy())
