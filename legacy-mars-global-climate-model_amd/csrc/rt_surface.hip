@@ -16,24 +16,37 @@ struct SurfArgs {
   double *directsol, *dnvflux;
 };
 
+// DSOL_COLS columns per 64-lane block.  The 119 terms of a column (its DETAU
+// read contiguously across lanes) are formed in parallel into LDS, then one
+// lane per column adds them in the reference's order (band, then Gauss point),
+// so the sum is the reference's bit for bit.  A term past DETAU > 5 adds +0,
+// which leaves the non-negative sum unchanged.
+constexpr int DSOL_COLS = 16, DSOL_TERMS = NG * NV;
 __global__ __launch_bounds__(64) void k_dsolflux(const RtConst *__restrict__ rc, SurfArgs a) {
-  const int col = blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= a.ncol) return;
-  const double ubar0 = a.acosz[col];
-  double directsol = 0.0;
-  if (ubar0 >= F32LIT(1.0e-5)) {  // code/comp3.f:336-345: sun down gives 0
-    const double *detau = a.detau + (size_t)col * NG * NV;  // DETAU(nw, ng)
-    for (int nw = 0; nw < NV; nw++) {
-      const double factor = ubar0 * a.sol[nw];
-      for (int ng = 0; ng < NG - 1; ng++) {
-        const double d = detau[ng * NV + nw];
-        if (d <= 5.0)
-          directsol = directsol + factor * mgcm_libm::mgcm_exp(-d / ubar0) * rc->gw[nw][ng] * (1.0 - rc->fzero[nw]);
+  __shared__ double term[DSOL_COLS][DSOL_TERMS];
+  const int col0 = blockIdx.x * DSOL_COLS, lane = threadIdx.x;
+  for (int t = lane; t < DSOL_COLS * DSOL_TERMS; t += 64) {
+    const int cc = t / DSOL_TERMS, e = t - cc * DSOL_TERMS;  // e = ng*NV + nw: DETAU(nw, ng) of the column
+    const int col = col0 + cc;
+    double v = 0.0;
+    if (col < a.ncol) {
+      const double ubar0 = a.acosz[col];
+      const int nw = e % NV, ng = e / NV;
+      const double d = a.detau[(size_t)col * DSOL_TERMS + e];
+      if (ubar0 >= F32LIT(1.0e-5) && d <= 5.0) {  // code/dsolflux.f:28-45; sun down: code/comp3.f:336-345
+        const double factor = ubar0 * a.sol[nw];
+        v = (ng < NG - 1) ? factor * mgcm_libm::mgcm_exp(-d / ubar0) * rc->gw[nw][ng] * (1.0 - rc->fzero[nw])
+                          : factor * mgcm_libm::mgcm_exp(-d / ubar0) * rc->fzero[nw];
       }
-      const double d = detau[(NG - 1) * NV + nw];
-      if (d <= 5.0) directsol = directsol + factor * mgcm_libm::mgcm_exp(-d / ubar0) * rc->fzero[nw];
     }
+    term[cc][e] = v;
   }
+  __syncthreads();
+  const int col = col0 + lane;
+  if (lane >= DSOL_COLS || col >= a.ncol) return;
+  double directsol = 0.0;
+  for (int nw = 0; nw < NV; nw++)
+    for (int ng = 0; ng < NG; ng++) directsol = directsol + term[lane][ng * NV + nw];
   a.directsol[col] = directsol;
   if (a.dnvflux) a.dnvflux[col] = a.diffvt[col] + directsol;  // DNVFLUX = DNDIFFV + DIRECTSOL
 }
@@ -49,7 +62,7 @@ hipError_t launch_dsolflux(const RtConst *rc, int ncol, const double *sol, const
   a.diffvt = diffvt;
   a.directsol = directsol;
   a.dnvflux = dnvflux;
-  hipLaunchKernelGGL(k_dsolflux, dim3((ncol + 63) / 64), dim3(64), 0, st, rc, a);
+  hipLaunchKernelGGL(k_dsolflux, dim3((ncol + DSOL_COLS - 1) / DSOL_COLS), dim3(64), 0, st, rc, a);
   return hipGetLastError();
 }
 
