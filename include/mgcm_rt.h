@@ -17,6 +17,13 @@
  * Arrays use the reference's Fortran layouts (first index fastest), one column
  * after another: nlev = 2*nlay+3 (L_LEVELS), nlayrad = nlay+1 (L_NLAYRAD).
  * Every entry point returns MGCM_RT_OK (0) or a negative status; none aborts.
+ *
+ * Contexts share no mutable state: several may run at once on different
+ * streams or host threads.  One context's workspaces (the continuum-point
+ * records, the host-pointer staging) are reused by each of its calls, so calls
+ * on the same context must not overlap: issue them on one stream, or
+ * synchronise between them.  A context's tables live on the HIP device that was
+ * current at mgcm_rt_init; call it from that device.
  */
 #ifndef MGCM_RT_H
 #define MGCM_RT_H
