@@ -360,6 +360,27 @@ def form_next_rows(args, rt_unused, stream_unused):
     return out
 
 
+def form_host_arrays(args, rt, stream_unused):
+    """The same grid through mgcm_rt_run_host, the entry a host with its fields in
+    host memory calls (the Fortran mgcm_rt_radiation): the library copies the
+    inputs in, runs, copies the outputs back and synchronises.  Wall time per call
+    with numpy (pageable) arrays: the PCIe-inclusive rate."""
+    from mgcm_rt import FP64, alloc_outputs, synth
+    _, c = synth.make_workload(args.seed, args.ncol, args.nlay)
+    out = alloc_outputs(args.ncol, args.nlay)
+    rt.run_host(c, FP64, out)
+    steps = max(1, min(args.steps, 5))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        rt.run_host(c, FP64, out)
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    nin = args.ncol * (5 * (2 * args.nlay + 3) + 2 * (2 * args.nlay + 4) + 3) * 8
+    nout = args.ncol * (7 * (args.nlay + 1) + 3 + 2 * 17 * 7) * 8
+    return {"workload": f"{args.ncol} columns x {args.nlay} layers, host (pageable) arrays in and out",
+            "value": args.ncol / (ms / 1e3), "unit": "columns/s", "ms_per_call": ms,
+            "bytes_in": nin, "bytes_out": nout, "copy_GBps_if_all_copy": (nin + nout) / (ms / 1e3) / 1e9}
+
+
 def form_surrogate(args, rt_unused, stream):
     """Learned IR surrogate with exact fallback on 64,800 fresh columns x 36 layers.
 
@@ -509,8 +530,8 @@ def main():
     forms = {}
     if not args.no_forms:
         del do
-        for name, fn in (("real_tables", form_real), ("next_rows", form_next_rows), ("mixed", form_mixed),
-                         ("batched", form_batched), ("surrogate", form_surrogate)):
+        for name, fn in (("real_tables", form_real), ("host_arrays", form_host_arrays), ("next_rows", form_next_rows),
+                         ("mixed", form_mixed), ("batched", form_batched), ("surrogate", form_surrogate)):
             if args.forms and name not in args.forms.split(","):
                 continue
             try:
