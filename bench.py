@@ -364,7 +364,9 @@ def form_host_arrays(args, rt, stream_unused):
     """The same grid through mgcm_rt_run_host, the entry a host with its fields in
     host memory calls (the Fortran mgcm_rt_radiation): the library copies the
     inputs in, runs, copies the outputs back and synchronises.  Wall time per call
-    with numpy (pageable) arrays: the PCIe-inclusive rate."""
+    with numpy (pageable) arrays: the PCIe-inclusive rate.  The library copies
+    chunks of columns through pinned staging with its own threads while the
+    device runs the chunk before."""
     from mgcm_rt import FP64, alloc_outputs, synth
     _, c = synth.make_workload(args.seed, args.ncol, args.nlay)
     out = alloc_outputs(args.ncol, args.nlay)
@@ -374,7 +376,7 @@ def form_host_arrays(args, rt, stream_unused):
     for _ in range(steps):
         rt.run_host(c, FP64, out)
     ms = (time.perf_counter() - t0) / steps * 1e3
-    nin = args.ncol * (5 * (2 * args.nlay + 3) + 2 * (2 * args.nlay + 4) + 3) * 8
+    nin = args.ncol * (5 * (2 * args.nlay + 3) + 2 * (2 * args.nlay + 4) + 3 + 2 * 17 * 7) * 8  # DETAU/CUMTAUV in/out
     nout = args.ncol * (7 * (args.nlay + 1) + 3 + 2 * 17 * 7) * 8
     return {"workload": f"{args.ncol} columns x {args.nlay} layers, host (pageable) arrays in and out",
             "value": args.ncol / (ms / 1e3), "unit": "columns/s", "ms_per_call": ms,

@@ -159,7 +159,11 @@ int mgcm_rt_run(mgcm_rt_ctx *ctx, const mgcm_rt_columns *in, const mgcm_rt_fluxe
                 int precision, void *stream);
 
 /* Same with host pointers: copies in, runs, copies out, synchronises, and
- * reports MGCM_RT_ERR_RANGE if any column was out of the tables' range. */
+ * reports MGCM_RT_ERR_RANGE if any column was out of the tables' range.
+ * A grid above 4,096 columns goes in chunks of whole columns: library threads
+ * (up to 8) copy each chunk to pinned staging while the device runs the one
+ * before, so the arrays must not be touched by other threads during the call.
+ * The context keeps its staging (device, up to 256 MB pinned) until destroyed. */
 int mgcm_rt_run_host(mgcm_rt_ctx *ctx, const mgcm_rt_columns *in, const mgcm_rt_fluxes *out,
                      int precision);
 

@@ -21,7 +21,7 @@ FC = os.environ.get("FC", "/opt/rocm/bin/amdflang")
 ARCH = os.environ.get("MGCM_RT_ARCH", "gfx950")
 
 COMMON = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
-          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-munsafe-fp-atomics"]
+          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-munsafe-fp-atomics", "-pthread"]
 
 
 def build(verbose: bool = False) -> str:
@@ -46,7 +46,7 @@ def build(verbose: bool = False) -> str:
                 sys.stderr.write(r.stderr)
                 raise subprocess.CalledProcessError(r.returncode, r.args)
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs
+    cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-pthread", "-o", tmp] + objs
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
     build_fortran(verbose)

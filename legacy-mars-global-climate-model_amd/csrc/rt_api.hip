@@ -10,11 +10,6 @@
 
 using namespace mgcm;
 
-#define HIP_TRY(x)                                   \
-  do {                                               \
-    if ((x) != hipSuccess) return MGCM_RT_ERR_DEVICE; \
-  } while (0)
-
 extern "C" const char *mgcm_rt_strerror(int s) {
   switch (s) {
     case MGCM_RT_OK: return "ok";
@@ -123,6 +118,7 @@ extern "C" int mgcm_rt_destroy(mgcm_rt_ctx *c) {
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   if (c->batch) mgcm::batch_free(c->batch);
+  if (c->host) mgcm::host_free(c->host);
   delete c;
   return MGCM_RT_OK;
 }
@@ -179,6 +175,22 @@ void fill_args(RunArgs &a, const mgcm_rt_columns *in, const mgcm_rt_fluxes *out,
 }
 }  // namespace mgcm
 
+namespace mgcm {
+hipError_t run_grid(const mgcm_rt_ctx *c, const mgcm_rt_columns *in, const mgcm_rt_fluxes *out, int path,
+                    int bands, double *clr, hipStream_t st) {
+  RunArgs a;
+  fill_args(a, in, out, c->d_flag);
+  a.bands = bands;
+  a.clr = clr;
+  hipError_t e = (path == MGCM_RT_FP64) ? launch_plain(c->dt, a, st) : launch_mixed(c->dt, a, st);
+  // surface fluxes for TEMPGR from this run's DETAU/DIFFVT (code/comp3.f:336-348)
+  if (e == hipSuccess && out->directsol)
+    e = launch_dsolflux(c->d_const, in->ncol, in->sol, in->acosz, out->detau, out->diffvt, out->directsol,
+                        out->dnvflux, st);
+  return e;
+}
+}  // namespace mgcm
+
 extern "C" int mgcm_rt_run(mgcm_rt_ctx *c, const mgcm_rt_columns *in, const mgcm_rt_fluxes *out,
                            int precision, void *stream) {
   if (!c) return MGCM_RT_ERR_ARG;
@@ -199,17 +211,8 @@ extern "C" int mgcm_rt_run(mgcm_rt_ctx *c, const mgcm_rt_columns *in, const mgcm
     if (hipMalloc(&c->d_clr, need) != hipSuccess) return MGCM_RT_ERR_NOMEM;
     c->clr_bytes = need;
   }
-  RunArgs a;
-  fill_args(a, in, out, c->d_flag);
-  a.bands = bands;
-  a.clr = c->d_clr;
-  hipError_t e = (path == MGCM_RT_FP64) ? launch_plain(c->dt, a, (hipStream_t)stream)
-                                        : launch_mixed(c->dt, a, (hipStream_t)stream);
-  // surface fluxes for TEMPGR from this run's DETAU/DIFFVT (code/comp3.f:336-348)
-  if (e == hipSuccess && out->directsol)
-    e = launch_dsolflux(c->d_const, in->ncol, in->sol, in->acosz, out->detau, out->diffvt, out->directsol,
-                        out->dnvflux, (hipStream_t)stream);
-  return e == hipSuccess ? MGCM_RT_OK : MGCM_RT_ERR_DEVICE;
+  return run_grid(c, in, out, path, bands, c->d_clr, (hipStream_t)stream) == hipSuccess ? MGCM_RT_OK
+                                                                                        : MGCM_RT_ERR_DEVICE;
 }
 
 extern "C" int mgcm_rt_check(mgcm_rt_ctx *c) {
@@ -219,119 +222,6 @@ extern "C" int mgcm_rt_check(mgcm_rt_ctx *c) {
   HIP_TRY(hipMemcpy(&f, c->d_flag, sizeof(int), hipMemcpyDeviceToHost));
   HIP_TRY(hipMemset(c->d_flag, 0, sizeof(int)));
   return f ? MGCM_RT_ERR_RANGE : MGCM_RT_OK;
-}
-
-extern "C" int mgcm_rt_run_host(mgcm_rt_ctx *c, const mgcm_rt_columns *in, const mgcm_rt_fluxes *out,
-                                int precision) {
-  if (!c) return MGCM_RT_ERR_ARG;
-  int s = check_columns(in, out);
-  if (s != MGCM_RT_OK) return s;
-  int path, bands;
-  s = decode_mode(precision, &path, &bands);
-  if (s != MGCM_RT_OK) return s;
-  if (in->ncol == 0) return MGCM_RT_OK;
-  const size_t ncol = in->ncol, nlev = 2 * (size_t)in->nlay + 3, nl = in->nlay + 1;
-  const size_t nin = ncol * (5 * nlev + 2 * (nlev + 1) + 3);
-  const size_t nout = ncol * (6 * nl + 3 + 2 * NG * NV + nl + 2);
-  // optional per-level aerosol optics: the non-null arrays in mgcm_rt_aerosol order
-  const mgcm_rt_aerosol *hae = in->aerosol;
-  const double *asrc[14] = {nullptr};
-  size_t alen[14] = {0}, naer = 0;
-  if (hae) {
-    const double *src[14] = {hae->qxv_dst, hae->qsv_dst, hae->gv_dst, hae->qxi_dst, hae->qsi_dst, hae->gi_dst,
-                             hae->qxv_cld, hae->qsv_cld, hae->gv_cld, hae->qxi_cld, hae->qsi_cld, hae->gi_cld,
-                             hae->qextref_dst, hae->qextref_cld};
-    for (int i = 0; i < 14; i++) {
-      asrc[i] = src[i];
-      alen[i] = src[i] ? ncol * (i < 12 ? (((i / 3) % 2 == 0) ? NV : NI) : 1) * (nlev + 1) : 0;
-      naer += alen[i];
-    }
-  }
-  const size_t bytes = (nin + nout + naer) * sizeof(double);
-  if (bytes > c->host_bytes) {
-    if (c->d_host) (void)hipFree(c->d_host);
-    c->d_host = nullptr;
-    c->host_bytes = 0;
-    if (hipMalloc(&c->d_host, bytes) != hipSuccess) return MGCM_RT_ERR_NOMEM;
-    c->host_bytes = bytes;
-  }
-  double *p = c->d_host;
-  mgcm_rt_columns din = *in;
-  mgcm_rt_fluxes dout;
-  double *dplev = p; p += ncol * nlev;
-  double *dtlev = p; p += ncol * nlev;
-  double *dpmid = p; p += ncol * nlev;
-  double *dtmid = p; p += ncol * nlev;
-  double *dqh2o = p; p += ncol * nlev;
-  double *dtauref = p; p += ncol * (nlev + 1);
-  double *dtaurefcld = p; p += ncol * (nlev + 1);
-  double *dals = p; p += ncol;
-  double *dalbi = p; p += ncol;
-  double *dacosz = p; p += ncol;
-  HIP_TRY(hipMemcpy(dplev, in->plev, ncol * nlev * 8, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(dtlev, in->tlev, ncol * nlev * 8, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(dpmid, in->pmid, ncol * nlev * 8, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(dtmid, in->tmid, ncol * nlev * 8, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(dqh2o, in->qh2o, ncol * nlev * 8, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(dtauref, in->tauref, ncol * (nlev + 1) * 8, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(dtaurefcld, in->taurefcld, ncol * (nlev + 1) * 8, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(dals, in->als, ncol * 8, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(dalbi, in->albi, ncol * 8, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(dacosz, in->acosz, ncol * 8, hipMemcpyHostToDevice));
-  din.plev = dplev; din.tlev = dtlev; din.pmid = dpmid; din.tmid = dtmid; din.qh2o = dqh2o;
-  din.tauref = dtauref; din.taurefcld = dtaurefcld; din.als = dals; din.albi = dalbi; din.acosz = dacosz;
-  mgcm_rt_aerosol dae = {};
-  if (hae) {
-    const double **dst[14] = {&dae.qxv_dst, &dae.qsv_dst, &dae.gv_dst, &dae.qxi_dst, &dae.qsi_dst, &dae.gi_dst,
-                              &dae.qxv_cld, &dae.qsv_cld, &dae.gv_cld, &dae.qxi_cld, &dae.qsi_cld, &dae.gi_cld,
-                              &dae.qextref_dst, &dae.qextref_cld};
-    for (int i = 0; i < 14; i++) {
-      if (!asrc[i]) continue;
-      HIP_TRY(hipMemcpy(p, asrc[i], alen[i] * 8, hipMemcpyHostToDevice));
-      *dst[i] = p;
-      p += alen[i];
-    }
-    din.aerosol = &dae;
-  }
-  dout.fmnetv = p; p += ncol * nl;
-  dout.fluxupv = p; p += ncol * nl;
-  dout.fluxdnv = p; p += ncol * nl;
-  dout.fmneti = p; p += ncol * nl;
-  dout.fluxupi = p; p += ncol * nl;
-  dout.fluxdni = p; p += ncol * nl;
-  dout.nfluxtopv = p; p += ncol;
-  dout.nfluxtopi = p; p += ncol;
-  dout.diffvt = p; p += ncol;
-  dout.detau = p; p += ncol * NG * NV;
-  dout.cumtauv = p; p += ncol * NG * NV;
-  dout.hr = out->hr ? p : nullptr;
-  p += ncol * nl;
-  dout.directsol = out->directsol ? p : nullptr;
-  p += ncol;
-  dout.dnvflux = (out->directsol && out->dnvflux) ? p : nullptr;
-  p += ncol;
-  // detau/cumtauv are in/out: sun-down columns keep the caller's values
-  HIP_TRY(hipMemcpy(dout.detau, out->detau, ncol * NG * NV * 8, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(dout.cumtauv, out->cumtauv, ncol * NG * NV * 8, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemset(c->d_flag, 0, sizeof(int)));
-  s = mgcm_rt_run(c, &din, &dout, precision, nullptr);
-  if (s != MGCM_RT_OK) return s;
-  HIP_TRY(hipDeviceSynchronize());
-  HIP_TRY(hipMemcpy(out->fmnetv, dout.fmnetv, ncol * nl * 8, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(out->fluxupv, dout.fluxupv, ncol * nl * 8, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(out->fluxdnv, dout.fluxdnv, ncol * nl * 8, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(out->fmneti, dout.fmneti, ncol * nl * 8, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(out->fluxupi, dout.fluxupi, ncol * nl * 8, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(out->fluxdni, dout.fluxdni, ncol * nl * 8, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(out->nfluxtopv, dout.nfluxtopv, ncol * 8, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(out->nfluxtopi, dout.nfluxtopi, ncol * 8, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(out->diffvt, dout.diffvt, ncol * 8, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(out->detau, dout.detau, ncol * NG * NV * 8, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(out->cumtauv, dout.cumtauv, ncol * NG * NV * 8, hipMemcpyDeviceToHost));
-  if (out->hr) HIP_TRY(hipMemcpy(out->hr, dout.hr, ncol * nl * 8, hipMemcpyDeviceToHost));
-  if (dout.directsol) HIP_TRY(hipMemcpy(out->directsol, dout.directsol, ncol * 8, hipMemcpyDeviceToHost));
-  if (dout.dnvflux) HIP_TRY(hipMemcpy(out->dnvflux, dout.dnvflux, ncol * 8, hipMemcpyDeviceToHost));
-  return mgcm_rt_check(c);
 }
 
 extern "C" int mgcm_rt_run_batched(mgcm_rt_ctx *c, int ngrids, const mgcm_rt_columns *grids,

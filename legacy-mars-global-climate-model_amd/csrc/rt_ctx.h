@@ -3,6 +3,11 @@
 #include "../../include/mgcm_rt.h"
 #include "rt_internal.h"
 
+#define HIP_TRY(x)                                   \
+  do {                                               \
+    if ((x) != hipSuccess) return MGCM_RT_ERR_DEVICE; \
+  } while (0)
+
 static_assert(mgcm::MAX_NLAY == MGCM_RT_MAX_NLAY, "layer limit of the kernels and of the C ABI");
 
 struct mgcm_rt_ctx {
@@ -20,6 +25,8 @@ struct mgcm_rt_ctx {
   double *d_host;
   // batched-path state (rt_batched.hip)
   void *batch;
+  // mgcm_rt_run_host's streams, pinned slots and copy threads (rt_host.hip)
+  void *host;
   // aerosol producers' size grid and optical tables (rt_aerosol.hip)
   double *d_aer;
 };
@@ -40,6 +47,11 @@ void fill_args(RunArgs &a, const mgcm_rt_columns *in, const mgcm_rt_fluxes *out,
 int run_batched_impl(mgcm_rt_ctx *ctx, int ngrids, const mgcm_rt_columns *grids, const mgcm_rt_fluxes *outs,
                      int precision, hipStream_t st);
 void batch_free(void *state);
+// mgcm_rt_run's launches on a grid whose arguments are checked, with a
+// continuum-point workspace of clr_workspace_bytes(in->ncol, in->nlay) bytes.
+hipError_t run_grid(const mgcm_rt_ctx *c, const mgcm_rt_columns *in, const mgcm_rt_fluxes *out, int path,
+                    int bands, double *clr, hipStream_t st);
+void host_free(void *state);
 // DSOLFLUX + DNVFLUX of a grid (rt_surface.hip); dnvflux may be null.
 hipError_t launch_dsolflux(const RtConst *rc, int ncol, const double *sol, const double *acosz,
                            const double *detau, const double *diffvt, double *directsol, double *dnvflux,
