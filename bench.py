@@ -132,8 +132,7 @@ def traffic_from_profile(ncol, nlay):
 
 def form_mixed(args, rt_unused, stream):
     """Mixed path on 64,800 columns x 36 layers with cloud ice, against the FP64
-    path on the same columns; the conditioning floor (the FP64 oracle with the
-    k-tables rounded to FP32) is measured on the first 4,096 columns."""
+    path on the same columns."""
     import torch
     from mgcm_rt import FP64, MIXED, Radiation, alloc_outputs, synth, to_device
     ncol, nlay = 64800, 36
@@ -152,35 +151,29 @@ def form_mixed(args, rt_unused, stream):
            "unit": "columns/s", "ms_per_step": msm, "fp64_ms_per_step": ms64, "speedup_vs_fp64": ms64 / msm,
            "max_err_K_per_sol": float(err.max()), "rms_err_K_per_sol": float(err.pow(2).mean().sqrt()),
            "max_flux_err_W_m2": flux, "median_abs_hr_K_per_sol": float(d64["hr"].abs().median())}
-    if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        from oracle import Oracle
-        sub = _sample(c, 4096)
-        o = Oracle()
-        a = o.run(t, sub)
-        t32 = dict(t)
-        for k in ("co2v", "co2i"):
-            t32[k] = t[k].astype(np.float32).astype(np.float64)
-        b = o.run(t32, sub)
-        out["floor_max_err_K_per_sol"] = float(np.abs(a["hr"] - b["hr"]).max())
-        out["max_err_first4096_K_per_sol"] = float(err[:4096].max())
     del d64, dm, dc
     return out
 
 
-def form_real(args, rt_unused, stream):
-    """The plain FP64 path on the reference's own k-distribution: the records of
-    run/data/CO2H2O_V_12_95_INTEL and CO2H2O_IR_12_95_INTEL (carried as data by
-    tests/golden/column_real24.npz) written back to files, read by the library,
-    refined by the device LAGINTERP; the same seeded columns as the headline."""
+def _real_tables(nlay):
+    """The reference's k-distribution: the records of run/data/CO2H2O_V_12_95_INTEL
+    and CO2H2O_IR_12_95_INTEL (carried as data by tests/golden/column_real24.npz)
+    written back to files, read by the library, refined by the device LAGINTERP."""
     import tempfile
 
-    from mgcm_rt import FP64, Radiation, alloc_outputs, realdata, synth, to_device
+    from mgcm_rt import realdata
     f = np.load(os.path.join(ROOT, "tests", "golden", "column_real24.npz"))
     with tempfile.TemporaryDirectory() as d:
         realdata.write_ktable(os.path.join(d, realdata.VIS_FILE), f["raw_v"], f["fzero_v"])
         realdata.write_ktable(os.path.join(d, realdata.IR_FILE), f["raw_i"], f["fzero_i"])
-        t = realdata.load_tables(d, args.nlay)
+        return realdata.load_tables(d, nlay)
+
+
+def form_real(args, rt_unused, stream):
+    """The plain FP64 path on the reference's own k-distribution (_real_tables);
+    the same seeded columns as the headline."""
+    from mgcm_rt import FP64, Radiation, alloc_outputs, synth, to_device
+    t = _real_tables(args.nlay)
     c = synth.make_columns(args.seed, args.ncol, args.nlay)
     rt = Radiation(t)
     dc = to_device(c)
@@ -190,6 +183,34 @@ def form_real(args, rt_unused, stream):
     return {"workload": f"{args.ncol} columns x {args.nlay} layers, the reference's k-distribution "
                         "(CO2H2O_*_12_95_INTEL via LAGINTERP)",
             "value": args.ncol / (ms / 1e3), "unit": "columns/s", "ms_per_step": ms}
+
+
+def form_mixed_real(args, rt_unused, stream):
+    """The mixed path on the reference's own k-distribution (_real_tables), in the
+    mixed workload's shape (64,800 columns x 36 layers, cloud ice, the same seeded
+    columns as form_mixed), against the FP64 path on the same columns."""
+    from mgcm_rt import FP64, MIXED, Radiation, alloc_outputs, synth, to_device
+    ncol, nlay = 64800, 36
+    t = _real_tables(nlay)
+    c = synth.make_columns(args.seed + 1, ncol, nlay, clouds=True)
+    rt = Radiation(t)
+    dc = to_device(c)
+    d64, dm = to_device(alloc_outputs(ncol, nlay)), to_device(alloc_outputs(ncol, nlay))
+    sh = stream.cuda_stream
+    ms64, _ = timed_steps(lambda: rt.run_device(dc, d64, FP64, sh), args.steps, args.warmup, stream)
+    msm, _ = timed_steps(lambda: rt.run_device(dc, dm, MIXED, sh), args.steps, args.warmup, stream)
+    rt.check()
+    err = (dm["hr"] - d64["hr"]).abs()
+    flux = max(float((dm[k] - d64[k]).abs().max()) for k in ("fmnetv", "fmneti", "fluxupv", "fluxdnv",
+                                                             "fluxupi", "fluxdni"))
+    out = {"workload": f"{ncol} columns x {nlay} layers, cloud ice, the reference's k-distribution",
+           "value": ncol / (msm / 1e3), "unit": "columns/s", "ms_per_step": msm, "fp64_ms_per_step": ms64,
+           "speedup_vs_fp64": ms64 / msm, "max_err_K_per_sol": float(err.max()),
+           "rms_err_K_per_sol": float(err.pow(2).mean().sqrt()), "max_flux_err_W_m2": flux,
+           "median_abs_hr_K_per_sol": float(d64["hr"].abs().median()),
+           "max_abs_hr_K_per_sol": float(d64["hr"].abs().max())}
+    del d64, dm, dc
+    return out
 
 
 def form_batched(args, rt, stream):
@@ -533,7 +554,7 @@ def main():
     if not args.no_forms:
         del do
         for name, fn in (("real_tables", form_real), ("host_arrays", form_host_arrays), ("next_rows", form_next_rows),
-                         ("mixed", form_mixed), ("batched", form_batched), ("surrogate", form_surrogate)):
+                         ("mixed", form_mixed), ("mixed_real", form_mixed_real), ("batched", form_batched), ("surrogate", form_surrogate)):
             if args.forms and name not in args.forms.split(","):
                 continue
             try:

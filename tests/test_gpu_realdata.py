@@ -103,3 +103,25 @@ def test_real_tables_match_oracle(oracle, real, data_dir, nlay, ncol, seed):
     assert ev.max() <= 1e-10
     assert len(above) <= 0.01 * ncol
     assert np.all(ei <= np.maximum(1e-10, 4.0 * sens))
+
+
+def test_mixed_path_on_real_tables(oracle, data_dir):
+    """The mixed path (k-tables stored in FP32, FP64 arithmetic) on the reference's
+    k-distribution, 36 layers with cloud ice, against the FP64 oracle: within twice
+    the floor that rounding the tables to FP32 sets for an all-FP64 evaluation
+    (plus 0.05 K/sol), and RMS below 0.05 K/sol."""
+    from mgcm_rt import MIXED, Radiation, realdata, synth
+    t = realdata.load_tables(data_dir, 36)
+    c = synth.make_columns(41, 1024, 36, clouds=True)
+    ref = oracle.run(t, c)
+    t32 = dict(t)
+    for k in ("co2v", "co2i"):
+        t32[k] = t[k].astype(np.float32).astype(np.float64)
+    floor = float(np.abs(oracle.run(t32, c)["hr"] - ref["hr"]).max())
+    got = Radiation(t).run_host(c, precision=MIXED)
+    err = np.abs(got["hr"] - ref["hr"])
+    rms = float(np.sqrt((err ** 2).mean()))
+    print(f"real tables, 36 layers: mixed max {err.max():.3e} K/sol rms {rms:.3e} (floor {floor:.3e}); "
+          f"|hr| median {np.median(np.abs(ref['hr'])):.3e} max {np.abs(ref['hr']).max():.3e} K/sol", flush=True)
+    assert err.max() <= 2.0 * floor + 0.05
+    assert rms <= 0.05
