@@ -45,16 +45,14 @@ constexpr unsigned PIN_IN_FLAGS = hipHostMallocDefault, PIN_OUT_FLAGS = hipHostM
 class CopyTeam {
  public:
   explicit CopyTeam(int n) : n_(n) {
-    for (int p = 1; p < n; p++) th_.emplace_back([this, p] { work(p); });
-  }
-  ~CopyTeam() {
-    {
-      std::lock_guard<std::mutex> l(m_);
-      stop_ = true;
+    try {
+      for (int p = 1; p < n; p++) th_.emplace_back([this, p] { work(p); });
+    } catch (...) {  // a thread that would not start: end the others, then report
+      stop();
+      throw;
     }
-    go_.notify_all();
-    for (auto &t : th_) t.join();
   }
+  ~CopyTeam() { stop(); }
   int size() const { return n_; }
   void run(const std::function<void(int)> &job) {
     {
@@ -86,6 +84,15 @@ class CopyTeam {
       std::lock_guard<std::mutex> l(m_);
       if (--pending_ == 0) done_.notify_one();
     }
+  }
+  void stop() {
+    {
+      std::lock_guard<std::mutex> l(m_);
+      stop_ = true;
+    }
+    go_.notify_all();
+    for (auto &t : th_) t.join();
+    th_.clear();
   }
   const int n_;
   std::vector<std::thread> th_;
@@ -142,7 +149,11 @@ HostPipe *pipe_get(mgcm_rt_ctx *c) {
          hipEventCreateWithFlags(&h->ev_down[s], hipEventDisableTiming) == hipSuccess;
   if (ok) {
     const unsigned hw = std::thread::hardware_concurrency();
-    h->team = new (std::nothrow) CopyTeam(std::max(1, std::min<int>(HOST_THREADS, hw ? (int)hw : 1)));
+    try {
+      h->team = new CopyTeam(std::max(1, std::min<int>(HOST_THREADS, hw ? (int)hw : 1)));
+    } catch (...) {
+      h->team = nullptr;
+    }
     ok = h->team != nullptr;
   }
   if (!ok) {
@@ -198,8 +209,8 @@ void host_free(void *p) {
 
 }  // namespace mgcm
 
-extern "C" int mgcm_rt_run_host(mgcm_rt_ctx *c, const mgcm_rt_columns *in, const mgcm_rt_fluxes *out,
-                                int precision) {
+namespace {
+int run_host(mgcm_rt_ctx *c, const mgcm_rt_columns *in, const mgcm_rt_fluxes *out, int precision) {
   if (!c) return MGCM_RT_ERR_ARG;
   int s = check_columns(in, out);
   if (s != MGCM_RT_OK) return s;
@@ -278,6 +289,7 @@ extern "C" int mgcm_rt_run_host(mgcm_rt_ctx *c, const mgcm_rt_columns *in, const
   auto chunk_cols = [&](size_t i) { return std::min(K, ncol - i * K); };
   auto region = [&](size_t i) { return c->d_host + i * K * region_pc; };  // chunks before i are all K wide
   std::vector<Seg> segs;
+  segs.reserve(na);  // no allocation once copies are in flight
   int st = MGCM_RT_OK;
   for (size_t i = 0; i <= nchunk && st == MGCM_RT_OK; i++) {
     if (i < nchunk) {
@@ -341,4 +353,15 @@ extern "C" int mgcm_rt_run_host(mgcm_rt_ctx *c, const mgcm_rt_columns *in, const
     return st;
   }
   return mgcm_rt_check(c);
+}
+}  // namespace
+
+extern "C" int mgcm_rt_run_host(mgcm_rt_ctx *c, const mgcm_rt_columns *in, const mgcm_rt_fluxes *out,
+                                int precision) {
+  try {
+    return run_host(c, in, out, precision);
+  } catch (...) {  // host memory exhausted: nothing of the call may stay in flight
+    (void)hipDeviceSynchronize();
+    return MGCM_RT_ERR_NOMEM;
+  }
 }
