@@ -110,11 +110,17 @@ struct Seg {
   size_t bytes;
 };
 
+constexpr size_t TEAM_MIN_BYTES = 1u << 20;  // smaller copies stay on the calling thread
+
 // Copies the concatenation of segs split into equal byte shares over the team.
 void team_copy(CopyTeam &team, const std::vector<Seg> &segs) {
   size_t total = 0;
   for (const Seg &s : segs) total += s.bytes;
   if (total == 0) return;
+  if (total < TEAM_MIN_BYTES) {  // waking the team would cost more than the copy
+    for (const Seg &s : segs) memcpy(s.dst, s.src, s.bytes);
+    return;
+  }
   const int n = team.size();
   team.run([&](int p) {
     size_t lo = total * p / n, hi = total * (p + 1) / n, base = 0;
