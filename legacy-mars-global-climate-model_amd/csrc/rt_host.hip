@@ -2,20 +2,19 @@
 // Fortran host's COMP3 replacement calls (code/comp3.f:637-738 over a grid).
 //
 // The grid is cut into chunks of whole columns.  Chunk i goes
-//   host arrays -> pinned slot (host threads) -> device (stream up)
-//   -> mgcm_rt_run on the chunk (stream run) -> pinned slot (stream down)
+//   host arrays -> pinned slot (host threads) -> device (stream up, DMA)
+//   -> mgcm_rt_run on the chunk (stream run) -> pinned slot (stream down, DMA)
 //   -> host arrays (host threads),
-// with two pinned slots each way, so that the host copies of chunk i+1 and
-// chunk i-1 overlap the run of chunk i and the DMA of its neighbours.  Every
+// with two pinned slots each way.  The host keeps two chunks queued ahead of
+// the one it drains, so its copies overlap the runs and the runs follow one
+// another on the device without a gap.  Every
 // chunk has its own region of the device staging buffer, so the device side
 // has no reuse hazards; the pinned slots are handed back by events.  A chunk is
 // a sub-grid: its columns are computed exactly as in a whole-grid run (columns
 // are independent), so the outputs are the same bits.
 //
-// The upload goes by the DMA engine; the runtime does the download with a copy
-// kernel, which takes about as much GPU time from the next chunk's run as the
-// copy lasts, so a call costs the runs plus the downloads plus one chunk's
-// upload and one chunk's host copies.
+// A call costs the chunks' runs plus one chunk's upload and host copy before
+// the first run and one chunk's download and host copy after the last.
 #include <algorithm>
 #include <condition_variable>
 #include <functional>
@@ -296,66 +295,80 @@ int run_host(mgcm_rt_ctx *c, const mgcm_rt_columns *in, const mgcm_rt_fluxes *ou
   auto region = [&](size_t i) { return c->d_host + i * K * region_pc; };  // chunks before i are all K wide
   std::vector<Seg> segs;
   segs.reserve(na);  // no allocation once copies are in flight
+  // host -> pinned in-slot i % 2 -> device, then the run; the slot is free once
+  // the upload of chunk i-2 has ended
+  auto start = [&](size_t i) -> int {
+    const int slot = (int)(i & 1);
+    const size_t k = chunk_cols(i), col0 = i * K;
+    double *r = region(i);
+    if (i >= 2 && hipEventSynchronize(h->ev_up[slot]) != hipSuccess) return MGCM_RT_ERR_DEVICE;
+    segs.clear();
+    for (int j = 0; j < io1; j++)
+      segs.push_back({h->pin_in[slot] + k * off[j] * 8, (const char *)(a[j].h_in + col0 * a[j].pc), k * a[j].pc * 8});
+    team_copy(*h->team, segs);
+    mgcm_rt_columns din = *in;
+    mgcm_rt_fluxes dout = {};
+    auto dev = [&](int j) { return r + k * off[j]; };
+    din.ncol = (int)k;
+    din.plev = dev(0); din.tlev = dev(1); din.pmid = dev(2); din.tmid = dev(3); din.qh2o = dev(4);
+    din.tauref = dev(5); din.taurefcld = dev(6); din.als = dev(7); din.albi = dev(8); din.acosz = dev(9);
+    if (hae) {
+      for (int q = 0; q < 14; q++) *aer_dev[q] = aer_at[q] >= 0 ? dev(aer_at[q]) : nullptr;
+      din.aerosol = &dae;
+    }
+    dout.detau = dev(io0); dout.cumtauv = dev(io0 + 1);
+    dout.fmnetv = dev(io1); dout.fluxupv = dev(io1 + 1); dout.fluxdnv = dev(io1 + 2);
+    dout.fmneti = dev(io1 + 3); dout.fluxupi = dev(io1 + 4); dout.fluxdni = dev(io1 + 5);
+    dout.nfluxtopv = dev(io1 + 6); dout.nfluxtopi = dev(io1 + 7); dout.diffvt = dev(io1 + 8);
+    dout.hr = hr_at >= 0 ? dev(hr_at) : nullptr;
+    dout.directsol = ds_at >= 0 ? dev(ds_at) : nullptr;
+    dout.dnvflux = dn_at >= 0 ? dev(dn_at) : nullptr;
+    if (hipMemcpyAsync(r, h->pin_in[slot], k * in_pc * 8, hipMemcpyHostToDevice, h->up) != hipSuccess ||
+        hipEventRecord(h->ev_up[slot], h->up) != hipSuccess || hipStreamWaitEvent(h->run, h->ev_up[slot], 0) != hipSuccess)
+      return MGCM_RT_ERR_DEVICE;
+    const int rs = mgcm_rt_run(c, &din, &dout, precision, h->run);
+    if (rs != MGCM_RT_OK) return rs;
+    return hipEventRecord(h->ev_run[slot], h->run) == hipSuccess ? MGCM_RT_OK : MGCM_RT_ERR_DEVICE;
+  };
+  // device -> pinned out-slot i % 2 (free: chunk i-2 has been drained).  A
+  // pitched copy, which the runtime hands to the DMA engine; a plain one it
+  // does with a copy kernel that slows the next chunk's run.
+  auto fetch = [&](size_t i) -> int {
+    const int slot = (int)(i & 1);
+    const size_t k = chunk_cols(i), row = out_pc * 8;  // k rows of one column's worth of bytes
+    return hipStreamWaitEvent(h->down, h->ev_run[slot], 0) == hipSuccess &&
+                   hipMemcpy2DAsync(h->pin_out[slot], row, region(i) + k * off[io0], row, row, k,
+                                    hipMemcpyDeviceToHost, h->down) == hipSuccess &&
+                   hipEventRecord(h->ev_down[slot], h->down) == hipSuccess
+               ? MGCM_RT_OK
+               : MGCM_RT_ERR_DEVICE;
+  };
+  // pinned out-slot i % 2 -> the caller's arrays
+  auto drain = [&](size_t i) -> int {
+    const int slot = (int)(i & 1);
+    const size_t k = chunk_cols(i), col0 = i * K;
+    if (hipEventSynchronize(h->ev_down[slot]) != hipSuccess) return MGCM_RT_ERR_DEVICE;
+    segs.clear();
+    for (int j = io0; j < na; j++)
+      segs.push_back({(char *)(a[j].h_out + col0 * a[j].pc), h->pin_out[slot] + k * (off[j] - off[io0]) * 8,
+                      k * a[j].pc * 8});
+    team_copy(*h->team, segs);
+    return MGCM_RT_OK;
+  };
+  // two chunks ahead: chunk i+2 is uploaded and queued before chunk i is
+  // drained, so the device never waits for the host's copies
   int st = MGCM_RT_OK;
-  for (size_t i = 0; i <= nchunk && st == MGCM_RT_OK; i++) {
-    if (i < nchunk) {
-      const int slot = (int)(i & 1);
-      const size_t k = chunk_cols(i), col0 = i * K;
-      double *r = region(i);
-      if (i >= 2 && hipEventSynchronize(h->ev_up[slot]) != hipSuccess) { st = MGCM_RT_ERR_DEVICE; break; }
-      segs.clear();
-      for (int j = 0; j < io1; j++)
-        segs.push_back({h->pin_in[slot] + k * off[j] * 8, (const char *)(a[j].h_in + col0 * a[j].pc),
-                        k * a[j].pc * 8});
-      team_copy(*h->team, segs);
-      mgcm_rt_columns din = *in;
-      mgcm_rt_fluxes dout = {};
-      auto dev = [&](int j) { return r + k * off[j]; };
-      din.ncol = (int)k;
-      din.plev = dev(0); din.tlev = dev(1); din.pmid = dev(2); din.tmid = dev(3); din.qh2o = dev(4);
-      din.tauref = dev(5); din.taurefcld = dev(6); din.als = dev(7); din.albi = dev(8); din.acosz = dev(9);
-      if (hae) {
-        for (int q = 0; q < 14; q++) *aer_dev[q] = aer_at[q] >= 0 ? dev(aer_at[q]) : nullptr;
-        din.aerosol = &dae;
-      }
-      dout.detau = dev(io0); dout.cumtauv = dev(io0 + 1);
-      dout.fmnetv = dev(io1); dout.fluxupv = dev(io1 + 1); dout.fluxdnv = dev(io1 + 2);
-      dout.fmneti = dev(io1 + 3); dout.fluxupi = dev(io1 + 4); dout.fluxdni = dev(io1 + 5);
-      dout.nfluxtopv = dev(io1 + 6); dout.nfluxtopi = dev(io1 + 7); dout.diffvt = dev(io1 + 8);
-      dout.hr = hr_at >= 0 ? dev(hr_at) : nullptr;
-      dout.directsol = ds_at >= 0 ? dev(ds_at) : nullptr;
-      dout.dnvflux = dn_at >= 0 ? dev(dn_at) : nullptr;
-      if (hipMemcpyAsync(r, h->pin_in[slot], k * in_pc * 8, hipMemcpyHostToDevice, h->up) != hipSuccess ||
-          hipEventRecord(h->ev_up[slot], h->up) != hipSuccess ||
-          hipStreamWaitEvent(h->run, h->ev_up[slot], 0) != hipSuccess) {
-        st = MGCM_RT_ERR_DEVICE;
-        break;
-      }
-      st = mgcm_rt_run(c, &din, &dout, precision, h->run);
-      if (st != MGCM_RT_OK) break;
-      if (hipEventRecord(h->ev_run[slot], h->run) != hipSuccess ||
-          hipStreamWaitEvent(h->down, h->ev_run[slot], 0) != hipSuccess ||
-          hipMemcpyAsync(h->pin_out[slot], dev(io0), k * out_pc * 8, hipMemcpyDeviceToHost, h->down) != hipSuccess ||
-          hipEventRecord(h->ev_down[slot], h->down) != hipSuccess) {
-        st = MGCM_RT_ERR_DEVICE;
-        break;
-      }
-    }
-    if (i >= 1) {  // chunk i-1 back to the caller's arrays
-      const size_t j0 = i - 1, k = chunk_cols(j0), col0 = j0 * K;
-      const int slot = (int)(j0 & 1);
-      if (hipEventSynchronize(h->ev_down[slot]) != hipSuccess) { st = MGCM_RT_ERR_DEVICE; break; }
-      segs.clear();
-      for (int j = io0; j < na; j++)
-        segs.push_back({(char *)(a[j].h_out + col0 * a[j].pc), h->pin_out[slot] + k * (off[j] - off[io0]) * 8,
-                        k * a[j].pc * 8});
-      team_copy(*h->team, segs);
-    }
+  for (size_t i = 0; i < std::min<size_t>(2, nchunk) && st == MGCM_RT_OK; i++) {
+    st = start(i);
+    if (st == MGCM_RT_OK) st = fetch(i);
+  }
+  for (size_t i = 0; i < nchunk && st == MGCM_RT_OK; i++) {
+    if (i + 2 < nchunk) st = start(i + 2);
+    if (st == MGCM_RT_OK) st = drain(i);
+    if (st == MGCM_RT_OK && i + 2 < nchunk) st = fetch(i + 2);
   }
   if (st != MGCM_RT_OK) {  // nothing in flight may outlive the call
-    (void)hipStreamSynchronize(h->up);
-    (void)hipStreamSynchronize(h->run);
-    (void)hipStreamSynchronize(h->down);
+    (void)hipDeviceSynchronize();
     return st;
   }
   return mgcm_rt_check(c);
