@@ -2,16 +2,19 @@
 // Fortran host's COMP3 replacement calls (code/comp3.f:637-738 over a grid).
 //
 // The grid is cut into chunks of whole columns.  Chunk i goes
-//   host arrays -> pinned slot (host threads) -> device (stream up, DMA)
-//   -> mgcm_rt_run on the chunk (stream run) -> pinned slot (stream down, DMA)
-//   -> host arrays (host threads),
+//   host arrays -> pinned slot (host threads) -> device (copy stream, DMA)
+//   -> the radiation on the chunk (run stream i % 2)
+//   -> pinned slot (copy stream, DMA) -> host arrays (host threads),
 // with two pinned slots each way.  The host keeps two chunks queued ahead of
-// the one it drains, so its copies overlap the runs and the runs follow one
-// another on the device without a gap.  Every
-// chunk has its own region of the device staging buffer, so the device side
-// has no reuse hazards; the pinned slots are handed back by events.  A chunk is
-// a sub-grid: its columns are computed exactly as in a whole-grid run (columns
-// are independent), so the outputs are the same bits.
+// the one it drains, so its copies overlap the runs.  The two run streams, each
+// with its own continuum-point workspace, let chunk i+1's kernels start while
+// chunk i's last waves finish.  Uploads and downloads share one stream: a
+// process has 4 hardware queues, and a fifth stream would put both run
+// streams on one queue.  Every chunk has its own region of the device staging
+// buffer, so the device side has no reuse hazards; pinned slots and workspaces
+// are handed back by stream order and events.  A chunk is a sub-grid: its
+// columns are computed exactly as in a whole-grid run (columns are
+// independent), so the outputs are the same bits.
 //
 // A call costs the chunks' runs plus one chunk's upload and host copy before
 // the first run and one chunk's download and host copy after the last.
@@ -134,8 +137,10 @@ void team_copy(CopyTeam &team, const std::vector<Seg> &segs) {
 
 struct HostPipe {
   CopyTeam *team = nullptr;
-  hipStream_t up = nullptr, run = nullptr, down = nullptr;
-  hipEvent_t ev_up[2] = {}, ev_run[2] = {}, ev_down[2] = {};
+  hipStream_t up = nullptr, run[2] = {};  // up: the copy stream, both directions
+  hipEvent_t ev_up[2] = {}, ev_run[4] = {}, ev_down[2] = {};  // ev_run: chunk i % 4
+  double *clr[2] = {};  // continuum-point workspace of each run stream
+  size_t clr_bytes = 0;
   char *pin_in[2] = {}, *pin_out[2] = {};
   size_t pin_in_bytes = 0, pin_out_bytes = 0;
 };
@@ -146,11 +151,11 @@ HostPipe *pipe_get(mgcm_rt_ctx *c) {
   if (!h) return nullptr;
   c->host = h;  // freed by host_free, also when set-up stops half way
   bool ok = hipStreamCreateWithFlags(&h->up, hipStreamNonBlocking) == hipSuccess &&
-            hipStreamCreateWithFlags(&h->run, hipStreamNonBlocking) == hipSuccess &&
-            hipStreamCreateWithFlags(&h->down, hipStreamNonBlocking) == hipSuccess;
+            hipStreamCreateWithFlags(&h->run[0], hipStreamNonBlocking) == hipSuccess &&
+            hipStreamCreateWithFlags(&h->run[1], hipStreamNonBlocking) == hipSuccess;
+  for (int s = 0; ok && s < 4; s++) ok = hipEventCreateWithFlags(&h->ev_run[s], hipEventDisableTiming) == hipSuccess;
   for (int s = 0; ok && s < 2; s++)
     ok = hipEventCreateWithFlags(&h->ev_up[s], hipEventDisableTiming) == hipSuccess &&
-         hipEventCreateWithFlags(&h->ev_run[s], hipEventDisableTiming) == hipSuccess &&
          hipEventCreateWithFlags(&h->ev_down[s], hipEventDisableTiming) == hipSuccess;
   if (ok) {
     const unsigned hw = std::thread::hardware_concurrency();
@@ -202,13 +207,14 @@ void host_free(void *p) {
   for (int s = 0; s < 2; s++) {
     if (h->pin_in[s]) (void)hipHostFree(h->pin_in[s]);
     if (h->pin_out[s]) (void)hipHostFree(h->pin_out[s]);
+    if (h->clr[s]) (void)hipFree(h->clr[s]);
     if (h->ev_up[s]) (void)hipEventDestroy(h->ev_up[s]);
-    if (h->ev_run[s]) (void)hipEventDestroy(h->ev_run[s]);
     if (h->ev_down[s]) (void)hipEventDestroy(h->ev_down[s]);
+    if (h->run[s]) (void)hipStreamDestroy(h->run[s]);
   }
+  for (hipEvent_t e : h->ev_run)
+    if (e) (void)hipEventDestroy(e);
   if (h->up) (void)hipStreamDestroy(h->up);
-  if (h->run) (void)hipStreamDestroy(h->run);
-  if (h->down) (void)hipStreamDestroy(h->down);
   delete h;
 }
 
@@ -289,7 +295,17 @@ int run_host(mgcm_rt_ctx *c, const mgcm_rt_columns *in, const mgcm_rt_fluxes *ou
     if (hipMalloc(&c->d_host, bytes) != hipSuccess) return MGCM_RT_ERR_NOMEM;
     c->host_bytes = bytes;
   }
-  HIP_TRY(hipMemsetAsync(c->d_flag, 0, sizeof(int), h->run));
+  if (clr_workspace_bytes((int)K, in->nlay) > h->clr_bytes) {
+    for (double *&p : h->clr) {
+      if (p) (void)hipFree(p);
+      p = nullptr;
+    }
+    h->clr_bytes = 0;
+    for (double *&p : h->clr)
+      if (hipMalloc((void **)&p, clr_workspace_bytes((int)K, in->nlay)) != hipSuccess) return MGCM_RT_ERR_NOMEM;
+    h->clr_bytes = clr_workspace_bytes((int)K, in->nlay);
+  }
+  HIP_TRY(hipMemsetAsync(c->d_flag, 0, sizeof(int), h->up));  // every run waits on the copy stream
 
   auto chunk_cols = [&](size_t i) { return std::min(K, ncol - i * K); };
   auto region = [&](size_t i) { return c->d_host + i * K * region_pc; };  // chunks before i are all K wide
@@ -324,22 +340,22 @@ int run_host(mgcm_rt_ctx *c, const mgcm_rt_columns *in, const mgcm_rt_fluxes *ou
     dout.directsol = ds_at >= 0 ? dev(ds_at) : nullptr;
     dout.dnvflux = dn_at >= 0 ? dev(dn_at) : nullptr;
     if (hipMemcpyAsync(r, h->pin_in[slot], k * in_pc * 8, hipMemcpyHostToDevice, h->up) != hipSuccess ||
-        hipEventRecord(h->ev_up[slot], h->up) != hipSuccess || hipStreamWaitEvent(h->run, h->ev_up[slot], 0) != hipSuccess)
+        hipEventRecord(h->ev_up[slot], h->up) != hipSuccess || hipStreamWaitEvent(h->run[slot], h->ev_up[slot], 0) != hipSuccess)
       return MGCM_RT_ERR_DEVICE;
-    const int rs = mgcm_rt_run(c, &din, &dout, precision, h->run);
-    if (rs != MGCM_RT_OK) return rs;
-    return hipEventRecord(h->ev_run[slot], h->run) == hipSuccess ? MGCM_RT_OK : MGCM_RT_ERR_DEVICE;
+    if (run_grid(c, &din, &dout, path, bands, h->clr[slot], h->run[slot]) != hipSuccess) return MGCM_RT_ERR_DEVICE;
+    return hipEventRecord(h->ev_run[i & 3], h->run[slot]) == hipSuccess ? MGCM_RT_OK : MGCM_RT_ERR_DEVICE;
   };
-  // device -> pinned out-slot i % 2 (free: chunk i-2 has been drained).  A
+  // device -> pinned out-slot i % 2 (free: chunk i-2 has been drained), after
+  // chunk i's run and chunk i+2's upload on the copy stream.  A
   // pitched copy, which the runtime hands to the DMA engine; a plain one it
   // does with a copy kernel that slows the next chunk's run.
   auto fetch = [&](size_t i) -> int {
     const int slot = (int)(i & 1);
     const size_t k = chunk_cols(i), row = out_pc * 8;  // k rows of one column's worth of bytes
-    return hipStreamWaitEvent(h->down, h->ev_run[slot], 0) == hipSuccess &&
+    return hipStreamWaitEvent(h->up, h->ev_run[i & 3], 0) == hipSuccess &&
                    hipMemcpy2DAsync(h->pin_out[slot], row, region(i) + k * off[io0], row, row, k,
-                                    hipMemcpyDeviceToHost, h->down) == hipSuccess &&
-                   hipEventRecord(h->ev_down[slot], h->down) == hipSuccess
+                                    hipMemcpyDeviceToHost, h->up) == hipSuccess &&
+                   hipEventRecord(h->ev_down[slot], h->up) == hipSuccess
                ? MGCM_RT_OK
                : MGCM_RT_ERR_DEVICE;
   };
@@ -356,16 +372,13 @@ int run_host(mgcm_rt_ctx *c, const mgcm_rt_columns *in, const mgcm_rt_fluxes *ou
     return MGCM_RT_OK;
   };
   // two chunks ahead: chunk i+2 is uploaded and queued before chunk i is
-  // drained, so the device never waits for the host's copies
+  // fetched and drained, so the device never waits for the host's copies
   int st = MGCM_RT_OK;
-  for (size_t i = 0; i < std::min<size_t>(2, nchunk) && st == MGCM_RT_OK; i++) {
-    st = start(i);
-    if (st == MGCM_RT_OK) st = fetch(i);
-  }
+  for (size_t i = 0; i < std::min<size_t>(2, nchunk) && st == MGCM_RT_OK; i++) st = start(i);
   for (size_t i = 0; i < nchunk && st == MGCM_RT_OK; i++) {
     if (i + 2 < nchunk) st = start(i + 2);
+    if (st == MGCM_RT_OK) st = fetch(i);
     if (st == MGCM_RT_OK) st = drain(i);
-    if (st == MGCM_RT_OK && i + 2 < nchunk) st = fetch(i + 2);
   }
   if (st != MGCM_RT_OK) {  // nothing in flight may outlive the call
     (void)hipDeviceSynchronize();
