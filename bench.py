@@ -105,29 +105,67 @@ def cpu_baseline(seed, cols, nsample):
                       f"reference OPTCV/SFLUXV/OPTCI/SFLUXI (amdflang -O2 build), one core, {dt:.2f} s wall"}
 
 
-def cpu_port(tables, cols):
-    """The C restatement with OpenMP on every host core, over the whole grid."""
+def cpu_port(tables, cols, keep=None):
+    """The C restatement with OpenMP on every host core, over the whole grid.  With
+    keep (a dict), its outputs are stored there for the parity check."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import Oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
     o = Oracle()
     o.run(tables, _sample(cols, 64), nthreads=threads)
     t0 = time.perf_counter()
-    o.run(tables, cols, nthreads=threads)
+    out = o.run(tables, cols, nthreads=threads)
     dt = time.perf_counter() - t0
+    if keep is not None:
+        keep.update(out)
     return {"value": int(cols["ncol"]) / dt, "unit": "columns/s", "cores": threads, "kind": "port",
             "sample": f"all {cols['ncol']} columns, C restatement with OpenMP, {dt:.2f} s wall"}
 
 
-def traffic_from_profile(ncol, nlay):
-    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary."""
-    path = os.path.join(ROOT, "profiles", f"k_rt_fp64_{ncol}x{nlay}_hbm.txt")
-    if not os.path.exists(path):
-        return None
-    for line in open(path):
-        if line.startswith("traffic_bytes_per_launch"):
-            return float(line.split("=")[1].split()[0])
-    return None
+PARITY_KEYS = ("fmnetv", "fluxupv", "fluxdnv", "fmneti", "fluxupi", "fluxdni", "nfluxtopv", "nfluxtopi",
+               "diffvt", "detau", "cumtauv", "hr")
+
+
+def parity(got: dict, want: dict) -> dict:
+    """Max over the grid of |got - want| / max(|want|, the column's max |want|), per output."""
+    errs = {}
+    for k in PARITY_KEYS:
+        g, w = np.asarray(got[k]), np.asarray(want[k])
+        g2, w2 = g.reshape(g.shape[0], -1), w.reshape(w.shape[0], -1)
+        s = np.max(np.abs(w2), axis=1, keepdims=True)
+        errs[k] = float(np.max(np.abs(g2 - w2) / np.maximum(np.maximum(np.abs(w2), s), 1e-300)))
+    return {"max_rel_err": max(errs.values()), "per_output": errs, "tolerance": 1e-10,
+            "pass": all(v <= 1e-10 for v in errs.values())}
+
+
+def library_sha256():
+    import hashlib
+    from mgcm_rt import LIB_PATH
+    with open(LIB_PATH, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()
+
+
+def profile_figures(ncol, nlay):
+    """Figures of the committed rocprofv3 summaries of this step, used only when they
+    were measured on the library that is loaded (their library_sha256 line): HBM bytes
+    per step (FETCH_SIZE/WRITE_SIZE, calibrated) and the FP64 VALU operations the
+    counters saw (SQ_INSTS_VALU_FLOPS_FP64)."""
+    sha = library_sha256()
+    res = {"traffic": None, "counter_fp64_flop": None, "profile_library_matches": False}
+    for name, key, tag in ((f"k_rt_fp64_{ncol}x{nlay}_hbm.txt", "traffic", "traffic_bytes_per_launch"),
+                           (f"k_rt_fp64_{ncol}x{nlay}.txt", "counter_fp64_flop", "counter_fp64_flop_per_step")):
+        path = os.path.join(ROOT, "profiles", name)
+        if not os.path.exists(path):
+            continue
+        lines = open(path).read().splitlines()
+        stamp = [ln.split("=")[1].strip() for ln in lines if ln.startswith("library_sha256")]
+        if not stamp or stamp[0] != sha:
+            continue
+        res["profile_library_matches"] = True
+        for ln in lines:
+            if ln.startswith(tag):
+                res[key] = float(ln.split("=")[1].split()[0])
+    return res
 
 
 def form_mixed(args, rt_unused, stream):
@@ -513,7 +551,9 @@ def main():
 
     fpc = OPS_PER_COLUMN.get((args.seed, args.nlay), {"flops": None, "exps": None, "sqrts": None})
     achieved = None if fpc["flops"] is None else fpc["flops"] * args.ncol / (ms / 1e3) / 1e12
-    traffic = traffic_from_profile(args.ncol, args.nlay)
+    prof = profile_figures(args.ncol, args.nlay)
+    traffic = prof["traffic"]
+    got = {k: do[k].cpu().numpy() for k in PARITY_KEYS}
     result = {
         "metric": "radiative columns/sec (plain FP64 column radiation, 64,800 columns x 24 layers)",
         "value": value,
@@ -531,13 +571,20 @@ def main():
                                "7 VIS + 5 IR bands x 17 Gauss points",
                    "ncol": args.ncol, "nlay": args.nlay, "seed": args.seed, "launches_per_step": 2,
                    "launches": "k_rt_clear2 (continuum point g=17, VIS and IR bands), k_rt (column kernel)"},
-        "roofline": {"bound": "mfma", "pipe": "FP64 vector ALU (MI355X FP64 vector peak = matrix peak)",
+        "roofline": {"bound": "valu", "pipe": "FP64 vector ALU (no matrix work on this path); "
+                                                "peak = MI355X FP64 vector 78.6 TFLOP/s",
                      "scope": "the whole step (its 2 launches); flops and traffic summed over them",
                      "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": None if achieved is None else achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
                      "hbm_GBps": None if traffic is None else traffic / (ms / 1e3) / 1e9,
                      "flops_per_column": fpc["flops"], "exp_per_column": fpc["exps"],
-                     "sqrt_per_column": fpc["sqrts"], "kernel_ms": ms},
+                     "sqrt_per_column": fpc["sqrts"], "kernel_ms": ms,
+                     "counter_fp64_TFLOPs": (None if prof["counter_fp64_flop"] is None
+                                             else prof["counter_fp64_flop"] / (ms / 1e3) / 1e12),
+                     "counter_note": "executed FP64 VALU operations per lane (SQ_INSTS_VALU_FLOPS_FP64 x 64: "
+                                     "add/mul/transcendental 1, FMA 2), including the expansions of "
+                                     "division, sqrt and exp; from the committed profile of the same library",
+                     "profile_library_matches": prof["profile_library_matches"]},
         "wall_ms_per_step": wall_ms,
     }
 
@@ -570,7 +617,9 @@ def main():
         except (FileNotFoundError, RuntimeError) as e:
             print(f"bench.py: reference build unavailable ({e}); timing the C restatement", file=sys.stderr)
             result["cpu_baseline"] = cpu_port(tables, _sample(cols, args.cpu_sample))
-        result["cpu_port"] = cpu_port(tables, cols)
+        want = {}
+        result["cpu_port"] = cpu_port(tables, cols, keep=want)
+        result["parity"] = parity(got, want)
     print(json.dumps(result), flush=True)
 
 

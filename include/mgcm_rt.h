@@ -303,11 +303,13 @@ typedef struct {
 int mgcm_rt_history_fields(const mgcm_rt_columns *in, const mgcm_rt_fluxes *out, const mgcm_rt_history *hist,
                            void *stream);
 
-/* Diagnostic: the device's exp, log10 and sqrt on n host arguments (x > 0 for
- * log10, x >= 0 for sqrt; other entries return 0).  The FP64 path reproduces the
- * reference's libm (exp and log10 of glibc 2.35, correctly rounded sqrt); this
- * entry lets a host compare them.  No counterpart in the reference. */
-int mgcm_rt_libm_check(const double *x, int n, double *exp_out, double *log10_out, double *sqrt_out);
+/* Diagnostic: the device's exp, log10, sqrt and log on n host arguments (x > 0
+ * for log10 and log, x >= 0 for sqrt; other entries return 0).  The FP64 path
+ * reproduces the reference's libm (exp, log and log10 of glibc 2.35, correctly
+ * rounded sqrt); this entry lets a host compare them bit for bit.  No
+ * counterpart in the reference. */
+int mgcm_rt_libm_check(const double *x, int n, double *exp_out, double *log10_out, double *sqrt_out,
+                       double *log_out);
 
 const char *mgcm_rt_strerror(int status);
 

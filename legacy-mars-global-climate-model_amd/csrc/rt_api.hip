@@ -378,29 +378,32 @@ extern "C" int mgcm_rt_laginterp(const double *pgref, const double *raw, int nba
 
 // Device results of the transcendental functions of the FP64 path, for the
 // parity check against the host C library (mgcm_rt_libm_check).
-__global__ void k_libm(const double *x, int n, double *e, double *l, double *q) {
+__global__ void k_libm(const double *x, int n, double *e, double *l, double *q, double *ln) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   e[i] = mgcm_libm::mgcm_exp(x[i]);
   l[i] = (x[i] > 0.0) ? mgcm_libm::mgcm_log10(x[i]) : 0.0;
   q[i] = (x[i] >= 0.0) ? sqrt(x[i]) : 0.0;
+  ln[i] = (x[i] > 0.0) ? mgcm_libm::mgcm_log(x[i]) : 0.0;
 }
 
-extern "C" int mgcm_rt_libm_check(const double *x, int n, double *exp_out, double *log10_out, double *sqrt_out) {
-  if (n < 0 || (n > 0 && (!x || !exp_out || !log10_out || !sqrt_out))) return MGCM_RT_ERR_ARG;
+extern "C" int mgcm_rt_libm_check(const double *x, int n, double *exp_out, double *log10_out, double *sqrt_out,
+                                  double *log_out) {
+  if (n < 0 || (n > 0 && (!x || !exp_out || !log10_out || !sqrt_out || !log_out))) return MGCM_RT_ERR_ARG;
   if (n == 0) return MGCM_RT_OK;
   double *d = nullptr;
   const size_t b = (size_t)n * sizeof(double);
-  HIP_TRY(hipMalloc(&d, 4 * b));
+  HIP_TRY(hipMalloc(&d, 5 * b));
   int st = MGCM_RT_OK;
   if (hipMemcpy(d, x, b, hipMemcpyHostToDevice) != hipSuccess) st = MGCM_RT_ERR_DEVICE;
   if (st == MGCM_RT_OK) {
     hipLaunchKernelGGL(k_libm, dim3((n + 255) / 256), dim3(256), 0, nullptr, d, n, d + n, d + 2 * (size_t)n,
-                       d + 3 * (size_t)n);
+                       d + 3 * (size_t)n, d + 4 * (size_t)n);
     if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
         hipMemcpy(exp_out, d + n, b, hipMemcpyDeviceToHost) != hipSuccess ||
         hipMemcpy(log10_out, d + 2 * (size_t)n, b, hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(sqrt_out, d + 3 * (size_t)n, b, hipMemcpyDeviceToHost) != hipSuccess)
+        hipMemcpy(sqrt_out, d + 3 * (size_t)n, b, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(log_out, d + 4 * (size_t)n, b, hipMemcpyDeviceToHost) != hipSuccess)
       st = MGCM_RT_ERR_DEVICE;
   }
   (void)hipFree(d);

@@ -14,11 +14,15 @@
 //    On FMA-capable x86-64 hosts glibc runs its FMA build; the fused
 //    operations below are the ones that build makes, so results are identical
 //    bit for bit (checked against libm on the host: 2.85M arguments, 0 differences).
+//  * mgcm_log: glibc's table-driven log (sysdeps/ieee754/dbl-64/e_log.c, the
+//    same ARM optimized-routines family): a polynomial near 1, elsewhere a
+//    128-entry {1/c, log c} table (rt_glibc_log_table.inc, gen_log_table.c)
+//    and a degree-6 polynomial for log1p.  Like glibc it is not correctly
+//    rounded (about 1.4 in 10^4 results differ from the nearest double); the
+//    restatement returns glibc's bits, misroundings included.
 //  * mgcm_log10: glibc's fdlibm wrapper (sysdeps/ieee754/dbl-64/e_log10.c):
-//    x = 2^e m, log10(x) = e*log10_2lo + ivln10*log(m) + e*log10_2hi, over a
-//    correctly rounded log (a 129-point table and a double-double atanh series,
-//    rt_log_table.inc generated by gen_log_table.py).  It equals glibc's
-//    log10 except where glibc's own log misrounds (about 3 in 10^4 arguments).
+//    x = 2^e m, log10(x) = e*log10_2lo + ivln10*log(m) + e*log10_2hi, over
+//    mgcm_log, so it returns glibc's log10 bit for bit.
 #pragma once
 #include <stdint.h>
 
@@ -27,7 +31,7 @@
 #define MGCM_FMA(a, b, c) __builtin_fma((a), (b), (c))
 #else
 #include <math.h>
-#define MGCM_HD static inline
+#define MGCM_HD inline
 #define MGCM_FMA(a, b, c) fma((a), (b), (c))
 #endif
 
@@ -58,24 +62,24 @@ MGCM_HD uint64_t exp_tab(int i) { return kExpTabHost[i]; }
 #endif
 
 #if defined(__HIPCC__)
-__device__ __constant__ static const uint64_t kLogTab[258] = {
-#include "rt_log_table.inc"
+__device__ __constant__ static const uint64_t kGlogTab[256] = {
+#include "rt_glibc_log_table.inc"
 };
-static const uint64_t kLogTabHost[258] = {
-#include "rt_log_table.inc"
+static const uint64_t kGlogTabHost[256] = {
+#include "rt_glibc_log_table.inc"
 };
-MGCM_HD uint64_t log_tab(int i) {
+MGCM_HD uint64_t glog_tab(int i) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  return kLogTab[i];
+  return kGlogTab[i];
 #else
-  return kLogTabHost[i];
+  return kGlogTabHost[i];
 #endif
 }
 #else
-static const uint64_t kLogTabHost[258] = {
-#include "rt_log_table.inc"
+static const uint64_t kGlogTabHost[256] = {
+#include "rt_glibc_log_table.inc"
 };
-MGCM_HD uint64_t log_tab(int i) { return kLogTabHost[i]; }
+MGCM_HD uint64_t glog_tab(int i) { return kGlogTabHost[i]; }
 #endif
 
 MGCM_HD double exp_special(double tmp, uint64_t sbits, uint64_t ki) {
@@ -176,88 +180,66 @@ __device__ __forceinline__ void load_lds_exptab(int tid, int nthreads) {
 constexpr int LDS_EXPTAB_BYTES = 256 * 8;
 #endif
 
-// ---- double-double helpers ----
-struct DD {
-  double h, l;
-};
-MGCM_HD DD dd_two_sum(double a, double b) {
-  const double s = a + b, bb = s - a;
-  return {s, (a - (s - bb)) + (b - bb)};
-}
-MGCM_HD DD dd_fast(double a, double b) {
-  const double s = a + b;
-  return {s, b - (s - a)};
-}
-MGCM_HD DD dd_prod(double a, double b) {
-  const double p = a * b;
-  return {p, MGCM_FMA(a, b, -p)};
-}
-MGCM_HD DD dd_add(DD a, DD b) {
-  const DD s = dd_two_sum(a.h, b.h);
-  return dd_fast(s.h, s.l + a.l + b.l);
-}
-MGCM_HD DD dd_mul(DD a, DD b) {
-  const DD p = dd_prod(a.h, b.h);
-  return dd_fast(p.h, p.l + (a.h * b.l + a.l * b.h));
-}
-MGCM_HD DD dd_div(DD a, DD b) {
-  const double q1 = a.h / b.h;
-  const DD p = dd_mul({q1, 0.0}, b);
-  const DD r = dd_add(a, {-p.h, -p.l});
-  return dd_fast(q1, r.h / b.h);
-}
-
-// k*ln2 + log(m) for m in [0.70703125, 1.4140625), rounded once from
-// double-double (2^-100 relative): correctly rounded but for ties closer than
-// that.  m = c * (1+s)/(1-s) with c the table point below m (above it under 1,
-// so c = 1 on both sides of 1): log(m) = log(c) + 2 atanh(s), |s| < 1/256, the
-// atanh series to s^13; log(c) is a double-double table (rt_log_table.inc).
-MGCM_HD double log_dd_round(double m, int k) {
-  int idx;
-  double c;
-  if (m >= 1.0) {
-    const int j = (int)((m - 1.0) * 128.0);
-    idx = j;
-    c = 1.0 + j * 0x1p-7;
-  } else {
-    const int j = (int)((1.0 - m) * 256.0);
-    idx = 53 + j;
-    c = 1.0 - j * 0x1p-8;
+// glibc's double log (sysdeps/ieee754/dbl-64/e_log.c, the Arm
+// optimized-routines method, FMA build).  Near 1 (x in [1 - 2^-4, 1 + 0x1.09p-4))
+// a degree-12 polynomial in r = x - 1 with the r^2/2 term split exactly;
+// elsewhere x = 2^k z, z in [0x1.6p-1, 0x1.6p0), log(x) = k ln2 + logc +
+// log1p(r), r = z invc - 1, with {invc, logc} of z's subinterval from
+// rt_glibc_log_table.inc (gen_log_table.c) and a degree-6 polynomial for
+// log1p(r) - r.  The fused operations are those the FMA build makes.
+MGCM_HD double glibc_log(double x) {
+  const double Ln2hi = 0x1.62e42fefa3800p-1, Ln2lo = 0x1.ef35793c76730p-45;
+  const double A0 = -0x1.0000000000001p-1, A1 = 0x1.555555551305bp-2, A2 = -0x1.fffffffeb459p-3,
+               A3 = 0x1.999b324f10111p-3, A4 = -0x1.55575e506c89fp-3;
+  uint64_t ix = asu(x);
+  const uint64_t lo1 = 0x3fee000000000000ull, hi1 = 0x3ff1090000000000ull;  // 1 - 2^-4, 1 + 0x1.09p-4
+  if (ix - lo1 < hi1 - lo1) {
+    if (ix == 0x3ff0000000000000ull) return 0.0;
+    const double B0 = -0x1p-1, B1 = 0x1.5555555555577p-2, B2 = -0x1.ffffffffffdcbp-3, B3 = 0x1.999999995dd0cp-3,
+                 B4 = -0x1.55555556745a7p-3, B5 = 0x1.24924a344de3p-3, B6 = -0x1.fffffa4423d65p-4,
+                 B7 = 0x1.c7184282ad6cap-4, B8 = -0x1.999eb43b068ffp-4, B9 = 0x1.78182f7afd085p-4,
+                 B10 = -0x1.5521375d145cdp-4;
+    const double r = x - 1.0, r2 = r * r, r3 = r * r2;
+    const double p3 = MGCM_FMA(r3, B10, MGCM_FMA(r2, B9, MGCM_FMA(r, B8, B7)));
+    const double p2 = MGCM_FMA(r3, p3, MGCM_FMA(r2, B6, MGCM_FMA(r, B5, B4)));
+    const double p1 = MGCM_FMA(r3, p2, MGCM_FMA(r2, B3, MGCM_FMA(r, B2, B1)));
+    double w = r * 0x1p27;
+    const double rhi = r + w - w, rlo = r - rhi;
+    w = rhi * rhi * B0;  // exact
+    const double hi = r + w;
+    double lo = r - hi + w;
+    lo = MGCM_FMA(B0 * rlo, rhi + r, lo);
+    const double y = MGCM_FMA(r3, p1, lo);
+    return y + hi;
   }
-  const DD s = dd_div({m - c, 0.0}, dd_two_sum(m, c));  // m - c is exact
-  const DD t = dd_mul(s, s);
-  DD p = {0.0, 0.0};
-  // unrolled: the coefficients 1/(2j+1) fold to double-double constants at
-  // compile time (the same IEEE operations, evaluated once)
-#pragma unroll
-  for (int j = 6; j >= 1; j--) p = dd_add(dd_mul(p, t), dd_div({1.0, 0.0}, {2.0 * j + 1.0, 0.0}));
-  p = dd_add(dd_mul(p, t), {1.0, 0.0});
-  DD L = dd_mul(s, p);
-  L.h *= 2.0;
-  L.l *= 2.0;
-  const DD logc = {asd(log_tab(2 * idx)), asd(log_tab(2 * idx + 1))};
-  const DD ln2 = {0x1.62e42fefa39efp-1, 0x1.abc9e3b39803fp-56};
-  const DD r = dd_add(dd_add(dd_mul({(double)k, 0.0}, ln2), logc), L);
-  return r.h + r.l;
+  const uint32_t top = (uint32_t)(ix >> 48);
+  if (top - 0x0010u >= 0x7ff0u - 0x0010u) {
+    if (ix * 2 == 0) return -1.0 / 0.0;                                   // log(+-0) = -inf
+    if (ix == 0x7ff0000000000000ull) return x;                            // log(inf) = inf
+    if ((top & 0x8000u) || (top & 0x7ff0u) == 0x7ff0u) return (x - x) / (x - x);  // x < 0 or NaN
+    ix = asu(x * 0x1p52);  // subnormal: normalise
+    ix -= 52ull << 52;
+  }
+  const uint64_t off = 0x3fe6000000000000ull;
+  const uint64_t tmp = ix - off;
+  const int i = (int)((tmp >> 45) % 128);
+  const int k = (int)((int64_t)tmp >> 52);
+  const double z = asd(ix - (tmp & (0xfffull << 52)));
+  const double invc = asd(glog_tab(2 * i)), logc = asd(glog_tab(2 * i + 1));
+  const double r = MGCM_FMA(z, invc, -1.0), kd = (double)k;
+  const double w = MGCM_FMA(kd, Ln2hi, logc);
+  const double hi = w + r;
+  const double lo = MGCM_FMA(kd, Ln2lo, (w - hi) + r);
+  const double r2 = r * r;
+  const double p = MGCM_FMA(r2, MGCM_FMA(r, A4, A3), MGCM_FMA(r, A2, A1));
+  return MGCM_FMA(r * r2, p, MGCM_FMA(r2, A0, lo)) + hi;
 }
 
-// Correctly rounded natural log of a positive normal double in [0.5, 2).
-MGCM_HD double cr_log_unit(double m) {
-  int k = 0;
-  if (m < 0.70703125) { m *= 2.0; k = -1; }
-  else if (m >= 1.4140625) { m *= 0.5; k = 1; }
-  return log_dd_round(m, k);
-}
+// The reference's LOG (the C library's log).
+MGCM_HD double mgcm_log(double x) { return glibc_log(x); }
 
-// Correctly rounded natural log of a positive normal double.
-MGCM_HD double mgcm_log(double x) {
-  const uint64_t u = asu(x);
-  int k = (int)((u >> 52) & 0x7ff) - 1023;
-  double m = asd((u & 0x000fffffffffffffull) | 0x3ff0000000000000ull);  // [1, 2)
-  if (m >= 1.4140625) { m *= 0.5; k += 1; }
-  return log_dd_round(m, k);
-}
-
+// glibc's log10 (sysdeps/ieee754/dbl-64/e_log10.c, fdlibm; built without FMA):
+// x = 2^e m, log10(x) = e*log10_2lo + ivln10*log(m) + e*log10_2hi.
 MGCM_HD double mgcm_log10(double x) {
   const double two54 = 1.80143985094819840000e+16, ivln10 = 4.34294481903251816668e-01,
                log10_2hi = 3.01029995663611771306e-01, log10_2lo = 3.69423907715893078616e-13;
@@ -278,7 +260,7 @@ MGCM_HD double mgcm_log10(double x) {
   hx = (hx & 0x000fffff) | ((0x3ff - i) << 20);
   const double y = (double)(k + i);
   const double m = asd(((uint64_t)(uint32_t)hx << 32) | (u & 0xffffffffull));
-  const double z = y * log10_2lo + ivln10 * cr_log_unit(m);
+  const double z = y * log10_2lo + ivln10 * glibc_log(m);
   return z + y * log10_2hi;
 }
 

@@ -140,7 +140,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         lib.mgcm_rt_opt_cld.argtypes = [C.c_void_p, C.POINTER(Tracers), C.POINTER(AerosolOut), C.c_void_p]
         lib.mgcm_rt_opt_dst_host.argtypes = [C.c_void_p, C.POINTER(Tracers), C.POINTER(AerosolOut)]
         lib.mgcm_rt_opt_cld_host.argtypes = [C.c_void_p, C.POINTER(Tracers), C.POINTER(AerosolOut)]
-        lib.mgcm_rt_libm_check.argtypes = [DP, C.c_int, DP, DP, DP]
+        lib.mgcm_rt_libm_check.argtypes = [DP, C.c_int, DP, DP, DP, DP]
         lib.mgcm_rt_strerror.restype = C.c_char_p
         lib.mgcm_rt_strerror.argtypes = [C.c_int]
         _lib = lib
@@ -384,12 +384,12 @@ def read_ktable(path: str, nbands: int) -> tuple:
 
 
 def libm_check(x) -> tuple:
-    """Device exp/log10/sqrt of x (diagnostic entry of the library)."""
+    """Device exp/log10/sqrt/log of x (diagnostic entry of the library)."""
     lib = load_library()
     x = _f64(x)
-    e, l, q = np.zeros_like(x), np.zeros_like(x), np.zeros_like(x)
-    _check(lib.mgcm_rt_libm_check(_dp(x), x.size, _dp(e), _dp(l), _dp(q)), "mgcm_rt_libm_check")
-    return e, l, q
+    e, l, q, ln = np.zeros_like(x), np.zeros_like(x), np.zeros_like(x), np.zeros_like(x)
+    _check(lib.mgcm_rt_libm_check(_dp(x), x.size, _dp(e), _dp(l), _dp(q), _dp(ln)), "mgcm_rt_libm_check")
+    return e, l, q, ln
 
 
 def to_device(d: dict, device="cuda:0") -> dict:

@@ -241,9 +241,56 @@ def _conrath_taudst(tautot, conr=0.03, rptau=6.1, psf=7.01, npdst=100):
     return prdst, taudst
 
 
-def make_columns(seed: int, ncol: int, nlay: int, clouds: bool = False) -> dict:
-    """Per-column inputs, per-column contiguous (Fortran order within a column)."""
-    rng = np.random.Generator(np.random.PCG64(seed + 7919))
+def _clib_log(x: np.ndarray) -> np.ndarray:
+    """Natural log by the C library (math.log calls it for positive finite x), element by
+    element: the reference's DLOG.  NumPy's log is its own vector code, whose last bit
+    differs from the C library's for some arguments."""
+    flat = np.ascontiguousarray(x, dtype=np.float64).ravel()
+    return np.fromiter(map(math.log, flat.tolist()), dtype=np.float64, count=flat.size).reshape(np.shape(x))
+
+
+def fillpt(pl: np.ndarray, ptrop: float, tg: np.ndarray, tstrat: np.ndarray, tl: np.ndarray) -> dict:
+    """FILLPT (code/fillpt.f90:30-72) for ncol columns: the GCM's level pressures PL and
+    temperatures TL ([ncol][L_LEVELS], TL at the even levels K = 4..L_LEVELS-1), ground
+    temperature TG and stratosphere temperature TSTRAT ([ncol]) -> PLEV, TLEV, PMID, TMID
+    on the radiation grid.  The odd levels' log-p interpolation is evaluated with the same
+    operations in the same order, DLOG being the C library's log."""
+    ncol, nlev = pl.shape
+    nlay = (nlev - 3) // 2
+    # level-major work arrays (row K-1 holds level K of every column)
+    plev = np.array(np.asarray(pl, dtype=np.float64).T, copy=True)
+    plev[0] = ptrop / 2.0
+    plev[1] = ptrop / 2.0
+    tlev = np.empty((nlev, ncol))
+    tlev[0:3] = np.asarray(tstrat)[None, :]
+    tlev[3:nlev - 1:2] = np.asarray(tl, dtype=np.float64).T[3:nlev - 1:2]  # K = 4, 6, ..., L_LEVELS-1
+    if nlev - 1 > 5:  # K = 5, 7, ..., L_LEVELS-2 (each depends only on even levels)
+        ks = np.arange(4, nlev - 2, 2)
+        num = _clib_log(plev[ks] / plev[ks + 1])
+        den = _clib_log(plev[ks - 1] / plev[ks + 1])
+        tlev[ks] = tlev[ks + 1] + (tlev[ks - 1] - tlev[ks + 1]) * num / den
+    tlev[nlev - 1] = tg
+    tmid = np.empty_like(tlev)
+    pmid = np.empty_like(plev)
+    tmid[0] = tlev[1]
+    tmid[1] = tlev[1]
+    pmid[0] = plev[0]
+    pmid[1] = plev[1]
+    tmid[2:2 * nlay + 2:2] = tlev[2:2 * nlay + 2:2]        # TMID(2L+1) = TLEV(2L+1)
+    tmid[3:2 * nlay + 3:2] = tlev[2:2 * nlay + 2:2]        # TMID(2L+2) = TLEV(2L+1)
+    pmid[2:2 * nlay + 2:2] = plev[2:2 * nlay + 2:2]
+    pmid[3:2 * nlay + 3:2] = plev[2:2 * nlay + 2:2]
+    tmid[nlev - 1] = tlev[nlev - 1]
+    pmid[nlev - 1] = plev[nlev - 1]
+    plev, tlev, pmid, tmid = (np.ascontiguousarray(a.T) for a in (plev, tlev, pmid, tmid))
+    return {"plev": plev, "tlev": tlev, "pmid": pmid, "tmid": tmid}
+
+
+def gcm_levels(rng: np.random.Generator, ncol: int, nlay: int) -> dict:
+    """The GCM-side fields FILLPT takes, drawn from rng: surface pressure ps (mbar), level
+    pressures PL = PTROP + sigma*ps ([ncol][L_LEVELS]; PL(1..2) unused), layer
+    temperatures TL at the even levels K = 2l+2 (a 140..250 K profile in log p plus
+    N(0, 5 K)), TSTRAT and TG."""
     nlev = 2 * nlay + 3
     sig = sigma_levels(nlay)
     ps = rng.uniform(4.0, 10.0, ncol)                         # mbar
@@ -252,40 +299,26 @@ def make_columns(seed: int, ncol: int, nlay: int, clouds: bool = False) -> dict:
     pl[:, 2:nlev - 1] = PTROP + sig[None, 2:nlev - 1] * ps[:, None]
     pl[:, nlev - 1] = PTROP + ps
     pl[:, 0] = PTROP / 2.0
-    # layer mid-point temperatures (K = 2l+2, 0-based index 2l+1... for l=1..nlay -> 2l+1)
-    kmid = np.arange(1, nlay + 1) * 2 + 1                     # 0-based index of K=2l+2
+    kmid = np.arange(1, nlay + 1) * 2 + 1                     # 0-based index of K = 2l+2
     lp = np.log(pl[:, kmid])
     den = lp[:, -1:] - lp[:, :1]                              # 0 for a single layer
     frac = np.divide(lp - lp[:, :1], den, out=np.full_like(lp, 0.5), where=den > 0)
-    tl = 140.0 + 110.0 * frac + rng.normal(0.0, 5.0, (ncol, nlay))
+    tlay = 140.0 + 110.0 * frac + rng.normal(0.0, 5.0, (ncol, nlay))
     tstrat = 140.0 + rng.normal(0.0, 5.0, ncol)
-    tg = tl[:, -1] + rng.uniform(-5.0, 15.0, ncol)
-    # FILLPT (code/fillpt.f90:30-72)
-    plev = pl.copy()
-    plev[:, 0] = PTROP / 2.0
-    plev[:, 1] = PTROP / 2.0
-    tlev = np.empty((ncol, nlev))
-    tlev[:, 0:3] = tstrat[:, None]
-    for l in range(nlay):
-        tlev[:, 2 * l + 3] = tl[:, l]                         # K = 2l+4 (even), 1-based
-    for k1 in range(5, nlev - 1, 2):                          # K = 5, 7, ..., L_LEVELS-2
-        k = k1 - 1
-        tlev[:, k] = tlev[:, k + 1] + (tlev[:, k - 1] - tlev[:, k + 1]) * np.log(
-            plev[:, k] / plev[:, k + 1]) / np.log(plev[:, k - 1] / plev[:, k + 1])
-    tlev[:, nlev - 1] = tg
-    tmid = np.empty_like(tlev)
-    pmid = np.empty_like(plev)
-    tmid[:, 0] = tlev[:, 1]
-    tmid[:, 1] = tlev[:, 1]
-    pmid[:, 0] = plev[:, 0]
-    pmid[:, 1] = plev[:, 1]
-    for l in range(1, nlay + 1):
-        tmid[:, 2 * l] = tlev[:, 2 * l]
-        tmid[:, 2 * l + 1] = tlev[:, 2 * l]
-        pmid[:, 2 * l] = plev[:, 2 * l]
-        pmid[:, 2 * l + 1] = plev[:, 2 * l]
-    tmid[:, nlev - 1] = tlev[:, nlev - 1]
-    pmid[:, nlev - 1] = plev[:, nlev - 1]
+    tg = tlay[:, -1] + rng.uniform(-5.0, 15.0, ncol)
+    tl = np.zeros((ncol, nlev))
+    tl[:, kmid] = tlay
+    return {"ps": ps, "pl": pl, "tl": tl, "tstrat": tstrat, "tg": tg, "sigma": sig}
+
+
+def make_columns(seed: int, ncol: int, nlay: int, clouds: bool = False) -> dict:
+    """Per-column inputs, per-column contiguous (Fortran order within a column)."""
+    rng = np.random.Generator(np.random.PCG64(seed + 7919))
+    nlev = 2 * nlay + 3
+    g = gcm_levels(rng, ncol, nlay)
+    ps, sig = g["ps"], g["sigma"]
+    f = fillpt(g["pl"], PTROP, g["tg"], g["tstrat"], g["tl"])
+    plev, tlev, pmid, tmid = f["plev"], f["tlev"], f["pmid"], f["tmid"]
     # water vapour (code/comp3.f:576-589)
     q0 = 10.0 ** rng.uniform(-7.0, -3.0, ncol)
     qf = 10.0 ** rng.uniform(-0.3, 0.3, (ncol, nlay))

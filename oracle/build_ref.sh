@@ -21,7 +21,7 @@ mkdir -p "$OUT"
 SRCS="modules.f90 tpindex.f90 optcv.f90 optci.f90 getdetau.f90 gfluxv.f90 gfluxi.f90
       dsolver.f90 dtridgl.f90 sfluxv.f90 sfluxi.f90 setspv.f90 setspi.f90 setrad.f90
       laginterp.f90 lagrange.f90 initcld.f opt_dst.f opt_cld.f dustprofile.f filltaucum.f
-      jsrchgt.f90 dsolflux.f"
+      jsrchgt.f90 dsolflux.f fillpt.f90"
 cd "$TMP"
 # -O2 without fast-math; x86-64 baseline (no FMA), as the reference's own gfortran -O3 build.
 for f in $SRCS; do

@@ -1240,3 +1240,14 @@ double or_dsolflux(const double *sol, double ubar0, const double *gw, const doub
   }
   return directsol;
 }
+
+void or_libm(int n, const double *x, double *e, double *l10, double *ln)
+{
+  int i;
+#pragma omp parallel for schedule(static)
+  for (i = 0; i < n; i++) {
+    e[i] = exp(x[i]);
+    l10[i] = x[i] > 0.0 ? log10(x[i]) : 0.0;
+    ln[i] = x[i] > 0.0 ? log(x[i]) : 0.0;
+  }
+}

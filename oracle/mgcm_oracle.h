@@ -148,6 +148,11 @@ void or_dust_columns(int ncol, int nlay, int npdst, const double *prdst, double 
 double or_dsolflux(const double *sol, double ubar0, const double *gw, const double *fzerov,
                    const double *detau);
 
+/* The host C library's exp, log10 and log of n arguments (the functions the
+ * reference's EXP, LOG10 and LOG call); entries with x <= 0 give 0 for the logs.
+ * The reference the device's restatements are compared with. */
+void or_libm(int n, const double *x, double *e, double *l10, double *ln);
+
 #ifdef __cplusplus
 }
 #endif

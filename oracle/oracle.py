@@ -107,6 +107,7 @@ class Oracle:
         self.lib.or_dsolflux.restype = C.c_double
         self.lib.or_dust_columns.argtypes = [C.c_int, C.c_int, C.c_int, DP, C.c_double, C.c_double] + [DP] * 7
         self.lib.or_opt_aer.argtypes = [C.c_int, C.POINTER(_AerTables), C.c_int, C.c_int] + [DP] * 11
+        self.lib.or_libm.argtypes = [C.c_int, DP, DP, DP, DP]
 
     @staticmethod
     def _tables(t: dict, keep: list) -> _Tables:
@@ -250,6 +251,13 @@ class Oracle:
         self.lib.or_setspv(*[_p(x) for x in a])
         return tuple(a)
 
+    def libm(self, x: np.ndarray) -> tuple:
+        """The host C library's exp, log10 and log of x (logs 0 where x <= 0)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        e, l10, ln = np.zeros_like(x), np.zeros_like(x), np.zeros_like(x)
+        self.lib.or_libm(int(x.size), _p(x), _p(e), _p(l10), _p(ln))
+        return e, l10, ln
+
 
 class Reference:
     """The reference Fortran (amdflang build), one column per call; 24 layers."""
@@ -275,6 +283,7 @@ class Reference:
         self.lib.ref_dust_setup.argtypes = [C.c_double] * 4 + [C.c_int, C.c_double, DP]
         self.lib.ref_filltaucum.argtypes = [C.c_int, C.c_int, C.c_double] + [DP] * 5
         self.lib.ref_dsolflux.argtypes = [DP, C.c_double, DP, DP, DP, DP]
+        self.lib.ref_fillpt.argtypes = [DP, C.c_double, C.c_double, C.c_double] + [DP] * 5
 
     def run(self, t: dict, c: dict, cols=None) -> dict:
         assert int(t["nlay"]) == self.nlay, "reference build is compiled for %d layers" % self.nlay
@@ -349,6 +358,20 @@ class Reference:
                              _p(o["qsi"]), _p(o["gv"]), _p(o["gi"]), _p(qextref), _p(tauref), _p(tau2))
         o.update(qextref=qextref, tauref=tauref, tau2=tau2)
         return o
+
+    def fillpt(self, pl, ptrop: float, tg, tstrat, tl) -> dict:
+        """FILLPT of each column: pl, tl [ncol][L_LEVELS], tg, tstrat [ncol]."""
+        ncol, nlev = pl.shape
+        assert nlev == 2 * self.nlay + 3
+        out = {k: np.zeros((ncol, nlev)) for k in ("plev", "tlev", "pmid", "tmid")}
+        for i in range(ncol):
+            row = {k: np.zeros(nlev) for k in out}
+            self.lib.ref_fillpt(_p(np.ascontiguousarray(pl[i])), ptrop, float(tg[i]), float(tstrat[i]),
+                                _p(np.ascontiguousarray(tl[i])), _p(row["plev"]), _p(row["tlev"]),
+                                _p(row["pmid"]), _p(row["tmid"]))
+            for k in out:
+                out[k][i] = row[k]
+        return out
 
     def dsolflux(self, sol, ubar0: float, gweight, fzerov, detau_col) -> float:
         out = np.zeros(1)

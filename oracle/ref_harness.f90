@@ -92,6 +92,18 @@
       end subroutine ref_filltaucum
 
 !     DSOLFLUX (code/dsolflux.f) of one column's DETAU(7,17).
+!     FILLPT (code/fillpt.f90) of one column: the GCM's PL, TL (L_LEVELS), TG,
+!     TSTRAT and PTROP -> the radiation grid's PLEV, TLEV, PMID, TMID.
+      subroutine ref_fillpt(pl, ptrop_, tg, tstrat, tl, plev, tlev, pmid, tmid) &
+           bind(C, name="ref_fillpt")
+      real(c_double), intent(in) :: pl(L_LEVELS), tl(L_LEVELS)
+      real(c_double), value :: ptrop_, tg, tstrat
+      real(c_double), intent(out) :: plev(L_LEVELS), tlev(L_LEVELS), pmid(L_LEVELS), tmid(L_LEVELS)
+      real(c_double) :: p_unused
+      p_unused = 0.0d0
+      call fillpt(pl, p_unused, ptrop_, tg, tstrat, tl, plev, tlev, pmid, tmid)
+      end subroutine ref_fillpt
+
       subroutine ref_dsolflux(sol, ubar0, gweight, fzerov, detau_col, directsol) &
            bind(C, name="ref_dsolflux")
       real(c_double), intent(in) :: sol(L_NSPECTV), gweight(L_NGAUSS), fzerov(L_NSPECTV)

@@ -105,6 +105,8 @@ def main(only=None) -> None:
             dust_profiles(ref)
         if "dsolflux_ref" in only:
             dsolflux_columns(ref)
+        if "fillpt_ref" in only:
+            fillpt_columns(ref)
         return
     out = {}
     for name, seed, weights in (("column_ref24", 101, "reference"), ("column_dir24", 102, "dirichlet")):
@@ -189,6 +191,16 @@ def dsolflux_columns(ref) -> None:
     ds = np.array([ref.dsolflux(t["sol"], acosz[i], synth.GWEIGHT_REF, t["fzerov"], detau[i]) if sun[i] else 0.0
                    for i in range(len(acosz))])
     np.savez_compressed(os.path.join(HERE, "dsolflux_ref.npz"), directsol=ds)
+
+
+def fillpt_columns(ref) -> None:
+    """fillpt_ref.npz: the reference's FILLPT on 64 columns of the generator's GCM-side
+    fields (seed 404, 24 layers): the inputs PL, TL, TG, TSTRAT and the outputs."""
+    rng = np.random.Generator(np.random.PCG64(404))
+    g = synth.gcm_levels(rng, 64, 24)
+    r = ref.fillpt(g["pl"], synth.PTROP, g["tg"], g["tstrat"], g["tl"])
+    np.savez_compressed(os.path.join(HERE, "fillpt_ref.npz"), pl=g["pl"], tl=g["tl"], tg=g["tg"],
+                        tstrat=g["tstrat"], ptrop=synth.PTROP, **{"out_" + k: v for k, v in r.items()})
 
 
 DUST_CASES = ((6.1, 0.0005, False, 0.0), (7.01, 0.0008, True, 300.0))  # PSF, PTROP, VARY_CONR, VPOUT

@@ -1,0 +1,37 @@
+"""The headline workload whole: every column of the 64,800-column x 24-layer grid
+(seed 2024, the bench's step) against the oracle at rel 1e-10, through the C ABI with
+the inputs resident on the device, as the bench runs it."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("fmnetv", "fluxupv", "fluxdnv", "fmneti", "fluxupi", "fluxdni", "nfluxtopv", "nfluxtopi",
+        "diffvt", "detau", "cumtauv", "hr")
+
+
+def test_headline_grid_every_column(oracle):
+    import torch
+    from mgcm_rt import FP64, Radiation, alloc_outputs, synth, to_device
+    ncol, nlay = 64800, 24
+    t, c = synth.make_workload(2024, ncol, nlay)
+    rt = Radiation(t)
+    dc = to_device(c)
+    do = to_device(alloc_outputs(ncol, nlay))
+    s = torch.cuda.current_stream()
+    rt.run_device(dc, do, FP64, s.cuda_stream)
+    torch.cuda.synchronize()
+    rt.check()
+    want = oracle.run(t, c)
+    worst = {}
+    for k in KEYS:
+        g = do[k].cpu().numpy()
+        # per column: |got - want| / max(|want|, the column's max |want|)
+        worst[k] = rel_err(g, want[k])
+    print({k: f"{v:.1e}" for k, v in worst.items()}, flush=True)
+    bad = {k: v for k, v in worst.items() if not v <= 1e-10}
+    assert not bad, bad

@@ -31,29 +31,33 @@ def test_plain_matches_oracle(oracle, nlay, weights, seed):
     print(nlay, weights, {k: f"{v:.1e}" for k, v in errs.items()}, flush=True)
 
 
-def test_device_libm_matches_host_libm():
-    """exp bit-identical to the host C library; log10 identical except where the host's
-    log misrounds (bounded rate); sqrt correctly rounded."""
-    import ctypes
-    import math
+def libm_arguments(n: int, seed: int) -> np.ndarray:
+    """Arguments of the FP64 path's transcendentals: exp arguments in [-40, 40] and over
+    its whole finite range; positive doubles of every binade (random bits), pressures
+    1e-7..1e5 mbar (TPINDEX's LOG10), values near 1 (the log's polynomial path)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    q = n // 5
+    bits = rng.integers(0x0010000000000000, 0x7FEFFFFFFFFFFFFF, q, dtype=np.int64)
+    return np.concatenate([rng.uniform(-40, 40, q), rng.uniform(-745, 709, q), bits.view(np.float64),
+                           10 ** rng.uniform(-7, 5, q), rng.uniform(0.9, 1.1, n - 4 * q),
+                           [0.0, 1.0, -35.0, 35.0, 1e-300, 5e-324, 2.2250738585072014e-308, 0.5, 2.0]])
+
+
+def test_device_libm_matches_host_libm(oracle):
+    """exp, log10 and log bit-identical to the host C library (the functions the
+    reference's EXP, LOG10 and LOG call) on 12 M arguments; sqrt correctly rounded."""
     from mgcm_rt import libm_check
-    rng = np.random.Generator(np.random.PCG64(77))
-    x = np.concatenate([rng.uniform(-40, 40, 200000), rng.uniform(-745, 709, 50000),
-                        10 ** rng.uniform(-3.6, 1.2, 100000), [0.0, 1.0, -35.0, 35.0, 1e-300]])
-    e, l, q = libm_check(x)
-    libm = ctypes.CDLL("libm.so.6")
-    for f in ("exp", "log10"):
-        getattr(libm, f).restype = ctypes.c_double
-        getattr(libm, f).argtypes = [ctypes.c_double]
-    he = np.array([libm.exp(v) for v in x])
-    assert np.array_equal(e, he), int(np.sum(e != he))
+    x = libm_arguments(12_000_000, 77)
+    e, l, q, ln = libm_check(x)
+    he, hl, hln = oracle.libm(x)
+    assert np.array_equal(e.view(np.int64), he.view(np.int64)), int(np.sum(e != he))
     pos = x > 0
-    hl = np.array([libm.log10(v) for v in x[pos]])
-    nbad = int(np.sum(l[pos] != hl))
-    assert nbad <= 1e-3 * pos.sum(), nbad
+    nb10 = int(np.sum(l[pos].view(np.int64) != hl[pos].view(np.int64)))
+    nbln = int(np.sum(ln[pos].view(np.int64) != hln[pos].view(np.int64)))
+    assert nb10 == 0 and nbln == 0, (nb10, nbln)
     nn = x >= 0
     assert np.array_equal(q[nn], np.sqrt(x[nn]))
-    print("log10 differences", nbad, "of", int(pos.sum()), flush=True)
+    print("exp/log10/log: 0 differences of", x.size, flush=True)
 
 
 def test_run_captured_in_a_hip_graph():

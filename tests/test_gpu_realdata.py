@@ -77,12 +77,10 @@ def column_errors(got: dict, want: dict, keys) -> np.ndarray:
 
 @pytest.mark.parametrize("nlay,ncol,seed", [(24, 1024, 31), (36, 512, 32), (16, 256, 33)])
 def test_real_tables_match_oracle(oracle, real, data_dir, nlay, ncol, seed):
-    """Every operation of the path is the reference's; the one libm difference is log10
-    (TPINDEX's log10 p), where the host's misrounds about 1 argument in 7,000.  On this
-    k-distribution the reference's IR solve is ill-conditioned in the thin top layers: one
-    ulp of PMID moves its IR fluxes by up to ~1e-8 and the top heating rates by ~1e-5.
-    So: VIS at 1e-10 in every column; IR at 1e-10 in at least 99% of the columns, and in
-    every column within 4x the oracle's own response to a one-ulp change of PMID."""
+    """On the reference's k-distribution the IR solve is ill-conditioned in the thin top
+    layers (one ulp of PMID moves its IR fluxes by up to ~1e-8 and the top heating rates
+    by ~1e-5), so parity needs every operation to be the reference's, TPINDEX's LOG10
+    included: VIS and IR at rel 1e-10 in every column."""
     from mgcm_rt import Radiation, realdata, synth
     t = realdata.load_tables(data_dir, nlay)
     c = synth.make_columns(seed, ncol, nlay, clouds=(nlay == 36))
@@ -93,16 +91,9 @@ def test_real_tables_match_oracle(oracle, real, data_dir, nlay, ncol, seed):
     got = Radiation(t).run_host(c)
     ev = column_errors(got, want, VIS_KEYS)
     ei = column_errors(got, want, IR_KEYS)
-    sens = np.zeros(ncol)
-    for d in (np.inf, -np.inf):
-        w2 = oracle.run(tor, c | {"pmid": np.nextafter(c["pmid"], d)})
-        sens = np.maximum(sens, column_errors(w2, want, IR_KEYS))
-    above = np.where(ei > 1e-10)[0]
-    print(nlay, ncol, f"VIS max {ev.max():.1e}; IR columns above 1e-10: {above.tolist()} "
-          f"err {ei[above]} one-ulp response {sens[above]}; median one-ulp response {np.median(sens):.1e}", flush=True)
+    print(nlay, ncol, f"VIS max {ev.max():.1e}, IR max {ei.max():.1e}", flush=True)
     assert ev.max() <= 1e-10
-    assert len(above) <= 0.01 * ncol
-    assert np.all(ei <= np.maximum(1e-10, 4.0 * sens))
+    assert ei.max() <= 1e-10, np.where(ei > 1e-10)[0].tolist()
 
 
 def test_mixed_path_on_real_tables(oracle, data_dir):
