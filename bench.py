@@ -255,11 +255,14 @@ def form_batched(args, rt, stream):
     """512 grids, U{2000..20000} columns, U{16..64} layers, in one
     mgcm_rt_run_batched call (plan, tuning and graph capture on the first call,
     then graph replays).  Inputs of grids with equal layer counts share one
-    column pool; outputs are per grid."""
+    column pool; outputs are per grid.  After the timed replays, 64 sampled
+    columns (8 from each of 8 grids spread over the layer range) are checked
+    against the oracle at rel 1e-10, and the tuner's choices are reported."""
     import torch
     from mgcm_rt import FP64, alloc_outputs, synth, to_device
     shapes = synth.ragged_shapes(args.seed, 512)
-    pools = {nl: to_device(p) for nl, p in synth.ragged_pools(args.seed, shapes).items()}
+    host_pools = synth.ragged_pools(args.seed, shapes)
+    pools = {nl: to_device(p) for nl, p in host_pools.items()}
     grids, outs = [], []
     for n, nl in shapes:
         g = {k: (v[:n] if hasattr(v, "shape") else v) for k, v in pools[nl].items()}
@@ -279,7 +282,27 @@ def form_batched(args, rt, stream):
     ncol = sum(n for n, _ in shapes)
     out = {"workload": f"512 grids, {ncol} columns, 16..64 layers", "value": ncol / (ms / 1e3),
            "unit": "columns/s", "ms_per_batch": ms, "steps": steps, "first_call_s": first,
-           "column_layers_per_s": sum(n * (nl + 1) for n, nl in shapes) / (ms / 1e3)}
+           "column_layers_per_s": sum(n * (nl + 1) for n, nl in shapes) / (ms / 1e3),
+           "tuning": rt.batched_tuning()}
+    try:  # sampled parity against the oracle
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from oracle import Oracle
+        o = Oracle()
+        order = sorted(range(len(shapes)), key=lambda i: shapes[i][1])
+        worst = 0.0
+        rng = np.random.Generator(np.random.PCG64(args.seed))
+        for i in [order[j] for j in np.linspace(0, len(order) - 1, 8).astype(int)]:
+            n, nl = shapes[i]
+            pick = np.sort(rng.choice(n, 8, replace=False))
+            sub = {k: (v[pick] if isinstance(v, np.ndarray) and v.ndim and v.shape[0] >= n else v)
+                   for k, v in host_pools[nl].items()}
+            sub["ncol"] = len(pick)
+            want = o.run(synth.make_tables(args.seed, nl), sub)
+            got = {k: outs[i][k][torch.from_numpy(pick).cuda()].cpu().numpy() for k in PARITY_KEYS}
+            worst = max(worst, parity(got, want)["max_rel_err"])
+        out["parity_sampled"] = {"columns": 64, "max_rel_err": worst, "tolerance": 1e-10, "pass": worst <= 1e-10}
+    except Exception as e:  # noqa: BLE001
+        out["parity_sampled"] = {"error": f"{type(e).__name__}: {e}"[:200]}
     del grids, outs, pools
     return out
 

@@ -51,7 +51,9 @@ enum {
   MGCM_RT_ERR_NLAY = -2,     /* nlay outside 1..MGCM_RT_MAX_NLAY */
   MGCM_RT_ERR_DEVICE = -3,   /* HIP runtime failure */
   MGCM_RT_ERR_NOMEM = -4,
-  MGCM_RT_ERR_RANGE = -5,    /* a temperature outside the Planck table (50.0..900.0 K) or a NaN input */
+  MGCM_RT_ERR_RANGE = -5,    /* a temperature outside the Planck table (50.0..900.0 K), or a NaN in
+                                PLEV, TLEV, PMID, TMID, QH2O, TAUREF, TAUREFCLD, ALS, ALBI or ACOSZ
+                                of a column (the per-level aerosol arrays are not checked) */
   MGCM_RT_ERR_PRECISION = -6, /* unknown precision mode */
   MGCM_RT_ERR_IO = -7         /* data file missing, unreadable or not in the expected record layout */
 };
@@ -159,7 +161,9 @@ int mgcm_rt_run(mgcm_rt_ctx *ctx, const mgcm_rt_columns *in, const mgcm_rt_fluxe
                 int precision, void *stream);
 
 /* Same with host pointers: copies in, runs, copies out, synchronises, and
- * reports MGCM_RT_ERR_RANGE if any column was out of the tables' range.
+ * reports MGCM_RT_ERR_RANGE if any column was out of the tables' range.  It
+ * ends with a range check on its own stream: like mgcm_rt_check, the result
+ * covers this call and any earlier run of the context not yet checked.
  * A grid above 4,096 columns goes in chunks of whole columns: library threads
  * (up to 8) copy each chunk to pinned staging while the device runs the one
  * before, so the arrays must not be touched by other threads during the call.
@@ -167,15 +171,42 @@ int mgcm_rt_run(mgcm_rt_ctx *ctx, const mgcm_rt_columns *in, const mgcm_rt_fluxe
 int mgcm_rt_run_host(mgcm_rt_ctx *ctx, const mgcm_rt_columns *in, const mgcm_rt_fluxes *out,
                      int precision);
 
-/* Range flag of the runs since the last call (synchronises the device);
- * MGCM_RT_OK or MGCM_RT_ERR_RANGE. */
+/* Range flag of the context's runs since the last check (synchronises the
+ * whole device, then clears the flag); MGCM_RT_OK or MGCM_RT_ERR_RANGE.  The
+ * reference has no such check: it reads outside PLANCKIR silently
+ * (code/sfluxi.f90:54-55). */
 int mgcm_rt_check(mgcm_rt_ctx *ctx);
+
+/* The same waiting for `stream` only (a hipStream_t; NULL = default stream):
+ * it covers the runs issued on that stream, and other runs that have ended. */
+int mgcm_rt_check_stream(mgcm_rt_ctx *ctx, void *stream);
 
 /* Many independent grids (ragged layer counts) in one persistent launch.
  * grids[i] / outs[i] hold device pointers.  Columns of all grids are packed
- * into one work list; the launch shape comes from the context's tuner. */
+ * into one work list per layer class; the launch shape comes from the
+ * context's tuner (mgcm_rt_batched_tuning).  Grids whose outs[i].directsol is
+ * set also get DSOLFLUX/DNVFLUX, inside the same graph.  The reference calls
+ * the radiation per column (code/comp3.f:411-415,637-738). */
 int mgcm_rt_run_batched(mgcm_rt_ctx *ctx, int ngrids, const mgcm_rt_columns *grids,
                         const mgcm_rt_fluxes *outs, int precision, void *stream);
+
+/* The batched tuner's choice per layer class of the last planned batch: the
+ * column kernel's instance (waves per SIMD) and persistent grid size, and the
+ * time of each candidate (waves per SIMD x multiple of that instance's resident
+ * capacity) on the first tuned_items work items (0 ms: not timed -- the list
+ * fits one resident round, or the candidate repeats the one before).  Returns
+ * the number of classes (fills at most maxcls).  No counterpart in the
+ * reference (it has no batched path). */
+typedef struct {
+  int max_layers;     /* grids with nlay <= max_layers (and above the class below) */
+  int items;          /* 4-column work items of the class */
+  int waves_per_simd; /* chosen instance: 1 or 2 */
+  int blocks;         /* chosen persistent grid size */
+  int tuned_items;    /* items the candidates were timed on */
+  int cand_waves[6], cand_mult[6];
+  float cand_ms[6];
+} mgcm_rt_batch_class;
+int mgcm_rt_batched_tuning(const mgcm_rt_ctx *ctx, int maxcls, mgcm_rt_batch_class *info);
 
 /* Init-time table builders (host in/out pointers), both equal to the
  * reference's tables bit for bit:

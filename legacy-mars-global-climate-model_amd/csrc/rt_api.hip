@@ -17,7 +17,7 @@ extern "C" const char *mgcm_rt_strerror(int s) {
     case MGCM_RT_ERR_NLAY: return "layer count outside 1..99";
     case MGCM_RT_ERR_DEVICE: return "HIP runtime error";
     case MGCM_RT_ERR_NOMEM: return "out of device memory";
-    case MGCM_RT_ERR_RANGE: return "temperature outside the Planck table (50.0..900.0 K)";
+    case MGCM_RT_ERR_RANGE: return "temperature outside the Planck table (50.0..900.0 K) or a NaN input";
     case MGCM_RT_ERR_PRECISION: return "unknown precision mode";
     case MGCM_RT_ERR_IO: return "data file missing, unreadable or not in the expected record layout";
     default: return "unknown status";
@@ -172,6 +172,8 @@ void fill_args(RunArgs &a, const mgcm_rt_columns *in, const mgcm_rt_fluxes *out,
   for (int i = 0; i < 12; i++) a.aq[i] = aq[i];
   a.qref_dst = ae ? ae->qextref_dst : nullptr;
   a.qref_cld = ae ? ae->qextref_cld : nullptr;
+  a.directsol = out->directsol;
+  a.dnvflux = out->directsol ? out->dnvflux : nullptr;
 }
 }  // namespace mgcm
 
@@ -202,8 +204,8 @@ extern "C" int mgcm_rt_run(mgcm_rt_ctx *c, const mgcm_rt_columns *in, const mgcm
   if (in->ncol == 0) return MGCM_RT_OK;
   const size_t need = clr_workspace_bytes(in->ncol, in->nlay);
   if (need > c->clr_bytes) {  // grow-only continuum-point workspace
-    if (c->d_clr) {
-      if (hipDeviceSynchronize() != hipSuccess) return MGCM_RT_ERR_DEVICE;
+    if (c->d_clr) {  // the context's earlier runs were issued on this stream (or have ended)
+      if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess) return MGCM_RT_ERR_DEVICE;
       (void)hipFree(c->d_clr);
     }
     c->d_clr = nullptr;
@@ -224,6 +226,21 @@ extern "C" int mgcm_rt_check(mgcm_rt_ctx *c) {
   return f ? MGCM_RT_ERR_RANGE : MGCM_RT_OK;
 }
 
+namespace mgcm {
+int check_on_stream(mgcm_rt_ctx *c, hipStream_t st) {
+  int f = 0;
+  HIP_TRY(hipMemcpyAsync(&f, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemsetAsync(c->d_flag, 0, sizeof(int), st));
+  HIP_TRY(hipStreamSynchronize(st));
+  return f ? MGCM_RT_ERR_RANGE : MGCM_RT_OK;
+}
+}  // namespace mgcm
+
+extern "C" int mgcm_rt_check_stream(mgcm_rt_ctx *c, void *stream) {
+  if (!c) return MGCM_RT_ERR_ARG;
+  return check_on_stream(c, (hipStream_t)stream);
+}
+
 extern "C" int mgcm_rt_run_batched(mgcm_rt_ctx *c, int ngrids, const mgcm_rt_columns *grids,
                                    const mgcm_rt_fluxes *outs, int precision, void *stream) {
   if (!c || ngrids < 0 || (ngrids > 0 && (!grids || !outs))) return MGCM_RT_ERR_ARG;
@@ -235,6 +252,11 @@ extern "C" int mgcm_rt_run_batched(mgcm_rt_ctx *c, int ngrids, const mgcm_rt_col
     if (s != MGCM_RT_OK) return s;
   }
   return run_batched_impl(c, ngrids, grids, outs, precision, (hipStream_t)stream);
+}
+
+extern "C" int mgcm_rt_batched_tuning(const mgcm_rt_ctx *c, int maxcls, mgcm_rt_batch_class *info) {
+  if (!c || maxcls < 0 || (maxcls > 0 && !info)) return MGCM_RT_ERR_ARG;
+  return batched_tuning_impl(c, maxcls, info);
 }
 
 // ---------------------------------------------------------------------------

@@ -5,10 +5,13 @@
 //   * grids are grouped by layer-count class (kernel instance NLM);
 //   * each class gets a work list of (grid, first column) 4-column blocks,
 //     grids with more layers first, columns in order;
-//   * one persistent kernel per class walks its list round-robin; the tuner
-//     picks the number of resident blocks per class on the first call of a
-//     shape (timed on a prefix of the list);
-//   * the class launches are captured once into a HIP graph.
+//   * one persistent kernel per class walks its list round-robin; on the first
+//     call of a shape the tuner times, per class, the column kernel's two
+//     instances (one or two waves per SIMD) at 1x, 2x and 4x their resident
+//     capacity on a prefix of the list and keeps the fastest;
+//   * DSOLFLUX of the grids that ask for surface fluxes is one more persistent
+//     launch over (grid, 16 columns) items;
+//   * the launches are captured once into a HIP graph.
 // Per call: the grids' argument blocks (device pointers) go to the device with
 // one asynchronous copy on the caller's stream, then the graph is launched on
 // that stream.  Nothing is allocated once a shape has been planned.
@@ -31,11 +34,20 @@ namespace {
 constexpr int NKIND = 3;
 constexpr int KIND_COLS[NKIND] = {4, 9, 12};
 
+// Tuner candidates of the column kernel: (waves per SIMD, multiple of the
+// instance's resident capacity).
+constexpr int NCAND = 6;
+constexpr int CAND_WAVES[NCAND] = {2, 2, 2, 1, 1, 1};
+constexpr int CAND_MULT[NCAND] = {1, 2, 4, 1, 2, 4};
+
 struct ClassPlan {
   int nwork[NKIND] = {0, 0, 0};
   size_t work_off[NKIND] = {0, 0, 0};  // offsets of the lists in d_work
   size_t lds = 0;                      // dynamic LDS per column-kernel block
   int nblocks[NKIND] = {0, 0, 0};      // persistent grid sizes (kind 0 from the tuner)
+  int waves = MGCM_FP64_WAVES;         // column kernel instance (from the tuner)
+  int tuned_prefix = 0;                // work items the candidates were timed on (0: not timed)
+  float cand_ms[NCAND] = {0, 0, 0, 0, 0, 0};  // per candidate; 0 where not timed
 };
 
 struct BatchState {
@@ -46,6 +58,8 @@ struct BatchState {
   size_t grids_cap = 0;
   uint2 *d_work = nullptr;
   size_t work_cap = 0;
+  size_t dsol_off = 0;                // DSOLFLUX items (grid, first column) in d_work
+  int dsol_nwork = 0;
   double *d_clr = nullptr;            // continuum-point records of every grid
   size_t clr_cap = 0;
   std::vector<size_t> clr_off;        // per grid, in doubles
@@ -62,55 +76,71 @@ void drop_graph(BatchState *b) {
   b->graph = nullptr;
 }
 
-hipError_t launch_class(int precision, int k, int kind, const DevTables &dt, const BatchState *b, int nwork,
-                        int nblocks, hipStream_t st) {
+hipError_t launch_class(int precision, int k, int kind, int waves, const DevTables &dt, const BatchState *b,
+                        int nwork, int nblocks, hipStream_t st) {
   const ClassPlan &p = b->cls[k];
   const uint2 *work = b->d_work + p.work_off[kind];
   return (precision & 0xf) == MGCM_RT_FP64
-             ? launch_batch_plain(k, kind, dt, b->d_grids, work, nwork, nblocks, p.lds, st)
-             : launch_batch_mixed(k, kind, dt, b->d_grids, work, nwork, nblocks, p.lds, st);
+             ? launch_batch_plain(k, kind, waves, dt, b->d_grids, work, nwork, nblocks, p.lds, st)
+             : launch_batch_mixed(k, kind, waves, dt, b->d_grids, work, nwork, nblocks, p.lds, st);
 }
 
-// Resident one-wave blocks per CU for `lds` bytes of LDS at 2 waves per SIMD.
-int resident_per_cu(size_t lds) {
+// Resident one-wave blocks per CU for `lds` bytes of LDS at `waves` waves per
+// SIMD (4 SIMDs per CU).
+int resident_per_cu(size_t lds, int waves) {
   const int by_lds = (int)((160 * 1024) / std::max<size_t>(lds, 1));
-  return std::max(1, std::min(8, by_lds));
+  return std::max(1, std::min(4 * waves, by_lds));
 }
 
-// Picks the persistent grid size of one class: 1x, 2x or 4x the resident
-// capacity, timed on a prefix of the work list (the prefix's outputs are
-// written here and rewritten by the real run).
-int tune_class(int precision, int k, const DevTables &dt, const BatchState *b, int ncu, hipStream_t st) {
-  const ClassPlan &p = b->cls[k];
-  const int cap = ncu * resident_per_cu(p.lds);
-  if (p.nwork[0] <= cap) return p.nwork[0];
-  const int prefix = std::min(p.nwork[0], 8 * cap);
+// Picks the column kernel's instance and persistent grid size of one class:
+// each candidate (waves per SIMD, multiple of that instance's resident
+// capacity) runs twice on a prefix of the work list, the second run timed; the
+// fastest is kept.  The prefix's outputs are written here and rewritten by the
+// real run.  A list that fits one resident round of an instance gets one block
+// per item.
+void tune_class(int precision, int k, const DevTables &dt, BatchState *b, int ncu, hipStream_t st) {
+  ClassPlan &p = b->cls[k];
+  const int cap2 = ncu * resident_per_cu(p.lds, 2);
+  p.waves = MGCM_FP64_WAVES;
+  p.nblocks[0] = std::min(p.nwork[0], cap2);
+  p.tuned_prefix = 0;
+  for (int c = 0; c < NCAND; c++) p.cand_ms[c] = 0.f;
+  if (p.nwork[0] <= cap2) {
+    p.nblocks[0] = p.nwork[0];
+    return;
+  }
+  const int prefix = std::min(p.nwork[0], 8 * cap2);
   hipEvent_t e0, e1;
-  if (hipEventCreate(&e0) != hipSuccess) return cap;
+  if (hipEventCreate(&e0) != hipSuccess) return;
   if (hipEventCreate(&e1) != hipSuccess) {
     (void)hipEventDestroy(e0);
-    return cap;
+    return;
   }
-  int best = cap;
   float best_ms = 1e30f;
-  const int cand[3] = {cap, 2 * cap, 4 * cap};
-  for (int c : cand) {
-    const int nb = std::min(c, prefix);
-    if (launch_class(precision, k, 0, dt, b, prefix, nb, st) != hipSuccess) break;  // warm-up
+  for (int c = 0; c < NCAND; c++) {
+    const int w = CAND_WAVES[c];
+    const int nb = std::min(CAND_MULT[c] * ncu * resident_per_cu(p.lds, w), prefix);
+    if (c > 0 && nb == std::min(CAND_MULT[c - 1] * ncu * resident_per_cu(p.lds, w), prefix) &&
+        CAND_WAVES[c - 1] == w)
+      continue;  // the same launch as the previous candidate
+    if (launch_class(precision, k, 0, w, dt, b, prefix, nb, st) != hipSuccess) break;  // warm-up
     (void)hipEventRecord(e0, st);
-    if (launch_class(precision, k, 0, dt, b, prefix, nb, st) != hipSuccess) break;
+    if (launch_class(precision, k, 0, w, dt, b, prefix, nb, st) != hipSuccess) break;
     (void)hipEventRecord(e1, st);
     if (hipEventSynchronize(e1) != hipSuccess) break;
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, e0, e1);
+    p.cand_ms[c] = ms;
     if (ms < best_ms) {
       best_ms = ms;
-      best = c;
+      p.waves = w;
+      p.nblocks[0] = nb;
     }
   }
+  p.tuned_prefix = prefix;
+  p.nblocks[0] = std::min(p.nblocks[0], p.nwork[0]);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  return std::min(best, p.nwork[0]);
 }
 
 int plan(BatchState *b, int ngrids, const mgcm_rt_columns *grids) {
@@ -136,6 +166,11 @@ int plan(BatchState *b, int ngrids, const mgcm_rt_columns *grids) {
     }
     p.lds = maxlay ? stage_bytes(maxlay) : 0;
   }
+  // DSOLFLUX items of every grid (skipped at run time where not asked for)
+  b->dsol_off = work.size();
+  for (int i = 0; i < ngrids; i++)
+    for (int col = 0; col < grids[i].ncol; col += DSOLFLUX_ITEM_COLS) work.push_back(make_uint2((unsigned)i, (unsigned)col));
+  b->dsol_nwork = (int)(work.size() - b->dsol_off);
   // continuum-point records, grid after grid
   b->clr_off.assign(ngrids, 0);
   size_t clr_total = 0;
@@ -247,13 +282,13 @@ int run_batched_impl(mgcm_rt_ctx *c, int ngrids, const mgcm_rt_columns *grids, c
     for (int k = 0; k < NCLASS; k++)
       for (int kind = 1; kind < NKIND; kind++)
         if ((bands & kind) && b->cls[k].nwork[kind] > 0) {
-          if (launch_class(precision, k, kind, c->dt, b, b->cls[k].nwork[kind],
+          if (launch_class(precision, k, kind, 0, c->dt, b, b->cls[k].nwork[kind],
                            std::min(b->cls[k].nwork[kind], 16 * ncu), st) != hipSuccess)
             return MGCM_RT_ERR_DEVICE;
         }
     for (int k = 0; k < NCLASS; k++) {
       ClassPlan &p = b->cls[k];
-      if (p.nwork[0] > 0) p.nblocks[0] = tune_class(precision, k, c->dt, b, ncu, st);
+      if (p.nwork[0] > 0) tune_class(precision, k, c->dt, b, ncu, st);
       for (int kind = 1; kind < NKIND; kind++) p.nblocks[kind] = std::min(p.nwork[kind], 16 * ncu);
     }
     // capture the class launches; the argument upload stays outside the graph
@@ -263,11 +298,16 @@ int run_batched_impl(mgcm_rt_ctx *c, int ngrids, const mgcm_rt_columns *grids, c
     for (int kind = 1; kind < NKIND && ok; kind++)
       for (int k = 0; k < NCLASS && ok; k++)
         if ((bands & kind) && b->cls[k].nwork[kind] > 0)
-          ok = launch_class(precision, k, kind, c->dt, b, b->cls[k].nwork[kind], b->cls[k].nblocks[kind], b->cap) ==
-               hipSuccess;
+          ok = launch_class(precision, k, kind, 0, c->dt, b, b->cls[k].nwork[kind], b->cls[k].nblocks[kind],
+                            b->cap) == hipSuccess;
     for (int k = 0; k < NCLASS && ok; k++)
       if (b->cls[k].nwork[0] > 0)
-        ok = launch_class(precision, k, 0, c->dt, b, b->cls[k].nwork[0], b->cls[k].nblocks[0], b->cap) == hipSuccess;
+        ok = launch_class(precision, k, 0, b->cls[k].waves, c->dt, b, b->cls[k].nwork[0], b->cls[k].nblocks[0],
+                          b->cap) == hipSuccess;
+    // surface fluxes for TEMPGR of the grids that ask for them (code/comp3.f:336-348)
+    if (ok && b->dsol_nwork > 0)
+      ok = launch_dsolflux_batch(c->d_const, b->d_grids, b->d_work + b->dsol_off, b->dsol_nwork,
+                                 std::min(b->dsol_nwork, 4 * ncu), b->cap) == hipSuccess;
     hipGraph_t g = nullptr;
     const bool ended = hipStreamEndCapture(b->cap, &g) == hipSuccess;
     if (!ok || !ended || !g) {
@@ -282,13 +322,32 @@ int run_batched_impl(mgcm_rt_ctx *c, int ngrids, const mgcm_rt_columns *grids, c
     b->sig = sig;
   }
   if (hipGraphLaunch(b->exec, st) != hipSuccess) return MGCM_RT_ERR_DEVICE;
-  // surface fluxes for TEMPGR of the grids that ask for them (code/comp3.f:336-348)
-  for (int i = 0; i < ngrids; i++)
-    if (outs[i].directsol && grids[i].ncol > 0 &&
-        launch_dsolflux(c->d_const, grids[i].ncol, grids[i].sol, grids[i].acosz, outs[i].detau, outs[i].diffvt,
-                        outs[i].directsol, outs[i].dnvflux, st) != hipSuccess)
-      return MGCM_RT_ERR_DEVICE;
   return MGCM_RT_OK;
+}
+
+int batched_tuning_impl(const mgcm_rt_ctx *c, int maxcls, mgcm_rt_batch_class *info) {
+  const BatchState *b = (const BatchState *)c->batch;
+  if (!b || b->sig.empty()) return 0;
+  int n = 0;
+  for (int k = 0; k < NCLASS; k++) {
+    const ClassPlan &p = b->cls[k];
+    if (p.nwork[0] == 0) continue;
+    if (n < maxcls && info) {
+      mgcm_rt_batch_class &o = info[n];
+      o.max_layers = KCLASS_NL[k] - 1;
+      o.items = p.nwork[0];
+      o.waves_per_simd = p.waves;
+      o.blocks = p.nblocks[0];
+      o.tuned_items = p.tuned_prefix;
+      for (int cnd = 0; cnd < NCAND; cnd++) {
+        o.cand_waves[cnd] = CAND_WAVES[cnd];
+        o.cand_mult[cnd] = CAND_MULT[cnd];
+        o.cand_ms[cnd] = p.cand_ms[cnd];
+      }
+    }
+    n++;
+  }
+  return n;
 }
 
 }  // namespace mgcm

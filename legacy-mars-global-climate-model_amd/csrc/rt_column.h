@@ -16,8 +16,9 @@
 //      72-75,121-162, code/sfluxi.f90:72-75,114-143);
 //   4. writes the fluxes and forms SUNTOT/IRTOTAL and the layer heating
 //      (code/comp3.f:677-681,735-738).
-// Nothing goes through global memory between the stages; the only HBM traffic
-// is the column inputs, the outputs and the k-tables (L2/MALL resident).
+// Between the stages, only the continuum-point records (written by the
+// continuum-point kernel, read in stage 3) go through global memory; the
+// per-item tridiagonal state lives in private (scratch) memory.
 #pragma once
 #include <type_traits>
 
@@ -289,12 +290,24 @@ __device__ __forceinline__ void column_block(const DevTables &dt, const RunArgs 
     const size_t o = (size_t)cl * nlev + K - 1;
     int mt, mp, nh;
     double tt, uu, wr;
-    tp_stage(rc, a.pmid[o], a.tmid[o], a.qh2o[o], tt, uu, mt, mp, nh, wr);
+    const double pm = a.pmid[o], tm = a.tmid[o], qw = a.qh2o[o], p0 = a.plev[o - 1], p1 = a.plev[o];
+    tp_stage(rc, pm, tm, qw, tt, uu, mt, mp, nh, wr);
     S.t[t] = tt;
     S.u[t] = uu;
     S.wr[t] = wr;
-    S.uk[t] = CMK * (a.plev[o] - a.plev[o - 1]);
+    S.uk[t] = CMK * (p1 - p0);
     S.base[t] = kcell(0, mt, mp, nh) | (nh == NH ? 1 : 0);
+    // a NaN among the column's inputs sets the range flag: the values this
+    // sub-level reads, plus (once per column) TAUREF/TAUREFCLD at levels 1 and
+    // L_LEVELS+1 and the column scalars (TLEV reaches the flag through the
+    // Planck index)
+    const size_t oa = (size_t)cl * (nlev + 1) + K - 1;
+    bool nan = isnan(pm) || isnan(tm) || isnan(qw) || isnan(p0) || isnan(p1) || isnan(a.tauref[oa]) ||
+               isnan(a.taurefcld[oa]);
+    if (K == 2)
+      nan = nan || isnan(a.tauref[oa - 1]) || isnan(a.taurefcld[oa - 1]) || isnan(a.tauref[oa + nsub]) ||
+            isnan(a.taurefcld[oa + nsub]) || isnan(a.acosz[cl]) || isnan(a.als[cl]) || isnan(a.albi[cl]);
+    if (nan) atomicOr(a.flag, 1);
   }
 
   Acc<SL> acc;
@@ -630,9 +643,8 @@ void k_rt(DevTables dt, RunArgs a) {
 // work list (grid index, first column) round-robin, largest layer counts first.
 // No counters: every block's share is fixed by its index.
 template <int NLM, bool MIXED>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MGCM_FP64_WAVES)))
-void k_rt_batch(DevTables dt, const RunArgs *__restrict__ grids, const uint2 *__restrict__ work, int nwork) {
-  extern __shared__ double smem[];
+__device__ __forceinline__ void batch_items(const DevTables &dt, const RunArgs *__restrict__ grids,
+                                            const uint2 *__restrict__ work, int nwork, double *smem) {
   mgcm_libm::load_lds_exptab(threadIdx.x, 64);
   for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
     __syncthreads();  // the previous item's reads of the staging are done
@@ -640,6 +652,22 @@ void k_rt_batch(DevTables dt, const RunArgs *__restrict__ grids, const uint2 *__
     const RunArgs a = grids[wk.x];
     column_block<NLM, MIXED>(dt, a, (int)wk.y, smem);
   }
+}
+// Two instances the batched tuner chooses between per layer class: two waves
+// per SIMD (at most 256 registers per lane) ...
+template <int NLM, bool MIXED>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MGCM_FP64_WAVES)))
+void k_rt_batch(DevTables dt, const RunArgs *__restrict__ grids, const uint2 *__restrict__ work, int nwork) {
+  extern __shared__ double smem[];
+  batch_items<NLM, MIXED>(dt, grids, work, nwork, smem);
+}
+// ... and one wave per SIMD (512 registers per lane: fewer spills of the
+// item state, half the waves in flight).
+template <int NLM, bool MIXED>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
+void k_rt_batch_w1(DevTables dt, const RunArgs *__restrict__ grids, const uint2 *__restrict__ work, int nwork) {
+  extern __shared__ double smem[];
+  batch_items<NLM, MIXED>(dt, grids, work, nwork, smem);
 }
 
 template <int NLM, bool MIXED>
@@ -682,31 +710,33 @@ inline int nl_class(int nl) {
   return NCLASS - 1;
 }
 
-// kind 0: column kernel; 1: VIS continuum points; 2: IR continuum points
+// kind 0: column kernel (waves per SIMD 1 or 2); 1: VIS continuum points; 2: IR continuum points
 template <int NLM, bool MIXED>
-static hipError_t launch_batch_nlm(int kind, const DevTables &dt, const RunArgs *grids, const uint2 *work, int nwork,
-                                   int nblocks, size_t lds, hipStream_t st) {
+static hipError_t launch_batch_nlm(int kind, int waves, const DevTables &dt, const RunArgs *grids, const uint2 *work,
+                                   int nwork, int nblocks, size_t lds, hipStream_t st) {
   if (kind == 1)
     hipLaunchKernelGGL((k_rt_clear_batch<NLM, true>), dim3(nblocks), dim3(64), mgcm_libm::LDS_EXPTAB_BYTES, st, dt,
                        grids, work, nwork);
   else if (kind == 2)
     hipLaunchKernelGGL((k_rt_clear_batch<NLM, false>), dim3(nblocks), dim3(64), mgcm_libm::LDS_EXPTAB_BYTES, st, dt,
                        grids, work, nwork);
+  else if (waves == 1)
+    hipLaunchKernelGGL((k_rt_batch_w1<NLM, MIXED>), dim3(nblocks), dim3(64), lds, st, dt, grids, work, nwork);
   else
     hipLaunchKernelGGL((k_rt_batch<NLM, MIXED>), dim3(nblocks), dim3(64), lds, st, dt, grids, work, nwork);
   return hipGetLastError();
 }
 
 template <bool MIXED>
-hipError_t launch_batch_class(int cls, int kind, const DevTables &dt, const RunArgs *grids, const uint2 *work,
-                              int nwork, int nblocks, size_t lds, hipStream_t st) {
+hipError_t launch_batch_class(int cls, int kind, int waves, const DevTables &dt, const RunArgs *grids,
+                              const uint2 *work, int nwork, int nblocks, size_t lds, hipStream_t st) {
   switch (cls) {
-    case 0: return launch_batch_nlm<17, MIXED>(kind, dt, grids, work, nwork, nblocks, lds, st);
-    case 1: return launch_batch_nlm<25, MIXED>(kind, dt, grids, work, nwork, nblocks, lds, st);
-    case 2: return launch_batch_nlm<37, MIXED>(kind, dt, grids, work, nwork, nblocks, lds, st);
-    case 3: return launch_batch_nlm<49, MIXED>(kind, dt, grids, work, nwork, nblocks, lds, st);
-    case 4: return launch_batch_nlm<65, MIXED>(kind, dt, grids, work, nwork, nblocks, lds, st);
-    default: return launch_batch_nlm<MAX_NLAY + 1, MIXED>(kind, dt, grids, work, nwork, nblocks, lds, st);
+    case 0: return launch_batch_nlm<17, MIXED>(kind, waves, dt, grids, work, nwork, nblocks, lds, st);
+    case 1: return launch_batch_nlm<25, MIXED>(kind, waves, dt, grids, work, nwork, nblocks, lds, st);
+    case 2: return launch_batch_nlm<37, MIXED>(kind, waves, dt, grids, work, nwork, nblocks, lds, st);
+    case 3: return launch_batch_nlm<49, MIXED>(kind, waves, dt, grids, work, nwork, nblocks, lds, st);
+    case 4: return launch_batch_nlm<65, MIXED>(kind, waves, dt, grids, work, nwork, nblocks, lds, st);
+    default: return launch_batch_nlm<MAX_NLAY + 1, MIXED>(kind, waves, dt, grids, work, nwork, nblocks, lds, st);
   }
 }
 

@@ -34,10 +34,11 @@ struct mgcm_rt_ctx {
 namespace mgcm {
 hipError_t launch_plain(const DevTables &dt, const RunArgs &a, hipStream_t st);
 hipError_t launch_mixed(const DevTables &dt, const RunArgs &a, hipStream_t st);
-hipError_t launch_batch_plain(int cls, int kind, const DevTables &dt, const RunArgs *grids, const uint2 *work,
-                              int nwork, int nblocks, size_t lds, hipStream_t st);
-hipError_t launch_batch_mixed(int cls, int kind, const DevTables &dt, const RunArgs *grids, const uint2 *work,
-                              int nwork, int nblocks, size_t lds, hipStream_t st);
+// waves: the column kernel instance's waves per SIMD (1 or 2); kind 0 only
+hipError_t launch_batch_plain(int cls, int kind, int waves, const DevTables &dt, const RunArgs *grids,
+                              const uint2 *work, int nwork, int nblocks, size_t lds, hipStream_t st);
+hipError_t launch_batch_mixed(int cls, int kind, int waves, const DevTables &dt, const RunArgs *grids,
+                              const uint2 *work, int nwork, int nblocks, size_t lds, hipStream_t st);
 size_t clr_workspace_bytes(int ncol, int nlay);
 // Splits the precision argument into the path (MGCM_RT_FP64/MIXED) and the
 // band mask; MGCM_RT_ERR_PRECISION for anything else.
@@ -47,6 +48,9 @@ void fill_args(RunArgs &a, const mgcm_rt_columns *in, const mgcm_rt_fluxes *out,
 int run_batched_impl(mgcm_rt_ctx *ctx, int ngrids, const mgcm_rt_columns *grids, const mgcm_rt_fluxes *outs,
                      int precision, hipStream_t st);
 void batch_free(void *state);
+// Waits for stream st only, then reads and clears the context's range flag.
+int check_on_stream(mgcm_rt_ctx *ctx, hipStream_t st);
+int batched_tuning_impl(const mgcm_rt_ctx *ctx, int maxcls, mgcm_rt_batch_class *info);
 // mgcm_rt_run's launches on a grid whose arguments are checked, with a
 // continuum-point workspace of clr_workspace_bytes(in->ncol, in->nlay) bytes.
 hipError_t run_grid(const mgcm_rt_ctx *c, const mgcm_rt_columns *in, const mgcm_rt_fluxes *out, int path,
@@ -56,4 +60,9 @@ void host_free(void *state);
 hipError_t launch_dsolflux(const RtConst *rc, int ncol, const double *sol, const double *acosz,
                            const double *detau, const double *diffvt, double *directsol, double *dnvflux,
                            hipStream_t st);
+// DSOLFLUX of many grids: a persistent launch over (grid, first column) items of
+// 16 columns; grids whose RunArgs::directsol is null are skipped.
+constexpr int DSOLFLUX_ITEM_COLS = 16;
+hipError_t launch_dsolflux_batch(const RtConst *rc, const RunArgs *grids, const uint2 *work, int nwork, int nblocks,
+                                 hipStream_t st);
 }  // namespace mgcm

@@ -280,8 +280,8 @@ int run_host(mgcm_rt_ctx *c, const mgcm_rt_columns *in, const mgcm_rt_fluxes *ou
   K = std::min(K, ncol);
   const size_t nchunk = (ncol + K - 1) / K;
 
-  // earlier work on the context is finished before its buffers are reused
-  HIP_TRY(hipDeviceSynchronize());
+  // the staging below belongs to this entry point alone, and its previous call
+  // returned only after all of its work had ended
   HostPipe *h = pipe_get(c);
   if (!h) return MGCM_RT_ERR_NOMEM;
   if (!pin_grow(h->pin_in, h->pin_in_bytes, K * in_pc * 8, PIN_IN_FLAGS) ||
@@ -305,7 +305,6 @@ int run_host(mgcm_rt_ctx *c, const mgcm_rt_columns *in, const mgcm_rt_fluxes *ou
       if (hipMalloc((void **)&p, clr_workspace_bytes((int)K, in->nlay)) != hipSuccess) return MGCM_RT_ERR_NOMEM;
     h->clr_bytes = clr_workspace_bytes((int)K, in->nlay);
   }
-  HIP_TRY(hipMemsetAsync(c->d_flag, 0, sizeof(int), h->up));  // every run waits on the copy stream
 
   auto chunk_cols = [&](size_t i) { return std::min(K, ncol - i * K); };
   auto region = [&](size_t i) { return c->d_host + i * K * region_pc; };  // chunks before i are all K wide
@@ -384,7 +383,9 @@ int run_host(mgcm_rt_ctx *c, const mgcm_rt_columns *in, const mgcm_rt_fluxes *ou
     (void)hipDeviceSynchronize();
     return st;
   }
-  return mgcm_rt_check(c);
+  // every run of the call has ended (its outputs were downloaded); the flag
+  // holds them and any earlier run of the context not yet checked
+  return check_on_stream(c, h->up);
 }
 }  // namespace
 

@@ -62,6 +62,9 @@ struct RunArgs {
   // of the water-ice cloud, [ncol][nb][nlev+1]; QEXTREFDST, QEXTREFCLD [ncol][nlev+1]
   const double *aq[12];
   const double *qref_dst, *qref_cld;
+  // DSOLFLUX's direct solar flux at the surface and DNVFLUX, [ncol], or null
+  // (the batched path's DSOLFLUX kernel reads them here)
+  double *directsol, *dnvflux;
 };
 
 struct DevTables {

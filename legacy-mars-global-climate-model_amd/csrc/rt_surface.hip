@@ -22,9 +22,9 @@ struct SurfArgs {
 // so the sum is the reference's bit for bit.  A term past DETAU > 5 adds +0,
 // which leaves the non-negative sum unchanged.
 constexpr int DSOL_COLS = 16, DSOL_TERMS = NG * NV;
-__global__ __launch_bounds__(64) void k_dsolflux(const RtConst *__restrict__ rc, SurfArgs a) {
-  __shared__ double term[DSOL_COLS][DSOL_TERMS];
-  const int col0 = blockIdx.x * DSOL_COLS, lane = threadIdx.x;
+__device__ __forceinline__ void dsolflux_block(const RtConst *__restrict__ rc, const SurfArgs &a, int col0,
+                                               double (*term)[DSOL_TERMS]) {
+  const int lane = threadIdx.x;
   for (int t = lane; t < DSOL_COLS * DSOL_TERMS; t += 64) {
     const int cc = t / DSOL_TERMS, e = t - cc * DSOL_TERMS;  // e = ng*NV + nw: DETAU(nw, ng) of the column
     const int col = col0 + cc;
@@ -43,12 +43,48 @@ __global__ __launch_bounds__(64) void k_dsolflux(const RtConst *__restrict__ rc,
   }
   __syncthreads();
   const int col = col0 + lane;
-  if (lane >= DSOL_COLS || col >= a.ncol) return;
-  double directsol = 0.0;
-  for (int nw = 0; nw < NV; nw++)
-    for (int ng = 0; ng < NG; ng++) directsol = directsol + term[lane][ng * NV + nw];
-  a.directsol[col] = directsol;
-  if (a.dnvflux) a.dnvflux[col] = a.diffvt[col] + directsol;  // DNVFLUX = DNDIFFV + DIRECTSOL
+  if (lane < DSOL_COLS && col < a.ncol) {
+    double directsol = 0.0;
+    for (int nw = 0; nw < NV; nw++)
+      for (int ng = 0; ng < NG; ng++) directsol = directsol + term[lane][ng * NV + nw];
+    a.directsol[col] = directsol;
+    if (a.dnvflux) a.dnvflux[col] = a.diffvt[col] + directsol;  // DNVFLUX = DNDIFFV + DIRECTSOL
+  }
+}
+
+__global__ __launch_bounds__(64) void k_dsolflux(const RtConst *__restrict__ rc, SurfArgs a) {
+  __shared__ double term[DSOL_COLS][DSOL_TERMS];
+  dsolflux_block(rc, a, blockIdx.x * DSOL_COLS, term);
+}
+
+// The batched path: a persistent grid of blocks walks a work list of (grid,
+// first column) 16-column items; grids that asked for no surface fluxes
+// (directsol null, uniform over an item) are passed over.
+__global__ __launch_bounds__(64) void k_dsolflux_batch(const RtConst *__restrict__ rc,
+                                                       const RunArgs *__restrict__ grids,
+                                                       const uint2 *__restrict__ work, int nwork) {
+  __shared__ double term[DSOL_COLS][DSOL_TERMS];
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const uint2 wk = work[w];
+    const RunArgs &g = grids[wk.x];
+    if (!g.directsol) continue;
+    SurfArgs a;
+    a.ncol = g.ncol;
+    for (int b = 0; b < NV; b++) a.sol[b] = g.sol[b];
+    a.acosz = g.acosz;
+    a.detau = g.detau;
+    a.diffvt = g.diffvt;
+    a.directsol = g.directsol;
+    a.dnvflux = g.dnvflux;
+    __syncthreads();  // the previous item's reads of term are done
+    dsolflux_block(rc, a, (int)wk.y, term);
+  }
+}
+
+hipError_t launch_dsolflux_batch(const RtConst *rc, const RunArgs *grids, const uint2 *work, int nwork, int nblocks,
+                                 hipStream_t st) {
+  hipLaunchKernelGGL(k_dsolflux_batch, dim3(nblocks), dim3(64), 0, st, rc, grids, work, nwork);
+  return hipGetLastError();
 }
 
 hipError_t launch_dsolflux(const RtConst *rc, int ncol, const double *sol, const double *acosz,
@@ -161,3 +197,4 @@ extern "C" int mgcm_rt_history_fields(const mgcm_rt_columns *in, const mgcm_rt_f
   hipLaunchKernelGGL(mgcm::k_history, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, a);
   return hipGetLastError() == hipSuccess ? MGCM_RT_OK : MGCM_RT_ERR_DEVICE;
 }
+static_assert(mgcm::DSOLFLUX_ITEM_COLS == mgcm::DSOL_COLS, "batched DSOLFLUX item width");

@@ -4,8 +4,12 @@
 ! results to a second stream file.
 !
 !   mgcm_rt_driver [--per-column] [--aerosol <tracer file> <aerosol data dir>]
+!                  [--dust <dust file>]
 !                  <case file> <result file> [data directory | -] [history file]
 !
+! --dust replaces the case file's TAUREF by the prescribed (Conrath) dust of
+! DUSTPROFILE + FILLTAUCUM (mgcm_rt_dust_profile).  Dust file (FP64 stream):
+! PSF, PTROP, RPTAU, then TAUTOT(ncol), CONR(ncol), P(ncol), SIGMA(L_LEVELS).
 ! --aerosol runs COMP3's active dust and water-ice clouds (code/comp3.f:595-631):
 ! INITCLD's tables from the four optical-property files in <aerosol data dir>,
 ! then OPT_DST and OPT_CLD on the tracers QTRACE(nlay, 6, ncol) (FP64 stream
@@ -65,8 +69,10 @@ program mgcm_rt_driver
   real(4), allocatable :: fuptopv(:,:), fdntopv(:,:), fupsurfv(:,:), fdnsurfv(:,:)
   real(4), allocatable :: fuptopir(:,:), fupsurfir(:,:), fdnsurfir(:,:)
   integer :: n, a0, nargs
-  logical :: percol, aeron
-  character(len=4096) :: ftracer, aerdir
+  logical :: percol, aeron, duston
+  character(len=4096) :: ftracer, aerdir, fdust
+  real(c_double) :: psf, ptrop, rptau
+  real(c_double), allocatable :: tautot(:), conr(:), psurf(:), sigma(:)
   real(c_double), allocatable, target :: qtrace(:,:,:)
   real(c_double), allocatable, target :: dqxv(:,:,:), dqsv(:,:,:), dgv(:,:,:), dqxi(:,:,:), dqsi(:,:,:), dgi(:,:,:)
   real(c_double), allocatable, target :: cqxv(:,:,:), cqsv(:,:,:), cgv(:,:,:), cqxi(:,:,:), cqsi(:,:,:), cgi(:,:,:)
@@ -78,6 +84,7 @@ program mgcm_rt_driver
   a0 = 0
   percol = .false.
   aeron = .false.
+  duston = .false.
   do while (a0 < nargs)
     call get_command_argument(a0 + 1, fin)
     if (trim(fin) == '--per-column') then
@@ -88,13 +95,17 @@ program mgcm_rt_driver
       call get_command_argument(a0 + 2, ftracer)
       call get_command_argument(a0 + 3, aerdir)
       a0 = a0 + 3
+    else if (trim(fin) == '--dust' .and. a0 + 2 <= nargs) then
+      duston = .true.
+      call get_command_argument(a0 + 2, fdust)
+      a0 = a0 + 2
     else
       exit
     end if
   end do
   if (nargs - a0 < 2 .or. nargs - a0 > 4) then
     write(*, '(a)') 'usage: mgcm_rt_driver [--per-column] [--aerosol <tracer file> <aerosol data dir>] '// &
-                    '<case file> <result file> [data directory | -] [history file]'
+                    '[--dust <dust file>] <case file> <result file> [data directory | -] [history file]'
     stop 2
   end if
   call get_command_argument(a0 + 1, fin)
@@ -131,7 +142,19 @@ program mgcm_rt_driver
   cumtauv = 0.0d0
 
   status = MGCM_RT_OK
-  if (len_trim(datadir) > 0) call mgcm_rt_laginterp_files(datadir, pgref, pfgasref, co2i, co2v, fzeroi, fzerov, status)
+  if (duston) then
+    ! COMP3 without active dust: the prescribed dust's TAUREF (code/comp3.f:591-593)
+    allocate(tautot(ncol), conr(ncol), psurf(ncol), sigma(nlev))
+    open(newunit=u, file=trim(fdust), access='stream', form='unformatted', status='old', iostat=ios)
+    if (ios /= 0) then
+      status = MGCM_RT_ERR_IO
+    else
+      read(u) psf, ptrop, rptau, tautot, conr, psurf, sigma
+      close(u)
+      call mgcm_rt_dust_profile(ncol, nlay, psf, ptrop, rptau, 100, tautot, conr, psurf, sigma, tauref, status)
+    end if
+  end if
+  if (status == MGCM_RT_OK .and. len_trim(datadir) > 0) call mgcm_rt_laginterp_files(datadir, pgref, pfgasref, co2i, co2v, fzeroi, fzerov, status)
   if (status == MGCM_RT_OK) &
     call mgcm_rt_setup(co2v, co2i, pfgasref, tgasref, wrefh2o, gwv, gwi, fzerov, fzeroi, tauray, dwni, &
                        planckir, qxvdst, qsvdst, gvdst, qxidst, qsidst, gidst, qxvcld, qsvcld, gvcld, &

@@ -71,11 +71,21 @@ module mgcm_rt_mod
     type(c_ptr) :: qxv, qsv, gv, qxi, qsi, gi, qextref, tauref, tau2
   end type
 
+  ! mirrors mgcm_rt_dust_columns: the prescribed (Conrath) dust of each column
+  ! (DUSTPROFILE + FILLTAUCUM, code/dustprofile.f, code/filltaucum.f)
+  type, bind(C), public :: mgcm_rt_dust_columns
+    integer(c_int) :: ncol, nlay
+    type(c_ptr) :: tautot, conr, p, sigma, prdst
+    integer(c_int) :: npdst
+    real(c_double) :: rptau, ptrop
+  end type
+
   integer, parameter, public :: MGCM_RT_AER_TABLE_DOUBLES = 11552
 
   public :: mgcm_rt_setup, mgcm_rt_radiation, mgcm_rt_message, mgcm_rt_laginterp_files, mgcm_rt_column
   public :: mgcm_rt_read_aerosol_tables, mgcm_rt_set_aerosol_tables, mgcm_rt_opt_dst_host, mgcm_rt_opt_cld_host
   public :: mgcm_rt_aerosol_setup, mgcm_rt_opt_aer
+  public :: mgcm_rt_dust_grid, mgcm_rt_dust_tauref_host, mgcm_rt_dust_profile
 
   interface
     integer(c_int) function mgcm_rt_init(tables, ctx) bind(C, name="mgcm_rt_init")
@@ -160,6 +170,17 @@ module mgcm_rt_mod
       type(c_ptr), value :: ctx
       type(mgcm_rt_tracers), intent(in) :: tr
       type(mgcm_rt_aerosol_out), intent(in) :: out
+    end function
+    integer(c_int) function mgcm_rt_dust_grid(psf, ptrop, npdst, prdst) bind(C, name="mgcm_rt_dust_grid")
+      import :: c_int, c_double
+      real(c_double), value :: psf, ptrop
+      integer(c_int), value :: npdst
+      real(c_double), intent(out) :: prdst(*)
+    end function
+    integer(c_int) function mgcm_rt_dust_tauref_host(dc, tauref, taudst) bind(C, name="mgcm_rt_dust_tauref_host")
+      import :: c_int, c_ptr, mgcm_rt_dust_columns
+      type(mgcm_rt_dust_columns), intent(in) :: dc
+      type(c_ptr), value :: tauref, taudst
     end function
     type(c_ptr) function mgcm_rt_strerror(status) bind(C, name="mgcm_rt_strerror")
       import :: c_int, c_ptr
@@ -371,6 +392,41 @@ contains
       status = mgcm_rt_opt_cld_host(ctx, tr, o)
     else
       status = mgcm_rt_opt_dst_host(ctx, tr, o)
+    end if
+  end subroutine
+
+  ! Prescribed dust as DUSTPROFILE and FILLTAUCUM give it (code/dustprofile.f:
+  ! 30-113, code/filltaucum.f:33-52): PRDST(npdst) from PSF and PTROP, then each
+  ! column's Conrath profile with opacity TAUTOT at RPTAU and parameter CONR,
+  ! surface pressure P = PS - PTROP, on the sub-levels of SIGMA(L_LEVELS):
+  ! TAUREF(L_LEVELS+1, ncol) and, optionally, TAUDST(npdst, ncol).
+  subroutine mgcm_rt_dust_profile(ncol, nlay, psf, ptrop, rptau, npdst, tautot, conr, p, sigma, tauref, &
+                                  status, taudst)
+    integer, intent(in) :: ncol, nlay, npdst
+    real(c_double), intent(in) :: psf, ptrop, rptau
+    real(c_double), intent(in), target :: tautot(ncol), conr(ncol), p(ncol), sigma(2*nlay+3)
+    real(c_double), intent(out), target :: tauref(2*nlay+4, ncol)
+    integer(c_int), intent(out) :: status
+    real(c_double), intent(out), target, optional :: taudst(npdst, ncol)
+    real(c_double), allocatable, target :: prdst(:)
+    type(mgcm_rt_dust_columns) :: dc
+    allocate(prdst(npdst))
+    status = mgcm_rt_dust_grid(psf, ptrop, int(npdst, c_int), prdst)
+    if (status /= MGCM_RT_OK) return
+    dc%ncol = ncol
+    dc%nlay = nlay
+    dc%tautot = c_loc(tautot)
+    dc%conr = c_loc(conr)
+    dc%p = c_loc(p)
+    dc%sigma = c_loc(sigma)
+    dc%prdst = c_loc(prdst)
+    dc%npdst = npdst
+    dc%rptau = rptau
+    dc%ptrop = ptrop
+    if (present(taudst)) then
+      status = mgcm_rt_dust_tauref_host(dc, c_loc(tauref), c_loc(taudst))
+    else
+      status = mgcm_rt_dust_tauref_host(dc, c_loc(tauref), c_null_ptr)
     end if
   end subroutine
 

@@ -94,6 +94,12 @@ class Fluxes(C.Structure):
         "nfluxtopv", "nfluxtopi", "diffvt", "detau", "cumtauv", "hr", "directsol", "dnvflux")]
 
 
+class BatchClass(C.Structure):
+    _fields_ = [("max_layers", C.c_int), ("items", C.c_int), ("waves_per_simd", C.c_int), ("blocks", C.c_int),
+                ("tuned_items", C.c_int), ("cand_waves", C.c_int * 6), ("cand_mult", C.c_int * 6),
+                ("cand_ms", C.c_float * 6)]
+
+
 IN_KEYS = ("plev", "tlev", "pmid", "tmid", "qh2o", "tauref", "taurefcld", "als", "albi", "acosz")
 OUT_KEYS = ("fmnetv", "fluxupv", "fluxdnv", "fmneti", "fluxupi", "fluxdni",
             "nfluxtopv", "nfluxtopi", "diffvt", "detau", "cumtauv", "hr")
@@ -128,6 +134,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         lib.mgcm_rt_run_batched.argtypes = [C.c_void_p, C.c_int, C.POINTER(Columns), C.POINTER(Fluxes),
                                             C.c_int, C.c_void_p]
         lib.mgcm_rt_check.argtypes = [C.c_void_p]
+        lib.mgcm_rt_check_stream.argtypes = [C.c_void_p, C.c_void_p]
+        lib.mgcm_rt_batched_tuning.argtypes = [C.c_void_p, C.c_int, C.POINTER(BatchClass)]
         lib.mgcm_rt_planck_table.argtypes = [DP, DP, DP]
         lib.mgcm_rt_laginterp.argtypes = [DP, DP, C.c_int, DP, DP]
         lib.mgcm_rt_read_ktable.argtypes = [C.c_char_p, C.c_int, DP, DP]
@@ -269,8 +277,24 @@ class Radiation:
                "mgcm_rt_run_batched")
         del structs
 
+    def batched_tuning(self) -> list:
+        """The batched tuner's choice per layer class of the last planned batch."""
+        info = (BatchClass * 8)()
+        n = self.lib.mgcm_rt_batched_tuning(self.ctx, 8, info)
+        _check(min(n, 0), "mgcm_rt_batched_tuning")
+        out = []
+        for c in info[:n]:
+            cands = [{"waves_per_simd": c.cand_waves[j], "mult": c.cand_mult[j], "ms": float(c.cand_ms[j])}
+                     for j in range(6) if c.cand_ms[j] > 0]
+            out.append({"max_layers": c.max_layers, "items": c.items, "waves_per_simd": c.waves_per_simd,
+                        "blocks": c.blocks, "tuned_items": c.tuned_items, "candidates": cands})
+        return out
+
     def check(self) -> None:
         _check(self.lib.mgcm_rt_check(self.ctx), "mgcm_rt_check")
+
+    def check_stream(self, stream: int = 0) -> None:
+        _check(self.lib.mgcm_rt_check_stream(self.ctx, C.c_void_p(stream)), "mgcm_rt_check_stream")
 
     def dsolflux(self, cols: dict, out: dict, stream: int = 0) -> dict:
         """DSOLFLUX and COMP3's DNVFLUX from a run's outputs (device tensors): returns

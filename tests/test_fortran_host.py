@@ -208,3 +208,33 @@ def test_fortran_driver_active_dust_and_clouds(tmp_path):
     for key in casefile.RESULT_KEYS:
         assert np.array_equal(got[key], want[key]), key
     assert np.any(d["tauref"] > 0) and np.any(k["tauref"] > 0)  # dust and cloud are both active
+
+
+@pytest.mark.gpu
+def test_fortran_driver_prescribed_dust(tmp_path):
+    """COMP3's prescribed-dust path from Fortran: PRDST (mgcm_rt_dust_grid) and each
+    column's TAUREF (mgcm_rt_dust_tauref_host) through the module's bind(C) interfaces
+    (mgcm_rt_dust_profile), then the radiation; equal bit for bit to the same chain
+    through the Python binding."""
+    from mgcm_rt import Radiation, casefile, dust, synth
+    nlay, ncol = 24, 96
+    t, c = synth.make_workload(31, ncol, nlay)
+    rng = np.random.Generator(np.random.PCG64(31))
+    psf, ptrop, rptau = 7.01, synth.PTROP, 6.1
+    tautot = rng.uniform(0.1, 1.0, ncol)
+    conr = np.array([dust.conrath_nu(0.0, j, 36, True) for j in rng.integers(1, 37, ncol)])
+    p = c["plev"][:, -1] - ptrop
+    sigma = synth.sigma_levels(nlay)
+    case, res, df = str(tmp_path / "case.bin"), str(tmp_path / "res.bin"), str(tmp_path / "dust.bin")
+    casefile.write_case(case, t, c, 0)
+    np.concatenate([[psf, ptrop, rptau], tautot, conr, p, sigma]).astype(np.float64).tofile(df)
+    r = subprocess.run([DRIVER, "--dust", df, case, res], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    status, got = casefile.read_result(res, ncol, nlay)
+    assert status == 0
+    prdst = dust.pressure_grid(psf, ptrop)
+    tr = dust.tauref(prdst, rptau, ptrop, tautot, conr, p, sigma, nlay)["tauref"]
+    assert np.any(tr > 0)
+    want = Radiation(t).run_host(c | {"tauref": tr})
+    for key in casefile.RESULT_KEYS:
+        assert np.array_equal(got[key], want[key]), key

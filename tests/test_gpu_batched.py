@@ -88,3 +88,61 @@ def test_batched_rejects_bad_layer_count():
     with pytest.raises(RadiationError) as e:
         _run(rt, [bad], [(4, 24)])
     assert e.value.status == -2
+
+
+# Per-class work lists longer than the resident capacity: the <= 24-layer class
+# has 50,000 columns, 12,500 4-column items -- more than 4x the 2,048 one-wave
+# blocks that two waves per SIMD keep resident on 256 CUs, so every persistent
+# block of every tuner candidate walks several items -- and 5,556 VIS (9-column)
+# and 4,167 IR (12-column) continuum-point items, more than the 4,096 blocks of
+# those launches.  The 36- and 60-layer classes exceed one resident round too.
+AT_SHAPE = [(25000, 24), (25000, 20), (8000, 36), (5000, 60), (3000, 16)]
+
+
+def test_batched_at_shape_multi_item(oracle):
+    """The persistent blocks' multi-item loop (staging reuse, the per-item barrier) and
+    the tuner's candidates under a check: every grid equals its own single-grid run bit
+    for bit, sampled columns match the oracle within 1e-10, and the DSOLFLUX/DNVFLUX of
+    the grids that ask for them (inside the batch's graph) equal mgcm_rt_dsolflux's."""
+    import torch
+    from mgcm_rt import Radiation, alloc_outputs, synth, to_device
+    t, cols = _batch(46, AT_SHAPE)
+    rt = Radiation(t)
+    dc = [to_device(c) for c in cols]
+    do = [to_device(alloc_outputs(n, nlay)) for n, nlay in AT_SHAPE]
+    for i in (0, 2, 3):  # surface fluxes for some grids only
+        do[i]["directsol"] = torch.full((AT_SHAPE[i][0],), -1.0, dtype=torch.float64, device="cuda")
+        do[i]["dnvflux"] = torch.full_like(do[i]["directsol"], -1.0)
+    s = torch.cuda.current_stream().cuda_stream
+    rt.run_batched(dc, do, 0, s)
+    torch.cuda.synchronize()
+    rt.check()
+    tune = rt.batched_tuning()
+    print(tune, flush=True)
+    c24 = [c for c in tune if c["max_layers"] == 24][0]
+    assert c24["items"] == 12500 and c24["blocks"] < c24["items"], c24  # several items per block
+    assert len(c24["candidates"]) >= 4 and c24["tuned_items"] > 0, c24
+    rng = np.random.Generator(np.random.PCG64(5))
+    for i, ((n, nlay), c, o) in enumerate(zip(AT_SHAPE, cols, do)):
+        got = {k: v.cpu().numpy() for k, v in o.items()}
+        single = rt.run_host(c)
+        for k in KEYS:
+            assert np.array_equal(got[k], single[k]), (n, nlay, k)
+        if "directsol" in got:
+            ref = rt.dsolflux(dc[i], {k: torch.from_numpy(v).cuda() for k, v in single.items()}, s)
+            torch.cuda.synchronize()
+            assert np.array_equal(got["directsol"], ref["directsol"].cpu().numpy()), (n, nlay)
+            assert np.array_equal(got["dnvflux"], ref["dnvflux"].cpu().numpy()), (n, nlay)
+        pick = np.sort(rng.choice(n, 48, replace=False))
+        sub = {k: (v[pick] if isinstance(v, np.ndarray) and v.shape[:1] == (n,) else v) for k, v in c.items()}
+        sub["ncol"] = len(pick)
+        want = oracle.run(synth.make_tables(46, nlay), sub)
+        for k in KEYS:
+            assert rel_err(got[k][pick], want[k]) <= 1e-10, (n, nlay, k)
+    # a replay of the same shape (graph, tuned launch shapes) gives the same bits
+    first = [{k: v.clone() for k, v in o.items()} for o in do]
+    rt.run_batched(dc, do, 0, s)
+    torch.cuda.synchronize()
+    for a, b in zip(first, do):
+        for k in a:
+            assert torch.equal(a[k], b[k]), k

@@ -73,6 +73,50 @@ def test_range_flag_reported():
     assert e.value.status == -5
 
 
+@pytest.mark.parametrize("key,where", [("plev", (2, 7)), ("tlev", (1, 30)), ("pmid", (3, 12)), ("tmid", (4, 40)),
+                                       ("qh2o", (5, 20)), ("tauref", (6, 0)), ("tauref", (6, 51)),
+                                       ("taurefcld", (0, 33)), ("taurefcld", (7, 51)), ("als", (1,)),
+                                       ("albi", (2,)), ("acosz", (3,))])
+def test_nan_input_sets_range_flag(key, where):
+    """A NaN in any column input the path reads -- every level of PLEV, TLEV, PMID, TMID,
+    QH2O it uses, TAUREF/TAUREFCLD including levels 1 and L_LEVELS+1, and the column's
+    ALS/ALBI/ACOSZ -- is reported as MGCM_RT_ERR_RANGE; the next clean call is OK."""
+    from mgcm_rt import Radiation, RadiationError, synth
+    t, c = synth.make_workload(73, 8, 24)
+    rt = Radiation(t)
+    bad = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in c.items()}
+    bad[key][where] = np.nan
+    with pytest.raises(RadiationError) as e:
+        rt.run_host(bad)
+    assert e.value.status == -5
+    rt.run_host(c)
+
+
+def test_range_flag_of_an_unchecked_device_run_is_kept():
+    """A range error of an asynchronous mgcm_rt_run that no check has read yet is reported
+    by the next check -- here the one mgcm_rt_run_host makes -- and then cleared;
+    mgcm_rt_check_stream reads it waiting on the run's stream only."""
+    import torch
+    from mgcm_rt import FP64, Radiation, RadiationError, alloc_outputs, synth, to_device
+    t, c = synth.make_workload(74, 16, 24)
+    rt = Radiation(t)
+    bad = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in c.items()}
+    bad["tlev"][3, 10] = 950.0
+    s = torch.cuda.Stream()
+    dc, do = to_device(bad), to_device(alloc_outputs(16, 24))
+    rt.run_device(dc, do, FP64, s.cuda_stream)   # no check: the flag stays pending
+    s.synchronize()
+    with pytest.raises(RadiationError) as e:
+        rt.run_host(c)                            # clean call, pending flag reported
+    assert e.value.status == -5
+    rt.run_host(c)                                # cleared by that check
+    rt.run_device(dc, do, FP64, s.cuda_stream)
+    with pytest.raises(RadiationError) as e:
+        rt.check_stream(s.cuda_stream)
+    assert e.value.status == -5
+    rt.check_stream(s.cuda_stream)
+
+
 def test_deterministic():
     from mgcm_rt import Radiation, synth
     t, c = synth.make_workload(73, 101, 24)
