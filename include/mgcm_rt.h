@@ -55,7 +55,8 @@ enum {
                                 PLEV, TLEV, PMID, TMID, QH2O, TAUREF, TAUREFCLD, ALS, ALBI or ACOSZ
                                 of a column (the per-level aerosol arrays are not checked) */
   MGCM_RT_ERR_PRECISION = -6, /* unknown precision mode */
-  MGCM_RT_ERR_IO = -7         /* data file missing, unreadable or not in the expected record layout */
+  MGCM_RT_ERR_IO = -7,        /* data file missing, unreadable or not in the expected record layout */
+  MGCM_RT_ERR_SCRATCH = -8    /* grid larger than mgcm_rt_run_noalloc's scratch holds */
 };
 
 enum {
@@ -159,6 +160,22 @@ int mgcm_rt_destroy(mgcm_rt_ctx *ctx);
  * it can be captured in a HIP graph. */
 int mgcm_rt_run(mgcm_rt_ctx *ctx, const mgcm_rt_columns *in, const mgcm_rt_fluxes *out,
                 int precision, void *stream);
+
+/* mgcm_rt_run for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * same kernels and results bit for bit, with the continuum-point records in a
+ * static device array of MGCM_RT_NOALLOC_SCRATCH_BYTES (128 MiB, held by every
+ * device the library runs on until the process ends) instead of the context's
+ * workspace.  Nothing is allocated, ever.  A grid needing more than the array
+ * -- ncol * 12 * (2*nlay+4) * 8 bytes: 26,886 columns at 24 layers, 3,461 at 99
+ * -- returns MGCM_RT_ERR_SCRATCH; run it in column slices (every array is per
+ * column, so a slice is a pointer offset).  All calls on one device share the
+ * array: calls that overlap on a device (any context, any stream) race, so
+ * issue them on one stream or synchronise between them.  No counterpart in the
+ * reference. */
+#define MGCM_RT_NOALLOC_SCRATCH_BYTES (128ull << 20)
+int mgcm_rt_run_noalloc(mgcm_rt_ctx *ctx, const mgcm_rt_columns *in, const mgcm_rt_fluxes *out,
+                        int precision, void *stream);
 
 /* Same with host pointers: copies in, runs, copies out, synchronises, and
  * reports MGCM_RT_ERR_RANGE if any column was out of the tables' range.  It

@@ -20,6 +20,8 @@ extern "C" const char *mgcm_rt_strerror(int s) {
     case MGCM_RT_ERR_RANGE: return "temperature outside the Planck table (50.0..900.0 K) or a NaN input";
     case MGCM_RT_ERR_PRECISION: return "unknown precision mode";
     case MGCM_RT_ERR_IO: return "data file missing, unreadable or not in the expected record layout";
+    case MGCM_RT_ERR_SCRATCH:
+      return "grid needs more than the no-allocation entry point's 128 MiB scratch (run it in column slices)";
     default: return "unknown status";
   }
 }
@@ -107,6 +109,9 @@ extern "C" int mgcm_rt_init(const mgcm_rt_tables *t, mgcm_rt_ctx **out) {
   c->dt.planck = c->d_planck;
   c->dt.kv32 = c->d_kv32;
   c->dt.ki32 = c->d_ki32;
+  // the no-allocation entry point's scratch on this device; a failed lookup
+  // leaves it unusable (MGCM_RT_ERR_DEVICE there), the other entries unaffected
+  (void)noalloc_setup(c);
   *out = c;
   return MGCM_RT_OK;
 }

@@ -29,6 +29,9 @@ struct mgcm_rt_ctx {
   void *host;
   // aerosol producers' size grid and optical tables (rt_aerosol.hip)
   double *d_aer;
+  // device address of the no-allocation entry point's scratch array on this
+  // context's device (rt_noalloc.hip; not owned: it lives until the process ends)
+  double *d_scratch;
 };
 
 namespace mgcm {
@@ -48,6 +51,9 @@ void fill_args(RunArgs &a, const mgcm_rt_columns *in, const mgcm_rt_fluxes *out,
 int run_batched_impl(mgcm_rt_ctx *ctx, int ngrids, const mgcm_rt_columns *grids, const mgcm_rt_fluxes *outs,
                      int precision, hipStream_t st);
 void batch_free(void *state);
+// Looks up the scratch array of mgcm_rt_run_noalloc on the current device
+// (once per device) into ctx->d_scratch.
+int noalloc_setup(mgcm_rt_ctx *ctx);
 // Waits for stream st only, then reads and clears the context's range flag.
 int check_on_stream(mgcm_rt_ctx *ctx, hipStream_t st);
 int batched_tuning_impl(const mgcm_rt_ctx *ctx, int maxcls, mgcm_rt_batch_class *info);

@@ -34,8 +34,9 @@ SKIP_VIS, SKIP_IR = 0x10, 0x20  # OR-ed into the precision argument: one part on
 DP = C.POINTER(C.c_double)
 
 STATUS = {0: "ok", -1: "invalid argument", -2: "layer count outside 1..99", -3: "HIP runtime error",
-          -4: "out of device memory", -5: "temperature outside the Planck table", -6: "unknown precision",
-          -7: "data file missing, unreadable or not in the expected record layout"}
+          -4: "out of device memory", -5: "temperature outside the Planck table or a NaN input",
+          -6: "unknown precision", -7: "data file missing, unreadable or not in the expected record layout",
+          -8: "grid larger than the no-allocation entry point's scratch"}
 
 
 class RadiationError(RuntimeError):
@@ -131,6 +132,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         lib.mgcm_rt_destroy.argtypes = [C.c_void_p]
         lib.mgcm_rt_run.argtypes = [C.c_void_p, C.POINTER(Columns), C.POINTER(Fluxes), C.c_int, C.c_void_p]
         lib.mgcm_rt_run_host.argtypes = [C.c_void_p, C.POINTER(Columns), C.POINTER(Fluxes), C.c_int]
+        lib.mgcm_rt_run_noalloc.argtypes = [C.c_void_p, C.POINTER(Columns), C.POINTER(Fluxes), C.c_int,
+                                            C.c_void_p]
         lib.mgcm_rt_run_batched.argtypes = [C.c_void_p, C.c_int, C.POINTER(Columns), C.POINTER(Fluxes),
                                             C.c_int, C.c_void_p]
         lib.mgcm_rt_check.argtypes = [C.c_void_p]
@@ -266,6 +269,15 @@ class Radiation:
         fs = flux_struct(out, ptr=ptr)
         _check(self.lib.mgcm_rt_run(self.ctx, C.byref(cs), C.byref(fs), int(precision), C.c_void_p(stream)),
                "mgcm_rt_run")
+
+    def run_noalloc(self, cols: dict, out: dict, precision: int = FP64, stream: int = 0) -> None:
+        """mgcm_rt_run_noalloc on device tensors: no device allocation; grids beyond the
+        static scratch raise RadiationError (MGCM_RT_ERR_SCRATCH)."""
+        ptr = lambda t: t.data_ptr()  # noqa: E731
+        cs = column_struct(cols, ptr=ptr, sol=self.sol)
+        fs = flux_struct(out, ptr=ptr)
+        _check(self.lib.mgcm_rt_run_noalloc(self.ctx, C.byref(cs), C.byref(fs), int(precision),
+                                            C.c_void_p(stream)), "mgcm_rt_run_noalloc")
 
     def run_batched(self, grids: list, outs: list, precision: int = FP64, stream: int = 0) -> None:
         ptr = lambda t: t.data_ptr()  # noqa: E731
