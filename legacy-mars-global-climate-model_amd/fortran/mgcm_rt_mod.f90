@@ -18,6 +18,8 @@ module mgcm_rt_mod
   integer(c_int), parameter, public :: MGCM_RT_ERR_DEVICE = -3, MGCM_RT_ERR_NOMEM = -4
   integer(c_int), parameter, public :: MGCM_RT_ERR_RANGE = -5, MGCM_RT_ERR_PRECISION = -6, MGCM_RT_ERR_IO = -7
   integer(c_int), parameter, public :: MGCM_RT_FP64 = 0, MGCM_RT_MIXED = 1
+  ! added to the precision: run one part only (the other part's outputs are not written)
+  integer(c_int), parameter, public :: MGCM_RT_SKIP_VIS = 16, MGCM_RT_SKIP_IR = 32
 
   ! mirrors mgcm_rt_tables (host pointers, copied by mgcm_rt_init)
   type, bind(C), public :: mgcm_rt_tables
