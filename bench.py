@@ -168,27 +168,58 @@ def profile_figures(ncol, nlay):
     return res
 
 
-def form_mixed(args, rt_unused, stream):
-    """Mixed path on 64,800 columns x 36 layers with cloud ice, against the FP64
-    path on the same columns."""
-    import torch
-    from mgcm_rt import FP64, MIXED, Radiation, alloc_outputs, synth, to_device
-    ncol, nlay = 64800, 36
+def _precision_forms():
+    """The reduced-precision forms measured on the mixed workload: the mixed path
+    (k-tables stored in FP32) and the precision study's components of the
+    FP32/FP64 split, one at a time and together, plus FP64 with a cheaper division."""
+    from mgcm_rt import MIXED, P_FASTDIV, P_INTERP32, P_TAB32, P_TAU32, P_TRANS32, STUDY
+    return (("tables_fp32 (the mixed path)", MIXED),
+            ("k_interpolation_fp32 (tables and arithmetic)", STUDY | P_TAB32 | P_INTERP32),
+            ("optical_depths_fp32", STUDY | P_TAU32),
+            ("transmissions_fp32", STUDY | P_TRANS32),
+            ("all_four_fp32 (the configuration's split)", STUDY | P_TAB32 | P_INTERP32 | P_TAU32 | P_TRANS32),
+            ("fp64_fast_division", STUDY | P_FASTDIV),
+            ("fp64_fast_division_tables_fp32", STUDY | P_FASTDIV | P_TAB32))
+
+
+def form_mixed(args, rt_unused, stream, tables=None, label="cloud ice"):
+    """Mixed precision on 64,800 columns x 36 layers with cloud ice, against the
+    FP64 path on the same columns (itself within 1e-10 of the oracle): per form,
+    ms per step, speedup, max and RMS heating error (K/sol) and max flux error
+    (W/m^2).  The headline value is the fastest form within the 0.05 K/sol bar
+    that beats FP64, or -- when none does -- the mixed path, flagged as missing
+    the bar."""
+    from mgcm_rt import FP64, Radiation, alloc_outputs, synth, to_device
+    ncol, nlay, bar = 64800, 36, 0.05
     t, c = synth.make_workload(args.seed + 1, ncol, nlay, clouds=True)
+    if tables is not None:
+        t = tables
     rt = Radiation(t)
     dc = to_device(c)
     d64, dm = to_device(alloc_outputs(ncol, nlay)), to_device(alloc_outputs(ncol, nlay))
     sh = stream.cuda_stream
     ms64, _ = timed_steps(lambda: rt.run_device(dc, d64, FP64, sh), args.steps, args.warmup, stream)
-    msm, _ = timed_steps(lambda: rt.run_device(dc, dm, MIXED, sh), args.steps, args.warmup, stream)
     rt.check()
-    err = (dm["hr"] - d64["hr"]).abs()
-    flux = max(float((dm[k] - d64[k]).abs().max()) for k in ("fmnetv", "fmneti", "fluxupv", "fluxdnv",
-                                                             "fluxupi", "fluxdni"))
-    out = {"workload": f"{ncol} columns x {nlay} layers, cloud ice", "value": ncol / (msm / 1e3),
-           "unit": "columns/s", "ms_per_step": msm, "fp64_ms_per_step": ms64, "speedup_vs_fp64": ms64 / msm,
-           "max_err_K_per_sol": float(err.max()), "rms_err_K_per_sol": float(err.pow(2).mean().sqrt()),
-           "max_flux_err_W_m2": flux, "median_abs_hr_K_per_sol": float(d64["hr"].abs().median())}
+    forms = {}
+    for name, mode in _precision_forms():
+        ms, _ = timed_steps(lambda: rt.run_device(dc, dm, mode, sh), args.steps, args.warmup, stream)
+        rt.check()
+        err = (dm["hr"] - d64["hr"]).abs()
+        flux = max(float((dm[k] - d64[k]).abs().max()) for k in ("fmnetv", "fmneti", "fluxupv", "fluxdnv",
+                                                                 "fluxupi", "fluxdni"))
+        forms[name] = {"ms_per_step": ms, "speedup_vs_fp64": ms64 / ms, "max_err_K_per_sol": float(err.max()),
+                       "rms_err_K_per_sol": float(err.pow(2).mean().sqrt()), "max_flux_err_W_m2": flux,
+                       "meets_bar": bool(float(err.max()) <= bar and ms < ms64)}
+    ok = [n for n, f in forms.items() if f["meets_bar"]]
+    head = min(ok, key=lambda n: forms[n]["ms_per_step"]) if ok else _precision_forms()[0][0]
+    f = forms[head]
+    out = {"workload": f"{ncol} columns x {nlay} layers, {label}", "form": head, "value": ncol / (f["ms_per_step"] / 1e3),
+           "unit": "columns/s", "ms_per_step": f["ms_per_step"], "fp64_ms_per_step": ms64,
+           "speedup_vs_fp64": f["speedup_vs_fp64"], "max_err_K_per_sol": f["max_err_K_per_sol"],
+           "rms_err_K_per_sol": f["rms_err_K_per_sol"], "max_flux_err_W_m2": f["max_flux_err_W_m2"],
+           "bar_K_per_sol": bar, "meets_bar": f["meets_bar"],
+           "median_abs_hr_K_per_sol": float(d64["hr"].abs().median()),
+           "max_abs_hr_K_per_sol": float(d64["hr"].abs().max()), "forms": forms}
     del d64, dm, dc
     return out
 
@@ -224,31 +255,10 @@ def form_real(args, rt_unused, stream):
 
 
 def form_mixed_real(args, rt_unused, stream):
-    """The mixed path on the reference's own k-distribution (_real_tables), in the
-    mixed workload's shape (64,800 columns x 36 layers, cloud ice, the same seeded
-    columns as form_mixed), against the FP64 path on the same columns."""
-    from mgcm_rt import FP64, MIXED, Radiation, alloc_outputs, synth, to_device
-    ncol, nlay = 64800, 36
-    t = _real_tables(nlay)
-    c = synth.make_columns(args.seed + 1, ncol, nlay, clouds=True)
-    rt = Radiation(t)
-    dc = to_device(c)
-    d64, dm = to_device(alloc_outputs(ncol, nlay)), to_device(alloc_outputs(ncol, nlay))
-    sh = stream.cuda_stream
-    ms64, _ = timed_steps(lambda: rt.run_device(dc, d64, FP64, sh), args.steps, args.warmup, stream)
-    msm, _ = timed_steps(lambda: rt.run_device(dc, dm, MIXED, sh), args.steps, args.warmup, stream)
-    rt.check()
-    err = (dm["hr"] - d64["hr"]).abs()
-    flux = max(float((dm[k] - d64[k]).abs().max()) for k in ("fmnetv", "fmneti", "fluxupv", "fluxdnv",
-                                                             "fluxupi", "fluxdni"))
-    out = {"workload": f"{ncol} columns x {nlay} layers, cloud ice, the reference's k-distribution",
-           "value": ncol / (msm / 1e3), "unit": "columns/s", "ms_per_step": msm, "fp64_ms_per_step": ms64,
-           "speedup_vs_fp64": ms64 / msm, "max_err_K_per_sol": float(err.max()),
-           "rms_err_K_per_sol": float(err.pow(2).mean().sqrt()), "max_flux_err_W_m2": flux,
-           "median_abs_hr_K_per_sol": float(d64["hr"].abs().median()),
-           "max_abs_hr_K_per_sol": float(d64["hr"].abs().max())}
-    del d64, dm, dc
-    return out
+    """The same forms on the reference's own k-distribution (_real_tables), in the
+    mixed workload's shape (the same seeded columns as form_mixed)."""
+    return form_mixed(args, rt_unused, stream, tables=_real_tables(36),
+                      label="cloud ice, the reference's k-distribution")
 
 
 def form_batched(args, rt, stream):

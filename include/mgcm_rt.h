@@ -62,6 +62,17 @@ enum {
 enum {
   MGCM_RT_FP64 = 0,       /* plain path: every operation in FP64, reference evaluation order */
   MGCM_RT_MIXED = 1,      /* k-tables read in FP32, all arithmetic FP64 */
+  /* Precision study (diagnostic; mgcm_rt_run/_run_host/_run_noalloc, grids of
+   * 25..36 layers): MGCM_RT_STUDY | component bits -- the column kernel with
+   * those components in cheaper arithmetic, the continuum point g = 17 in FP64.
+   * Built combinations: INTERP32 (with TAB32), TAU32, TRANS32, all four, FASTDIV,
+   * FASTDIV | TAB32.  Other combinations return MGCM_RT_ERR_DEVICE. */
+  MGCM_RT_STUDY = 2,
+  MGCM_RT_P_TAB32 = 0x100,    /* k-tables stored in FP32 */
+  MGCM_RT_P_INTERP32 = 0x200, /* k-interpolation arithmetic in FP32 */
+  MGCM_RT_P_TAU32 = 0x400,    /* sub-level optical depths rounded to FP32 */
+  MGCM_RT_P_TRANS32 = 0x800,  /* layer transmissions exp(-lambda dtau), exp(-tau/mu0) in FP32 */
+  MGCM_RT_P_FASTDIV = 0x1000, /* FP64 division by reciprocal + 2 Newton steps (not IEEE-rounded) */
   /* OR-ed into the precision argument: compute one part only.  The other
    * part's fluxes are zero, its DETAU/CUMTAUV untouched, and the heating holds
    * the computed part alone (the IR surrogate's exact VIS and fallback IR). */

@@ -130,11 +130,13 @@ extern "C" int mgcm_rt_destroy(mgcm_rt_ctx *c) {
 
 namespace mgcm {
 int decode_mode(int precision, int *path, int *bands) {
-  const int p = precision & 0xf, skip = precision & ~0xf;
-  if ((p != MGCM_RT_FP64 && p != MGCM_RT_MIXED) || (skip & ~(MGCM_RT_SKIP_VIS | MGCM_RT_SKIP_IR)) ||
+  const int p = precision & 0xf, pol = (precision >> 8) & 0x1f;
+  const int skip = precision & ~0xf & ~(0x1f << 8);
+  if ((p != MGCM_RT_FP64 && p != MGCM_RT_MIXED && p != MGCM_RT_STUDY) || (p != MGCM_RT_STUDY && pol) ||
+      (p == MGCM_RT_STUDY && !pol) || (skip & ~(MGCM_RT_SKIP_VIS | MGCM_RT_SKIP_IR)) ||
       skip == (MGCM_RT_SKIP_VIS | MGCM_RT_SKIP_IR))
     return MGCM_RT_ERR_PRECISION;
-  *path = p;
+  *path = p | (pol << 8);
   *bands = ((skip & MGCM_RT_SKIP_VIS) ? 0 : 1) | ((skip & MGCM_RT_SKIP_IR) ? 0 : 2);
   return MGCM_RT_OK;
 }
@@ -189,7 +191,10 @@ hipError_t run_grid(const mgcm_rt_ctx *c, const mgcm_rt_columns *in, const mgcm_
   fill_args(a, in, out, c->d_flag);
   a.bands = bands;
   a.clr = clr;
-  hipError_t e = (path == MGCM_RT_FP64) ? launch_plain(c->dt, a, st) : launch_mixed(c->dt, a, st);
+  const int p = path & 0xf;
+  hipError_t e = (p == MGCM_RT_FP64)    ? launch_plain(c->dt, a, st)
+                 : (p == MGCM_RT_MIXED) ? launch_mixed(c->dt, a, st)
+                                        : launch_study(path >> 8, c->dt, a, st);
   // surface fluxes for TEMPGR from this run's DETAU/DIFFVT (code/comp3.f:336-348)
   if (e == hipSuccess && out->directsol)
     e = launch_dsolflux(c->d_const, in->ncol, in->sol, in->acosz, out->detau, out->diffvt, out->directsol,
@@ -252,6 +257,7 @@ extern "C" int mgcm_rt_run_batched(mgcm_rt_ctx *c, int ngrids, const mgcm_rt_col
   int path, bands;
   const int sm = decode_mode(precision, &path, &bands);
   if (sm != MGCM_RT_OK) return sm;
+  if ((path & 0xf) == MGCM_RT_STUDY) return MGCM_RT_ERR_PRECISION;  // single grids only
   for (int i = 0; i < ngrids; i++) {
     const int s = check_columns(&grids[i], &outs[i]);
     if (s != MGCM_RT_OK) return s;

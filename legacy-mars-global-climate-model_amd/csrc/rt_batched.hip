@@ -247,6 +247,7 @@ int run_batched_impl(mgcm_rt_ctx *c, int ngrids, const mgcm_rt_columns *grids, c
   int path, bands;
   const int sm = decode_mode(precision, &path, &bands);
   if (sm != MGCM_RT_OK) return sm;
+  if ((path & 0xf) == MGCM_RT_STUDY) return MGCM_RT_ERR_PRECISION;  // single grids only
   long long total = 0;
   for (int i = 0; i < ngrids; i++) {
     const int s = check_columns(&grids[i], &outs[i]);

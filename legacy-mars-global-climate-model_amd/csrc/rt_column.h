@@ -108,9 +108,9 @@ static __device__ void tp_stage(const RtConst *__restrict__ rc, double pw, doubl
 // Gas point source: k-interpolation (code/optcv.f90:142-167, code/optci.f90:
 // 143-167) from the g-fastest table; the 16 lanes of a column read one 128-byte
 // line per corner.
-template <bool VIS, bool F32 = false>
+template <bool VIS, int P = 0>
 struct GasSrc {
-  using kt = typename std::conditional<F32, float, double>::type;  // table storage type
+  using kt = typename std::conditional<(P & P_TAB32) != 0, float, double>::type;  // table storage type
   const kt *__restrict__ tab;      // band's table
   const double *t_, *u_, *wr_, *uk_, *c1_, *c2_;  // this column's staged sub-levels (LDS)
   const int *base_;
@@ -144,14 +144,19 @@ struct GasSrc {
     const R ans = lc0 * k1 + lc1 * k2 + lc2 * k3 + lc3 * k4;
     return (R)uk_[K] * ans;
   }
-  // DTAUK(K) = gas + continuum, in FP64 on both paths: the mixed path only
-  // stores the table in FP32 (FP32 arithmetic here costs accuracy -- 1 - W of
-  // near-conservative layers -- and runs no faster than FP64 on this VALU)
+  // DTAUK(K) = gas + continuum (code/optcv.f90:160-167, code/optci.f90:160-167):
+  // FP64 unless the policy moves the interpolation (P_INTERP32) or the
+  // resulting optical depth (P_TAU32) to FP32
   __device__ __forceinline__ double finish(int K, const Raw &r) {
-    const double taugas = taugas_of<double>(K, r.a0, r.a1, r.b0, r.b1, r.c0, r.c1, r.d0, r.d1);
+    double taugas;
+    if (P & P_INTERP32)
+      taugas = (double)taugas_of<float>(K, (float)r.a0, (float)r.a1, (float)r.b0, (float)r.b1, (float)r.c0,
+                                        (float)r.c1, (float)r.d0, (float)r.d1);
+    else
+      taugas = taugas_of<double>(K, r.a0, r.a1, r.b0, r.b1, r.c0, r.c1, r.d0, r.d1);
     taugsurf = taugsurf + taugas;
-    if (VIS) return taugas + c1_[K];
-    return taugas + c1_[K] + c2_[K];
+    const double d = VIS ? taugas + c1_[K] : taugas + c1_[K] + c2_[K];
+    return (P & P_TAU32) ? (double)(float)d : d;
   }
   // TAUGSURF of the FP64 table tab64 (this lane's g), summed in the order of
   // the plain path: the mixed path settles TLIMIT decisions near the threshold
@@ -175,11 +180,11 @@ struct GasSrc {
 // FP64-table one; closer than 1e-4 to the limit it is recomputed from the FP64
 // table (all lanes of the wave take part, the others keep their decision), so
 // both paths skip the same Gauss points.
-template <bool MIXED, class Src>
+template <int P, class Src>
 __device__ __forceinline__ bool below_tlimit(const Src &src, const double *tab64, int nlev, double tlim,
                                              bool active) {
   double tg = src.taugsurf;
-  if (MIXED) {
+  if (P & (P_TAB32 | P_INTERP32)) {
     const bool near = active && fabs(tg - tlim) <= 1.0e-4 * tlim;
     if (__any(near)) {
       const double t64 = src.taugsurf_fp64(tab64, nlev);
@@ -266,13 +271,15 @@ __device__ __forceinline__ double acc_get(const double *arr, int n) {
 #define MGCM_FP64_WAVES 2
 #endif
 
-// MIXED: the k-tables are read in FP32 (half the gather bytes and corner
-// registers); all arithmetic, the interpolation included, is FP64.
+// P: the precision policy (P_* bits; 0 = plain FP64, P_TAB32 = the mixed path,
+// whose k-tables are read in FP32 -- half the gather bytes and corner
+// registers -- with all arithmetic, the interpolation included, in FP64).
 // The whole radiation call of columns col0..col0+3 of grid a, by one wave.
 // The exp table must be in LDS; the staging after it is rewritten.
-template <int NLM, bool MIXED>
+template <int NLM, int P>
 __device__ __forceinline__ void column_block(const DevTables &dt, const RunArgs &a, int col0, double *smem) {
-  using kt = typename std::conditional<MIXED, float, double>::type;
+  constexpr bool T32 = (P & P_TAB32) != 0;
+  using kt = typename std::conditional<T32, float, double>::type;
   constexpr int SL = (NLM + 15) / 16;
   const int nl = a.nlay + 1, nlev = 2 * a.nlay + 3, nsub = nlev - 1;
   const int lane = threadIdx.x, c = lane >> 4, g0 = lane & 15;
@@ -379,11 +386,11 @@ __device__ __forceinline__ void column_block(const DevTables &dt, const RunArgs 
     if (VISB) {
       const int nw = b;
       const size_t toff = (size_t)nw * NP * NH * NT * NGAS;
-      GasSrc<true, MIXED> src(MIXED ? (const kt *)(dt.kv32 + toff) : (const kt *)(dt.kv + toff), S, c, nsub, g0);
+      GasSrc<true, P> src(T32 ? (const kt *)(dt.kv32 + toff) : (const kt *)(dt.kv + toff), S, c, nsub, g0);
       VisState<NLM> s;
       double cumtau = 0.0, detau = 0.0;
       if (valid && sun) detau = vis_optics<NLM>(nl, src, StagedLayerAer{scat, cosb}, s, cumtau);
-      skip = !(valid && sun) || below_tlimit<MIXED>(src, dt.kv + toff + g0, nlev, TLIMITS, valid && sun);
+      skip = !(valid && sun) || below_tlimit<P>(src, dt.kv + toff + g0, nlev, TLIMITS, valid && sun);
       if (valid && sun) {  // GETDETAU output and TAUCUMV(L_LEVELS) (code/comp3.f:664-668)
         a.detau[(size_t)col * NG * NV + g0 * NV + nw] = detau;
         a.cumtauv[(size_t)col * NG * NV + g0 * NV + nw] = cumtau;
@@ -401,7 +408,7 @@ __device__ __forceinline__ void column_block(const DevTables &dt, const RunArgs 
         };
         double fluxup = 0.0, fluxdn = 0.0, diffv = 0.0;
         const double bsurf = a.als[colc] * u0 * a.sol[nw] * rt_exp35(-dmin(detau / u0, MAXEXP));
-        vis_solve<NLM>(nl, s, cbar, u0, a.sol[nw], a.als[colc], 0.0, bsurf, accf, fluxup, fluxdn, diffv);
+        vis_solve<NLM, P>(nl, s, cbar, u0, a.sol[nw], a.als[colc], 0.0, bsurf, accf, fluxup, fluxdn, diffv);
         topv = topv + group16_sum(skip ? 0.0 : (fluxup - fluxdn) * gw * onemf);
         difv = difv + group16_sum(skip ? 0.0 : diffv * gw * onemf);
       }
@@ -409,10 +416,10 @@ __device__ __forceinline__ void column_block(const DevTables &dt, const RunArgs 
       const int nw = b - NV;
       const double dw = rc->dwni[nw];
       const size_t toff = (size_t)nw * NP * NH * NT * NGAS;
-      GasSrc<false, MIXED> src(MIXED ? (const kt *)(dt.ki32 + toff) : (const kt *)(dt.ki + toff), S, c, nsub, g0);
+      GasSrc<false, P> src(T32 ? (const kt *)(dt.ki32 + toff) : (const kt *)(dt.ki + toff), S, c, nsub, g0);
       IrState<NLM> s;
       if (valid) ir_optics<NLM>(nl, src, StagedLayerAer{scat, cosb}, s);
-      skip = !valid || below_tlimit<MIXED>(src, dt.ki + toff + g0, nlev, TLIMITI, valid);
+      skip = !valid || below_tlimit<P>(src, dt.ki + toff + g0, nlev, TLIMITI, valid);
       if (__any(!skip)) {
         const double *bk = S.bk + c * nlev;
         auto accf = [&](int n, double fp, double fm) {
@@ -428,7 +435,7 @@ __device__ __forceinline__ void column_block(const DevTables &dt, const RunArgs 
         const double bsurf = (1. - rsf) * bk[nlev - 1];
         const double tautop = s.wd[1].y * plev[1] / (plev[3] - plev[1]);
         const double btop = (1.0 - rt_exp(-tautop * 2.0)) * bk[1];  // /UBARI, UBARI = 0.5
-        const double ftopup = ir_solve<NLM>(nl, s, cosb, bk, rsf, btop, bsurf, accf);
+        const double ftopup = ir_solve<NLM, P>(nl, s, cosb, bk, rsf, btop, bsurf, accf);
         topi = topi + group16_sum(skip ? 0.0 : ftopup * dw * gw * onemf);
       }
     }
@@ -630,19 +637,19 @@ __global__ __launch_bounds__(64) void k_rt_clear_batch(DevTables dt, const RunAr
 }
 
 // One grid: block b owns columns 4b..4b+3.
-template <int NLM, bool MIXED>
+template <int NLM, int P>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MGCM_FP64_WAVES)))
 void k_rt(DevTables dt, RunArgs a) {
   extern __shared__ double smem[];
   mgcm_libm::load_lds_exptab(threadIdx.x, 64);
   __syncthreads();
-  column_block<NLM, MIXED>(dt, a, blockIdx.x * 4, smem);
+  column_block<NLM, P>(dt, a, blockIdx.x * 4, smem);
 }
 
 // Many grids of one layer-count class: a persistent grid of blocks walks the
 // work list (grid index, first column) round-robin, largest layer counts first.
 // No counters: every block's share is fixed by its index.
-template <int NLM, bool MIXED>
+template <int NLM, int P>
 __device__ __forceinline__ void batch_items(const DevTables &dt, const RunArgs *__restrict__ grids,
                                             const uint2 *__restrict__ work, int nwork, double *smem) {
   mgcm_libm::load_lds_exptab(threadIdx.x, 64);
@@ -650,27 +657,27 @@ __device__ __forceinline__ void batch_items(const DevTables &dt, const RunArgs *
     __syncthreads();  // the previous item's reads of the staging are done
     const uint2 wk = work[w];
     const RunArgs a = grids[wk.x];
-    column_block<NLM, MIXED>(dt, a, (int)wk.y, smem);
+    column_block<NLM, P>(dt, a, (int)wk.y, smem);
   }
 }
 // Two instances the batched tuner chooses between per layer class: two waves
 // per SIMD (at most 256 registers per lane) ...
-template <int NLM, bool MIXED>
+template <int NLM, int P>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MGCM_FP64_WAVES)))
 void k_rt_batch(DevTables dt, const RunArgs *__restrict__ grids, const uint2 *__restrict__ work, int nwork) {
   extern __shared__ double smem[];
-  batch_items<NLM, MIXED>(dt, grids, work, nwork, smem);
+  batch_items<NLM, P>(dt, grids, work, nwork, smem);
 }
 // ... and one wave per SIMD (512 registers per lane: fewer spills of the
 // item state, half the waves in flight).
-template <int NLM, bool MIXED>
+template <int NLM, int P>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
 void k_rt_batch_w1(DevTables dt, const RunArgs *__restrict__ grids, const uint2 *__restrict__ work, int nwork) {
   extern __shared__ double smem[];
-  batch_items<NLM, MIXED>(dt, grids, work, nwork, smem);
+  batch_items<NLM, P>(dt, grids, work, nwork, smem);
 }
 
-template <int NLM, bool MIXED>
+template <int NLM, int P>
 static hipError_t launch_nlm(const DevTables &dt, const RunArgs &a, hipStream_t st) {
   // continuum points first (their records feed the column kernel's sums)
   const int nvis = (a.ncol + 8) / 9, nir = (a.ncol + 11) / 12;
@@ -682,22 +689,22 @@ static hipError_t launch_nlm(const DevTables &dt, const RunArgs &a, hipStream_t 
     hipLaunchKernelGGL((k_rt_clear<NLM, false>), dim3(nir), dim3(64), mgcm_libm::LDS_EXPTAB_BYTES, st, dt, a);
   const size_t lds = stage_bytes(a.nlay);
   const unsigned gx = (unsigned)((a.ncol + 3) / 4);
-  hipLaunchKernelGGL((k_rt<NLM, MIXED>), dim3(gx), dim3(64), lds, st, dt, a);
+  hipLaunchKernelGGL((k_rt<NLM, P>), dim3(gx), dim3(64), lds, st, dt, a);
   return hipGetLastError();
 }
 
 // One grid: the continuum-point kernels, then the column kernel (4 columns per
 // wave), each instance sized for the grid's layer count.  a.clr must hold
 // clr_bytes(a.ncol, a.nlay).
-template <bool MIXED>
+template <int P>
 hipError_t launch_column(const DevTables &dt, const RunArgs &a, hipStream_t st) {
   const int nl = a.nlay + 1;
-  if (nl <= 17) return launch_nlm<17, MIXED>(dt, a, st);
-  if (nl <= 25) return launch_nlm<25, MIXED>(dt, a, st);
-  if (nl <= 37) return launch_nlm<37, MIXED>(dt, a, st);
-  if (nl <= 49) return launch_nlm<49, MIXED>(dt, a, st);
-  if (nl <= 65) return launch_nlm<65, MIXED>(dt, a, st);
-  return launch_nlm<MAX_NLAY + 1, MIXED>(dt, a, st);
+  if (nl <= 17) return launch_nlm<17, P>(dt, a, st);
+  if (nl <= 25) return launch_nlm<25, P>(dt, a, st);
+  if (nl <= 37) return launch_nlm<37, P>(dt, a, st);
+  if (nl <= 49) return launch_nlm<49, P>(dt, a, st);
+  if (nl <= 65) return launch_nlm<65, P>(dt, a, st);
+  return launch_nlm<MAX_NLAY + 1, P>(dt, a, st);
 }
 
 // Layer-count classes of the batched path: kernel instance NLM = KCLASS_NL[k]
@@ -711,7 +718,7 @@ inline int nl_class(int nl) {
 }
 
 // kind 0: column kernel (waves per SIMD 1 or 2); 1: VIS continuum points; 2: IR continuum points
-template <int NLM, bool MIXED>
+template <int NLM, int P>
 static hipError_t launch_batch_nlm(int kind, int waves, const DevTables &dt, const RunArgs *grids, const uint2 *work,
                                    int nwork, int nblocks, size_t lds, hipStream_t st) {
   if (kind == 1)
@@ -721,22 +728,22 @@ static hipError_t launch_batch_nlm(int kind, int waves, const DevTables &dt, con
     hipLaunchKernelGGL((k_rt_clear_batch<NLM, false>), dim3(nblocks), dim3(64), mgcm_libm::LDS_EXPTAB_BYTES, st, dt,
                        grids, work, nwork);
   else if (waves == 1)
-    hipLaunchKernelGGL((k_rt_batch_w1<NLM, MIXED>), dim3(nblocks), dim3(64), lds, st, dt, grids, work, nwork);
+    hipLaunchKernelGGL((k_rt_batch_w1<NLM, P>), dim3(nblocks), dim3(64), lds, st, dt, grids, work, nwork);
   else
-    hipLaunchKernelGGL((k_rt_batch<NLM, MIXED>), dim3(nblocks), dim3(64), lds, st, dt, grids, work, nwork);
+    hipLaunchKernelGGL((k_rt_batch<NLM, P>), dim3(nblocks), dim3(64), lds, st, dt, grids, work, nwork);
   return hipGetLastError();
 }
 
-template <bool MIXED>
+template <int P>
 hipError_t launch_batch_class(int cls, int kind, int waves, const DevTables &dt, const RunArgs *grids,
                               const uint2 *work, int nwork, int nblocks, size_t lds, hipStream_t st) {
   switch (cls) {
-    case 0: return launch_batch_nlm<17, MIXED>(kind, waves, dt, grids, work, nwork, nblocks, lds, st);
-    case 1: return launch_batch_nlm<25, MIXED>(kind, waves, dt, grids, work, nwork, nblocks, lds, st);
-    case 2: return launch_batch_nlm<37, MIXED>(kind, waves, dt, grids, work, nwork, nblocks, lds, st);
-    case 3: return launch_batch_nlm<49, MIXED>(kind, waves, dt, grids, work, nwork, nblocks, lds, st);
-    case 4: return launch_batch_nlm<65, MIXED>(kind, waves, dt, grids, work, nwork, nblocks, lds, st);
-    default: return launch_batch_nlm<MAX_NLAY + 1, MIXED>(kind, waves, dt, grids, work, nwork, nblocks, lds, st);
+    case 0: return launch_batch_nlm<17, P>(kind, waves, dt, grids, work, nwork, nblocks, lds, st);
+    case 1: return launch_batch_nlm<25, P>(kind, waves, dt, grids, work, nwork, nblocks, lds, st);
+    case 2: return launch_batch_nlm<37, P>(kind, waves, dt, grids, work, nwork, nblocks, lds, st);
+    case 3: return launch_batch_nlm<49, P>(kind, waves, dt, grids, work, nwork, nblocks, lds, st);
+    case 4: return launch_batch_nlm<65, P>(kind, waves, dt, grids, work, nwork, nblocks, lds, st);
+    default: return launch_batch_nlm<MAX_NLAY + 1, P>(kind, waves, dt, grids, work, nwork, nblocks, lds, st);
   }
 }
 

@@ -37,14 +37,18 @@ struct mgcm_rt_ctx {
 namespace mgcm {
 hipError_t launch_plain(const DevTables &dt, const RunArgs &a, hipStream_t st);
 hipError_t launch_mixed(const DevTables &dt, const RunArgs &a, hipStream_t st);
+// The precision study's column kernels (rt_study.hip): policy = P_* bits;
+// hipErrorInvalidValue outside the 25..36-layer class or for a policy not built.
+hipError_t launch_study(int policy, const DevTables &dt, const RunArgs &a, hipStream_t st);
 // waves: the column kernel instance's waves per SIMD (1 or 2); kind 0 only
 hipError_t launch_batch_plain(int cls, int kind, int waves, const DevTables &dt, const RunArgs *grids,
                               const uint2 *work, int nwork, int nblocks, size_t lds, hipStream_t st);
 hipError_t launch_batch_mixed(int cls, int kind, int waves, const DevTables &dt, const RunArgs *grids,
                               const uint2 *work, int nwork, int nblocks, size_t lds, hipStream_t st);
 size_t clr_workspace_bytes(int ncol, int nlay);
-// Splits the precision argument into the path (MGCM_RT_FP64/MIXED) and the
-// band mask; MGCM_RT_ERR_PRECISION for anything else.
+// Splits the precision argument into the path (MGCM_RT_FP64/MIXED, or
+// MGCM_RT_STUDY with its policy bits in path >> 8) and the band mask;
+// MGCM_RT_ERR_PRECISION for anything else.
 int decode_mode(int precision, int *path, int *bands);
 int check_columns(const mgcm_rt_columns *in, const mgcm_rt_fluxes *out);
 void fill_args(RunArgs &a, const mgcm_rt_columns *in, const mgcm_rt_fluxes *out, int *flag);

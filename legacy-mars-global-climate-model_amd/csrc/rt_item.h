@@ -12,6 +12,40 @@ namespace mgcm {
 __device__ __forceinline__ double rt_exp(double x) { return mgcm_libm::mgcm_exp_lds(x); }
 // exp of an argument clamped to [-35, 35] (MAXEXP): branch-free, same bits
 __device__ __forceinline__ double rt_exp35(double x) { return mgcm_libm::mgcm_exp_lds_mid(x); }
+
+// Precision policy of the column kernel, a template parameter P of its parts.
+// P = 0 is the plain path: every operation FP64 in the reference's order.  The
+// other bits move one component of the work to cheaper arithmetic; the
+// reduced-precision forms and the precision study are combinations of them.
+enum : int {
+  P_TAB32 = 1,     // k-tables stored in FP32 (read as FP32, interpolated in FP64)
+  P_INTERP32 = 2,  // k-interpolation arithmetic in FP32 (OPTCV/OPTCI's corner combination)
+  P_TAU32 = 4,     // sub-level optical depths DTAUK rounded to FP32
+  P_TRANS32 = 8,   // layer transmissions exp(-lambda dtau), exp(-tau/mu0) in FP32
+  P_FASTDIV = 16,  // FP64 division as a * (1/b), 1/b from v_rcp_f64 and two Newton steps
+};
+
+// 1/b to within 1 ulp: the hardware reciprocal estimate refined by two
+// Newton-Raphson steps (each doubles its correct bits).
+__device__ __forceinline__ double rcp_nr(double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  double e = __builtin_fma(-b, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-b, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+// Division of the two-stream solves under policy P (IEEE a/b on the plain path).
+template <int P>
+__device__ __forceinline__ double pdiv(double a, double b) {
+  if (P & P_FASTDIV) return a * rcp_nr(b);
+  return a / b;
+}
+// exp of a clamped argument under policy P.
+template <int P>
+__device__ __forceinline__ double pexp35(double x) {
+  if (P & P_TRANS32) return (double)expf((float)x);
+  return rt_exp35(x);
+}
 }  // namespace mgcm
 
 namespace mgcm {
@@ -205,6 +239,7 @@ struct VisState {
 struct VisLayer {
   double g1, g2, g3, lam, gama, am, ap;
 };
+template <int P = 0>
 __device__ __forceinline__ VisLayer vis_layer(double w0, double cb, double u0, double f0pi) {
   const double s3 = 0x1.bb67aep+0;   // SQRT(3.0) in REAL(4) (code/gfluxv.f90:114-116)
   const double s3h = 0x1.bb67aep-1;  // SQRT(3.0)*0.5 in REAL(4)
@@ -213,12 +248,12 @@ __device__ __forceinline__ VisLayer vis_layer(double w0, double cb, double u0, d
   v.g2 = (s3 * w0 * 0.5) * (1.0 - cb);
   v.g3 = 0.5 * (1.0 - s3 * cb * u0);
   v.lam = sqrt(v.g1 * v.g1 - v.g2 * v.g2);
-  v.gama = (v.g1 - v.lam) / v.g2;
+  v.gama = pdiv<P>(v.g1 - v.lam, v.g2);
   const double g4 = 1.0 - v.g3;
-  double denom = v.lam * v.lam - 1.0 / (u0 * u0);
+  double denom = v.lam * v.lam - pdiv<P>(1.0, u0 * u0);
   if (denom == 0.0) denom = F32LIT(1.e-10);
-  v.am = f0pi * w0 * (g4 * (v.g1 + 1.0 / u0) + v.g2 * v.g3) / denom;
-  v.ap = f0pi * w0 * (v.g3 * (v.g1 - 1.0 / u0) + v.g2 * g4) / denom;
+  v.am = pdiv<P>(f0pi * w0 * (g4 * (v.g1 + pdiv<P>(1.0, u0)) + v.g2 * v.g3), denom);
+  v.ap = pdiv<P>(f0pi * w0 * (v.g3 * (v.g1 - pdiv<P>(1.0, u0)) + v.g2 * g4), denom);
   return v;
 }
 
@@ -282,7 +317,7 @@ __device__ __forceinline__ double vis_optics(int nl, Src &src, const LA &la, Vis
 // GFLUXV (code/gfluxv.f90:108-280) + DSOLVER/DTRIDGL (code/dsolver.f90,
 // code/dtridgl.f90) on a scaled item.  Calls acc(L, fmidp, fmidm) for each
 // layer mid-point and returns the top fluxes and DIFFV.
-template <int NLM, typename Acc>
+template <int NLM, int P = 0, typename Acc>
 __device__ __forceinline__ void vis_solve(int nl, VisState<NLM> &s, const double *__restrict__ cbar, double u0,
                                           double f0pi, double rsf, double btop,
                           double bsurf, Acc &acc, double &fluxup, double &fluxdn, double &diffv) {
@@ -290,7 +325,7 @@ __device__ __forceinline__ void vis_solve(int nl, VisState<NLM> &s, const double
   // the row below and the transmission at the layer bottom (the top of the
   // layer below, the same argument) are carried in registers.
   double gama_n1 = 0.0, cpm1_n1 = 0.0, cmm1_n1 = 0.0, as1 = 0.0, ds1 = 0.0;
-  double xbot = rt_exp35(-dmin(s.tbot / u0, MAXEXP));
+  double xbot = pexp35<P>(-dmin(pdiv<P>(s.tbot, u0), MAXEXP));
   // the next layer's state is loaded one iteration ahead
   Pair wt_n = s.wt[nl];
   double cb_n = cbar[nl - 1], dtau_n = s.dh[nl].x;
@@ -301,21 +336,21 @@ __device__ __forceinline__ void vis_solve(int nl, VisState<NLM> &s, const double
       cb_n = cbar[n - 2];
       dtau_n = s.dh[n - 1].x;
     }
-    const VisLayer v = vis_layer(w0c, cbc, u0, f0pi);
+    const VisLayer v = vis_layer<P>(w0c, cbc, u0, f0pi);
     const double lam = v.lam, gama = v.gama, am = v.am, ap = v.ap;
-    const double xtop = rt_exp35(-dmin(tauc / u0, MAXEXP));
+    const double xtop = pexp35<P>(-dmin(pdiv<P>(tauc, u0), MAXEXP));
     const double cpm1 = ap * xtop, cmm1 = am * xtop;
     const double cp = ap * xbot, cm = am * xbot;
-    const double ep = rt_exp35(dmin(MAXEXP, lam * dtauc));
-    const double em = 1.0 / ep;
+    const double ep = pexp35<P>(dmin(MAXEXP, lam * dtauc));
+    const double em = pdiv<P>(1.0, ep);
     const double e1 = ep + gama * em, e2 = ep - gama * em;
     const double e3 = gama * ep + em, e4 = gama * ep - em;
     if (n == nl) {  // surface row (dsolver.f90:82-85)
       const int i = 2 * nl;
       const double af = e1 - rsf * e3, bf = e2 - rsf * e4;
       const double df = bsurf - cp + rsf * cm;
-      as1 = af / bf;
-      ds1 = df / bf;
+      as1 = pdiv<P>(af, bf);
+      ds1 = pdiv<P>(df, bf);
       s.row[i] = Pair{as1, ds1};
     } else {
       {  // odd row 2n+1 (dsolver.f90:74-80)
@@ -324,7 +359,7 @@ __device__ __forceinline__ void vis_solve(int nl, VisState<NLM> &s, const double
         const double bf = (e1 - e3) * (1.0 + gama_n1);
         const double cf = (e1 + e3) * (gama_n1 - 1.0);
         const double df = e3 * (cpm1_n1 - cp) + e1 * (cm - cmm1_n1);
-        const double x = 1.0 / (bf - cf * as1);
+        const double x = pdiv<P>(1.0, bf - cf * as1);
         as1 = af * x;
         ds1 = (df - cf * ds1) * x;
         s.row[i] = Pair{as1, ds1};
@@ -335,7 +370,7 @@ __device__ __forceinline__ void vis_solve(int nl, VisState<NLM> &s, const double
         const double bf = (e2 + e4) * (gama_n1 - 1.0);
         const double cf = 2.0 * (1.0 - gama_n1 * gama_n1);
         const double df = (gama_n1 - 1.0) * (cpm1_n1 - cp) + (1.0 - gama_n1) * (cm - cmm1_n1);
-        const double x = 1.0 / (bf - cf * as1);
+        const double x = pdiv<P>(1.0, bf - cf * as1);
         as1 = af * x;
         ds1 = (df - cf * ds1) * x;
         s.row[i] = Pair{as1, ds1};
@@ -343,7 +378,7 @@ __device__ __forceinline__ void vis_solve(int nl, VisState<NLM> &s, const double
     }
     if (n == 1) {  // top row (dsolver.f90:54-57)
       const double af = 0.0, bf = gama + 1.0, cf = gama - 1.0, df = btop - cmm1;
-      const double x = 1.0 / (bf - cf * as1);
+      const double x = pdiv<P>(1.0, bf - cf * as1);
       s.row[1] = Pair{af * x, (df - cf * ds1) * x};
     }
     gama_n1 = gama;
@@ -378,23 +413,23 @@ __device__ __forceinline__ void vis_solve(int nl, VisState<NLM> &s, const double
     const double xk1 = xa + xb;
     double xk2 = xa - xb;
     xk2 = xk2_clip(xa, xk2);  // dsolver.f90:101
-    const VisLayer v = vis_layer(w0c, cbc, u0, f0pi);
+    const VisLayer v = vis_layer<P>(w0c, cbc, u0, f0pi);
     const double lam = v.lam, gama = v.gama, am = v.am, ap = v.ap;
     double fmidp, fmidm;
     if (n < nl) {  // gfluxv.f90:170-206
-      const double ep = rt_exp35(dmin(MAXEXP, lam * (hmc - ttc)));
-      const double em = 1.0 / ep;
+      const double ep = pexp35<P>(dmin(MAXEXP, lam * (hmc - ttc)));
+      const double em = pdiv<P>(1.0, ep);
       const double taumid = hmc;
-      const double xm = rt_exp35(-dmin(taumid / u0, MAXEXP));
+      const double xm = pexp35<P>(-dmin(pdiv<P>(taumid, u0), MAXEXP));
       const double cpmid = ap * xm, cmmid = am * xm;
       fmidp = xk1 * ep + gama * xk2 * em + cpmid;
       fmidm = xk1 * ep * gama + xk2 * em + cmmid;
       fmidm = fmidm + u0 * f0pi * xm;
     } else {  // bottom layer, full sub-layer (gfluxv.f90:243-280)
-      const double ep = rt_exp35(dmin(MAXEXP, lam * (tau_bot - tau_lastl)));
-      const double em = 1.0 / ep;
+      const double ep = pexp35<P>(dmin(MAXEXP, lam * (tau_bot - tau_lastl)));
+      const double em = pdiv<P>(1.0, ep);
       const double taumid = dmin(tau_bot, MAXEXP);
-      const double xm = rt_exp35(-dmin(taumid / u0, MAXEXP));
+      const double xm = pexp35<P>(-dmin(pdiv<P>(taumid, u0), MAXEXP));
       const double cpmid = ap * xm, cmmid = am * xm;
       fmidp = xk1 * ep + gama * xk2 * em + cpmid;
       fmidm = xk1 * ep * gama + xk2 * em + cmmid;
@@ -408,7 +443,7 @@ __device__ __forceinline__ void vis_solve(int nl, VisState<NLM> &s, const double
     const double ep = 1.0, em = 1.0;
     fluxup = xk1_1 * ep + gama_1 * xk2_1 * em + ap_1;
     fluxdn = xk1_1 * ep * gama_1 + xk2_1 * em + am_1;
-    fluxdn = fluxdn + u0 * f0pi * rt_exp35(-dmin(0.0 / u0, MAXEXP));
+    fluxdn = fluxdn + u0 * f0pi * pexp35<P>(-dmin(pdiv<P>(0.0, u0), MAXEXP));
   }
 }
 
@@ -428,14 +463,15 @@ struct IrState {
 struct IrLayer {
   double lam, gama, term, b0, b1;
 };
+template <int P = 0>
 __device__ __forceinline__ IrLayer ir_layer(double w0, double cb, double dtau, double bt, double bb) {
   IrLayer v;
-  const double alpha = sqrt((1.0 - w0) / (1.0 - w0 * cb));
+  const double alpha = sqrt(pdiv<P>(1.0 - w0, 1.0 - w0 * cb));
   v.lam = alpha * (1.0 - w0 * cb) * 2.0;  // /UBARI with UBARI = 0.5: exact
-  v.b1 = (bb - bt) / dtau;
+  v.b1 = pdiv<P>(bb - bt, dtau);
   v.b0 = bt;
-  v.gama = (1.0 - alpha) / (1.0 + alpha);
-  v.term = UBARI / (1.0 - w0 * cb);
+  v.gama = pdiv<P>(1.0 - alpha, 1.0 + alpha);
+  v.term = pdiv<P>(UBARI, 1.0 - w0 * cb);
   return v;
 }
 
@@ -469,7 +505,7 @@ __device__ __forceinline__ void ir_optics(int nl, Src &src, const LA &la, IrStat
 
 // GFLUXI (code/gfluxi.f90:69-194) + DSOLVER/DTRIDGL.  bk(K) = Planck value of
 // TLEV(K).  Calls acc(L, fmidp, fmidm); returns FTOPUP.
-template <int NLM, typename Acc>
+template <int NLM, int P = 0, typename Acc>
 __device__ __forceinline__ double ir_solve(int nl, IrState<NLM> &s, const double *__restrict__ cosb,
                                            const double *__restrict__ bk, double rsf,
                            double btop, double bsurf, Acc &acc) {
@@ -486,22 +522,22 @@ __device__ __forceinline__ double ir_solve(int nl, IrState<NLM> &s, const double
       bb_n = bk[2 * n - 1];
     }
     // TLEV(2L), TLEV(2L+2); the bottom layer uses TLEV(2L), TLEV(2L+1)
-    const IrLayer v = ir_layer(w0c, cbc, dtau, btc, bbc);
+    const IrLayer v = ir_layer<P>(w0c, cbc, dtau, btc, bbc);
     const double lam = v.lam, gama = v.gama, term = v.term, b0 = v.b0, b1 = v.b1;
     const double cp = b0 + b1 * dtau + b1 * term;
     const double cm = b0 + b1 * dtau - b1 * term;
     const double cpm1 = b0 + b1 * term;
     const double cmm1 = b0 - b1 * term;
-    const double ep = rt_exp35(dmin(lam * dtau, MAXEXP));
-    const double em = 1.0 / ep;
+    const double ep = pexp35<P>(dmin(lam * dtau, MAXEXP));
+    const double em = pdiv<P>(1.0, ep);
     const double e1 = ep + gama * em, e2 = ep - gama * em;
     const double e3 = gama * ep + em, e4 = gama * ep - em;
     if (n == nl) {
       const int i = 2 * nl;
       const double af = e1 - rsf * e3, bf = e2 - rsf * e4;
       const double df = bsurf - cp + rsf * cm;
-      as1 = af / bf;
-      ds1 = df / bf;
+      as1 = pdiv<P>(af, bf);
+      ds1 = pdiv<P>(df, bf);
       s.row[i] = Pair{as1, ds1};
     } else {
       {
@@ -510,7 +546,7 @@ __device__ __forceinline__ double ir_solve(int nl, IrState<NLM> &s, const double
         const double bf = (e1 - e3) * (1.0 + gama_n1);
         const double cf = (e1 + e3) * (gama_n1 - 1.0);
         const double df = e3 * (cpm1_n1 - cp) + e1 * (cm - cmm1_n1);
-        const double x = 1.0 / (bf - cf * as1);
+        const double x = pdiv<P>(1.0, bf - cf * as1);
         as1 = af * x;
         ds1 = (df - cf * ds1) * x;
         s.row[i] = Pair{as1, ds1};
@@ -521,7 +557,7 @@ __device__ __forceinline__ double ir_solve(int nl, IrState<NLM> &s, const double
         const double bf = (e2 + e4) * (gama_n1 - 1.0);
         const double cf = 2.0 * (1.0 - gama_n1 * gama_n1);
         const double df = (gama_n1 - 1.0) * (cpm1_n1 - cp) + (1.0 - gama_n1) * (cm - cmm1_n1);
-        const double x = 1.0 / (bf - cf * as1);
+        const double x = pdiv<P>(1.0, bf - cf * as1);
         as1 = af * x;
         ds1 = (df - cf * ds1) * x;
         s.row[i] = Pair{as1, ds1};
@@ -529,7 +565,7 @@ __device__ __forceinline__ double ir_solve(int nl, IrState<NLM> &s, const double
     }
     if (n == 1) {
       const double af = 0.0, bf = gama + 1.0, cf = gama - 1.0, df = btop - cmm1;
-      const double x = 1.0 / (bf - cf * as1);
+      const double x = pdiv<P>(1.0, bf - cf * as1);
       s.row[1] = Pair{af * x, (df - cf * ds1) * x};
     }
     gama_n1 = gama;
@@ -562,11 +598,11 @@ __device__ __forceinline__ double ir_solve(int nl, IrState<NLM> &s, const double
     const double xk1 = xa + xb;
     double xk2 = xa - xb;
     xk2 = xk2_clip(xa, xk2);  // dsolver.f90:101
-    const IrLayer v = ir_layer(w0c, cbc, dtc, btc, bbc);
+    const IrLayer v = ir_layer<P>(w0c, cbc, dtc, btc, bbc);
     const double lam = v.lam, gama = v.gama, b0 = v.b0, b1 = v.b1, term = v.term;
     const double dk = (n < nl) ? dmc : dtc;
-    const double ep = rt_exp35(dmin(lam * dk, MAXEXP));
-    const double em = 1.0 / ep;
+    const double ep = pexp35<P>(dmin(lam * dk, MAXEXP));
+    const double em = pdiv<P>(1.0, ep);
     const double cpmid = b0 + b1 * dk + b1 * term;
     const double cmmid = b0 + b1 * dk - b1 * term;
     double fmidp = xk1 * ep + gama * xk2 * em + cpmid;

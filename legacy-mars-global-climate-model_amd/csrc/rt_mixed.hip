@@ -5,10 +5,10 @@
 
 namespace mgcm {
 hipError_t launch_mixed(const DevTables &dt, const RunArgs &a, hipStream_t st) {
-  return launch_column<true>(dt, a, st);
+  return launch_column<P_TAB32>(dt, a, st);
 }
 hipError_t launch_batch_mixed(int cls, int kind, int waves, const DevTables &dt, const RunArgs *grids,
                               const uint2 *work, int nwork, int nblocks, size_t lds, hipStream_t st) {
-  return launch_batch_class<true>(cls, kind, waves, dt, grids, work, nwork, nblocks, lds, st);
+  return launch_batch_class<P_TAB32>(cls, kind, waves, dt, grids, work, nwork, nblocks, lds, st);
 }
 }  // namespace mgcm

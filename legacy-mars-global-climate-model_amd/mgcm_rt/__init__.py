@@ -29,7 +29,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmgcm_rt.so")
 
 NV, NI, NG = 7, 5, 17
-FP64, MIXED = 0, 1
+FP64, MIXED, STUDY = 0, 1, 2
+# precision-study component bits, OR-ed with STUDY (grids of 25..36 layers): k-tables
+# in FP32, k-interpolation arithmetic in FP32, sub-level optical depths in FP32,
+# layer transmissions in FP32, FP64 division by a Newton-refined reciprocal
+P_TAB32, P_INTERP32, P_TAU32, P_TRANS32, P_FASTDIV = 0x100, 0x200, 0x400, 0x800, 0x1000
 SKIP_VIS, SKIP_IR = 0x10, 0x20  # OR-ed into the precision argument: one part only
 DP = C.POINTER(C.c_double)
 

@@ -12,6 +12,8 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
+from conftest import rel_err
+
 pytestmark = pytest.mark.gpu
 
 
@@ -41,3 +43,43 @@ def test_mixed_within_conditioning_floor(oracle, nlay, clouds, seed):
         assert float(np.abs(got[k] - ref[k]).max()) <= 2.0 * floor + 1e-3, k
     sun = c["acosz"] >= np.float32(1e-5)
     assert np.allclose(got["detau"][sun], ref["detau"][sun], rtol=1e-5, atol=1e-12)
+
+
+def test_precision_study_modes():
+    """The precision study's column kernels (MGCM_RT_STUDY | component bits): every
+    built combination runs and moves the result (its component is active), FP64 with
+    the Newton-refined reciprocal -- a few ulps per division, which the ill-conditioned
+    solves amplify to ~1e-5 in some fluxes -- stays within the 0.05 K/sol bar of the
+    FP64 heating, and the study is refused outside the 25..36-layer class and on the
+    batched path."""
+    import torch
+    from mgcm_rt import (FP64, P_FASTDIV, P_INTERP32, P_TAB32, P_TAU32, P_TRANS32, STUDY, Radiation,
+                         RadiationError, alloc_outputs, synth, to_device)
+    t, c = synth.make_workload(35, 600, 36, clouds=True)
+    rt = Radiation(t)
+    dc = to_device(c)
+    s = torch.cuda.current_stream().cuda_stream
+    ref = to_device(alloc_outputs(600, 36))
+    rt.run_device(dc, ref, FP64, s)
+    modes = {"interp": STUDY | P_TAB32 | P_INTERP32, "tau": STUDY | P_TAU32, "trans": STUDY | P_TRANS32,
+             "all": STUDY | P_TAB32 | P_INTERP32 | P_TAU32 | P_TRANS32, "fastdiv": STUDY | P_FASTDIV,
+             "fastdiv_tab": STUDY | P_FASTDIV | P_TAB32}
+    for name, mode in modes.items():
+        o = to_device(alloc_outputs(600, 36))
+        rt.run_device(dc, o, mode, s)
+        torch.cuda.synchronize()
+        rt.check()
+        e = max(rel_err(o[k].cpu().numpy(), ref[k].cpu().numpy()) for k in ("fmnetv", "fmneti", "fluxupv", "fluxupi"))
+        dh = float((o["hr"] - ref["hr"]).abs().max())
+        print(name, f"flux rel {e:.1e}, max |dhr| {dh:.3e} K/sol", flush=True)
+        assert all(bool(torch.isfinite(v).all()) for v in o.values()), name
+        assert e > 0.0, name                       # the component is active
+        if name == "fastdiv":
+            assert dh <= 0.05, dh
+    o = to_device(alloc_outputs(600, 24))
+    _, c24 = synth.make_workload(36, 600, 24)
+    with pytest.raises(RadiationError):
+        rt.run_device(to_device(c24), o, STUDY | P_TRANS32, s)
+    with pytest.raises(RadiationError) as err:
+        rt.run_batched([dc], [to_device(alloc_outputs(600, 36))], STUDY | P_TRANS32, s)
+    assert err.value.status == -6
