@@ -89,7 +89,8 @@ def setspv():
     dwnv = bwnv[1:] - bwnv[:-1]
     wl = _f32(1.0e4) / wnov
     p0 = 9.423e6
-    tauray = ((_f32(8.7) / GRAV) * (_f32(1.527) * (1.0 + _f32(0.013) / (wl * wl)) / ((wl * wl) * (wl * wl)))
+    # WL**4 as the reference build evaluates it: ((WL*WL)*WL)*WL
+    tauray = ((_f32(8.7) / GRAV) * (_f32(1.527) * (1.0 + _f32(0.013) / (wl * wl)) / (((wl * wl) * wl) * wl))
               * SCALEP / p0)
     return wnov, dwnv, solar, tauray
 

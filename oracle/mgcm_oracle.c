@@ -993,7 +993,7 @@ void or_setspv(double *wnov, double *dwnv, double *solarf, double *tauray)
     solarf[m] = (double)solar[m];
     wl = (double)1.0e+4f / wnov[m];
     tauray[m] = ((double)8.7f / GRAV) *
-                ((double)1.527f * (1.0 + (double)0.013f / (wl * wl)) / ((wl * wl) * (wl * wl))) * SCALEP / p0;
+                ((double)1.527f * (1.0 + (double)0.013f / (wl * wl)) / (((wl * wl) * wl) * wl)) * SCALEP / p0;
   }
 }
 

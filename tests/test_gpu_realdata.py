@@ -52,8 +52,9 @@ def test_plain_path_on_real_tables_matches_reference(real, data_dir):
     """Device tables + plain FP64 path against the reference's outputs on its own k-distribution."""
     from mgcm_rt import Radiation, realdata
     t = realdata.load_tables(data_dir, 24)
-    t["solarf"], t["tauray"] = real["solarf"], real["tauray"]
-    t["sol"] = real["solarf"] / 2.4277
+    # the start-up tables are the reference's own, bit for bit (SETSPV: code/setspv.f90:67-105)
+    assert np.array_equal(t["solarf"], real["solarf"]) and np.array_equal(t["tauray"], real["tauray"])
+    t["sol"] = real["solarf"] / 2.4277  # SOLARF/RSDIST of the fixture's orbit (code/comp3.f:300-304)
     got = Radiation(t).run_host(fixture_columns(real))
     want = {k: real["out_" + k] for k in ("fmnetv", "fluxupv", "fluxdnv", "fmneti", "fluxupi", "fluxdni", "nfluxtopv",
                                           "nfluxtopi", "diffvt", "detau", "cumtauv", "hr")}
