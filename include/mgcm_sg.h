@@ -60,6 +60,17 @@ int mgcm_sg_train_step(mgcm_sg *sg, int batch, const float *x, const float *t, f
  * MGCM_SG_NPARAM); parameters unchanged.  Synchronises. */
 int mgcm_sg_gradients(mgcm_sg *sg, int batch, const float *x, const float *t, float *grads);
 
+/* The consistency check of the surrogate form, on the device: for each of ncol
+ * columns, the outputs of two networks y0, y1 ([ncol][MGCM_SG_NOUT]; y1 may be
+ * NULL) de-standardised as h = scale * sinh(y * tstd + tmean) (FP64; tmean,
+ * tstd [MGCM_SG_NOUT] FP64), h of y0 into h [ncol][MGCM_SG_NOUT], the largest
+ * |h0 - h1| into dis [ncol], and into inr [ncol] 1 where every standardised
+ * feature x [ncol][MGCM_SG_NIN] lies in [xlo, xhi] (FP32 [MGCM_SG_NIN]), else 0.
+ * Asynchronous on stream. */
+int mgcm_sg_check(int ncol, const float *y0, const float *y1, const float *x, const double *tmean,
+                  const double *tstd, const float *xlo, const float *xhi, double scale, double *h, double *dis,
+                  int *inr, void *stream);
+
 /* Standardised features of ncol columns of a grid with nlay = 36 layers,
  * device pointers in the layouts of mgcm_rt_columns; mean/inv_std: device
  * [MGCM_SG_NIN]; x: device [ncol][MGCM_SG_NIN]. */

@@ -188,18 +188,34 @@ __global__ void k_loss(int B, const float *__restrict__ y, const float *__restri
   if (threadIdx.x == 0) atomicAdd(loss, part[0] / ((float)B * NOUT));
 }
 
-// db[n] = sum over rows of dZ[b][n]: 4 row groups per column, combined in a
-// fixed order (deterministic).
-__global__ void k_colsum(int B, int N, const bf16 *__restrict__ dz, long ld, float *__restrict__ db) {
-  __shared__ float part[4][64];
+// db[n] = sum over rows of dZ[b][n], deterministic and spread over the CUs:
+// stage 1 -- block (column chunk, row slice p) sums rows p*RS.. of 64 columns
+// in 4 row groups combined in a fixed order into part[p][n]; stage 2 -- one
+// thread per column adds the slices in order.
+constexpr int COLSUM_ROWS = 64;  // rows per slice
+__global__ void k_colsum_part(int B, int N, const bf16 *__restrict__ dz, long ld, float *__restrict__ part) {
+  __shared__ float acc[4][64];
   const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
-  const int n = blockIdx.x * 64 + c;
+  const int n = blockIdx.x * 64 + c, p = blockIdx.y;
+  const int b1 = min(B, (p + 1) * COLSUM_ROWS);
   float s = 0.f;
   if (n < N)
-    for (int b = g; b < B; b += 4) s += (float)dz[(long)b * ld + n];
-  part[g][c] = s;
+    for (int b = p * COLSUM_ROWS + g; b < b1; b += 4) s += (float)dz[(long)b * ld + n];
+  acc[g][c] = s;
   __syncthreads();
-  if (g == 0 && n < N) db[n] = ((part[0][c] + part[1][c]) + part[2][c]) + part[3][c];
+  if (g == 0 && n < N) part[(long)p * N + n] = ((acc[0][c] + acc[1][c]) + acc[2][c]) + acc[3][c];
+}
+__global__ void k_colsum_fin(int N, int P, const float *__restrict__ part, float *__restrict__ db) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  float s = 0.f;
+  for (int p = 0; p < P; p++) s += part[(long)p * N + n];
+  db[n] = s;
+}
+static void colsum(int B, int N, const bf16 *dz, long ld, float *part, float *db, hipStream_t st) {
+  const int P = (B + COLSUM_ROWS - 1) / COLSUM_ROWS;
+  hipLaunchKernelGGL(k_colsum_part, dim3((N + 63) / 64, P), dim3(256), 0, st, B, N, dz, ld, part);
+  hipLaunchKernelGGL(k_colsum_fin, dim3((N + 255) / 256), dim3(256), 0, st, N, P, part, db);
 }
 
 __global__ void k_adam(long n, float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
@@ -263,6 +279,7 @@ struct mgcm_sg {
   float *p, *g, *m, *v;        // FP32 master parameters, gradients, Adam moments
   bf16 *wt[4];                 // BF16 W^T per layer: [NH][KIN], [NH][NH], [NH][NH], [NOUT_PAD][NH]
   bf16 *x0, *h[3], *dy, *dz[2];
+  float *part;  // bias-gradient partial sums, [ceil(B/COLSUM_ROWS)][NH]
   float *z[3], *y;
 };
 
@@ -303,7 +320,7 @@ hipError_t backward(mgcm_sg *s, int B, const float *t, float *loss, hipStream_t 
     Epi e{s->g + OW4, NOUT, nullptr, nullptr, 0, nullptr, 0};
     hipError_t r = gemm<false, false, EPI_F32>(NH, NOUT, B, s->h[2], 1, NH, s->dy, NOUT_PAD, 1, e, st);
     if (r != hipSuccess) return r;
-    hipLaunchKernelGGL(k_colsum, dim3(1), dim3(256), 0, st, B, NOUT, s->dy, (long)NOUT_PAD, s->g + OB4);
+    colsum(B, NOUT, s->dy, (long)NOUT_PAD, s->part, s->g + OB4, st);
     Epi d{nullptr, 0, nullptr, s->dz[0], NH, s->z[2], NH};
     r = gemm<true, false, EPI_DGELU>(B, NH, NOUT_PAD, s->dy, NOUT_PAD, 1, s->wt[3], NH, 1, d, st);
     if (r != hipSuccess) return r;
@@ -316,7 +333,7 @@ hipError_t backward(mgcm_sg *s, int B, const float *t, float *loss, hipStream_t 
     Epi e{s->g + OW[l], NH, nullptr, nullptr, 0, nullptr, 0};
     hipError_t r = gemm<false, false, EPI_F32>(kin, NH, B, hin, 1, kin, s->dz[cur], NH, 1, e, st);
     if (r != hipSuccess) return r;
-    hipLaunchKernelGGL(k_colsum, dim3(NH / 64), dim3(256), 0, st, B, NH, s->dz[cur], (long)NH, s->g + OB[l]);
+    colsum(B, NH, s->dz[cur], (long)NH, s->part, s->g + OB[l], st);
     if (l > 0) {
       Epi d{nullptr, 0, nullptr, s->dz[1 - cur], NH, s->z[l - 1], NH};
       r = gemm<true, false, EPI_DGELU>(B, NH, NH, s->dz[cur], NH, 1, s->wt[l], KL[l], 1, d, st);
@@ -350,7 +367,7 @@ struct Rng {
 extern "C" int mgcm_sg_destroy(mgcm_sg *s) {
   if (!s) return MGCM_RT_ERR_ARG;
   void *bufs[] = {s->p, s->g, s->m, s->v, s->wt[0], s->wt[1], s->wt[2], s->wt[3], s->x0, s->h[0], s->h[1],
-                  s->h[2], s->dy, s->dz[0], s->dz[1], s->z[0], s->z[1], s->z[2], s->y};
+                  s->h[2], s->dy, s->dz[0], s->dz[1], s->z[0], s->z[1], s->z[2], s->y, s->part};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   delete s;
@@ -394,6 +411,7 @@ extern "C" int mgcm_sg_create(int max_batch, unsigned long long seed, mgcm_sg **
   for (int l = 0; l < 3 && ok; l++)
     ok = hipMalloc(&s->h[l], B * NH * 2) == hipSuccess && hipMalloc(&s->z[l], B * NH * 4) == hipSuccess;
   for (int l = 0; l < 2 && ok; l++) ok = hipMalloc(&s->dz[l], B * NH * 2) == hipSuccess;
+  ok = ok && hipMalloc(&s->part, (long)((B + COLSUM_ROWS - 1) / COLSUM_ROWS) * NH * 4) == hipSuccess;
   if (!ok) {
     mgcm_sg_destroy(s);
     return MGCM_RT_ERR_NOMEM;
@@ -455,6 +473,44 @@ extern "C" int mgcm_sg_features(int ncol, int nlay, const double *plev, const do
     return MGCM_RT_ERR_ARG;
   hipLaunchKernelGGL(k_features, dim3(nblk(ncol, 128)), dim3(128), 0, (hipStream_t)stream, ncol, nlay, plev, tlev,
                      pmid, tmid, qh2o, tauref, albi, mean, inv_std, x);
+  return hipGetLastError() == hipSuccess ? MGCM_RT_OK : MGCM_RT_ERR_DEVICE;
+}
+
+// The surrogate's check, one thread per column: both networks' outputs
+// de-standardised, h_k = scale * sinh(y_k * tstd + tmean) (the inverse of the
+// asinh target transform, FP64), their largest disagreement over the outputs,
+// and whether every feature lies in the trusted range [xlo, xhi].
+__global__ void k_check(int ncol, const float *__restrict__ y0, const float *__restrict__ y1,
+                        const float *__restrict__ x, const double *__restrict__ tmean,
+                        const double *__restrict__ tstd, const float *__restrict__ xlo,
+                        const float *__restrict__ xhi, double scale, double *__restrict__ h,
+                        double *__restrict__ dis, int *__restrict__ inr) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ncol) return;
+  double d = 0.0;
+  for (int j = 0; j < NOUT; j++) {
+    const double a = scale * sinh((double)y0[(long)i * NOUT + j] * tstd[j] + tmean[j]);
+    const double b = y1 ? scale * sinh((double)y1[(long)i * NOUT + j] * tstd[j] + tmean[j]) : a;
+    h[(long)i * NOUT + j] = a;
+    d = fmax(d, fabs(a - b));
+  }
+  dis[i] = d;
+  int ok = 1;
+  for (int k = 0; k < NIN; k++) {
+    const float v = x[(long)i * NIN + k];
+    ok &= (v >= xlo[k]) & (v <= xhi[k]);
+  }
+  inr[i] = ok;
+}
+
+extern "C" int mgcm_sg_check(int ncol, const float *y0, const float *y1, const float *x, const double *tmean,
+                             const double *tstd, const float *xlo, const float *xhi, double scale, double *h,
+                             double *dis, int *inr, void *stream) {
+  if (ncol < 0) return MGCM_RT_ERR_ARG;
+  if (ncol == 0) return MGCM_RT_OK;
+  if (!y0 || !x || !tmean || !tstd || !xlo || !xhi || !h || !dis || !inr) return MGCM_RT_ERR_ARG;
+  hipLaunchKernelGGL(k_check, dim3(nblk(ncol, 128)), dim3(128), 0, (hipStream_t)stream, ncol, y0, y1, x, tmean,
+                     tstd, xlo, xhi, scale, h, dis, inr);
   return hipGetLastError() == hipSuccess ? MGCM_RT_OK : MGCM_RT_ERR_DEVICE;
 }
 

@@ -43,6 +43,7 @@ def _bind(lib):
     lib.mgcm_sg_gradients.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, FP]
     lib.mgcm_sg_features.argtypes = [C.c_int, C.c_int] + [C.c_void_p] * 10 + [C.c_void_p]
     lib.mgcm_sg_gemm_check.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.mgcm_sg_check.argtypes = [C.c_int] + [C.c_void_p] * 7 + [C.c_double] + [C.c_void_p] * 4
     lib._sg_bound = True
     return lib
 
@@ -137,8 +138,11 @@ def exact_ir_heating(rt: Radiation, dc: dict, stream: int = 0):
     out["detau"] = torch.zeros((ncol, 17, 7), dtype=torch.float64, device=dev)
     out["cumtauv"] = torch.zeros((ncol, 17, 7), dtype=torch.float64, device=dev)
     out["hr"] = None
+    out["hr"] = torch.empty((ncol, nlay + 1), dtype=torch.float64, device=dev)
     rt.run_device(dc, out, FP64 | SKIP_VIS, stream)
-    return ir_heating(out["fmneti"], dc["plev"])
+    # with the VIS part skipped the kernel's heating is the IR part alone: the same
+    # operations as ir_heating(fmneti, plev) for GCM layers 1..36
+    return out["hr"][:, 1:]
 
 
 class Surrogate:
@@ -163,10 +167,17 @@ class Surrogate:
                 b1 = min(ncol, b0 + net.max_batch)
                 net.forward(x[b0:b1], y[b0:b1], stream)
             ys.append(y)
-        h = [SCALE * torch.sinh(y.double() * self.stats["tstd"] + self.stats["tmean"]) for y in ys]
-        dis = (h[0] - h[1]).abs().amax(dim=1) if len(h) > 1 else torch.zeros(ncol, dtype=torch.float64, device=dev)
-        inr = ((x >= self.stats["xlo"]) & (x <= self.stats["xhi"])).all(dim=1)
-        return h[0], dis, inr
+        # de-standardisation, disagreement and range check in one device kernel
+        h = torch.empty((ncol, NOUT), dtype=torch.float64, device=dev)
+        dis = torch.empty(ncol, dtype=torch.float64, device=dev)
+        inr = torch.empty(ncol, dtype=torch.int32, device=dev)
+        lib = _bind(load_library())
+        st = self.stats
+        _check(lib.mgcm_sg_check(ncol, ys[0].data_ptr(), ys[1].data_ptr() if len(ys) > 1 else None, x.data_ptr(),
+                                 st["tmean"].data_ptr(), st["tstd"].data_ptr(), st["xlo"].data_ptr(),
+                                 st["xhi"].data_ptr(), SCALE, h.data_ptr(), dis.data_ptr(), inr.data_ptr(),
+                                 C.c_void_p(stream)), "mgcm_sg_check")
+        return h, dis, inr.bool()
 
     def heating(self, rt: Radiation, dc: dict, stream: int = 0):
         """IR heating with exact fallback: (heating [ncol][36], fallback mask [ncol])."""
