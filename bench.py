@@ -549,6 +549,7 @@ def form_surrogate(args, rt_unused, stream):
     return {"workload": f"{ncol} fresh clear-sky columns x {nlay} layers; trained on {ntrain} columns",
             "value": ncol / (ms / 1e3), "unit": "columns/s", "ms_per_step": ms, "fp64_ms_per_step": ms64,
             "speedup_vs_fp64": ms64 / ms, "bar_K_per_sol": bar, "threshold": sg.threshold,
+            "meets_bar": bool(float(err.max()) <= bar and ms < ms64),
             "fallback_fraction": float(res["bad"].double().mean()),
             "max_err_K_per_sol": float(err.max()), "rms_err_K_per_sol": float(err.pow(2).mean().sqrt()),
             "surrogate_only": {"median_err_K_per_sol": float(perr.median()),
