@@ -317,6 +317,50 @@ def form_batched(args, rt, stream):
     return out
 
 
+def form_small_grid(args, rt, stream):
+    """The reference's own CPU case on the GPU: 2,160 columns (a 36 x 60 grid) x
+    24 layers, one single-grid call per step, at each column-kernel tile (1, 2
+    or 4 band slots per column: 4, 2 or 1 columns per wave) and at the
+    library's automatic choice.  Every tile's outputs are compared bit for bit
+    with the one-slot tile's, and the automatic tile's with the oracle at rel
+    1e-10.  At 4 columns per wave this grid is 540 waves, a quarter of the
+    device's resident slots."""
+    import torch
+    from mgcm_rt import FP64, alloc_outputs, synth, to_device
+    if args.nlay != 24:
+        return {"skipped": "the context's tables are built for another layer count"}
+    ncol = 2160
+    c = synth.make_columns(args.seed, ncol, 24)
+    dc, do = to_device(c), to_device(alloc_outputs(ncol, 24))
+    sh = stream.cuda_stream
+    tiles, first = {}, None
+    same = True
+    for tile in (1, 2, 4):
+        rt.set_tile(tile)
+        ms, _ = timed_steps(lambda: rt.run_device(dc, do, FP64, sh), 50, 5, stream)
+        got = {k: v.cpu().numpy() for k, v in do.items()}
+        if first is None:
+            first = got
+        else:
+            same = same and all(np.array_equal(got[k], first[k]) for k in PARITY_KEYS)
+        tiles[str(tile)] = {"ms_per_step": ms, "columns_per_s": ncol / (ms / 1e3)}
+    rt.set_tile(0)
+    auto = rt.tile_for(ncol, 24)
+    ms, _ = timed_steps(lambda: rt.run_device(dc, do, FP64, sh), 50, 5, stream)
+    rt.check()
+    out = {"workload": f"{ncol} columns (36 x 60) x 24 layers, plain FP64, one call per step",
+           "auto_tile": auto, "ms_per_step": ms, "value": ncol / (ms / 1e3), "unit": "columns/s",
+           "speedup_vs_tile1": tiles["1"]["ms_per_step"] / ms, "tiles": tiles, "tiles_bit_identical": same}
+    try:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from oracle import Oracle
+        want = Oracle().run(synth.make_tables(args.seed, 24), c)
+        out["parity"] = parity({k: v.cpu().numpy() for k, v in do.items()}, want)
+    except Exception as e:  # noqa: BLE001
+        out["parity"] = {"error": f"{type(e).__name__}: {e}"[:200]}
+    return out
+
+
 def _graph_ms(fn, stream, reps=20, replays=5):
     """GPU time of one fn() call: reps calls captured in a HIP graph, replayed;
     a kernel of a few microseconds is not hidden behind Python launch overhead."""
@@ -633,7 +677,8 @@ def main():
     forms = {}
     if not args.no_forms:
         del do
-        for name, fn in (("real_tables", form_real), ("host_arrays", form_host_arrays), ("next_rows", form_next_rows),
+        for name, fn in (("real_tables", form_real), ("host_arrays", form_host_arrays), ("small_grid", form_small_grid),
+                         ("next_rows", form_next_rows),
                          ("mixed", form_mixed), ("mixed_real", form_mixed_real), ("batched", form_batched), ("surrogate", form_surrogate)):
             if args.forms and name not in args.forms.split(","):
                 continue

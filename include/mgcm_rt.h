@@ -219,22 +219,39 @@ int mgcm_rt_run_batched(mgcm_rt_ctx *ctx, int ngrids, const mgcm_rt_columns *gri
                         const mgcm_rt_fluxes *outs, int precision, void *stream);
 
 /* The batched tuner's choice per layer class of the last planned batch: the
- * column kernel's instance (waves per SIMD) and persistent grid size, and the
- * time of each candidate (waves per SIMD x multiple of that instance's resident
- * capacity) on the first tuned_items work items (0 ms: not timed -- the list
- * fits one resident round, or the candidate repeats the one before).  Returns
- * the number of classes (fills at most maxcls).  No counterpart in the
+ * column kernel's tile (band slots per column), instance (waves per SIMD) and
+ * persistent grid size, and the time of each candidate (tile x waves per SIMD
+ * x multiple of that instance's resident capacity) on the first tuned_items
+ * work items (0 ms: not timed -- the candidate repeats an earlier launch).
+ * Returns the number of classes (fills at most maxcls).  No counterpart in the
  * reference (it has no batched path). */
+#define MGCM_RT_NCAND 10
 typedef struct {
   int max_layers;     /* grids with nlay <= max_layers (and above the class below) */
   int items;          /* 4-column work items of the class */
+  int tile;           /* chosen band slots per column: 1, 2 or 4 */
   int waves_per_simd; /* chosen instance: 1 or 2 */
   int blocks;         /* chosen persistent grid size */
   int tuned_items;    /* items the candidates were timed on */
-  int cand_waves[6], cand_mult[6];
-  float cand_ms[6];
+  int cand_tile[MGCM_RT_NCAND], cand_waves[MGCM_RT_NCAND], cand_mult[MGCM_RT_NCAND];
+  float cand_ms[MGCM_RT_NCAND];
 } mgcm_rt_batch_class;
 int mgcm_rt_batched_tuning(const mgcm_rt_ctx *ctx, int maxcls, mgcm_rt_batch_class *info);
+
+/* The column kernel's tile for single-grid calls (mgcm_rt_run, _noalloc,
+ * _host): band_slots = 1 (4 columns per wave, each wave solving one band of a
+ * column per 16-lane row at a time), 2 (2 columns, 2 bands at a time) or 4
+ * (1 column, 4 bands at a time); 0 (the default) lets the library choose per
+ * grid from its column count, layer count and the device's CU count.  Every
+ * tile gives the same bits; more slots give a small grid more waves at some
+ * extra work per column (empty band slots).  MGCM_RT_ERR_ARG for another
+ * value.  No counterpart in the reference (its radiation is serial per
+ * column, code/comp3.f:411-415).  A non-zero tile also restricts the batched
+ * tuner to that tile's candidates (the next mgcm_rt_run_batched re-plans). */
+int mgcm_rt_set_tile(mgcm_rt_ctx *ctx, int band_slots);
+/* The tile a single-grid call of ncol columns x nlay layers runs with (1, 2
+ * or 4), or MGCM_RT_ERR_ARG. */
+int mgcm_rt_tile_for(const mgcm_rt_ctx *ctx, int ncol, int nlay);
 
 /* Init-time table builders (host in/out pointers), both equal to the
  * reference's tables bit for bit:

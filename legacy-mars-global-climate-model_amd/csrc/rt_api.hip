@@ -5,6 +5,7 @@
 #include <string.h>
 #include <vector>
 
+#include "rt_column.h"
 #include "rt_ctx.h"
 #include "rt_libm.h"
 
@@ -54,6 +55,12 @@ extern "C" int mgcm_rt_init(const mgcm_rt_tables *t, mgcm_rt_ctx **out) {
   mgcm_rt_ctx *c = new (std::nothrow) mgcm_rt_ctx();
   if (!c) return MGCM_RT_ERR_NOMEM;
   if (hipGetDevice(&c->device) != hipSuccess) { delete c; return MGCM_RT_ERR_DEVICE; }
+  if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess ||
+      c->ncu <= 0) {
+    delete c;
+    return MGCM_RT_ERR_DEVICE;
+  }
+  c->tile = 0;
 
   RtConst h;
   memset(&h, 0, sizeof(h));
@@ -192,8 +199,9 @@ hipError_t run_grid(const mgcm_rt_ctx *c, const mgcm_rt_columns *in, const mgcm_
   a.bands = bands;
   a.clr = clr;
   const int p = path & 0xf;
-  hipError_t e = (p == MGCM_RT_FP64)    ? launch_plain(c->dt, a, st)
-                 : (p == MGCM_RT_MIXED) ? launch_mixed(c->dt, a, st)
+  const int tile = c->tile ? c->tile : auto_tile(in->ncol, in->nlay, c->ncu);
+  hipError_t e = (p == MGCM_RT_FP64)    ? launch_plain(c->dt, a, tile, st)
+                 : (p == MGCM_RT_MIXED) ? launch_mixed(c->dt, a, tile, st)
                                         : launch_study(path >> 8, c->dt, a, st);
   // surface fluxes for TEMPGR from this run's DETAU/DIFFVT (code/comp3.f:336-348)
   if (e == hipSuccess && out->directsol)
@@ -268,6 +276,17 @@ extern "C" int mgcm_rt_run_batched(mgcm_rt_ctx *c, int ngrids, const mgcm_rt_col
 extern "C" int mgcm_rt_batched_tuning(const mgcm_rt_ctx *c, int maxcls, mgcm_rt_batch_class *info) {
   if (!c || maxcls < 0 || (maxcls > 0 && !info)) return MGCM_RT_ERR_ARG;
   return batched_tuning_impl(c, maxcls, info);
+}
+
+extern "C" int mgcm_rt_set_tile(mgcm_rt_ctx *c, int band_slots) {
+  if (!c || !(band_slots == 0 || band_slots == 1 || band_slots == 2 || band_slots == 4)) return MGCM_RT_ERR_ARG;
+  c->tile = band_slots;
+  return MGCM_RT_OK;
+}
+
+extern "C" int mgcm_rt_tile_for(const mgcm_rt_ctx *c, int ncol, int nlay) {
+  if (!c || ncol < 0 || nlay < 1 || nlay > MAX_NLAY) return MGCM_RT_ERR_ARG;
+  return c->tile ? c->tile : auto_tile(ncol, nlay, c->ncu);
 }
 
 // ---------------------------------------------------------------------------

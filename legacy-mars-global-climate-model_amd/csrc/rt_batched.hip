@@ -6,8 +6,9 @@
 //   * each class gets a work list of (grid, first column) 4-column blocks,
 //     grids with more layers first, columns in order;
 //   * one persistent kernel per class walks its list round-robin; on the first
-//     call of a shape the tuner times, per class, the column kernel's two
-//     instances (one or two waves per SIMD) at 1x, 2x and 4x their resident
+//     call of a shape the tuner times, per class, the column kernel's tiles
+//     (1, 2 or 4 band slots per column: 4, 2 or 1 columns per wave) and
+//     instances (one or two waves per SIMD) at multiples of their resident
 //     capacity on a prefix of the list and keeps the fastest;
 //   * DSOLFLUX of the grids that ask for surface fluxes is one more persistent
 //     launch over (grid, 16 columns) items;
@@ -34,20 +35,22 @@ namespace {
 constexpr int NKIND = 3;
 constexpr int KIND_COLS[NKIND] = {4, 9, 12};
 
-// Tuner candidates of the column kernel: (waves per SIMD, multiple of the
-// instance's resident capacity).
-constexpr int NCAND = 6;
-constexpr int CAND_WAVES[NCAND] = {2, 2, 2, 1, 1, 1};
-constexpr int CAND_MULT[NCAND] = {1, 2, 4, 1, 2, 4};
+// Tuner candidates of the column kernel: (band slots per column, waves per
+// SIMD, multiple of the instance's resident capacity).
+constexpr int NCAND = MGCM_RT_NCAND;
+constexpr int CAND_TILE[NCAND] = {1, 1, 1, 1, 1, 1, 2, 2, 4, 4};
+constexpr int CAND_WAVES[NCAND] = {2, 2, 2, 1, 1, 1, 2, 2, 2, 2};
+constexpr int CAND_MULT[NCAND] = {1, 2, 4, 1, 2, 4, 1, 2, 1, 2};
 
 struct ClassPlan {
   int nwork[NKIND] = {0, 0, 0};
   size_t work_off[NKIND] = {0, 0, 0};  // offsets of the lists in d_work
   size_t lds = 0;                      // dynamic LDS per column-kernel block
   int nblocks[NKIND] = {0, 0, 0};      // persistent grid sizes (kind 0 from the tuner)
+  int tile = 1;                        // column kernel tile (from the tuner)
   int waves = MGCM_FP64_WAVES;         // column kernel instance (from the tuner)
   int tuned_prefix = 0;                // work items the candidates were timed on (0: not timed)
-  float cand_ms[NCAND] = {0, 0, 0, 0, 0, 0};  // per candidate; 0 where not timed
+  float cand_ms[NCAND] = {};           // per candidate; 0 where not timed
 };
 
 struct BatchState {
@@ -76,39 +79,30 @@ void drop_graph(BatchState *b) {
   b->graph = nullptr;
 }
 
-hipError_t launch_class(int precision, int k, int kind, int waves, const DevTables &dt, const BatchState *b,
-                        int nwork, int nblocks, hipStream_t st) {
+hipError_t launch_class(int precision, int k, int kind, int waves, int tile, const DevTables &dt,
+                        const BatchState *b, int nwork, int nblocks, hipStream_t st) {
   const ClassPlan &p = b->cls[k];
   const uint2 *work = b->d_work + p.work_off[kind];
   return (precision & 0xf) == MGCM_RT_FP64
-             ? launch_batch_plain(k, kind, waves, dt, b->d_grids, work, nwork, nblocks, p.lds, st)
-             : launch_batch_mixed(k, kind, waves, dt, b->d_grids, work, nwork, nblocks, p.lds, st);
+             ? launch_batch_plain(k, kind, waves, tile, dt, b->d_grids, work, nwork, nblocks, p.lds, st)
+             : launch_batch_mixed(k, kind, waves, tile, dt, b->d_grids, work, nwork, nblocks, p.lds, st);
 }
 
-// Resident one-wave blocks per CU for `lds` bytes of LDS at `waves` waves per
-// SIMD (4 SIMDs per CU).
-int resident_per_cu(size_t lds, int waves) {
-  const int by_lds = (int)((160 * 1024) / std::max<size_t>(lds, 1));
-  return std::max(1, std::min(4 * waves, by_lds));
-}
-
-// Picks the column kernel's instance and persistent grid size of one class:
-// each candidate (waves per SIMD, multiple of that instance's resident
-// capacity) runs twice on a prefix of the work list, the second run timed; the
-// fastest is kept.  The prefix's outputs are written here and rewritten by the
-// real run.  A list that fits one resident round of an instance gets one block
-// per item.
-void tune_class(int precision, int k, const DevTables &dt, BatchState *b, int ncu, hipStream_t st) {
+// Picks the column kernel's tile, instance and persistent grid size of one
+// class: each candidate (band slots per column, waves per SIMD, multiple of
+// that instance's resident capacity, at most one block per sub-item) runs
+// twice on a prefix of the work list, the second run timed; the fastest is
+// kept (only the candidates of tile force_tile when it is not 0).  The
+// prefix's outputs are written here and rewritten by the real run.
+void tune_class(int precision, int k, const DevTables &dt, BatchState *b, int ncu, int force_tile,
+                hipStream_t st) {
   ClassPlan &p = b->cls[k];
   const int cap2 = ncu * resident_per_cu(p.lds, 2);
+  p.tile = force_tile ? force_tile : 1;
   p.waves = MGCM_FP64_WAVES;
-  p.nblocks[0] = std::min(p.nwork[0], cap2);
+  p.nblocks[0] = std::min(p.nwork[0] * p.tile, cap2);
   p.tuned_prefix = 0;
   for (int c = 0; c < NCAND; c++) p.cand_ms[c] = 0.f;
-  if (p.nwork[0] <= cap2) {
-    p.nblocks[0] = p.nwork[0];
-    return;
-  }
   const int prefix = std::min(p.nwork[0], 8 * cap2);
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess) return;
@@ -116,16 +110,19 @@ void tune_class(int precision, int k, const DevTables &dt, BatchState *b, int nc
     (void)hipEventDestroy(e0);
     return;
   }
+  auto blocks_of = [&](int c) {
+    return std::min(CAND_MULT[c] * ncu * resident_per_cu(p.lds, CAND_WAVES[c]), prefix * CAND_TILE[c]);
+  };
   float best_ms = 1e30f;
   for (int c = 0; c < NCAND; c++) {
-    const int w = CAND_WAVES[c];
-    const int nb = std::min(CAND_MULT[c] * ncu * resident_per_cu(p.lds, w), prefix);
-    if (c > 0 && nb == std::min(CAND_MULT[c - 1] * ncu * resident_per_cu(p.lds, w), prefix) &&
-        CAND_WAVES[c - 1] == w)
-      continue;  // the same launch as the previous candidate
-    if (launch_class(precision, k, 0, w, dt, b, prefix, nb, st) != hipSuccess) break;  // warm-up
+    const int s = CAND_TILE[c], w = CAND_WAVES[c], nb = blocks_of(c);
+    if (force_tile && s != force_tile) continue;
+    bool repeat = false;  // the same launch as an earlier candidate
+    for (int e = 0; e < c; e++) repeat = repeat || (CAND_TILE[e] == s && CAND_WAVES[e] == w && blocks_of(e) == nb);
+    if (repeat) continue;
+    if (launch_class(precision, k, 0, w, s, dt, b, prefix, nb, st) != hipSuccess) break;  // warm-up
     (void)hipEventRecord(e0, st);
-    if (launch_class(precision, k, 0, w, dt, b, prefix, nb, st) != hipSuccess) break;
+    if (launch_class(precision, k, 0, w, s, dt, b, prefix, nb, st) != hipSuccess) break;
     (void)hipEventRecord(e1, st);
     if (hipEventSynchronize(e1) != hipSuccess) break;
     float ms = 0.f;
@@ -133,12 +130,13 @@ void tune_class(int precision, int k, const DevTables &dt, BatchState *b, int nc
     p.cand_ms[c] = ms;
     if (ms < best_ms) {
       best_ms = ms;
+      p.tile = s;
       p.waves = w;
       p.nblocks[0] = nb;
     }
   }
   p.tuned_prefix = prefix;
-  p.nblocks[0] = std::min(p.nblocks[0], p.nwork[0]);
+  p.nblocks[0] = std::min(p.nblocks[0], p.nwork[0] * p.tile);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
 }
@@ -260,8 +258,9 @@ int run_batched_impl(mgcm_rt_ctx *c, int ngrids, const mgcm_rt_columns *grids, c
   if (!b) return MGCM_RT_ERR_NOMEM;
 
   std::vector<long long> sig;
-  sig.reserve(2 + 2 * (size_t)ngrids);
+  sig.reserve(3 + 2 * (size_t)ngrids);
   sig.push_back(precision);
+  sig.push_back(c->tile);
   sig.push_back(ngrids);
   for (int i = 0; i < ngrids; i++) {
     sig.push_back(grids[i].ncol);
@@ -283,13 +282,13 @@ int run_batched_impl(mgcm_rt_ctx *c, int ngrids, const mgcm_rt_columns *grids, c
     for (int k = 0; k < NCLASS; k++)
       for (int kind = 1; kind < NKIND; kind++)
         if ((bands & kind) && b->cls[k].nwork[kind] > 0) {
-          if (launch_class(precision, k, kind, 0, c->dt, b, b->cls[k].nwork[kind],
+          if (launch_class(precision, k, kind, 0, 1, c->dt, b, b->cls[k].nwork[kind],
                            std::min(b->cls[k].nwork[kind], 16 * ncu), st) != hipSuccess)
             return MGCM_RT_ERR_DEVICE;
         }
     for (int k = 0; k < NCLASS; k++) {
       ClassPlan &p = b->cls[k];
-      if (p.nwork[0] > 0) tune_class(precision, k, c->dt, b, ncu, st);
+      if (p.nwork[0] > 0) tune_class(precision, k, c->dt, b, ncu, c->tile, st);
       for (int kind = 1; kind < NKIND; kind++) p.nblocks[kind] = std::min(p.nwork[kind], 16 * ncu);
     }
     // capture the class launches; the argument upload stays outside the graph
@@ -299,12 +298,12 @@ int run_batched_impl(mgcm_rt_ctx *c, int ngrids, const mgcm_rt_columns *grids, c
     for (int kind = 1; kind < NKIND && ok; kind++)
       for (int k = 0; k < NCLASS && ok; k++)
         if ((bands & kind) && b->cls[k].nwork[kind] > 0)
-          ok = launch_class(precision, k, kind, 0, c->dt, b, b->cls[k].nwork[kind], b->cls[k].nblocks[kind],
+          ok = launch_class(precision, k, kind, 0, 1, c->dt, b, b->cls[k].nwork[kind], b->cls[k].nblocks[kind],
                             b->cap) == hipSuccess;
     for (int k = 0; k < NCLASS && ok; k++)
       if (b->cls[k].nwork[0] > 0)
-        ok = launch_class(precision, k, 0, b->cls[k].waves, c->dt, b, b->cls[k].nwork[0], b->cls[k].nblocks[0],
-                          b->cap) == hipSuccess;
+        ok = launch_class(precision, k, 0, b->cls[k].waves, b->cls[k].tile, c->dt, b, b->cls[k].nwork[0],
+                          b->cls[k].nblocks[0], b->cap) == hipSuccess;
     // surface fluxes for TEMPGR of the grids that ask for them (code/comp3.f:336-348)
     if (ok && b->dsol_nwork > 0)
       ok = launch_dsolflux_batch(c->d_const, b->d_grids, b->d_work + b->dsol_off, b->dsol_nwork,
@@ -337,10 +336,12 @@ int batched_tuning_impl(const mgcm_rt_ctx *c, int maxcls, mgcm_rt_batch_class *i
       mgcm_rt_batch_class &o = info[n];
       o.max_layers = KCLASS_NL[k] - 1;
       o.items = p.nwork[0];
+      o.tile = p.tile;
       o.waves_per_simd = p.waves;
       o.blocks = p.nblocks[0];
       o.tuned_items = p.tuned_prefix;
       for (int cnd = 0; cnd < NCAND; cnd++) {
+        o.cand_tile[cnd] = CAND_TILE[cnd];
         o.cand_waves[cnd] = CAND_WAVES[cnd];
         o.cand_mult[cnd] = CAND_MULT[cnd];
         o.cand_ms[cnd] = p.cand_ms[cnd];

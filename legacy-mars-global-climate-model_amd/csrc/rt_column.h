@@ -2,14 +2,20 @@
 // one launch per grid, instantiated for the plain FP64 path (rt_plain.hip) and
 // for the mixed-precision path (rt_mixed.hip).
 //
-// One wave (64 lanes) owns 4 columns for the whole radiation call:
-//   1. stages the 4 columns' TPINDEX results (band independent) in LDS;
-//   2. for each of the 7 VIS and 5 IR bands: stages the band's continuum and
-//      layer optics, then lane (c, g) solves the (column c, band, Gauss point
-//      g = 1..16) item -- OPTCV/OPTCI layer assembly, delta-Eddington scaling,
-//      GFLUXV/GFLUXI with DSOLVER/DTRIDGL -- and the 16 lanes of a column sum
-//      their weighted layer fluxes in a fixed order into per-column
-//      accumulators held in registers (lane g keeps the levels L == g+1 mod 16);
+// One wave (64 lanes, four 16-lane rows) owns C = 4/S columns for the whole
+// radiation call, S in {1, 2, 4} band slots per column (the tile: C columns x
+// S bands x 16 Gauss points; S = 1 for large grids, more slots give small
+// grids more waves):
+//   1. stages the C columns' TPINDEX results (band independent) in LDS;
+//   2. in rounds of S bands (the 7 VIS bands first, then the 5 IR bands; a
+//      round is all VIS or all IR): row (c, s) stages band round*S+s of
+//      column c -- continuum and layer optics -- then lane (c, s, g) solves
+//      the (column, band, Gauss point g = 1..16) item -- OPTCV/OPTCI layer
+//      assembly, delta-Eddington scaling, GFLUXV/GFLUXI with DSOLVER/DTRIDGL
+//      -- and the 16 lanes of a row sum their weighted layer fluxes in a fixed
+//      order; the column's slot-0 row adds its S rows' sums in band order to
+//      per-column accumulators held in registers (lane g keeps the levels
+//      L == g+1 mod 16), so every tile gives the same bits;
 //   3. solves the continuum-only Gauss point 17 of every (column, band) -- the
 //      VIS ones in one pass, the IR ones in a second -- weighted with FZERO
 //      plus the weight of the band's TLIMIT-skipped points (code/sfluxv.f90:
@@ -26,14 +32,15 @@
 
 namespace mgcm {
 
-// LDS staging of a wave's 4 columns.
+// LDS staging of a wave: [4] is per column (at most 4) for the TPINDEX
+// results and skip masks, per 16-lane row (the row's band) for the band optics.
 struct Stage {
-  double *t, *u, *wr, *uk;     // [4][nsub]  TPINDEX weights T (log p) and U (T), WRATIO, Cmk*dp
-  double *c1, *c2;             // [4][nsub]  VIS: TRAY+TAEROS+TCLOUD; IR: TAEROS, TCLOUD
-  double *scat, *cosb, *cbar;  // [4][nl]    layer scattering tau, asymmetry, COSBAR = g/(1+g)
-  double *bk;                  // [4][nlev]  IR: PLANCKIR(nw, int(TLEV(K)*10-499))
-  int *base;                   // [4][nsub]  table cell offset; bit 0 set when NH2O = L_REFH2O
-  uint32_t *skip;              // [4][NB]    TLIMIT skip masks
+  double *t, *u, *wr, *uk;     // [4][nsub]  column: TPINDEX weights T (log p) and U (T), WRATIO, Cmk*dp
+  double *c1, *c2;             // [4][nsub]  row: VIS: TRAY+TAEROS+TCLOUD; IR: TAEROS, TCLOUD
+  double *scat, *cosb, *cbar;  // [4][nl]    row: layer scattering tau, asymmetry, COSBAR = g/(1+g)
+  double *bk;                  // [4][nlev]  row: IR: PLANCKIR(nw, int(TLEV(K)*10-499))
+  int *base;                   // [4][nsub]  column: table cell offset; bit 0 set when NH2O = L_REFH2O
+  uint32_t *skip;              // [4][NB]    column: TLIMIT skip masks
 };
 
 // Dynamic LDS of one wave: the exp table, then the staging.
@@ -112,12 +119,12 @@ template <bool VIS, int P = 0>
 struct GasSrc {
   using kt = typename std::conditional<(P & P_TAB32) != 0, float, double>::type;  // table storage type
   const kt *__restrict__ tab;      // band's table
-  const double *t_, *u_, *wr_, *uk_, *c1_, *c2_;  // this column's staged sub-levels (LDS)
+  const double *t_, *u_, *wr_, *uk_, *c1_, *c2_;  // this column's (row's) staged sub-levels (LDS)
   const int *base_;
   double taugsurf;
-  __device__ GasSrc(const kt *tab_, const Stage &S, int c, int nsub, int g0)
+  __device__ GasSrc(const kt *tab_, const Stage &S, int c, int row, int nsub, int g0)
       : tab(tab_ + g0), t_(S.t + c * nsub - 2), u_(S.u + c * nsub - 2), wr_(S.wr + c * nsub - 2),
-        uk_(S.uk + c * nsub - 2), c1_(S.c1 + c * nsub - 2), c2_(S.c2 + c * nsub - 2),
+        uk_(S.uk + c * nsub - 2), c1_(S.c1 + row * nsub - 2), c2_(S.c2 + row * nsub - 2),
         base_(S.base + c * nsub - 2), taugsurf(0.0) {}
   // the 8 corners (p0,T0),(p1,T0),(p1,T1),(p0,T1) x (NH2O, NH2O+1) of sub-level K;
   // issued a layer ahead of their use (fetch), combined in K order (finish)
@@ -267,6 +274,37 @@ __device__ __forceinline__ double acc_get(const double *arr, int n) {
   return r;
 }
 
+// The row sums of a column's S band slots (rows 16 apart, bands in increasing
+// order) added one after the other, as band after band: on the column's slot-0
+// row the result is ((acc + v_0) + v_1) + ...; the other rows' results are
+// not used.  The reads are wave-wide (every lane takes part).
+template <int S>
+__device__ __forceinline__ double tile_sum(double acc, double v) {
+  acc = acc + v;
+#pragma unroll
+  for (int j = 1; j < S; j++) acc = acc + __shfl(v, (int)threadIdx.x + 16 * j);
+  return acc;
+}
+
+template <int S, int SL>
+__device__ __forceinline__ void tile_add(double *arr, int n, int g0, double v) {
+  acc_add<SL>(arr, n, g0, v);
+#pragma unroll
+  for (int j = 1; j < S; j++) acc_add<SL>(arr, n, g0, __shfl(v, (int)threadIdx.x + 16 * j));
+}
+
+// Band rounds of the tile with S slots per column: VIS rounds, then IR rounds.
+template <int S>
+struct Tile {
+  static_assert(S == 1 || S == 2 || S == 4, "band slots per column");
+  static constexpr int C = 4 / S;                        // columns per wave
+  static constexpr int RV = (NV + S - 1) / S, RI = (NI + S - 1) / S;
+  // band of slot s in round k, or -1 past the last band of the part
+  __host__ __device__ static constexpr int band(int k, int s) {
+    return k < RV ? (k * S + s < NV ? k * S + s : -1) : ((k - RV) * S + s < NI ? NV + (k - RV) * S + s : -1);
+  }
+};
+
 #ifndef MGCM_FP64_WAVES
 #define MGCM_FP64_WAVES 2
 #endif
@@ -274,23 +312,29 @@ __device__ __forceinline__ double acc_get(const double *arr, int n) {
 // P: the precision policy (P_* bits; 0 = plain FP64, P_TAB32 = the mixed path,
 // whose k-tables are read in FP32 -- half the gather bytes and corner
 // registers -- with all arithmetic, the interpolation included, in FP64).
-// The whole radiation call of columns col0..col0+3 of grid a, by one wave.
+// S: band slots per column (the tile); the block owns Tile<S>::C columns.
+// The whole radiation call of columns col0..col0+C-1 of grid a, by one wave.
 // The exp table must be in LDS; the staging after it is rewritten.
-template <int NLM, int P>
+template <int NLM, int P, int S>
 __device__ __forceinline__ void column_block(const DevTables &dt, const RunArgs &a, int col0, double *smem) {
   constexpr bool T32 = (P & P_TAB32) != 0;
   using kt = typename std::conditional<T32, float, double>::type;
+  using TL = Tile<S>;
+  constexpr int C = TL::C;
   constexpr int SL = (NLM + 15) / 16;
+  if (col0 >= a.ncol) return;  // a batched sub-item past the grid's last column (block-uniform)
   const int nl = a.nlay + 1, nlev = 2 * a.nlay + 3, nsub = nlev - 1;
-  const int lane = threadIdx.x, c = lane >> 4, g0 = lane & 15;
+  const int lane = threadIdx.x, row = lane >> 4, g0 = lane & 15;
+  const int c = row / S, slot = row % S;  // this row's column in the block and band slot
+  const bool lead = slot == 0;            // the row that holds the column's sums
   const int col = col0 + c;
   const bool valid = col < a.ncol;
   const int colc = valid ? col : a.ncol - 1;
   const RtConst *rc = dt.c;
-  Stage S = carve(smem, nsub, nl, nlev);
+  Stage St = carve(smem, nsub, nl, nlev);
 
-  // 1. TPINDEX of every sub-level K = 2..L_LEVELS of the 4 columns
-  for (int t = lane; t < 4 * nsub; t += 64) {
+  // 1. TPINDEX of every sub-level K = 2..L_LEVELS of the C columns
+  for (int t = lane; t < C * nsub; t += 64) {
     const int cc = t / nsub, K = t % nsub + 2;
     const int cl = col0 + cc;
     if (cl >= a.ncol) continue;
@@ -299,11 +343,11 @@ __device__ __forceinline__ void column_block(const DevTables &dt, const RunArgs 
     double tt, uu, wr;
     const double pm = a.pmid[o], tm = a.tmid[o], qw = a.qh2o[o], p0 = a.plev[o - 1], p1 = a.plev[o];
     tp_stage(rc, pm, tm, qw, tt, uu, mt, mp, nh, wr);
-    S.t[t] = tt;
-    S.u[t] = uu;
-    S.wr[t] = wr;
-    S.uk[t] = CMK * (p1 - p0);
-    S.base[t] = kcell(0, mt, mp, nh) | (nh == NH ? 1 : 0);
+    St.t[t] = tt;
+    St.u[t] = uu;
+    St.wr[t] = wr;
+    St.uk[t] = CMK * (p1 - p0);
+    St.base[t] = kcell(0, mt, mp, nh) | (nh == NH ? 1 : 0);
     // a NaN among the column's inputs sets the range flag: the values this
     // sub-level reads, plus (once per column) TAUREF/TAUREFCLD at levels 1 and
     // L_LEVELS+1 and the column scalars (TLEV reaches the flag through the
@@ -326,109 +370,123 @@ __device__ __forceinline__ void column_block(const DevTables &dt, const RunArgs 
   const double u0 = a.acosz[colc];
   const bool sun = u0 >= F32LIT(1.0e-5);  // code/comp3.f:637
 
-  // 2. k-distribution points, band by band
-  for (int b = 0; b < NB; b++) {
-    const bool VISB = b < NV;
-    const double fz = rc->fzero[b];
+  // band of row rr in round k, or -1 where the row has none or the band is
+  // not solved: FZERO >= 0.99 (code/sfluxv.f90:64-65, code/sfluxi.f90:67-68)
+  // or a part the caller skipped (it contributes nothing)
+  auto row_band = [&](int k, int rr) {
+    const int b = TL::band(k, rr % S);
+    if (b < 0) return -1;
+    if (rc->fzero[b] >= F32LIT(0.99) || !(a.bands & (b < NV ? 1 : 2))) return -1;
+    return b;
+  };
+
+  // 2. k-distribution points, S bands per column per round
+  for (int k = 0; k < TL::RV + TL::RI; k++) {
+    const bool VISR = k < TL::RV;
+    const int bnom = TL::band(k, slot);  // this row's band slot (-1: none)
+    const int bact = row_band(k, row);   // ... if solved
+    const int b = bact >= 0 ? bact : (VISR ? 0 : NV);  // a band index to read constants with
     __syncthreads();
-    // code/sfluxv.f90:64-65, code/sfluxi.f90:67-68; a part the caller skipped contributes nothing
-    if (fz >= F32LIT(0.99) || !(a.bands & (VISB ? 1 : 2))) {
-      if (g0 == 0) S.skip[c * NB + b] = 0u;
+    if (!__any(bact >= 0)) {  // no row solves a band this round
+      if (g0 == 0 && bnom >= 0) St.skip[c * NB + bnom] = 0u;
       continue;
     }
-    for (int t = lane; t < 4 * nsub; t += 64) {  // continuum optics of the band
-      const int cc = t / nsub, K = t % nsub + 2;
-      const int cl = col0 + cc;
-      if (cl >= a.ncol) continue;
+    for (int t = lane; t < 4 * nsub; t += 64) {  // continuum optics of each row's band
+      const int rr = t / nsub, K = t % nsub + 2;
+      const int cl = col0 + rr / S, bb = row_band(k, rr);
+      if (cl >= a.ncol || bb < 0) continue;
       const double *pl = a.plev + (size_t)cl * nlev;
       const double *tr = a.tauref + (size_t)cl * (nlev + 1);
       const double *tc = a.taurefcld + (size_t)cl * (nlev + 1);
       const Aer ae = column_aerosol(a, rc, cl);
-      if (VISB) {
-        const SubAer s = sub_aerosol<true>(ae, b, pl, tr, tc, K);
-        S.c1[t] = s.tray + s.taeros + s.tcloud;
+      if (VISR) {
+        const SubAer s = sub_aerosol<true>(ae, bb, pl, tr, tc, K);
+        St.c1[t] = s.tray + s.taeros + s.tcloud;
       } else {
-        const SubAer s = sub_aerosol<false>(ae, b, pl, tr, tc, K);
-        S.c1[t] = s.taeros;
-        S.c2[t] = s.tcloud;
+        const SubAer s = sub_aerosol<false>(ae, bb, pl, tr, tc, K);
+        St.c1[t] = s.taeros;
+        St.c2[t] = s.tcloud;
       }
     }
     for (int t = lane; t < 4 * nl; t += 64) {
-      const int cc = t / nl, L = t % nl + 1;
-      const int cl = col0 + cc;
-      if (cl >= a.ncol) continue;
+      const int rr = t / nl, L = t % nl + 1;
+      const int cl = col0 + rr / S, bb = row_band(k, rr);
+      if (cl >= a.ncol || bb < 0) continue;
       const double *pl = a.plev + (size_t)cl * nlev;
       const double *tr = a.tauref + (size_t)cl * (nlev + 1);
       const double *tc = a.taurefcld + (size_t)cl * (nlev + 1);
       const Aer ae = column_aerosol(a, rc, cl);
       double sc, cb;
-      if (VISB) layer_aerosol<true>(ae, b, pl, tr, tc, nl, L, sc, cb);
-      else layer_aerosol<false>(ae, b, pl, tr, tc, nl, L, sc, cb);
-      S.scat[t] = sc;
-      S.cosb[t] = cb;
-      S.cbar[t] = cb / (1.0 + cb);  // COSBAR of the delta-Eddington scaling (code/gfluxv.f90:86)
+      if (VISR) layer_aerosol<true>(ae, bb, pl, tr, tc, nl, L, sc, cb);
+      else layer_aerosol<false>(ae, bb, pl, tr, tc, nl, L, sc, cb);
+      St.scat[t] = sc;
+      St.cosb[t] = cb;
+      St.cbar[t] = cb / (1.0 + cb);  // COSBAR of the delta-Eddington scaling (code/gfluxv.f90:86)
     }
-    if (!VISB) {
+    if (!VISR) {
       for (int t = lane; t < 4 * nlev; t += 64) {
-        const int cc = t / nlev, K = t % nlev + 1;
-        const int cl = col0 + cc;
-        if (cl >= a.ncol) continue;
+        const int rr = t / nlev, K = t % nlev + 1;
+        const int cl = col0 + rr / S, bb = row_band(k, rr);
+        if (cl >= a.ncol || bb < 0) continue;
         const int nt = planck_index(a.tlev[(size_t)cl * nlev + K - 1], a.flag);
-        S.bk[t] = dt.planck[(size_t)(nt - 1) * NI + (b - NV)];
+        St.bk[t] = dt.planck[(size_t)(nt - 1) * NI + (bb - NV)];
       }
     }
     __syncthreads();
 
+    const double fz = rc->fzero[b];
     const double gw = rc->gw[b][g0];
     const double onemf = 1.0 - fz;
-    const double *scat = S.scat + c * nl, *cosb = S.cosb + c * nl, *cbar = S.cbar + c * nl;
+    const double *scat = St.scat + row * nl, *cosb = St.cosb + row * nl, *cbar = St.cbar + row * nl;
     bool skip;
-    if (VISB) {
+    if (VISR) {
       const int nw = b;
       const size_t toff = (size_t)nw * NP * NH * NT * NGAS;
-      GasSrc<true, P> src(T32 ? (const kt *)(dt.kv32 + toff) : (const kt *)(dt.kv + toff), S, c, nsub, g0);
+      GasSrc<true, P> src(T32 ? (const kt *)(dt.kv32 + toff) : (const kt *)(dt.kv + toff), St, c, row, nsub, g0);
       VisState<NLM> s;
       double cumtau = 0.0, detau = 0.0;
-      if (valid && sun) detau = vis_optics<NLM>(nl, src, StagedLayerAer{scat, cosb}, s, cumtau);
-      skip = !(valid && sun) || below_tlimit<P>(src, dt.kv + toff + g0, nlev, TLIMITS, valid && sun);
-      if (valid && sun) {  // GETDETAU output and TAUCUMV(L_LEVELS) (code/comp3.f:664-668)
+      const bool go = valid && sun && bact >= 0;
+      if (go) detau = vis_optics<NLM>(nl, src, StagedLayerAer{scat, cosb}, s, cumtau);
+      skip = !go || below_tlimit<P>(src, dt.kv + toff + g0, nlev, TLIMITS, go);
+      if (go) {  // GETDETAU output and TAUCUMV(L_LEVELS) (code/comp3.f:664-668)
         a.detau[(size_t)col * NG * NV + g0 * NV + nw] = detau;
         a.cumtauv[(size_t)col * NG * NV + g0 * NV + nw] = cumtau;
       }
       // the 16-lane sums need the whole wave: the solve runs wave-uniformly and
-      // skipped lanes (TLIMIT, sun down, past the last column) contribute zeros
+      // skipped lanes (TLIMIT, sun down, past the last column, no band) contribute zeros
       if (__any(!skip)) {
         auto accf = [&](int n, double fp, double fm) {
           const double net = skip ? 0.0 : (fp - fm) * gw * onemf;
           const double up = skip ? 0.0 : fp * gw * onemf;
           const double dn = skip ? 0.0 : fm * gw * onemf;
-          acc_add<SL>(acc.vnet, n, g0, group16_sum(net));
-          acc_add<SL>(acc.vup, n, g0, group16_sum(up));
-          acc_add<SL>(acc.vdn, n, g0, group16_sum(dn));
+          tile_add<S, SL>(acc.vnet, n, g0, group16_sum(net));
+          tile_add<S, SL>(acc.vup, n, g0, group16_sum(up));
+          tile_add<S, SL>(acc.vdn, n, g0, group16_sum(dn));
         };
         double fluxup = 0.0, fluxdn = 0.0, diffv = 0.0;
         const double bsurf = a.als[colc] * u0 * a.sol[nw] * rt_exp35(-dmin(detau / u0, MAXEXP));
         vis_solve<NLM, P>(nl, s, cbar, u0, a.sol[nw], a.als[colc], 0.0, bsurf, accf, fluxup, fluxdn, diffv);
-        topv = topv + group16_sum(skip ? 0.0 : (fluxup - fluxdn) * gw * onemf);
-        difv = difv + group16_sum(skip ? 0.0 : diffv * gw * onemf);
+        topv = tile_sum<S>(topv, group16_sum(skip ? 0.0 : (fluxup - fluxdn) * gw * onemf));
+        difv = tile_sum<S>(difv, group16_sum(skip ? 0.0 : diffv * gw * onemf));
       }
     } else {
       const int nw = b - NV;
       const double dw = rc->dwni[nw];
       const size_t toff = (size_t)nw * NP * NH * NT * NGAS;
-      GasSrc<false, P> src(T32 ? (const kt *)(dt.ki32 + toff) : (const kt *)(dt.ki + toff), S, c, nsub, g0);
+      GasSrc<false, P> src(T32 ? (const kt *)(dt.ki32 + toff) : (const kt *)(dt.ki + toff), St, c, row, nsub, g0);
       IrState<NLM> s;
-      if (valid) ir_optics<NLM>(nl, src, StagedLayerAer{scat, cosb}, s);
-      skip = !valid || below_tlimit<P>(src, dt.ki + toff + g0, nlev, TLIMITI, valid);
+      const bool go = valid && bact >= 0;
+      if (go) ir_optics<NLM>(nl, src, StagedLayerAer{scat, cosb}, s);
+      skip = !go || below_tlimit<P>(src, dt.ki + toff + g0, nlev, TLIMITI, go);
       if (__any(!skip)) {
-        const double *bk = S.bk + c * nlev;
+        const double *bk = St.bk + row * nlev;
         auto accf = [&](int n, double fp, double fm) {
           const double net = skip ? 0.0 : (fp - fm) * dw * gw * onemf;
           const double up = skip ? 0.0 : fp * dw * gw * onemf;
           const double dn = skip ? 0.0 : fm * dw * gw * onemf;
-          acc_add<SL>(acc.inet, n, g0, group16_sum(net));
-          acc_add<SL>(acc.iup, n, g0, group16_sum(up));
-          acc_add<SL>(acc.idn, n, g0, group16_sum(dn));
+          tile_add<S, SL>(acc.inet, n, g0, group16_sum(net));
+          tile_add<S, SL>(acc.iup, n, g0, group16_sum(up));
+          tile_add<S, SL>(acc.idn, n, g0, group16_sum(dn));
         };
         // boundary conditions (code/sfluxi.f90:61-62,81-82)
         const double rsf = a.albi[colc];
@@ -436,24 +494,25 @@ __device__ __forceinline__ void column_block(const DevTables &dt, const RunArgs 
         const double tautop = s.wd[1].y * plev[1] / (plev[3] - plev[1]);
         const double btop = (1.0 - rt_exp(-tautop * 2.0)) * bk[1];  // /UBARI, UBARI = 0.5
         const double ftopup = ir_solve<NLM, P>(nl, s, cosb, bk, rsf, btop, bsurf, accf);
-        topi = topi + group16_sum(skip ? 0.0 : ftopup * dw * gw * onemf);
+        topi = tile_sum<S>(topi, group16_sum(skip ? 0.0 : ftopup * dw * gw * onemf));
       }
     }
     const unsigned long long m = __ballot(skip);
-    if (g0 == 0) S.skip[c * NB + b] = (uint32_t)((m >> (16 * c)) & 0xffffu);
+    if (g0 == 0 && bnom >= 0) St.skip[c * NB + bnom] = bact >= 0 ? (uint32_t)((m >> (16 * row)) & 0xffffu) : 0u;
   }
   __syncthreads();
 
   // 3. continuum point g = 17 of every (column, band): solved by k_rt_clear
   //    into a.clr, weighted here with FZERO plus the weights of the band's
   //    TLIMIT-skipped points, in g order, and added band by band
-  //    (code/sfluxv.f90:72-75,121-162, code/sfluxi.f90:72-75,114-143)
-  {
+  //    (code/sfluxv.f90:72-75,121-162, code/sfluxi.f90:72-75,114-143); by the
+  //    column's slot-0 row
+  if (lead) {
     double wz[NB];
     for (int b = 0; b < NB; b++) {
       double fzero = rc->fzero[b];
       if (!(fzero >= F32LIT(0.99))) {
-        const uint32_t mk = S.skip[c * NB + b];
+        const uint32_t mk = St.skip[c * NB + b];
         for (int g = 0; g < NGAS; g++)
           if (mk & (1u << g)) fzero = fzero + (1.0 - rc->fzero[b]) * rc->gw[b][g];
       }
@@ -504,33 +563,35 @@ __device__ __forceinline__ void column_block(const DevTables &dt, const RunArgs 
     }
   }
 
-  // 4. outputs, net-flux differences and heating
+  // 4. outputs, net-flux differences and heating (slot-0 rows)
   __syncthreads();
-  double *fv = S.t;  // reuse the staging: [4][nl] net VIS, [4][nl] net IR
-  double *fi = S.t + 4 * nl;
+  double *fv = St.t;  // reuse the staging: [C][nl] net VIS, [C][nl] net IR
+  double *fi = St.t + 4 * nl;
   if (!sun) { topv = 0.0; difv = 0.0; }
-  for (int n = 1; n <= nl; n++) {
-    if (((n - 1) & 15) == g0) {
-      const double nv = sun ? acc_get<SL>(acc.vnet, n) : 0.0;
-      const double uv = sun ? acc_get<SL>(acc.vup, n) : 0.0;
-      const double dv = sun ? acc_get<SL>(acc.vdn, n) : 0.0;
-      const double ni = acc_get<SL>(acc.inet, n), ui = acc_get<SL>(acc.iup, n), di = acc_get<SL>(acc.idn, n);
-      fv[c * nl + n - 1] = nv;
-      fi[c * nl + n - 1] = ni;
-      if (valid) {
-        const size_t o = (size_t)col * nl + n - 1;
-        a.fmnetv[o] = nv; a.fluxupv[o] = uv; a.fluxdnv[o] = dv;
-        a.fmneti[o] = ni; a.fluxupi[o] = ui; a.fluxdni[o] = di;
+  if (lead) {
+    for (int n = 1; n <= nl; n++) {
+      if (((n - 1) & 15) == g0) {
+        const double nv = sun ? acc_get<SL>(acc.vnet, n) : 0.0;
+        const double uv = sun ? acc_get<SL>(acc.vup, n) : 0.0;
+        const double dv = sun ? acc_get<SL>(acc.vdn, n) : 0.0;
+        const double ni = acc_get<SL>(acc.inet, n), ui = acc_get<SL>(acc.iup, n), di = acc_get<SL>(acc.idn, n);
+        fv[c * nl + n - 1] = nv;
+        fi[c * nl + n - 1] = ni;
+        if (valid) {
+          const size_t o = (size_t)col * nl + n - 1;
+          a.fmnetv[o] = nv; a.fluxupv[o] = uv; a.fluxdnv[o] = dv;
+          a.fmneti[o] = ni; a.fluxupi[o] = ui; a.fluxdni[o] = di;
+        }
       }
     }
-  }
-  if (g0 == 0 && valid) {
-    a.nfluxtopv[col] = topv;
-    a.nfluxtopi[col] = topi;
-    a.diffvt[col] = difv;
+    if (g0 == 0 && valid) {
+      a.nfluxtopv[col] = topv;
+      a.nfluxtopi[col] = topi;
+      a.diffvt[col] = difv;
+    }
   }
   __syncthreads();
-  if (a.hr && valid) {
+  if (a.hr && valid && lead) {
     for (int l = g0; l <= a.nlay; l += 16) {
       const int L = l + 1;  // SUNTOT(2L+1), IRTOTAL(2L+1)
       const double nv = fv[c * nl + L - 1], ni = fi[c * nl + L - 1];
@@ -636,48 +697,49 @@ __global__ __launch_bounds__(64) void k_rt_clear_batch(DevTables dt, const RunAr
   }
 }
 
-// One grid: block b owns columns 4b..4b+3.
-template <int NLM, int P>
+// One grid: block b owns columns Cb..Cb+C-1 (C = 4/S).
+template <int NLM, int P, int S>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MGCM_FP64_WAVES)))
 void k_rt(DevTables dt, RunArgs a) {
   extern __shared__ double smem[];
   mgcm_libm::load_lds_exptab(threadIdx.x, 64);
   __syncthreads();
-  column_block<NLM, P>(dt, a, blockIdx.x * 4, smem);
+  column_block<NLM, P, S>(dt, a, blockIdx.x * Tile<S>::C, smem);
 }
 
 // Many grids of one layer-count class: a persistent grid of blocks walks the
-// work list (grid index, first column) round-robin, largest layer counts first.
+// work list (grid index, first column of 4) round-robin, largest layer counts
+// first; with S band slots per column each item is S sub-items of 4/S columns.
 // No counters: every block's share is fixed by its index.
-template <int NLM, int P>
+template <int NLM, int P, int S>
 __device__ __forceinline__ void batch_items(const DevTables &dt, const RunArgs *__restrict__ grids,
                                             const uint2 *__restrict__ work, int nwork, double *smem) {
   mgcm_libm::load_lds_exptab(threadIdx.x, 64);
-  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+  for (int v = blockIdx.x; v < nwork * S; v += gridDim.x) {
     __syncthreads();  // the previous item's reads of the staging are done
-    const uint2 wk = work[w];
+    const uint2 wk = work[v / S];
     const RunArgs a = grids[wk.x];
-    column_block<NLM, P>(dt, a, (int)wk.y, smem);
+    column_block<NLM, P, S>(dt, a, (int)wk.y + (v % S) * Tile<S>::C, smem);
   }
 }
-// Two instances the batched tuner chooses between per layer class: two waves
-// per SIMD (at most 256 registers per lane) ...
-template <int NLM, int P>
+// The instances the batched tuner chooses between per layer class: two waves
+// per SIMD (at most 256 registers per lane) at every tile ...
+template <int NLM, int P, int S>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MGCM_FP64_WAVES)))
 void k_rt_batch(DevTables dt, const RunArgs *__restrict__ grids, const uint2 *__restrict__ work, int nwork) {
   extern __shared__ double smem[];
-  batch_items<NLM, P>(dt, grids, work, nwork, smem);
+  batch_items<NLM, P, S>(dt, grids, work, nwork, smem);
 }
 // ... and one wave per SIMD (512 registers per lane: fewer spills of the
-// item state, half the waves in flight).
+// item state, half the waves in flight), one band slot per column.
 template <int NLM, int P>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
 void k_rt_batch_w1(DevTables dt, const RunArgs *__restrict__ grids, const uint2 *__restrict__ work, int nwork) {
   extern __shared__ double smem[];
-  batch_items<NLM, P>(dt, grids, work, nwork, smem);
+  batch_items<NLM, P, 1>(dt, grids, work, nwork, smem);
 }
 
-template <int NLM, int P>
+template <int NLM, int P, int S>
 static hipError_t launch_nlm(const DevTables &dt, const RunArgs &a, hipStream_t st) {
   // continuum points first (their records feed the column kernel's sums)
   const int nvis = (a.ncol + 8) / 9, nir = (a.ncol + 11) / 12;
@@ -688,23 +750,56 @@ static hipError_t launch_nlm(const DevTables &dt, const RunArgs &a, hipStream_t 
   else if (a.bands & 2)
     hipLaunchKernelGGL((k_rt_clear<NLM, false>), dim3(nir), dim3(64), mgcm_libm::LDS_EXPTAB_BYTES, st, dt, a);
   const size_t lds = stage_bytes(a.nlay);
-  const unsigned gx = (unsigned)((a.ncol + 3) / 4);
-  hipLaunchKernelGGL((k_rt<NLM, P>), dim3(gx), dim3(64), lds, st, dt, a);
+  constexpr int C = Tile<S>::C;
+  const unsigned gx = (unsigned)((a.ncol + C - 1) / C);
+  hipLaunchKernelGGL((k_rt<NLM, P, S>), dim3(gx), dim3(64), lds, st, dt, a);
   return hipGetLastError();
 }
 
-// One grid: the continuum-point kernels, then the column kernel (4 columns per
-// wave), each instance sized for the grid's layer count.  a.clr must hold
+// One grid: the continuum-point kernels, then the column kernel (4/S columns
+// per wave), each instance sized for the grid's layer count.  a.clr must hold
 // clr_bytes(a.ncol, a.nlay).
-template <int P>
+template <int P, int S>
 hipError_t launch_column(const DevTables &dt, const RunArgs &a, hipStream_t st) {
   const int nl = a.nlay + 1;
-  if (nl <= 17) return launch_nlm<17, P>(dt, a, st);
-  if (nl <= 25) return launch_nlm<25, P>(dt, a, st);
-  if (nl <= 37) return launch_nlm<37, P>(dt, a, st);
-  if (nl <= 49) return launch_nlm<49, P>(dt, a, st);
-  if (nl <= 65) return launch_nlm<65, P>(dt, a, st);
-  return launch_nlm<MAX_NLAY + 1, P>(dt, a, st);
+  if (nl <= 17) return launch_nlm<17, P, S>(dt, a, st);
+  if (nl <= 25) return launch_nlm<25, P, S>(dt, a, st);
+  if (nl <= 37) return launch_nlm<37, P, S>(dt, a, st);
+  if (nl <= 49) return launch_nlm<49, P, S>(dt, a, st);
+  if (nl <= 65) return launch_nlm<65, P, S>(dt, a, st);
+  return launch_nlm<MAX_NLAY + 1, P, S>(dt, a, st);
+}
+
+// Resident one-wave blocks per CU for `lds` bytes of LDS at `waves` waves per
+// SIMD (4 SIMDs per CU, 160 KiB of LDS).
+inline int resident_per_cu(size_t lds, int waves) {
+  const int by_lds = (int)((160 * 1024) / (lds > 0 ? lds : 1));
+  const int by_waves = 4 * waves;
+  return by_lds < 1 ? 1 : (by_lds < by_waves ? by_lds : by_waves);
+}
+
+// Band slots per column of a single-grid call when the caller leaves the
+// choice to the library: the tile with the least (rounds of resident waves) x
+// (latency of one wave), S = 1 on ties.  A wave's latency at S = 1, 2, 4 is
+// 1 : 0.75 : 0.53 (measured on the MI355X with one resident round: 540
+// columns of 24 layers take 0.715, 0.535 and 0.376 ms); the band rounds alone
+// (12, 7, 4) would say 1 : 0.58 : 0.33 -- the staging, the continuum-point
+// sums and the empty band slots do not shrink with S.  Large grids fill the
+// device at S = 1, the least work per column.
+inline int auto_tile(int ncol, int nlay, int ncu) {
+  const long long slots = (long long)ncu * resident_per_cu(stage_bytes(nlay), MGCM_FP64_WAVES);
+  constexpr int tiles[3] = {1, 2, 4}, latency[3] = {100, 75, 53};
+  int best = 1;
+  long long best_cost = -1;
+  for (int i = 0; i < 3; i++) {
+    const long long waves = ((long long)ncol * tiles[i] + 3) / 4;
+    const long long cost = ((waves + slots - 1) / slots) * latency[i];
+    if (best_cost < 0 || cost < best_cost) {
+      best = tiles[i];
+      best_cost = cost;
+    }
+  }
+  return best;
 }
 
 // Layer-count classes of the batched path: kernel instance NLM = KCLASS_NL[k]
@@ -717,33 +812,39 @@ inline int nl_class(int nl) {
   return NCLASS - 1;
 }
 
-// kind 0: column kernel (waves per SIMD 1 or 2); 1: VIS continuum points; 2: IR continuum points
-template <int NLM, int P>
+// kind 0: column kernel (waves per SIMD 1 or 2; one wave per SIMD at S = 1
+// only); 1: VIS continuum points; 2: IR continuum points (S = 1 instances)
+template <int NLM, int P, int S>
 static hipError_t launch_batch_nlm(int kind, int waves, const DevTables &dt, const RunArgs *grids, const uint2 *work,
                                    int nwork, int nblocks, size_t lds, hipStream_t st) {
-  if (kind == 1)
-    hipLaunchKernelGGL((k_rt_clear_batch<NLM, true>), dim3(nblocks), dim3(64), mgcm_libm::LDS_EXPTAB_BYTES, st, dt,
-                       grids, work, nwork);
-  else if (kind == 2)
-    hipLaunchKernelGGL((k_rt_clear_batch<NLM, false>), dim3(nblocks), dim3(64), mgcm_libm::LDS_EXPTAB_BYTES, st, dt,
-                       grids, work, nwork);
-  else if (waves == 1)
-    hipLaunchKernelGGL((k_rt_batch_w1<NLM, P>), dim3(nblocks), dim3(64), lds, st, dt, grids, work, nwork);
-  else
-    hipLaunchKernelGGL((k_rt_batch<NLM, P>), dim3(nblocks), dim3(64), lds, st, dt, grids, work, nwork);
+  if constexpr (S == 1) {
+    if (kind == 1)
+      hipLaunchKernelGGL((k_rt_clear_batch<NLM, true>), dim3(nblocks), dim3(64), mgcm_libm::LDS_EXPTAB_BYTES, st, dt,
+                         grids, work, nwork);
+    else if (kind == 2)
+      hipLaunchKernelGGL((k_rt_clear_batch<NLM, false>), dim3(nblocks), dim3(64), mgcm_libm::LDS_EXPTAB_BYTES, st,
+                         dt, grids, work, nwork);
+    else if (waves == 1)
+      hipLaunchKernelGGL((k_rt_batch_w1<NLM, P>), dim3(nblocks), dim3(64), lds, st, dt, grids, work, nwork);
+    else
+      hipLaunchKernelGGL((k_rt_batch<NLM, P, 1>), dim3(nblocks), dim3(64), lds, st, dt, grids, work, nwork);
+  } else {
+    if (kind != 0 || waves != MGCM_FP64_WAVES) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_rt_batch<NLM, P, S>), dim3(nblocks), dim3(64), lds, st, dt, grids, work, nwork);
+  }
   return hipGetLastError();
 }
 
-template <int P>
+template <int P, int S>
 hipError_t launch_batch_class(int cls, int kind, int waves, const DevTables &dt, const RunArgs *grids,
                               const uint2 *work, int nwork, int nblocks, size_t lds, hipStream_t st) {
   switch (cls) {
-    case 0: return launch_batch_nlm<17, P>(kind, waves, dt, grids, work, nwork, nblocks, lds, st);
-    case 1: return launch_batch_nlm<25, P>(kind, waves, dt, grids, work, nwork, nblocks, lds, st);
-    case 2: return launch_batch_nlm<37, P>(kind, waves, dt, grids, work, nwork, nblocks, lds, st);
-    case 3: return launch_batch_nlm<49, P>(kind, waves, dt, grids, work, nwork, nblocks, lds, st);
-    case 4: return launch_batch_nlm<65, P>(kind, waves, dt, grids, work, nwork, nblocks, lds, st);
-    default: return launch_batch_nlm<MAX_NLAY + 1, P>(kind, waves, dt, grids, work, nwork, nblocks, lds, st);
+    case 0: return launch_batch_nlm<17, P, S>(kind, waves, dt, grids, work, nwork, nblocks, lds, st);
+    case 1: return launch_batch_nlm<25, P, S>(kind, waves, dt, grids, work, nwork, nblocks, lds, st);
+    case 2: return launch_batch_nlm<37, P, S>(kind, waves, dt, grids, work, nwork, nblocks, lds, st);
+    case 3: return launch_batch_nlm<49, P, S>(kind, waves, dt, grids, work, nwork, nblocks, lds, st);
+    case 4: return launch_batch_nlm<65, P, S>(kind, waves, dt, grids, work, nwork, nblocks, lds, st);
+    default: return launch_batch_nlm<MAX_NLAY + 1, P, S>(kind, waves, dt, grids, work, nwork, nblocks, lds, st);
   }
 }
 

@@ -32,19 +32,32 @@ struct mgcm_rt_ctx {
   // device address of the no-allocation entry point's scratch array on this
   // context's device (rt_noalloc.hip; not owned: it lives until the process ends)
   double *d_scratch;
+  // column kernel tile of single-grid calls: band slots per column (1, 2, 4;
+  // 0 = chosen per grid by auto_tile) and the device's CU count
+  int tile;
+  int ncu;
 };
 
 namespace mgcm {
-hipError_t launch_plain(const DevTables &dt, const RunArgs &a, hipStream_t st);
-hipError_t launch_mixed(const DevTables &dt, const RunArgs &a, hipStream_t st);
+// tile: band slots per column of the column kernel (1, 2 or 4)
+hipError_t launch_plain(const DevTables &dt, const RunArgs &a, int tile, hipStream_t st);
+hipError_t launch_mixed(const DevTables &dt, const RunArgs &a, int tile, hipStream_t st);
+hipError_t launch_plain_tile(const DevTables &dt, const RunArgs &a, int tile, hipStream_t st);  // tile 2, 4
+hipError_t launch_mixed_tile(const DevTables &dt, const RunArgs &a, int tile, hipStream_t st);
 // The precision study's column kernels (rt_study.hip): policy = P_* bits;
 // hipErrorInvalidValue outside the 25..36-layer class or for a policy not built.
 hipError_t launch_study(int policy, const DevTables &dt, const RunArgs &a, hipStream_t st);
-// waves: the column kernel instance's waves per SIMD (1 or 2); kind 0 only
-hipError_t launch_batch_plain(int cls, int kind, int waves, const DevTables &dt, const RunArgs *grids,
+// waves, tile: the column kernel instance's waves per SIMD (1 or 2) and band
+// slots per column (1, 2 or 4; one wave per SIMD at tile 1 only); kind 0 only.
+// nwork counts 4-column items.
+hipError_t launch_batch_plain(int cls, int kind, int waves, int tile, const DevTables &dt, const RunArgs *grids,
                               const uint2 *work, int nwork, int nblocks, size_t lds, hipStream_t st);
-hipError_t launch_batch_mixed(int cls, int kind, int waves, const DevTables &dt, const RunArgs *grids,
+hipError_t launch_batch_mixed(int cls, int kind, int waves, int tile, const DevTables &dt, const RunArgs *grids,
                               const uint2 *work, int nwork, int nblocks, size_t lds, hipStream_t st);
+hipError_t launch_batch_plain_tile(int cls, int kind, int waves, int tile, const DevTables &dt, const RunArgs *grids,
+                                   const uint2 *work, int nwork, int nblocks, size_t lds, hipStream_t st);
+hipError_t launch_batch_mixed_tile(int cls, int kind, int waves, int tile, const DevTables &dt, const RunArgs *grids,
+                                   const uint2 *work, int nwork, int nblocks, size_t lds, hipStream_t st);
 size_t clr_workspace_bytes(int ncol, int nlay);
 // Splits the precision argument into the path (MGCM_RT_FP64/MIXED, or
 // MGCM_RT_STUDY with its policy bits in path >> 8) and the band mask;

@@ -17,12 +17,12 @@ hipError_t launch_study(int policy, const DevTables &dt, const RunArgs &a, hipSt
   const int nl = a.nlay + 1;
   if (nl <= 25 || nl > 37) return hipErrorInvalidValue;
   switch (policy) {
-    case S_INTERP: return launch_nlm<37, S_INTERP>(dt, a, st);
-    case P_TAU32: return launch_nlm<37, P_TAU32>(dt, a, st);
-    case P_TRANS32: return launch_nlm<37, P_TRANS32>(dt, a, st);
-    case S_ALL: return launch_nlm<37, S_ALL>(dt, a, st);
-    case P_FASTDIV: return launch_nlm<37, P_FASTDIV>(dt, a, st);
-    case S_FAST_T32: return launch_nlm<37, S_FAST_T32>(dt, a, st);
+    case S_INTERP: return launch_nlm<37, S_INTERP, 1>(dt, a, st);
+    case P_TAU32: return launch_nlm<37, P_TAU32, 1>(dt, a, st);
+    case P_TRANS32: return launch_nlm<37, P_TRANS32, 1>(dt, a, st);
+    case S_ALL: return launch_nlm<37, S_ALL, 1>(dt, a, st);
+    case P_FASTDIV: return launch_nlm<37, P_FASTDIV, 1>(dt, a, st);
+    case S_FAST_T32: return launch_nlm<37, S_FAST_T32, 1>(dt, a, st);
     default: return hipErrorInvalidValue;
   }
 }

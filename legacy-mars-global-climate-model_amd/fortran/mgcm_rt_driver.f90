@@ -4,8 +4,11 @@
 ! results to a second stream file.
 !
 !   mgcm_rt_driver [--per-column] [--aerosol <tracer file> <aerosol data dir>]
-!                  [--dust <dust file>]
+!                  [--dust <dust file>] [--tile <band slots>]
 !                  <case file> <result file> [data directory | -] [history file]
+!
+! --tile sets the column kernel's tile (mgcm_rt_set_tile: 1, 2 or 4 band
+! slots per column, 0 = the library's choice, the default).
 !
 ! --dust replaces the case file's TAUREF by the prescribed (Conrath) dust of
 ! DUSTPROFILE + FILLTAUCUM (mgcm_rt_dust_profile).  Dust file (FP64 stream):
@@ -70,6 +73,8 @@ program mgcm_rt_driver
   real(4), allocatable :: fuptopir(:,:), fupsurfir(:,:), fdnsurfir(:,:)
   integer :: n, a0, nargs
   logical :: percol, aeron, duston
+  integer(c_int) :: tile
+  character(len=16) :: targ
   character(len=4096) :: ftracer, aerdir, fdust
   real(c_double) :: psf, ptrop, rptau
   real(c_double), allocatable :: tautot(:), conr(:), psurf(:), sigma(:)
@@ -85,6 +90,7 @@ program mgcm_rt_driver
   percol = .false.
   aeron = .false.
   duston = .false.
+  tile = 0
   do while (a0 < nargs)
     call get_command_argument(a0 + 1, fin)
     if (trim(fin) == '--per-column') then
@@ -99,13 +105,18 @@ program mgcm_rt_driver
       duston = .true.
       call get_command_argument(a0 + 2, fdust)
       a0 = a0 + 2
+    else if (trim(fin) == '--tile' .and. a0 + 2 <= nargs) then
+      call get_command_argument(a0 + 2, targ)
+      read(targ, *) tile
+      a0 = a0 + 2
     else
       exit
     end if
   end do
   if (nargs - a0 < 2 .or. nargs - a0 > 4) then
     write(*, '(a)') 'usage: mgcm_rt_driver [--per-column] [--aerosol <tracer file> <aerosol data dir>] '// &
-                    '[--dust <dust file>] <case file> <result file> [data directory | -] [history file]'
+                    '[--dust <dust file>] [--tile <band slots>] <case file> <result file> '// &
+                    '[data directory | -] [history file]'
     stop 2
   end if
   call get_command_argument(a0 + 1, fin)
@@ -159,6 +170,7 @@ program mgcm_rt_driver
     call mgcm_rt_setup(co2v, co2i, pfgasref, tgasref, wrefh2o, gwv, gwi, fzerov, fzeroi, tauray, dwni, &
                        planckir, qxvdst, qsvdst, gvdst, qxidst, qsidst, gidst, qxvcld, qsvcld, gvcld, &
                        qxicld, qsicld, gicld, qextrefdst, qextrefcld, ctx, status)
+  if (status == MGCM_RT_OK) status = mgcm_rt_set_tile(ctx, tile)
   if (status == MGCM_RT_OK .and. aeron) then
     ! COMP3's active dust and clouds: OPT_DST, OPT_CLD, then their outputs into the radiation
     allocate(qtrace(nlay, 6, ncol))

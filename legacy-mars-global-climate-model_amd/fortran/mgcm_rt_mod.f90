@@ -62,6 +62,7 @@ module mgcm_rt_mod
 
   public :: mgcm_rt_init, mgcm_rt_destroy, mgcm_rt_run, mgcm_rt_run_host, mgcm_rt_check, mgcm_rt_history_fields
   public :: mgcm_rt_run_batched, mgcm_rt_planck_table, mgcm_rt_laginterp, mgcm_rt_read_ktable, mgcm_rt_strerror
+  public :: mgcm_rt_set_tile
   ! mirrors mgcm_rt_tracers: QTRACE(nlay, 6) and PL(L_LEVELS) of each column
   type, bind(C), public :: mgcm_rt_tracers
     integer(c_int) :: ncol, nlay
@@ -117,6 +118,12 @@ module mgcm_rt_mod
     integer(c_int) function mgcm_rt_check(ctx) bind(C, name="mgcm_rt_check")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
+    end function
+    ! column kernel tile of single-grid calls: 1, 2 or 4 band slots per column, 0 = automatic
+    integer(c_int) function mgcm_rt_set_tile(ctx, band_slots) bind(C, name="mgcm_rt_set_tile")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: band_slots
     end function
     integer(c_int) function mgcm_rt_run_batched(ctx, ngrids, grids, outs, precision, stream) &
         bind(C, name="mgcm_rt_run_batched")

@@ -99,10 +99,13 @@ class Fluxes(C.Structure):
         "nfluxtopv", "nfluxtopi", "diffvt", "detau", "cumtauv", "hr", "directsol", "dnvflux")]
 
 
+NCAND = 10  # MGCM_RT_NCAND
+
+
 class BatchClass(C.Structure):
-    _fields_ = [("max_layers", C.c_int), ("items", C.c_int), ("waves_per_simd", C.c_int), ("blocks", C.c_int),
-                ("tuned_items", C.c_int), ("cand_waves", C.c_int * 6), ("cand_mult", C.c_int * 6),
-                ("cand_ms", C.c_float * 6)]
+    _fields_ = [("max_layers", C.c_int), ("items", C.c_int), ("tile", C.c_int), ("waves_per_simd", C.c_int),
+                ("blocks", C.c_int), ("tuned_items", C.c_int), ("cand_tile", C.c_int * NCAND),
+                ("cand_waves", C.c_int * NCAND), ("cand_mult", C.c_int * NCAND), ("cand_ms", C.c_float * NCAND)]
 
 
 IN_KEYS = ("plev", "tlev", "pmid", "tmid", "qh2o", "tauref", "taurefcld", "als", "albi", "acosz")
@@ -143,6 +146,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         lib.mgcm_rt_check.argtypes = [C.c_void_p]
         lib.mgcm_rt_check_stream.argtypes = [C.c_void_p, C.c_void_p]
         lib.mgcm_rt_batched_tuning.argtypes = [C.c_void_p, C.c_int, C.POINTER(BatchClass)]
+        lib.mgcm_rt_set_tile.argtypes = [C.c_void_p, C.c_int]
+        lib.mgcm_rt_tile_for.argtypes = [C.c_void_p, C.c_int, C.c_int]
         lib.mgcm_rt_planck_table.argtypes = [DP, DP, DP]
         lib.mgcm_rt_laginterp.argtypes = [DP, DP, C.c_int, DP, DP]
         lib.mgcm_rt_read_ktable.argtypes = [C.c_char_p, C.c_int, DP, DP]
@@ -300,11 +305,23 @@ class Radiation:
         _check(min(n, 0), "mgcm_rt_batched_tuning")
         out = []
         for c in info[:n]:
-            cands = [{"waves_per_simd": c.cand_waves[j], "mult": c.cand_mult[j], "ms": float(c.cand_ms[j])}
-                     for j in range(6) if c.cand_ms[j] > 0]
-            out.append({"max_layers": c.max_layers, "items": c.items, "waves_per_simd": c.waves_per_simd,
-                        "blocks": c.blocks, "tuned_items": c.tuned_items, "candidates": cands})
+            cands = [{"tile": c.cand_tile[j], "waves_per_simd": c.cand_waves[j], "mult": c.cand_mult[j],
+                      "ms": float(c.cand_ms[j])} for j in range(NCAND) if c.cand_ms[j] > 0]
+            out.append({"max_layers": c.max_layers, "items": c.items, "tile": c.tile,
+                        "waves_per_simd": c.waves_per_simd, "blocks": c.blocks, "tuned_items": c.tuned_items,
+                        "candidates": cands})
         return out
+
+    def set_tile(self, band_slots: int) -> None:
+        """Column kernel tile of single-grid calls: 1, 2 or 4 band slots per
+        column, 0 = chosen per grid by the library (mgcm_rt_set_tile)."""
+        _check(self.lib.mgcm_rt_set_tile(self.ctx, int(band_slots)), "mgcm_rt_set_tile")
+
+    def tile_for(self, ncol: int, nlay: int) -> int:
+        """The tile a single-grid call of ncol x nlay runs with."""
+        t = self.lib.mgcm_rt_tile_for(self.ctx, int(ncol), int(nlay))
+        _check(min(t, 0), "mgcm_rt_tile_for")
+        return t
 
     def check(self) -> None:
         _check(self.lib.mgcm_rt_check(self.ctx), "mgcm_rt_check")

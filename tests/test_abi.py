@@ -44,6 +44,8 @@ def test_strerror_and_null_arguments():
     assert lib.mgcm_rt_run(None, None, None, 0, None) == -1
     assert lib.mgcm_rt_run_host(None, None, None, 0) == -1
     assert lib.mgcm_rt_destroy(None) in (0, -1)
+    assert lib.mgcm_rt_set_tile(None, 1) == -1
+    assert lib.mgcm_rt_tile_for(None, 10, 24) == -1
 
 
 def test_python_binding_fails_loudly_without_library(tmp_path):

@@ -172,6 +172,31 @@ def test_fortran_per_column_shim_equals_batch(tmp_path):
 
 
 @pytest.mark.gpu
+def test_fortran_tiles_equal(tmp_path):
+    """The column kernel's tiles chosen from Fortran (mgcm_rt_set_tile through
+    --tile) give the same bits, batch and per-column calls alike."""
+    from mgcm_rt import casefile, synth
+    t, c = synth.make_workload(24, 13, 24)
+    case = str(tmp_path / "case.bin")
+    casefile.write_case(case, t, c, 0)
+    outs = {}
+    for mode in ("batch", "column"):
+        for tile in (1, 2, 4):
+            res = str(tmp_path / f"res_{mode}_{tile}.bin")
+            args = [DRIVER, "--tile", str(tile)] + (["--per-column"] if mode == "column" else []) + [case, res]
+            r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+            assert r.returncode == 0, r.stdout + r.stderr
+            status, outs[mode, tile] = casefile.read_result(res, 13, 24)
+            assert status == 0
+    for mode in ("batch", "column"):
+        for tile in (2, 4):
+            for k in casefile.RESULT_KEYS:
+                if mode == "column" and k == "hr":
+                    continue
+                assert np.array_equal(outs[mode, tile][k], outs[mode, 1][k]), (mode, tile, k)
+
+
+@pytest.mark.gpu
 def test_fortran_driver_active_dust_and_clouds(tmp_path):
     """COMP3's active dust and clouds from Fortran: INITCLD's tables read from the
     reference's optical-property files, OPT_DST and OPT_CLD on host arrays

@@ -120,8 +120,9 @@ def test_batched_at_shape_multi_item(oracle):
     tune = rt.batched_tuning()
     print(tune, flush=True)
     c24 = [c for c in tune if c["max_layers"] == 24][0]
-    assert c24["items"] == 12500 and c24["blocks"] < c24["items"], c24  # several items per block
-    assert len(c24["candidates"]) >= 4 and c24["tuned_items"] > 0, c24
+    assert c24["items"] == 12500 and c24["blocks"] < c24["items"] * c24["tile"], c24  # several items per block
+    assert len(c24["candidates"]) >= 8 and c24["tuned_items"] > 0, c24
+    assert {c["tile"] for c in c24["candidates"]} == {1, 2, 4}, c24
     rng = np.random.Generator(np.random.PCG64(5))
     for i, ((n, nlay), c, o) in enumerate(zip(AT_SHAPE, cols, do)):
         got = {k: v.cpu().numpy() for k, v in o.items()}
