@@ -697,9 +697,15 @@ __global__ __launch_bounds__(64) void k_rt_clear_batch(DevTables dt, const RunAr
   }
 }
 
+// Waves per SIMD of the single-grid column kernel: two up to 36 layers, one
+// above (512 registers per lane for the larger item state; the batched tuner
+// measures the one-wave instance 7-8% faster at 48 and 64 layers and the
+// two-wave one 2-16% faster at 16..36 layers).
+__host__ __device__ constexpr int column_waves(int nlm) { return nlm <= 37 ? MGCM_FP64_WAVES : 1; }
+
 // One grid: block b owns columns Cb..Cb+C-1 (C = 4/S).
 template <int NLM, int P, int S>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MGCM_FP64_WAVES)))
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(column_waves(NLM), column_waves(NLM))))
 void k_rt(DevTables dt, RunArgs a) {
   extern __shared__ double smem[];
   mgcm_libm::load_lds_exptab(threadIdx.x, 64);
@@ -787,7 +793,9 @@ inline int resident_per_cu(size_t lds, int waves) {
 // sums and the empty band slots do not shrink with S.  Large grids fill the
 // device at S = 1, the least work per column.
 inline int auto_tile(int ncol, int nlay, int ncu) {
-  const long long slots = (long long)ncu * resident_per_cu(stage_bytes(nlay), MGCM_FP64_WAVES);
+  const int nl = nlay + 1;
+  const int nlm = nl <= 17 ? 17 : nl <= 25 ? 25 : nl <= 37 ? 37 : nl <= 49 ? 49 : nl <= 65 ? 65 : MAX_NLAY + 1;
+  const long long slots = (long long)ncu * resident_per_cu(stage_bytes(nlay), column_waves(nlm));
   constexpr int tiles[3] = {1, 2, 4}, latency[3] = {100, 75, 53};
   int best = 1;
   long long best_cost = -1;
