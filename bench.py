@@ -317,6 +317,10 @@ def form_batched(args, rt, stream):
     return out
 
 
+SWEEP = [(24, 540), (24, 1080), (24, 2160), (24, 4320), (24, 8100), (24, 64800),
+         (36, 2160), (36, 8100), (36, 64800)]
+
+
 def form_small_grid(args, rt, stream):
     """The reference's own CPU case on the GPU: 2,160 columns (a 36 x 60 grid) x
     24 layers, one single-grid call per step, at each column-kernel tile (1, 2
@@ -324,33 +328,38 @@ def form_small_grid(args, rt, stream):
     library's automatic choice.  Every tile's outputs are compared bit for bit
     with the one-slot tile's, and the automatic tile's with the oracle at rel
     1e-10.  At 4 columns per wave this grid is 540 waves, a quarter of the
-    device's resident slots."""
+    device's resident slots.  "sweep" repeats the per-tile timing over SWEEP's
+    (layers, columns) shapes, with the automatic choice of each."""
     import torch
-    from mgcm_rt import FP64, alloc_outputs, synth, to_device
+    from mgcm_rt import FP64, Radiation, alloc_outputs, synth, to_device
     if args.nlay != 24:
         return {"skipped": "the context's tables are built for another layer count"}
+    sh = stream.cuda_stream
+
+    def per_tile(ctx, dc, do, steps):
+        res, first, same = {}, None, True
+        for tile in (1, 2, 4):
+            ctx.set_tile(tile)
+            ms, _ = timed_steps(lambda: ctx.run_device(dc, do, FP64, sh), steps, 3, stream)
+            got = {k: do[k].cpu().numpy() for k in PARITY_KEYS}
+            if first is None:
+                first = got
+            else:
+                same = same and all(np.array_equal(got[k], first[k]) for k in PARITY_KEYS)
+            res[str(tile)] = ms
+        ctx.set_tile(0)
+        return res, same
+
     ncol = 2160
     c = synth.make_columns(args.seed, ncol, 24)
     dc, do = to_device(c), to_device(alloc_outputs(ncol, 24))
-    sh = stream.cuda_stream
-    tiles, first = {}, None
-    same = True
-    for tile in (1, 2, 4):
-        rt.set_tile(tile)
-        ms, _ = timed_steps(lambda: rt.run_device(dc, do, FP64, sh), 50, 5, stream)
-        got = {k: v.cpu().numpy() for k, v in do.items()}
-        if first is None:
-            first = got
-        else:
-            same = same and all(np.array_equal(got[k], first[k]) for k in PARITY_KEYS)
-        tiles[str(tile)] = {"ms_per_step": ms, "columns_per_s": ncol / (ms / 1e3)}
-    rt.set_tile(0)
+    tiles, same = per_tile(rt, dc, do, 50)
     auto = rt.tile_for(ncol, 24)
     ms, _ = timed_steps(lambda: rt.run_device(dc, do, FP64, sh), 50, 5, stream)
     rt.check()
     out = {"workload": f"{ncol} columns (36 x 60) x 24 layers, plain FP64, one call per step",
            "auto_tile": auto, "ms_per_step": ms, "value": ncol / (ms / 1e3), "unit": "columns/s",
-           "speedup_vs_tile1": tiles["1"]["ms_per_step"] / ms, "tiles": tiles, "tiles_bit_identical": same}
+           "speedup_vs_tile1": tiles["1"] / ms, "tiles_ms": tiles, "tiles_bit_identical": same}
     try:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         from oracle import Oracle
@@ -358,6 +367,24 @@ def form_small_grid(args, rt, stream):
         out["parity"] = parity({k: v.cpu().numpy() for k, v in do.items()}, want)
     except Exception as e:  # noqa: BLE001
         out["parity"] = {"error": f"{type(e).__name__}: {e}"[:200]}
+    del dc, do
+    sweep = []
+    ctxs = {24: rt}
+    for nlay, n in SWEEP:
+        if nlay not in ctxs:
+            ctxs[nlay] = Radiation(synth.make_tables(args.seed, nlay))
+        ctx = ctxs[nlay]
+        cs = synth.make_columns(args.seed, n, nlay)
+        dcs, dos = to_device(cs), to_device(alloc_outputs(n, nlay))
+        t, same_s = per_tile(ctx, dcs, dos, 50 if n < 20000 else 10)
+        best = min(t, key=t.get)
+        pick = ctx.tile_for(n, nlay)
+        sweep.append({"nlay": nlay, "ncol": n, "ms": t, "auto_tile": pick, "fastest_tile": int(best),
+                      "auto_vs_fastest": t[str(pick)] / t[best], "bit_identical": same_s})
+        del dcs, dos
+    ctxs.pop(24)
+    del ctxs
+    out["sweep"] = sweep
     return out
 
 
