@@ -21,7 +21,7 @@ import os
 
 import numpy as np
 
-from . import SKIP_VIS, FP64, Radiation, load_library, _check  # noqa: F401
+from . import SKIP_IR, SKIP_VIS, FP64, Radiation, load_library, _check  # noqa: F401
 
 NLAY = 36
 NIN, NOUT, NPARAM = 147, 36, 160 * 512 + 512 + 2 * (512 * 512 + 512) + 512 * 36 + 36
@@ -192,6 +192,41 @@ class Surrogate:
             h = h.clone()
             h[idx] = exact_ir_heating(rt, sub, stream)
         return h, bad
+
+
+    def total_heating(self, rt: Radiation, dc: dict, out: dict, stream: int = 0, combined_above: float = 0.5):
+        """The learned form's radiation call: heating of GCM layers 1..36 (K/sol, VIS
+        exact, IR from the network where the check trusts it, exact elsewhere).  out
+        receives the plain path's outputs of the run made (mgcm_rt output dict).
+
+        With few fallback columns, a VIS-only run of the grid and an IR-only run of the
+        fallback columns (gathered) cost least.  With more than combined_above of the
+        columns falling back, one combined plain run of the grid (VIS and IR together,
+        one continuum-point launch, no gather) costs less than those two runs; its
+        heating is then taken whole for the fallback columns, and for trusted columns
+        its IR part (from FMNETI) is replaced by the network's.
+        Returns (heating [ncol][36], fallback mask [ncol], combined: bool)."""
+        import torch
+        ncol = int(dc["ncol"])
+        h, dis, inr = self.predict(dc, stream)
+        bad = (~inr) | (dis > self.threshold)
+        nbad = int(bad.sum())
+        if nbad > combined_above * ncol:
+            rt.run_device(dc, out, FP64, stream)
+            full = out["hr"][:, 1:]
+            if nbad == ncol:
+                return full, bad, True
+            exact_ir = ir_heating(out["fmneti"], dc["plev"])
+            return torch.where(bad[:, None], full, (full - exact_ir) + h), bad, True
+        rt.run_device(dc, out, FP64 | SKIP_IR, stream)
+        if nbad:
+            idx = torch.nonzero(bad).flatten()
+            sub = {k: (v.index_select(0, idx).contiguous() if hasattr(v, "index_select") else v)
+                   for k, v in dc.items()}
+            sub["ncol"] = nbad
+            h = h.clone()
+            h[idx] = exact_ir_heating(rt, sub, stream)
+        return out["hr"][:, 1:] + h, bad, False
 
 
 def calibrate(disagreement, err_max, bar: float) -> float:

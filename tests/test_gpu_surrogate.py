@@ -154,3 +154,37 @@ def test_fallback_gives_exact_heating_where_flagged():
     h0, bad0 = sg0.heating(rt, dc)
     assert bool(bad0.all()) and torch.equal(h0, exact)
 
+
+
+def test_total_heating_both_fallback_schedules():
+    """The learned form's whole call: with every column falling back it makes one
+    combined plain run, whose heating it returns bit for bit; with every column
+    trusted it makes a VIS-only run and adds the network's IR heating; in between
+    (below the switch) the fallback columns get the VIS-only run's heating plus
+    their exact IR heating."""
+    import torch
+    from mgcm_rt import FP64, SKIP_IR, alloc_outputs, synth, to_device
+    from mgcm_rt.surrogate import Net, Surrogate, exact_ir_heating
+    rt, t, stats, net, _, _, _ = _trained(ncol=2048, epochs=3)
+    net2 = Net(512, seed=13)
+    c = synth.make_columns(56, 300, 36)
+    c["qh2o"][:20] = 0.3                     # far outside the training range: falls back
+    dc = to_device(c)
+    full = to_device(alloc_outputs(300, 36))
+    rt.run_device(dc, full, FP64)
+    vis = to_device(alloc_outputs(300, 36))
+    rt.run_device(dc, vis, FP64 | SKIP_IR)
+    exact = exact_ir_heating(rt, dc)
+    pred, _, _ = Surrogate(stats, [net, net2], threshold=float("inf")).predict(dc)
+    out = to_device(alloc_outputs(300, 36))
+    h, bad, combined = Surrogate(stats, [net, net2], threshold=0.0).total_heating(rt, dc, out)
+    assert combined and bool(bad.all()) and torch.equal(h, full["hr"][:, 1:])
+    out = to_device(alloc_outputs(300, 36))
+    h, bad, combined = Surrogate(stats, [net, net2], threshold=float("inf")).total_heating(rt, dc, out)
+    # columns outside the (small) training run's feature range fall back too
+    assert bool(bad[:20].all()) and int(bad.sum()) < 150 and not combined, int(bad.sum())
+    assert torch.equal(h[~bad], vis["hr"][~bad][:, 1:] + pred[~bad])
+    assert torch.equal(h[bad], vis["hr"][bad][:, 1:] + exact[bad])
+    # the VIS-only heating plus the exact IR heating is the plain heating to rounding
+    hb, fb = h[bad], full["hr"][bad][:, 1:]
+    assert float((hb - fb).abs().max()) <= 1e-9 * float(fb.abs().max())
