@@ -16,9 +16,8 @@
 ! arrays QXVDST ... GICLD, QEXTREFDST, QEXTREFCLD, which OPT_DST/OPT_CLD fill
 ! for active dust and clouds and INI_OPTDST/INI_OPTCLD otherwise -- to the
 ! library through its aerosol argument.  The library context is set up on the
-! first call from the tables RADSETUP left in radcommon_h (one GWEIGHT set,
-! repeated per band).  A failing call stops the program with the library's
-! message.
+! first call (comp3_rt_ctx.f90).  A failing call stops the program with the
+! library's message.
 !
 ! With the environment variable MGCM_COMP3_DUMP naming a file, every call
 ! appends its outputs to it (stream, native endian): int32 J, I, part (1 VIS,
@@ -26,46 +25,20 @@
 ! NFLUXTOPV, DIFFVT, TAUCUMV(L_LEVELS,:,:) and DETAU(J,I,:,:) (7 x 17 each);
 ! IR: FMNETI, FLUXUPI, FLUXDNI, NFLUXTOPI.  The drop-in test compares these
 ! records with the same GCM's run on the reference's own routines.
-module comp3_rt_state
-  use iso_c_binding
+module comp3_rt_dump
   implicit none
-  type(c_ptr), save :: rtctx = c_null_ptr
-  logical, save :: dump_on = .false.
+  logical, save :: looked = .false., dump_on = .false.
   integer, save :: dump_unit = 0
-end module comp3_rt_state
-
-subroutine comp3_rt_setup()
-  use iso_c_binding
-  use mgcm_rt_mod
-  use comp3_rt_state
-  use grid_h
-  use radinc_h
-  use radcommon_h
-  use fccsave_h
-  implicit none
-  real(c_double) :: gwv(L_NGAUSS, L_NSPECTV), gwi(L_NGAUSS, L_NSPECTI)
-  integer(c_int) :: st
-  integer :: b
-  character(len=4096) :: path
-  integer :: plen, pst
-  do b = 1, L_NSPECTV
-    gwv(:, b) = GWEIGHT
-  end do
-  do b = 1, L_NSPECTI
-    gwi(:, b) = GWEIGHT
-  end do
-  call mgcm_rt_setup(CO2V, CO2I, PFGASREF, TGASREF, WREFH2O, gwv, gwi, FZEROV, FZEROI, TAURAY, DWNI, PLANCKIR, &
-                     QXVDST(1, :), QSVDST(1, :), GVDST(1, :), QXIDST(1, :), QSIDST(1, :), GIDST(1, :),            &
-                     QXVCLD(1, :), QSVCLD(1, :), GVCLD(1, :), QXICLD(1, :), QSICLD(1, :), GICLD(1, :),            &
-                     QEXTREFDST(1), QEXTREFCLD(1), rtctx, st)
-  if (st /= MGCM_RT_OK) then
-    write(*, '(a)') 'comp3_rt: mgcm_rt_setup: '//trim(mgcm_rt_message(st))
-    stop 1
-  end if
-  call get_environment_variable('MGCM_COMP3_DUMP', path, plen, pst)
-  dump_on = pst == 0 .and. plen > 0
-  if (dump_on) open(newunit=dump_unit, file=path(1:plen), access='stream', form='unformatted', status='replace')
-end subroutine comp3_rt_setup
+contains
+  subroutine dump_setup()
+    character(len=4096) :: path
+    integer :: plen, pst
+    looked = .true.
+    call get_environment_variable('MGCM_COMP3_DUMP', path, plen, pst)
+    dump_on = pst == 0 .and. plen > 0
+    if (dump_on) open(newunit=dump_unit, file=path(1:plen), access='stream', form='unformatted', status='replace')
+  end subroutine dump_setup
+end module comp3_rt_dump
 
 ! One column, one part, through mgcm_rt_radiation with the column's per-level aerosol optics.
 subroutine comp3_rt_column(part, plev, tlev, pmid, tmid, qh2o, tauref, taurefcld, als, albi, acosz, sol, &
@@ -133,7 +106,7 @@ end subroutine comp3_rt_column
 
 subroutine mgcm_comp3_vis(plev, tlev, pmid, tmid, qh2o, tauref, taurefcld, als, acosz, sol, taucumv, &
                           nfluxtopv, fmnetv, fluxupv, fluxdnv, diffvt, j, i)
-  use comp3_rt_state
+  use comp3_rt_dump
   use grid_h
   use radinc_h
   use radcommon_h, only: DETAU
@@ -144,6 +117,7 @@ subroutine mgcm_comp3_vis(plev, tlev, pmid, tmid, qh2o, tauref, taurefcld, als, 
   real*8, intent(out) :: nfluxtopv, fmnetv(L_NLAYRAD), fluxupv(L_NLAYRAD), fluxdnv(L_NLAYRAD), diffvt
   integer, intent(in) :: j, i
   real*8 :: det(L_NSPECTV, L_NGAUSS), cum(L_NSPECTV, L_NGAUSS)
+  if (.not. looked) call dump_setup()
   det = DETAU(j, i, :, :)
   cum = taucumv(L_LEVELS, :, :)
   call comp3_rt_column(1, plev, tlev, pmid, tmid, qh2o, tauref, taurefcld, als, 0.0d0, acosz, sol, &
@@ -156,7 +130,7 @@ end subroutine mgcm_comp3_vis
 
 subroutine mgcm_comp3_ir(plev, tlev, pmid, tmid, qh2o, tauref, taurefcld, albi, nfluxtopi, fmneti, fluxupi, &
                          fluxdni, j, i)
-  use comp3_rt_state
+  use comp3_rt_dump
   use grid_h
   use radinc_h
   implicit none
@@ -165,6 +139,7 @@ subroutine mgcm_comp3_ir(plev, tlev, pmid, tmid, qh2o, tauref, taurefcld, albi, 
   real*8, intent(out) :: nfluxtopi, fmneti(L_NLAYRAD), fluxupi(L_NLAYRAD), fluxdni(L_NLAYRAD)
   integer, intent(in) :: j, i
   real*8 :: det(L_NSPECTV, L_NGAUSS), cum(L_NSPECTV, L_NGAUSS), sol(L_NSPECTV), dif
+  if (.not. looked) call dump_setup()
   det = 0.0d0
   cum = 0.0d0
   sol = 0.0d0

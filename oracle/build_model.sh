@@ -8,8 +8,15 @@
 #                same calls, the same arguments) -- behaves as gcm_ref, bit for bit;
 #   gcm_mi355x   the same rewritten COMP3 with the library's glue
 #                (legacy-mars-global-climate-model_amd/fortran/comp3_rt.f90): the
-#                radiation on the MI355X through mgcm_rt_mod / libmgcm_rt.so.
-# Both glue builds write every radiation call's outputs to $MGCM_COMP3_DUMP when
+#                radiation on the MI355X through mgcm_rt_mod / libmgcm_rt.so,
+#                one column per call;
+#   gcm_refsplit COMP3's column loop in two passes around one radiation batch
+#                (fortran/comp3_split.f90), the batch run by the reference's own
+#                routines column by column (oracle/comp3_ref_split.f90) --
+#                behaves as gcm_ref, bit for bit;
+#   gcm_mi355x_split  the same two-pass COMP3, the batch one mgcm_rt_radiation
+#                call on the MI355X (fortran/comp3_rt_batch.f90).
+# Every glue build writes every radiation call's outputs to $MGCM_COMP3_DUMP when
 # it is set.  Also copies run/data and writes a one-hour tutorial namelist
 # (oracle/_ref/gcm_run/), which the test runs in its own directories.
 #
@@ -21,7 +28,11 @@
 #   comp3.f  (glue builds) the calls OPTCV (code/comp3.f:658-662) and SFLUXV
 #            (:672-675) become one call of mgcm_comp3_vis, OPTCI (:724-727) and
 #            SFLUXI (:731-733) one call of mgcm_comp3_ir; the lines are checked
-#            before they are replaced.
+#            before they are replaced.  (split builds) the column loop (DO 500 /
+#            DO 490, :411-1044) becomes two loops: the first runs :417-636, the
+#            albedos (:642-654 under the sun-up test, :712-720), records the
+#            column's radiation inputs and saves its state; after the batch the
+#            second restores each column and runs :637-1042 with the glue calls.
 # The rewritten sources live in a temporary directory and are deleted; only the
 # linked programs land in oracle/_ref/.  Exits 0 without building when the
 # reference tree is absent.
@@ -75,6 +86,52 @@ NR == 724 {
 }
 (NR >= 658 && NR <= 662) || (NR >= 672 && NR <= 675) || (NR >= 724 && NR <= 727) || (NR >= 731 && NR <= 733) { next }
 { print }' "$CODE/comp3.f" > "$TMP/comp3_glue.f"
+# the same rewrite keeping the original line numbers (each replaced line blank,
+# the glue call on one long line), for the split build's line ranges
+awk '
+NR == 658 { print "                 call mgcm_comp3_vis(PLEV,TLEV,PMID,TMID,QH2O,TAUREF,"; next }
+NR == 659 { print "     *                TAUREFCLD,ALS,ACOSZ,SOL,TAUCUMV,NFLUXTOPV,FMNETV,"; next }
+NR == 660 { print "     *                fluxupv,fluxdnv,diffvt,jcmn,icmn)"; next }
+NR == 724 { print "               call mgcm_comp3_ir(PLEV,TLEV,PMID,TMID,QH2O,TAUREF,"; next }
+NR == 725 { print "     *                TAUREFCLD,ALBI,NFLUXTOPI,FMNETI,FLUXUPI,FLUXDNI,"; next }
+NR == 726 { print "     *                jcmn,icmn)"; next }
+(NR >= 658 && NR <= 662) || (NR >= 672 && NR <= 675) || (NR >= 724 && NR <= 727) || (NR >= 731 && NR <= 733) { print ""; next }
+{ print }' "$CODE/comp3.f" > "$TMP/comp3_glue_full.f"
+
+# comp3.f in two passes around one radiation batch
+grep -q -E '^ +DO 500 I=1,IM *$' <(sed -n 411p "$CODE/comp3.f") || { echo "build_model: comp3.f:411 is not DO 500" >&2; exit 1; }
+grep -q -E '^ +DO 490 J=1,JM-1 *$' <(sed -n 415p "$CODE/comp3.f") || { echo "build_model: comp3.f:415 is not DO 490" >&2; exit 1; }
+grep -q -E '^ +if\(acosz.ge.1.0e-5\) then *$' <(sed -n 637p "$CODE/comp3.f") || { echo "build_model: comp3.f:637" >&2; exit 1; }
+grep -q -E '^ +ALS += ALSP' <(sed -n 642p "$CODE/comp3.f") || { echo "build_model: comp3.f:642" >&2; exit 1; }
+grep -q -E '^ +surfalb\(J,I\) = ALS *$' <(sed -n 654p "$CODE/comp3.f") || { echo "build_model: comp3.f:654" >&2; exit 1; }
+grep -q -E '^ +ALBI += 1.0D0-EGOGND *$' <(sed -n 712p "$CODE/comp3.f") || { echo "build_model: comp3.f:712" >&2; exit 1; }
+grep -q -E '^ +ENDIF *$' <(sed -n 720p "$CODE/comp3.f") || { echo "build_model: comp3.f:720" >&2; exit 1; }
+grep -q -E '^  490 +CONTINUE *$' <(sed -n 1043p "$CODE/comp3.f") || { echo "build_model: comp3.f:1043" >&2; exit 1; }
+grep -q -E '^  500 +CONTINUE *$' <(sed -n 1044p "$CODE/comp3.f") || { echo "build_model: comp3.f:1044" >&2; exit 1; }
+{
+  sed -n '1,410p' "$CODE/comp3.f"
+  echo "         call comp3_split_pass1_begin()"
+  sed -n '411,636p' "$CODE/comp3.f"
+  echo "               if(acosz.ge.1.0e-5) then"
+  sed -n '642,654p' "$CODE/comp3.f"
+  echo "               end if"
+  sed -n '712,720p' "$CODE/comp3.f"
+  echo "               call mgcm_comp3_record(PLEV,TLEV,PMID,TMID,QH2O,"
+  echo "     *              TAUREF,TAUREFCLD,ALS,ALBI,ACOSZ,SOL,jcmn,icmn)"
+  echo "               call comp3_split_save(jcmn,icmn,QPI,QPIG,TETAORG,"
+  echo "     *              QPISAV,dustref,PLEV,TLEV,PMID,TMID,QH2O)"
+  sed -n '1043,1044p' "$CODE/comp3.f"
+  echo "         call comp3_split_pass1_end()"
+  echo "         call comp3_split_radiate()"
+  echo "         DO 501 I=1,IM"
+  echo "            DO 491 J=1,JM-1"
+  echo "               call comp3_split_restore(J,I,QPI,QPIG,TETAORG,"
+  echo "     *              QPISAV,dustref,PLEV,TLEV,PMID,TMID,QH2O)"
+  awk 'NR >= 637 && NR <= 1042' "$TMP/comp3_glue_full.f"
+  echo "  491       CONTINUE"
+  echo "  501    CONTINUE"
+  sed -n '1045,$p' "$CODE/comp3.f"
+} > "$TMP/comp3_split.f"
 
 cd "$TMP/obj"
 FFLAGS="-O2 -w -J $TMP/obj"
@@ -87,13 +144,22 @@ printf '%s\n' $SRCS | grep -v -E '^(modules\.f90|main\.f|comp3\.f)$' |
 "$FC" $FFLAGS -c "$TMP/comp3_glue.f" -o "$TMP/comp3_glue.o"
 "$FC" $FFLAGS -c "$HERE/comp3_ref_glue.f90" -o "$TMP/comp3_ref_glue.o"
 "$FC" $FFLAGS -c "$PKG/fortran/mgcm_rt_mod.f90" -o "$TMP/mgcm_rt_mod.o"
+"$FC" $FFLAGS -c "$PKG/fortran/comp3_rt_ctx.f90" -o "$TMP/comp3_rt_ctx.o"
 "$FC" $FFLAGS -c "$PKG/fortran/comp3_rt.f90" -o "$TMP/comp3_rt.o"
+"$FC" $FFLAGS -c "$PKG/fortran/comp3_split.f90" -o "$TMP/comp3_split.o"
+"$FC" $FFLAGS -c "$PKG/fortran/comp3_rt_batch.f90" -o "$TMP/comp3_rt_batch.o"
+"$FC" $FFLAGS -c "$HERE/comp3_ref_split.f90" -o "$TMP/comp3_ref_split.o"
+"$FC" $FFLAGS -c "$TMP/comp3_split.f" -o "$TMP/comp3_splitloop.o"
 COMMON=$(ls *.o)
+RPATH='$ORIGIN/../../legacy-mars-global-climate-model_amd/mgcm_rt'
 "$FC" -o "$OUT/gcm_ref.tmp" $COMMON "$TMP/comp3_orig.o"
 "$FC" -o "$OUT/gcm_refglue.tmp" $COMMON "$TMP/comp3_glue.o" "$TMP/comp3_ref_glue.o"
-"$FC" -o "$OUT/gcm_mi355x.tmp" $COMMON "$TMP/comp3_glue.o" "$TMP/comp3_rt.o" "$TMP/mgcm_rt_mod.o" \
-  -L "$LIBDIR" -lmgcm_rt -Wl,-rpath,'$ORIGIN/../../legacy-mars-global-climate-model_amd/mgcm_rt'
-for b in gcm_ref gcm_refglue gcm_mi355x; do mv "$OUT/$b.tmp" "$OUT/$b"; done
+"$FC" -o "$OUT/gcm_mi355x.tmp" $COMMON "$TMP/comp3_glue.o" "$TMP/comp3_rt.o" "$TMP/comp3_rt_ctx.o" \
+  "$TMP/mgcm_rt_mod.o" -L "$LIBDIR" -lmgcm_rt -Wl,-rpath,"$RPATH"
+"$FC" -o "$OUT/gcm_refsplit.tmp" $COMMON "$TMP/comp3_splitloop.o" "$TMP/comp3_split.o" "$TMP/comp3_ref_split.o"
+"$FC" -o "$OUT/gcm_mi355x_split.tmp" $COMMON "$TMP/comp3_splitloop.o" "$TMP/comp3_split.o" \
+  "$TMP/comp3_rt_batch.o" "$TMP/comp3_rt_ctx.o" "$TMP/mgcm_rt_mod.o" -L "$LIBDIR" -lmgcm_rt -Wl,-rpath,"$RPATH"
+for b in gcm_ref gcm_refglue gcm_mi355x gcm_refsplit gcm_mi355x_split; do mv "$OUT/$b.tmp" "$OUT/$b"; done
 
 # run directory: the model's data files and the tutorial namelist cut to one hour
 # of model time (30 steps of 2 minutes, COMP3's radiation every 8 steps),
@@ -104,4 +170,4 @@ cp "$REF"/run/data/* "$RUN/data/"
 grep -q -E '^ +taue *= *[0-9.]+ +tauh *= *[0-9.]+' "$REF/tutorial/mars_tutorial"
 sed -E 's/^( +)taue *= *[0-9.]+ +tauh *= *[0-9.]+/\1taue = 1.0  tauh = 0.5/' "$REF/tutorial/mars_tutorial" \
   > "$RUN/mars_1h"
-echo "build_model: $OUT/gcm_ref $OUT/gcm_refglue $OUT/gcm_mi355x, run directory $RUN"
+echo "build_model: gcm_ref gcm_refglue gcm_mi355x gcm_refsplit gcm_mi355x_split in $OUT, run directory $RUN"

@@ -66,3 +66,30 @@ def test_gcm_with_mi355x_radiation(tmp_path, monkeypatch):
         print(f"history output {t}: max rel diff {max(row.values()):.1e} (controls {cmax:.1e})",
               {k: f"{v:.1e}" for k, v in row.items() if v > 0}, flush=True)
         assert max(row.values()) <= 10.0 * cmax + 1e-7, (t, max(row.values()), cmax)
+
+
+@pytest.mark.skipif(not model_run.built("gcm_mi355x", "gcm_mi355x_split"),
+                    reason="GCM builds not present (oracle/build_model.sh)")
+def test_gcm_two_pass_batched_equals_per_column(tmp_path):
+    """COMP3's column loop in two passes around ONE mgcm_rt_radiation call per
+    radiation step (all 2,100 columns, VIS and IR together) gives the GCM of the
+    per-column glue (two library calls per column) bit for bit -- every radiation
+    call's outputs and the history after the hour: columns are independent, and the
+    library's tiles and one-part runs give the same bits.  The wall times of both
+    forms and of the unmodified reference GCM (one host core) are printed."""
+    import time
+    t0 = time.perf_counter()
+    per = model_run.run_model("gcm_mi355x", str(tmp_path / "percol"), timeout=600)
+    t1 = time.perf_counter()
+    bat = model_run.run_model("gcm_mi355x_split", str(tmp_path / "batch"), timeout=600)
+    t2 = time.perf_counter()
+    model_run.run_model("gcm_ref", str(tmp_path / "ref"), dump=False)
+    t3 = time.perf_counter()
+    print(f"one model hour: per-column glue {t1 - t0:.2f} s, two-pass batch {t2 - t1:.2f} s, "
+          f"the reference GCM on one host core {t3 - t2:.2f} s", flush=True)
+    assert len(per["dump"]) == len(bat["dump"]) > 10000
+    for x, y in zip(per["dump"], bat["dump"]):
+        assert x[:3] == y[:3] and np.array_equal(x[3], y[3]), (x[:3], y[:3])
+    diffs = model_run.history_diff(per, bat)
+    bad = [(t, k, v) for t, row in enumerate(diffs) for k, v in row.items() if v != 0.0]
+    assert not bad, bad[:10]
