@@ -351,6 +351,11 @@ int mgcm_rt_dust_tauref_host(const mgcm_rt_dust_columns *in, double *tauref, dou
  * and FZEROV are the context's.  Asynchronous on stream. */
 int mgcm_rt_dsolflux(mgcm_rt_ctx *ctx, int ncol, const double *sol, const double *acosz, const double *detau,
                      const double *diffvt, double *directsol, double *dnvflux, void *stream);
+/* The same with host pointers (copies in, runs on the null stream, copies out,
+ * synchronises): COMP3's per-step TEMPGR loop (code/comp3.f:311-351) over a
+ * host-resident DETAU in one call. */
+int mgcm_rt_dsolflux_host(mgcm_rt_ctx *ctx, int ncol, const double *sol, const double *acosz, const double *detau,
+                          const double *diffvt, double *directsol, double *dnvflux);
 
 /* Radiation diagnostics of the history file, from the outputs of mgcm_rt_run.
  * The REAL*4 fields COMP3 stores per column (code/comp3.f:654,703-706,745-747;

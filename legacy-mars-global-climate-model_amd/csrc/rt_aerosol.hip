@@ -308,7 +308,8 @@ extern "C" int mgcm_rt_opt_dst(mgcm_rt_ctx *c, const mgcm_rt_tracers *in, const 
 // Host-pointer variants: stage the arrays in the context's grow-only device
 // block (the one mgcm_rt_run_host uses), run on the null stream, copy back,
 // synchronise.
-static double *staging(mgcm_rt_ctx *c, size_t bytes) {
+namespace mgcm {
+double *host_staging(mgcm_rt_ctx *c, size_t bytes) {
   if (bytes > c->host_bytes) {
     if (c->d_host) (void)hipFree(c->d_host);
     c->d_host = nullptr;
@@ -318,6 +319,7 @@ static double *staging(mgcm_rt_ctx *c, size_t bytes) {
   }
   return c->d_host;
 }
+}  // namespace mgcm
 
 static int opt_aer_host(bool cloud, mgcm_rt_ctx *c, const mgcm_rt_tracers *in, const mgcm_rt_aerosol_out *out) {
   if (!c || !in || !out) return MGCM_RT_ERR_ARG;
@@ -336,7 +338,7 @@ static int opt_aer_host(bool cloud, mgcm_rt_ctx *c, const mgcm_rt_tracers *in, c
     if (!hout[i]) return MGCM_RT_ERR_ARG;
   size_t total = nq + npl;
   for (int i = 0; i < 9; i++) total += hout[i] ? nout[i] : 0;
-  double *p = staging(c, total * sizeof(double));
+  double *p = mgcm::host_staging(c, total * sizeof(double));
   if (!p) return MGCM_RT_ERR_NOMEM;
   mgcm_rt_tracers din = *in;
   din.qtrace = p;

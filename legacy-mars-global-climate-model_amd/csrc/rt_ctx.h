@@ -88,4 +88,7 @@ hipError_t launch_dsolflux(const RtConst *rc, int ncol, const double *sol, const
 constexpr int DSOLFLUX_ITEM_COLS = 16;
 hipError_t launch_dsolflux_batch(const RtConst *rc, const RunArgs *grids, const uint2 *work, int nwork, int nblocks,
                                  hipStream_t st);
+// The context's grow-only device block for the host-pointer entry points
+// (shared with mgcm_rt_run_host's staging); null when it cannot grow.
+double *host_staging(mgcm_rt_ctx *ctx, size_t bytes);
 }  // namespace mgcm

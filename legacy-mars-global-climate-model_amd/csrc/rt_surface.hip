@@ -116,6 +116,27 @@ extern "C" int mgcm_rt_dsolflux(mgcm_rt_ctx *c, int ncol, const double *sol, con
              : MGCM_RT_ERR_DEVICE;
 }
 
+extern "C" int mgcm_rt_dsolflux_host(mgcm_rt_ctx *c, int ncol, const double *sol, const double *acosz,
+                                     const double *detau, const double *diffvt, double *directsol,
+                                     double *dnvflux) {
+  if (!c || !sol || ncol < 0) return MGCM_RT_ERR_ARG;
+  if (ncol == 0) return MGCM_RT_OK;
+  if (!acosz || !detau || !directsol || (dnvflux && !diffvt)) return MGCM_RT_ERR_ARG;
+  using namespace mgcm;
+  const size_t n = (size_t)ncol, nd = n * NG * NV;
+  double *p = host_staging(c, (n + nd + (dnvflux ? 3 : 1) * n) * sizeof(double));
+  if (!p) return MGCM_RT_ERR_NOMEM;
+  double *d_acosz = p, *d_detau = p + n, *d_dir = d_detau + nd;
+  double *d_dif = dnvflux ? d_dir + n : nullptr, *d_dnv = dnvflux ? d_dir + 2 * n : nullptr;
+  HIP_TRY(hipMemcpy(d_acosz, acosz, n * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(d_detau, detau, nd * sizeof(double), hipMemcpyHostToDevice));
+  if (dnvflux) HIP_TRY(hipMemcpy(d_dif, diffvt, n * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(launch_dsolflux(c->d_const, ncol, sol, d_acosz, d_detau, d_dif, d_dir, d_dnv, nullptr));
+  HIP_TRY(hipMemcpy(directsol, d_dir, n * sizeof(double), hipMemcpyDeviceToHost));
+  if (dnvflux) HIP_TRY(hipMemcpy(dnvflux, d_dnv, n * sizeof(double), hipMemcpyDeviceToHost));
+  return MGCM_RT_OK;
+}
+
 namespace mgcm {
 
 struct HistArgs {

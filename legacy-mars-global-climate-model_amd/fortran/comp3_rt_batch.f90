@@ -41,3 +41,37 @@ subroutine comp3_split_radiate()
     stop 1
   end if
 end subroutine comp3_split_radiate
+
+! COMP3's TEMPGR loop (code/comp3.f:311-351) on the MI355X: DSOLFLUX of every
+! recorded column from the stored DETAU and its current ACOSZ, and DNVFLUX(J,I) =
+! DNDIFFV(J,I) + DIRECTSOL, in one mgcm_rt_dsolflux_host call.
+subroutine mgcm_comp3_tempgr(sol, dnvflux)
+  use iso_c_binding
+  use mgcm_rt_mod
+  use comp3_rt_state
+  use comp3_split_state
+  use radinc_h
+  use radcommon_h, only: DETAU
+  use standard_h, only: DNDIFFV
+  implicit none
+  real*8, intent(in) :: sol(L_NSPECTV)
+  real*8, intent(inout) :: dnvflux(L_J, L_I)  ! COMP3's DNVFLUX
+  real(c_double), allocatable :: det(:, :, :), dif(:), dir(:), dnv(:)
+  integer(c_int) :: st
+  integer :: c
+  if (ntg == 0) return
+  if (.not. c_associated(rtctx)) call comp3_rt_setup()
+  allocate(det(L_NSPECTV, L_NGAUSS, ntg), dif(ntg), dir(ntg), dnv(ntg))
+  do c = 1, ntg
+    det(:, :, c) = DETAU(tg_j(c), tg_i(c), :, :)
+    dif(c) = DNDIFFV(tg_j(c), tg_i(c))
+  end do
+  st = mgcm_rt_dsolflux_host(rtctx, int(ntg, c_int), sol, tg_acosz, det, dif, dir, dnv)
+  if (st /= MGCM_RT_OK) then
+    write(*, '(a)') 'comp3_rt_batch: mgcm_rt_dsolflux_host: '//trim(mgcm_rt_message(st))
+    stop 1
+  end if
+  do c = 1, ntg
+    dnvflux(tg_j(c), tg_i(c)) = dnv(c)
+  end do
+end subroutine mgcm_comp3_tempgr

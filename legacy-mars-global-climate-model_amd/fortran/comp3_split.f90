@@ -28,6 +28,11 @@
 ! order in both passes.  ALS keeps across pass 1 the value it had before it
 ! (COMP3 sets it with the sun up only).
 !
+! COMP3's per-step loop for TEMPGR (code/comp3.f:311-351: DSOLFLUX from the
+! stored DETAU and the current sun, DNVFLUX = DNDIFFV + DIRECTSOL) goes the same
+! way: mgcm_comp3_tempgr_record keeps each column's ACOSZ, and
+! mgcm_comp3_tempgr(SOL, DNVFLUX) computes every column's DNVFLUX in one call.
+!
 ! With the environment variable MGCM_COMP3_DUMP naming a file, pass 2 writes
 ! the records of the per-column glue (comp3_rt.f90) in the same order.
 module comp3_split_state
@@ -66,6 +71,10 @@ module comp3_split_state
   real*8, save :: o_hr(L_NLAYRAD, NCMAX)
   logical, save :: looked = .false., dump_on = .false.
   integer, save :: dump_unit = 0
+  ! COMP3's per-step TEMPGR loop (code/comp3.f:311-351): the columns' ACOSZ, in loop order
+  integer, save :: ntg = 0
+  integer, save :: tg_j(NCMAX), tg_i(NCMAX)
+  real*8, save :: tg_acosz(NCMAX)
 contains
   subroutine dump_setup()
     character(len=4096) :: path
@@ -264,3 +273,16 @@ subroutine mgcm_comp3_ir(plev, tlev, pmid, tmid, qh2o, tauref, taurefcld, albi, 
   if (dump_on) write(dump_unit) int(j, 4), int(i, 4), 2_4, int(3 * L_NLAYRAD + 1, 4), fmneti, fluxupi, &
                                        fluxdni, nfluxtopi
 end subroutine mgcm_comp3_ir
+
+! COMP3's TEMPGR loop, per column: its ACOSZ (code/comp3.f:320-323).
+subroutine mgcm_comp3_tempgr_record(acosz, j, i)
+  use comp3_split_state
+  implicit none
+  real*8, intent(in) :: acosz
+  integer, intent(in) :: j, i
+  if (j == 1 .and. i == 1) ntg = 0
+  ntg = ntg + 1
+  tg_j(ntg) = j
+  tg_i(ntg) = i
+  tg_acosz(ntg) = acosz
+end subroutine mgcm_comp3_tempgr_record

@@ -62,7 +62,7 @@ module mgcm_rt_mod
 
   public :: mgcm_rt_init, mgcm_rt_destroy, mgcm_rt_run, mgcm_rt_run_host, mgcm_rt_check, mgcm_rt_history_fields
   public :: mgcm_rt_run_batched, mgcm_rt_planck_table, mgcm_rt_laginterp, mgcm_rt_read_ktable, mgcm_rt_strerror
-  public :: mgcm_rt_set_tile
+  public :: mgcm_rt_set_tile, mgcm_rt_dsolflux_host
   ! mirrors mgcm_rt_tracers: QTRACE(nlay, 6) and PL(L_LEVELS) of each column
   type, bind(C), public :: mgcm_rt_tracers
     integer(c_int) :: ncol, nlay
@@ -118,6 +118,16 @@ module mgcm_rt_mod
     integer(c_int) function mgcm_rt_check(ctx) bind(C, name="mgcm_rt_check")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
+    end function
+    ! DSOLFLUX + DNVFLUX = DIFFVT + DIRECTSOL of ncol columns from host arrays
+    ! (code/comp3.f:311-351): detau(7, 17, ncol) as DETAU(J,I,:,:) of each column
+    integer(c_int) function mgcm_rt_dsolflux_host(ctx, ncol, sol, acosz, detau, diffvt, directsol, dnvflux) &
+        bind(C, name="mgcm_rt_dsolflux_host")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ncol
+      real(c_double), intent(in) :: sol(*), acosz(*), detau(*), diffvt(*)
+      real(c_double), intent(out) :: directsol(*), dnvflux(*)
     end function
     ! column kernel tile of single-grid calls: 1, 2 or 4 band slots per column, 0 = automatic
     integer(c_int) function mgcm_rt_set_tile(ctx, band_slots) bind(C, name="mgcm_rt_set_tile")

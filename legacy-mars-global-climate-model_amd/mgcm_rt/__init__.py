@@ -153,6 +153,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         lib.mgcm_rt_read_ktable.argtypes = [C.c_char_p, C.c_int, DP, DP]
         lib.mgcm_rt_read_aerosol_tables.argtypes = [C.c_char_p, DP]
         lib.mgcm_rt_dsolflux.argtypes = [C.c_void_p, C.c_int, DP] + [C.c_void_p] * 6
+        lib.mgcm_rt_dsolflux_host.argtypes = [C.c_void_p, C.c_int, DP, DP, DP, DP, DP, DP]
         lib.mgcm_rt_history_fields.argtypes = [C.POINTER(Columns), C.POINTER(Fluxes), C.POINTER(History),
                                                C.c_void_p]
         lib.mgcm_rt_set_aerosol_tables.argtypes = [C.c_void_p, DP]
@@ -340,6 +341,17 @@ class Radiation:
         _check(self.lib.mgcm_rt_dsolflux(self.ctx, n, _dp(sol), cols["acosz"].data_ptr(), out["detau"].data_ptr(),
                                          out["diffvt"].data_ptr(), ds.data_ptr(), dn.data_ptr(), C.c_void_p(stream)),
                "mgcm_rt_dsolflux")
+        return {"directsol": ds, "dnvflux": dn}
+
+    def dsolflux_host(self, acosz, detau, diffvt) -> dict:
+        """mgcm_rt_dsolflux_host: DSOLFLUX and DNVFLUX from host arrays (numpy):
+        acosz [ncol], detau [ncol][17][7], diffvt [ncol]."""
+        acosz, detau, diffvt = _f64(acosz), _f64(detau), _f64(diffvt)
+        n = acosz.shape[0]
+        ds, dn = np.zeros(n), np.zeros(n)
+        sol = _f64(self.sol)
+        _check(self.lib.mgcm_rt_dsolflux_host(self.ctx, n, _dp(sol), _dp(acosz), _dp(detau), _dp(diffvt), _dp(ds),
+                                              _dp(dn)), "mgcm_rt_dsolflux_host")
         return {"directsol": ds, "dnvflux": dn}
 
     def history_fields(self, cols: dict, out: dict, hist: dict | None = None, p=None, sigma=None,

@@ -32,7 +32,9 @@
 #            DO 490, :411-1044) becomes two loops: the first runs :417-636, the
 #            albedos (:642-654 under the sun-up test, :712-720), records the
 #            column's radiation inputs and saves its state; after the batch the
-#            second restores each column and runs :637-1042 with the glue calls.
+#            second restores each column and runs :637-1042 with the glue calls;
+#            TEMPGR's DSOLFLUX loop (:311-351) records each column's ACOSZ and
+#            one call (mgcm_comp3_tempgr) forms every DNVFLUX.
 # The rewritten sources live in a temporary directory and are deleted; only the
 # linked programs land in oracle/_ref/.  Exits 0 without building when the
 # reference tree is absent.
@@ -108,8 +110,15 @@ grep -q -E '^ +ALBI += 1.0D0-EGOGND *$' <(sed -n 712p "$CODE/comp3.f") || { echo
 grep -q -E '^ +ENDIF *$' <(sed -n 720p "$CODE/comp3.f") || { echo "build_model: comp3.f:720" >&2; exit 1; }
 grep -q -E '^  490 +CONTINUE *$' <(sed -n 1043p "$CODE/comp3.f") || { echo "build_model: comp3.f:1043" >&2; exit 1; }
 grep -q -E '^  500 +CONTINUE *$' <(sed -n 1044p "$CODE/comp3.f") || { echo "build_model: comp3.f:1044" >&2; exit 1; }
+grep -q -E '^ +if\(acosz.ge.1.0e-5\) then *$' <(sed -n 335p "$CODE/comp3.f") || { echo "build_model: comp3.f:335" >&2; exit 1; }
+grep -q -E '^ +DNVFLUX\(J,I\) += DNDIFFV\(J,I\) \+ DIRECTSOL *$' <(sed -n 348p "$CODE/comp3.f") || { echo "build_model: comp3.f:348" >&2; exit 1; }
+grep -q -E '^ +END DO +! I-loop' <(sed -n 351p "$CODE/comp3.f") || { echo "build_model: comp3.f:351" >&2; exit 1; }
 {
-  sed -n '1,410p' "$CODE/comp3.f"
+  sed -n '1,334p' "$CODE/comp3.f"
+  echo "          call mgcm_comp3_tempgr_record(ACOSZ,J,I)"
+  sed -n '349,351p' "$CODE/comp3.f"
+  echo "      call mgcm_comp3_tempgr(SOL,DNVFLUX)"
+  sed -n '352,410p' "$CODE/comp3.f"
   echo "         call comp3_split_pass1_begin()"
   sed -n '411,636p' "$CODE/comp3.f"
   echo "               if(acosz.ge.1.0e-5) then"

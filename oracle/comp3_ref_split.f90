@@ -43,3 +43,24 @@ subroutine comp3_split_radiate()
     end do
   end do
 end subroutine comp3_split_radiate
+
+! COMP3's TEMPGR loop (code/comp3.f:335-348) over the recorded columns with the
+! reference's DSOLFLUX.
+subroutine mgcm_comp3_tempgr(sol, dnvflux)
+  use comp3_split_state
+  use radcommon_h, only: GWEIGHT, FZEROV, DETAU
+  use standard_h, only: DNDIFFV
+  implicit none
+  real*8, intent(in) :: sol(L_NSPECTV)
+  real*8, intent(inout) :: dnvflux(L_J, L_I)  ! COMP3's DNVFLUX
+  real*8 :: directsol
+  integer :: c
+  do c = 1, ntg
+    if (tg_acosz(c) .ge. 1.0e-5) then
+      call dsolflux(sol, tg_acosz(c), GWEIGHT, FZEROV, DETAU, tg_j(c), tg_i(c), directsol)
+    else
+      directsol = 0.0
+    end if
+    dnvflux(tg_j(c), tg_i(c)) = DNDIFFV(tg_j(c), tg_i(c)) + directsol
+  end do
+end subroutine mgcm_comp3_tempgr

@@ -73,3 +73,21 @@ def test_surface_outputs_of_the_run_calls():
     torch.cuda.synchronize()
     assert np.array_equal(outs[0]["directsol"].cpu().numpy(), sep["directsol"])
     assert np.array_equal(outs[0]["dnvflux"].cpu().numpy(), sep["dnvflux"])
+
+
+def test_dsolflux_host_equals_device_entry():
+    """mgcm_rt_dsolflux_host (host arrays: COMP3's per-step TEMPGR loop in one call)
+    gives mgcm_rt_dsolflux's bits, sun-down columns included."""
+    import torch
+    from mgcm_rt import Radiation, alloc_outputs, synth, to_device
+    t, c = synth.make_workload(81, 400, 24)
+    c["acosz"][::7] = 0.0
+    rt = Radiation(t)
+    dc, do = to_device(c), to_device(alloc_outputs(400, 24))
+    rt.run_device(dc, do)
+    dev = rt.dsolflux(dc, do)
+    torch.cuda.synchronize()
+    host = rt.dsolflux_host(c["acosz"], do["detau"].cpu().numpy(), do["diffvt"].cpu().numpy())
+    for k in ("directsol", "dnvflux"):
+        assert np.array_equal(host[k], dev[k].cpu().numpy()), k
+    assert not np.any(host["directsol"][::7])
