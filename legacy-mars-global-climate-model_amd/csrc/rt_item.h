@@ -4,6 +4,8 @@
 // Every FP64 statement follows the reference's evaluation order; files that
 // include this header are compiled with -ffp-contract=off.
 #pragma once
+#include <type_traits>
+
 #include "rt_internal.h"
 #include "rt_libm.h"
 
@@ -257,21 +259,32 @@ __device__ __forceinline__ VisLayer vis_layer(double w0, double cb, double u0, d
   return v;
 }
 
+// Compile-time buffer index of the layer loops below.  Each loop reads the
+// next layer's inputs one layer ahead into the other of two buffers and runs
+// two layers per trip, so no value is copied between buffers at the loop's
+// end (such a copy would wait there for the loads it moves).
+using Buf0 = std::integral_constant<int, 0>;
+using Buf1 = std::integral_constant<int, 1>;
+
 // VIS optics of one item + delta-Eddington scaling (OPTCV layer assembly
 // code/optcv.f90:189-274 fused with GETDETAU/GFLUXV code/gfluxv.f90:76-103).
 // Returns DETAU; cumtau receives TAUCUMV(L_LEVELS).
 template <int NLM, typename Src, typename LA>
 __device__ __forceinline__ double vis_optics(int nl, Src &src, const LA &la, VisState<NLM> &s, double &cumtau) {
   // sub-level inputs are fetched one layer ahead of their use
-  typename Src::Raw r2 = src.fetch(2), ra = src.fetch(3), rb = src.fetch(4);
+  typename Src::Raw r2 = src.fetch(2), ra[2], rb[2];
+  ra[0] = src.fetch(3);
+  rb[0] = src.fetch(4);
   // sub-level 2 (top half of the cap)
   const double d2 = src.finish(2, r2);
   double tcv = d2;  // TAUCUMV(2)
   double fprev = 0.0, tcv_prev_level = 0.0, tauL = 0.0;  // tauL = TAU(L), carried in a register
-  for (int L = 1; L <= nl - 1; L++) {
-    const double a = src.finish(2 * L + 1, ra), b = src.finish(2 * L + 2, rb);
-    ra = src.fetch(2 * L + 3);                   // L = nl-1: the bottom sub-level L_LEVELS
-    if (L < nl - 1) rb = src.fetch(2 * L + 4);
+  // layers 1..nl-1 (two sub-levels each)
+  auto mid = [&](auto buf, int L) {
+    constexpr int X = decltype(buf)::value;
+    const double a = src.finish(2 * L + 1, ra[X]), b = src.finish(2 * L + 2, rb[X]);
+    ra[1 - X] = src.fetch(2 * L + 3);                   // L = nl-1: the bottom sub-level L_LEVELS
+    if (L < nl - 1) rb[1 - X] = src.fetch(2 * L + 4);
     const double dtauv = a + b;
     const double tcA = tcv + a;  // TAUCUMV(2L+1)
     const double tcB = tcA + b;  // TAUCUMV(2L+2)
@@ -295,10 +308,12 @@ __device__ __forceinline__ double vis_optics(int nl, Src &src, const LA &la, Vis
     fprev = F;
     tcv_prev_level = tcB;  // TAUCUMV(2L+2), the base of the next layer's mid level
     tcv = tcB;
-  }
-  {
+  };
+  // the bottom layer nl (one sub-level)
+  auto bottom = [&](auto buf) {
+    constexpr int X = decltype(buf)::value;
     const int L = nl;
-    const double a = src.finish(2 * L + 1, ra);
+    const double a = src.finish(2 * L + 1, ra[X]);
     const double dtauv = a;
     cumtau = tcv + a;  // TAUCUMV(L_LEVELS)
     double Sc, C;
@@ -310,6 +325,17 @@ __device__ __forceinline__ double vis_optics(int nl, Src &src, const LA &la, Vis
     s.dh[L] = Pair{dtau, 0.0};  // the bottom layer has no HM
     tauL = tauL + dtau;
     s.tbot = tauL;
+  };
+  int L = 1;
+  for (; L + 1 <= nl - 1; L += 2) {
+    mid(Buf0{}, L);
+    mid(Buf1{}, L + 1);
+  }
+  if (L <= nl - 1) {
+    mid(Buf0{}, L);
+    bottom(Buf1{});
+  } else {
+    bottom(Buf0{});
   }
   return tauL;
 }
@@ -326,16 +352,16 @@ __device__ __forceinline__ void vis_solve(int nl, VisState<NLM> &s, const double
   // layer below, the same argument) are carried in registers.
   double gama_n1 = 0.0, cpm1_n1 = 0.0, cmm1_n1 = 0.0, as1 = 0.0, ds1 = 0.0;
   double xbot = pexp35<P>(-dmin(pdiv<P>(s.tbot, u0), MAXEXP));
-  // the next layer's state is loaded one iteration ahead
-  Pair wt_n = s.wt[nl];
-  double cb_n = cbar[nl - 1], dtau_n = s.dh[nl].x;
-  for (int n = nl; n >= 1; n--) {
-    const double w0c = wt_n.x, cbc = cb_n, tauc = wt_n.y, dtauc = dtau_n;
-    if (n > 1) {
-      wt_n = s.wt[n - 1];
-      cb_n = cbar[n - 2];
-      dtau_n = s.dh[n - 1].x;
-    }
+  // the next layer's state is loaded one layer ahead, into the other buffer
+  struct In {
+    Pair wt;
+    double cb, dtau;
+  } in[2];
+  in[0] = In{s.wt[nl], cbar[nl - 1], s.dh[nl].x};
+  auto elim = [&](auto buf, int n) {
+    constexpr int X = decltype(buf)::value;
+    const double w0c = in[X].wt.x, cbc = in[X].cb, tauc = in[X].wt.y, dtauc = in[X].dtau;
+    if (n > 1) in[1 - X] = In{s.wt[n - 1], cbar[n - 2], s.dh[n - 1].x};
     const VisLayer v = vis_layer<P>(w0c, cbc, u0, f0pi);
     const double lam = v.lam, gama = v.gama, am = v.am, ap = v.ap;
     const double xtop = pexp35<P>(-dmin(pdiv<P>(tauc, u0), MAXEXP));
@@ -385,27 +411,32 @@ __device__ __forceinline__ void vis_solve(int nl, VisState<NLM> &s, const double
     cpm1_n1 = cpm1;
     cmm1_n1 = cmm1;
     xbot = xtop;
+  };
+  int n = nl;
+  for (; n >= 2; n -= 2) {
+    elim(Buf0{}, n);
+    elim(Buf1{}, n - 1);
   }
-  // top-down: substitution, unmixing and the layer fluxes
-  Pair r1_n = s.row[1], r2_n = s.row[2];
-  double xkprev = r1_n.y;
+  if (n == 1) elim(Buf0{}, 1);
+  // top-down: substitution, unmixing and the layer fluxes.  The row pair,
+  // layer state and sub-level depth of the next layer are loaded one layer
+  // ahead, into the other buffer.
+  struct Sub {
+    Pair r1, r2;
+    double w0, cb, hm, tt;
+  } sub[2];
+  sub[0] = Sub{s.row[1], s.row[2], s.wt[1].x, cbar[0], s.dh[1].y, s.tm1};
+  double xkprev = sub[0].r1.y;
   double xk1_1 = 0.0, xk2_1 = 0.0, gama_1 = 0.0, am_1 = 0.0, ap_1 = 0.0;
-  // row pair, layer state and sub-level depth of the next layer, loaded one iteration ahead
-  double w0_n = s.wt[1].x, tt_n = s.tm1;
-  cb_n = cbar[0];
-  double hm_n = s.dh[1].y;
   const double tau_bot = s.tbot, tau_lastl = s.wt[nl].y;
-  for (int n = 1; n <= nl; n++) {
-    const double as1c = r1_n.x, ds1c = r1_n.y, as2c = r2_n.x, ds2c = r2_n.y;
-    const double w0c = w0_n, cbc = cb_n, hmc = hm_n, ttc = tt_n;
+  auto subst = [&](auto buf, int n) {
+    constexpr int X = decltype(buf)::value;
+    const double as1c = sub[X].r1.x, ds1c = sub[X].r1.y, as2c = sub[X].r2.x, ds2c = sub[X].r2.y;
+    const double w0c = sub[X].w0, cbc = sub[X].cb, hmc = sub[X].hm, ttc = sub[X].tt;
     if (n < nl) {
-      r1_n = s.row[2 * n + 1];
-      r2_n = s.row[2 * n + 2];
+      const Pair r1 = s.row[2 * n + 1], r2 = s.row[2 * n + 2];
       const Pair wt = s.wt[n + 1];
-      w0_n = wt.x;
-      tt_n = wt.y;
-      cb_n = cbar[n];
-      hm_n = s.dh[n + 1].y;
+      sub[1 - X] = Sub{r1, r2, wt.x, cbar[n], s.dh[n + 1].y, wt.y};
     }
     const double xa = (n == 1) ? ds1c : ds1c - as1c * xkprev;
     const double xb = ds2c - as2c * xa;
@@ -438,7 +469,12 @@ __device__ __forceinline__ void vis_solve(int nl, VisState<NLM> &s, const double
     }
     acc(n, fmidp, fmidm);
     if (n == 1) { xk1_1 = xk1; xk2_1 = xk2; gama_1 = gama; am_1 = am; ap_1 = ap; }
+  };
+  for (n = 1; n + 1 <= nl; n += 2) {
+    subst(Buf0{}, n);
+    subst(Buf1{}, n + 1);
   }
+  if (n == nl) subst(Buf0{}, nl);
   {  // top of the atmosphere (gfluxv.f90:210-238)
     const double ep = 1.0, em = 1.0;
     fluxup = xk1_1 * ep + gama_1 * xk2_1 * em + ap_1;
@@ -480,13 +516,16 @@ template <int NLM, typename Src, typename LA>
 __device__ __forceinline__ void ir_optics(int nl, Src &src, const LA &la, IrState<NLM> &s) {
   const double f1em9 = F32LIT(1.0e-9);
   // sub-level inputs are fetched one layer ahead of their use
-  typename Src::Raw r2 = src.fetch(2), ra = src.fetch(3), rb = src.fetch(4);
+  typename Src::Raw r2 = src.fetch(2), ra[2], rb[2];
+  ra[0] = src.fetch(3);
+  rb[0] = src.fetch(4);
   double tcv = src.finish(2, r2);  // TAUCUMI(2)
-  for (int L = 1; L <= nl; L++) {
-    const double a = src.finish(2 * L + 1, ra);
-    const double b = (L < nl) ? src.finish(2 * L + 2, rb) : 0.0;  // DTAUKI(L_LEVELS+1) = 0
-    if (L < nl) ra = src.fetch(2 * L + 3);
-    if (L < nl - 1) rb = src.fetch(2 * L + 4);
+  auto layer = [&](auto buf, int L) {
+    constexpr int X = decltype(buf)::value;
+    const double a = src.finish(2 * L + 1, ra[X]);
+    const double b = (L < nl) ? src.finish(2 * L + 2, rb[X]) : 0.0;  // DTAUKI(L_LEVELS+1) = 0
+    if (L < nl) ra[1 - X] = src.fetch(2 * L + 3);
+    if (L < nl - 1) rb[1 - X] = src.fetch(2 * L + 4);
     double dt = a + b + 1.e-50;
     const double sc = la.scat(L);  // every layer: a live provider keeps its COSB here
     double w;
@@ -500,7 +539,13 @@ __device__ __forceinline__ void ir_optics(int nl, Src &src, const LA &la, IrStat
     s.dmid[L] = tcA - tcv;       // TAUCUMI(2L+1) - TAUCUMI(2L)
     tcv = (L < nl) ? tcA + b : tcA;
     s.wd[L] = Pair{w, dt};
+  };
+  int L = 1;
+  for (; L + 1 <= nl; L += 2) {
+    layer(Buf0{}, L);
+    layer(Buf1{}, L + 1);
   }
+  if (L == nl) layer(Buf0{}, L);
 }
 
 // GFLUXI (code/gfluxi.f90:69-194) + DSOLVER/DTRIDGL.  bk(K) = Planck value of
@@ -510,17 +555,16 @@ __device__ __forceinline__ double ir_solve(int nl, IrState<NLM> &s, const double
                                            const double *__restrict__ bk, double rsf,
                            double btop, double bsurf, Acc &acc) {
   double gama_n1 = 0.0, cpm1_n1 = 0.0, cmm1_n1 = 0.0, as1 = 0.0, ds1 = 0.0;
-  // the next layer's state is loaded one iteration ahead
-  Pair wd_n = s.wd[nl];
-  double cb_n = cosb[nl - 1], bt_n = bk[2 * nl - 1], bb_n = bk[2 * nl];
-  for (int n = nl; n >= 1; n--) {
-    const double dtau = wd_n.y, w0c = wd_n.x, cbc = cb_n, btc = bt_n, bbc = bb_n;
-    if (n > 1) {
-      wd_n = s.wd[n - 1];
-      cb_n = cosb[n - 2];
-      bt_n = bk[2 * n - 3];
-      bb_n = bk[2 * n - 1];
-    }
+  // the next layer's state is loaded one layer ahead, into the other buffer
+  struct In {
+    Pair wd;
+    double cb, bt, bb;
+  } in[2];
+  in[0] = In{s.wd[nl], cosb[nl - 1], bk[2 * nl - 1], bk[2 * nl]};
+  auto elim = [&](auto buf, int n) {
+    constexpr int X = decltype(buf)::value;
+    const double dtau = in[X].wd.y, w0c = in[X].wd.x, cbc = in[X].cb, btc = in[X].bt, bbc = in[X].bb;
+    if (n > 1) in[1 - X] = In{s.wd[n - 1], cosb[n - 2], bk[2 * n - 3], bk[2 * n - 1]};
     // TLEV(2L), TLEV(2L+2); the bottom layer uses TLEV(2L), TLEV(2L+1)
     const IrLayer v = ir_layer<P>(w0c, cbc, dtau, btc, bbc);
     const double lam = v.lam, gama = v.gama, term = v.term, b0 = v.b0, b1 = v.b1;
@@ -571,27 +615,29 @@ __device__ __forceinline__ double ir_solve(int nl, IrState<NLM> &s, const double
     gama_n1 = gama;
     cpm1_n1 = cpm1;
     cmm1_n1 = cmm1;
+  };
+  int n = nl;
+  for (; n >= 2; n -= 2) {
+    elim(Buf0{}, n);
+    elim(Buf1{}, n - 1);
   }
-  Pair r1_n = s.row[1], r2_n = s.row[2];
-  double xkprev = r1_n.y;
+  if (n == 1) elim(Buf0{}, 1);
+  // top-down; the next layer's rows and state are loaded one layer ahead, into the other buffer
+  struct Sub {
+    Pair r1, r2, wd;
+    double cb, dm, bt, bb;
+  } sub[2];
+  sub[0] = Sub{s.row[1], s.row[2], s.wd[1], cosb[0], s.dmid[1], bk[1], (1 < nl) ? bk[3] : bk[2]};
+  double xkprev = sub[0].r1.y;
   double xk1_1 = 0.0, xk2_1 = 0.0, gama_1 = 0.0, b0_1 = 0.0, b1_1 = 0.0, term_1 = 0.0;
-  wd_n = s.wd[1];
-  cb_n = cosb[0];
-  bt_n = bk[1];
-  bb_n = (1 < nl) ? bk[3] : bk[2];
-  double dm_n = s.dmid[1];
-  for (int n = 1; n <= nl; n++) {
-    const double as1c = r1_n.x, ds1c = r1_n.y, as2c = r2_n.x, ds2c = r2_n.y;
-    const double w0c = wd_n.x, cbc = cb_n, dtc = wd_n.y, dmc = dm_n, btc = bt_n, bbc = bb_n;
-    if (n < nl) {
-      r1_n = s.row[2 * n + 1];
-      r2_n = s.row[2 * n + 2];
-      wd_n = s.wd[n + 1];
-      cb_n = cosb[n];
-      dm_n = s.dmid[n + 1];
-      bt_n = bk[2 * n + 1];
-      bb_n = (n + 1 < nl) ? bk[2 * n + 3] : bk[2 * n + 2];
-    }
+  auto subst = [&](auto buf, int n) {
+    constexpr int X = decltype(buf)::value;
+    const double as1c = sub[X].r1.x, ds1c = sub[X].r1.y, as2c = sub[X].r2.x, ds2c = sub[X].r2.y;
+    const double w0c = sub[X].wd.x, cbc = sub[X].cb, dtc = sub[X].wd.y, dmc = sub[X].dm, btc = sub[X].bt,
+                 bbc = sub[X].bb;
+    if (n < nl)
+      sub[1 - X] = Sub{s.row[2 * n + 1], s.row[2 * n + 2], s.wd[n + 1], cosb[n], s.dmid[n + 1], bk[2 * n + 1],
+                       (n + 1 < nl) ? bk[2 * n + 3] : bk[2 * n + 2]};
     const double xa = (n == 1) ? ds1c : ds1c - as1c * xkprev;
     const double xb = ds2c - as2c * xa;
     xkprev = xb;
@@ -611,7 +657,12 @@ __device__ __forceinline__ double ir_solve(int nl, IrState<NLM> &s, const double
     fmidm = fmidm * PI;
     acc(n, fmidp, fmidm);
     if (n == 1) { xk1_1 = xk1; xk2_1 = xk2; gama_1 = gama; b0_1 = b0; b1_1 = b1; term_1 = term; }
+  };
+  for (n = 1; n + 1 <= nl; n += 2) {
+    subst(Buf0{}, n);
+    subst(Buf1{}, n + 1);
   }
+  if (n == nl) subst(Buf0{}, nl);
   const double ep = 1.0, em = 1.0;
   const double cpmid = b0_1 + b1_1 * term_1;
   const double cmmid = b0_1 - b1_1 * term_1;
