@@ -315,7 +315,8 @@ struct Tile {
 // S: band slots per column (the tile); the block owns Tile<S>::C columns.
 // The whole radiation call of columns col0..col0+C-1 of grid a, by one wave.
 // The exp table must be in LDS; the staging after it is rewritten.
-template <int NLM, int P, int S>
+// D: how many layers ahead the solves load the item state (vis_solve).
+template <int NLM, int P, int S, int D = 1>
 __device__ __forceinline__ void column_block(const DevTables &dt, const RunArgs &a, int col0, double *smem) {
   constexpr bool T32 = (P & P_TAB32) != 0;
   using kt = typename std::conditional<T32, float, double>::type;
@@ -465,7 +466,7 @@ __device__ __forceinline__ void column_block(const DevTables &dt, const RunArgs 
         };
         double fluxup = 0.0, fluxdn = 0.0, diffv = 0.0;
         const double bsurf = a.als[colc] * u0 * a.sol[nw] * rt_exp35(-dmin(detau / u0, MAXEXP));
-        vis_solve<NLM, P>(nl, s, cbar, u0, a.sol[nw], a.als[colc], 0.0, bsurf, accf, fluxup, fluxdn, diffv);
+        vis_solve<NLM, P, D>(nl, s, cbar, u0, a.sol[nw], a.als[colc], 0.0, bsurf, accf, fluxup, fluxdn, diffv);
         topv = tile_sum<S>(topv, group16_sum(skip ? 0.0 : (fluxup - fluxdn) * gw * onemf));
         difv = tile_sum<S>(difv, group16_sum(skip ? 0.0 : diffv * gw * onemf));
       }
@@ -493,7 +494,7 @@ __device__ __forceinline__ void column_block(const DevTables &dt, const RunArgs 
         const double bsurf = (1. - rsf) * bk[nlev - 1];
         const double tautop = s.wd[1].y * plev[1] / (plev[3] - plev[1]);
         const double btop = (1.0 - rt_exp(-tautop * 2.0)) * bk[1];  // /UBARI, UBARI = 0.5
-        const double ftopup = ir_solve<NLM, P>(nl, s, cosb, bk, rsf, btop, bsurf, accf);
+        const double ftopup = ir_solve<NLM, P, D>(nl, s, cosb, bk, rsf, btop, bsurf, accf);
         topi = tile_sum<S>(topi, group16_sum(skip ? 0.0 : ftopup * dw * gw * onemf));
       }
     }
@@ -702,6 +703,10 @@ __global__ __launch_bounds__(64) void k_rt_clear_batch(DevTables dt, const RunAr
 // measures the one-wave instance 7-8% faster at 48 and 64 layers and the
 // two-wave one 2-16% faster at 16..36 layers).
 __host__ __device__ constexpr int column_waves(int nlm) { return nlm <= 37 ? MGCM_FP64_WAVES : 1; }
+// Layers ahead that the solves load the item state: two at one wave per SIMD
+// (its 512 registers hold the third buffer; 4-6% faster from 48 layers up),
+// one at two waves (a third buffer spills there).
+__host__ __device__ constexpr int state_ahead(int waves) { return waves == 1 ? 2 : 1; }
 
 // One grid: block b owns columns Cb..Cb+C-1 (C = 4/S).
 template <int NLM, int P, int S>
@@ -710,14 +715,14 @@ void k_rt(DevTables dt, RunArgs a) {
   extern __shared__ double smem[];
   mgcm_libm::load_lds_exptab(threadIdx.x, 64);
   __syncthreads();
-  column_block<NLM, P, S>(dt, a, blockIdx.x * Tile<S>::C, smem);
+  column_block<NLM, P, S, state_ahead(column_waves(NLM))>(dt, a, blockIdx.x * Tile<S>::C, smem);
 }
 
 // Many grids of one layer-count class: a persistent grid of blocks walks the
 // work list (grid index, first column of 4) round-robin, largest layer counts
 // first; with S band slots per column each item is S sub-items of 4/S columns.
 // No counters: every block's share is fixed by its index.
-template <int NLM, int P, int S>
+template <int NLM, int P, int S, int D>
 __device__ __forceinline__ void batch_items(const DevTables &dt, const RunArgs *__restrict__ grids,
                                             const uint2 *__restrict__ work, int nwork, double *smem) {
   mgcm_libm::load_lds_exptab(threadIdx.x, 64);
@@ -725,7 +730,7 @@ __device__ __forceinline__ void batch_items(const DevTables &dt, const RunArgs *
     __syncthreads();  // the previous item's reads of the staging are done
     const uint2 wk = work[v / S];
     const RunArgs a = grids[wk.x];
-    column_block<NLM, P, S>(dt, a, (int)wk.y + (v % S) * Tile<S>::C, smem);
+    column_block<NLM, P, S, D>(dt, a, (int)wk.y + (v % S) * Tile<S>::C, smem);
   }
 }
 // The instances the batched tuner chooses between per layer class: two waves
@@ -734,7 +739,7 @@ template <int NLM, int P, int S>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MGCM_FP64_WAVES)))
 void k_rt_batch(DevTables dt, const RunArgs *__restrict__ grids, const uint2 *__restrict__ work, int nwork) {
   extern __shared__ double smem[];
-  batch_items<NLM, P, S>(dt, grids, work, nwork, smem);
+  batch_items<NLM, P, S, state_ahead(MGCM_FP64_WAVES)>(dt, grids, work, nwork, smem);
 }
 // ... and one wave per SIMD (512 registers per lane: fewer spills of the
 // item state, half the waves in flight), one band slot per column.
@@ -742,7 +747,7 @@ template <int NLM, int P>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
 void k_rt_batch_w1(DevTables dt, const RunArgs *__restrict__ grids, const uint2 *__restrict__ work, int nwork) {
   extern __shared__ double smem[];
-  batch_items<NLM, P, 1>(dt, grids, work, nwork, smem);
+  batch_items<NLM, P, 1, state_ahead(1)>(dt, grids, work, nwork, smem);
 }
 
 template <int NLM, int P, int S>

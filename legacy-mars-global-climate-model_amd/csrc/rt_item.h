@@ -259,12 +259,50 @@ __device__ __forceinline__ VisLayer vis_layer(double w0, double cb, double u0, d
   return v;
 }
 
-// Compile-time buffer index of the layer loops below.  Each loop reads the
-// next layer's inputs one layer ahead into the other of two buffers and runs
-// two layers per trip, so no value is copied between buffers at the loop's
-// end (such a copy would wait there for the loads it moves).
+// Compile-time buffer index of the layer loops below.  Each loop reads a
+// later layer's inputs ahead of their use into its own buffer and runs as many
+// layers per trip as it has buffers, so no value is copied between buffers at
+// the loop's end (such a copy would wait there for the loads it moves).
 using Buf0 = std::integral_constant<int, 0>;
 using Buf1 = std::integral_constant<int, 1>;
+
+// f(buffer k, n - k) for k = 0..NB-1: NB layers of a downward loop from n ...
+template <typename F, int... K>
+__device__ __forceinline__ void layers_down(F &f, int n, std::integer_sequence<int, K...>) {
+  (f(std::integral_constant<int, K>{}, n - K), ...);
+}
+// ... and f(buffer k, n + k) for an upward one.
+template <typename F, int... K>
+__device__ __forceinline__ void layers_up(F &f, int n, std::integer_sequence<int, K...>) {
+  (f(std::integral_constant<int, K>{}, n + K), ...);
+}
+// A downward loop over layers nl..1 in trips of NB layers, then the last
+// nl % NB layers.
+template <int NB, typename F>
+__device__ __forceinline__ void sweep_down(F &f, int nl) {
+  int n = nl;
+  for (; n >= NB; n -= NB) layers_down(f, n, std::make_integer_sequence<int, NB>{});
+  if constexpr (NB > 2) {
+    if (n == 2) {
+      layers_down(f, 2, std::make_integer_sequence<int, 2>{});
+      return;
+    }
+  }
+  if (n == 1) layers_down(f, 1, std::make_integer_sequence<int, 1>{});
+}
+// An upward loop over layers 1..nl in trips of NB layers, then the rest.
+template <int NB, typename F>
+__device__ __forceinline__ void sweep_up(F &f, int nl) {
+  int n = 1;
+  for (; n + NB - 1 <= nl; n += NB) layers_up(f, n, std::make_integer_sequence<int, NB>{});
+  if constexpr (NB > 2) {
+    if (n + 1 == nl) {
+      layers_up(f, n, std::make_integer_sequence<int, 2>{});
+      return;
+    }
+  }
+  if (n == nl) layers_up(f, n, std::make_integer_sequence<int, 1>{});
+}
 
 // VIS optics of one item + delta-Eddington scaling (OPTCV layer assembly
 // code/optcv.f90:189-274 fused with GETDETAU/GFLUXV code/gfluxv.f90:76-103).
@@ -343,7 +381,9 @@ __device__ __forceinline__ double vis_optics(int nl, Src &src, const LA &la, Vis
 // GFLUXV (code/gfluxv.f90:108-280) + DSOLVER/DTRIDGL (code/dsolver.f90,
 // code/dtridgl.f90) on a scaled item.  Calls acc(L, fmidp, fmidm) for each
 // layer mid-point and returns the top fluxes and DIFFV.
-template <int NLM, int P = 0, typename Acc>
+// D: how many layers ahead the two passes load the item state (1 or 2; two
+// suits one wave per SIMD, whose 512 registers hold the third buffer).
+template <int NLM, int P = 0, int D = 1, typename Acc>
 __device__ __forceinline__ void vis_solve(int nl, VisState<NLM> &s, const double *__restrict__ cbar, double u0,
                                           double f0pi, double rsf, double btop,
                           double bsurf, Acc &acc, double &fluxup, double &fluxdn, double &diffv) {
@@ -352,16 +392,20 @@ __device__ __forceinline__ void vis_solve(int nl, VisState<NLM> &s, const double
   // layer below, the same argument) are carried in registers.
   double gama_n1 = 0.0, cpm1_n1 = 0.0, cmm1_n1 = 0.0, as1 = 0.0, ds1 = 0.0;
   double xbot = pexp35<P>(-dmin(pdiv<P>(s.tbot, u0), MAXEXP));
-  // the next layer's state is loaded one layer ahead, into the other buffer
+  // layer n - D's state is loaded while layer n is eliminated, into its own buffer
+  constexpr int NBUF = D + 1;
   struct In {
     Pair wt;
     double cb, dtau;
-  } in[2];
-  in[0] = In{s.wt[nl], cbar[nl - 1], s.dh[nl].x};
+  } in[NBUF];
+  auto ld = [&](int m) { return In{s.wt[m], cbar[m - 1], s.dh[m].x}; };
+#pragma unroll
+  for (int j = 0; j < D; j++)
+    if (nl - j >= 1) in[j] = ld(nl - j);
   auto elim = [&](auto buf, int n) {
     constexpr int X = decltype(buf)::value;
     const double w0c = in[X].wt.x, cbc = in[X].cb, tauc = in[X].wt.y, dtauc = in[X].dtau;
-    if (n > 1) in[1 - X] = In{s.wt[n - 1], cbar[n - 2], s.dh[n - 1].x};
+    if (n - D >= 1) in[(X + D) % NBUF] = ld(n - D);
     const VisLayer v = vis_layer<P>(w0c, cbc, u0, f0pi);
     const double lam = v.lam, gama = v.gama, am = v.am, ap = v.ap;
     const double xtop = pexp35<P>(-dmin(pdiv<P>(tauc, u0), MAXEXP));
@@ -412,20 +456,23 @@ __device__ __forceinline__ void vis_solve(int nl, VisState<NLM> &s, const double
     cmm1_n1 = cmm1;
     xbot = xtop;
   };
-  int n = nl;
-  for (; n >= 2; n -= 2) {
-    elim(Buf0{}, n);
-    elim(Buf1{}, n - 1);
-  }
-  if (n == 1) elim(Buf0{}, 1);
+  sweep_down<NBUF>(elim, nl);
   // top-down: substitution, unmixing and the layer fluxes.  The row pair,
-  // layer state and sub-level depth of the next layer are loaded one layer
-  // ahead, into the other buffer.
+  // layer state and sub-level depth of layer n + D are loaded while layer n is
+  // solved, into its own buffer.
   struct Sub {
     Pair r1, r2;
     double w0, cb, hm, tt;
-  } sub[2];
+  } sub[NBUF];
+  auto lds = [&](int m) {  // m >= 2
+    const Pair r1 = s.row[2 * m - 1], r2 = s.row[2 * m];
+    const Pair wt = s.wt[m];
+    return Sub{r1, r2, wt.x, cbar[m - 1], s.dh[m].y, wt.y};
+  };
   sub[0] = Sub{s.row[1], s.row[2], s.wt[1].x, cbar[0], s.dh[1].y, s.tm1};
+#pragma unroll
+  for (int j = 1; j < D; j++)
+    if (1 + j <= nl) sub[j] = lds(1 + j);
   double xkprev = sub[0].r1.y;
   double xk1_1 = 0.0, xk2_1 = 0.0, gama_1 = 0.0, am_1 = 0.0, ap_1 = 0.0;
   const double tau_bot = s.tbot, tau_lastl = s.wt[nl].y;
@@ -433,11 +480,7 @@ __device__ __forceinline__ void vis_solve(int nl, VisState<NLM> &s, const double
     constexpr int X = decltype(buf)::value;
     const double as1c = sub[X].r1.x, ds1c = sub[X].r1.y, as2c = sub[X].r2.x, ds2c = sub[X].r2.y;
     const double w0c = sub[X].w0, cbc = sub[X].cb, hmc = sub[X].hm, ttc = sub[X].tt;
-    if (n < nl) {
-      const Pair r1 = s.row[2 * n + 1], r2 = s.row[2 * n + 2];
-      const Pair wt = s.wt[n + 1];
-      sub[1 - X] = Sub{r1, r2, wt.x, cbar[n], s.dh[n + 1].y, wt.y};
-    }
+    if (n + D <= nl) sub[(X + D) % NBUF] = lds(n + D);
     const double xa = (n == 1) ? ds1c : ds1c - as1c * xkprev;
     const double xb = ds2c - as2c * xa;
     xkprev = xb;
@@ -470,11 +513,7 @@ __device__ __forceinline__ void vis_solve(int nl, VisState<NLM> &s, const double
     acc(n, fmidp, fmidm);
     if (n == 1) { xk1_1 = xk1; xk2_1 = xk2; gama_1 = gama; am_1 = am; ap_1 = ap; }
   };
-  for (n = 1; n + 1 <= nl; n += 2) {
-    subst(Buf0{}, n);
-    subst(Buf1{}, n + 1);
-  }
-  if (n == nl) subst(Buf0{}, nl);
+  sweep_up<NBUF>(subst, nl);
   {  // top of the atmosphere (gfluxv.f90:210-238)
     const double ep = 1.0, em = 1.0;
     fluxup = xk1_1 * ep + gama_1 * xk2_1 * em + ap_1;
@@ -550,22 +589,27 @@ __device__ __forceinline__ void ir_optics(int nl, Src &src, const LA &la, IrStat
 
 // GFLUXI (code/gfluxi.f90:69-194) + DSOLVER/DTRIDGL.  bk(K) = Planck value of
 // TLEV(K).  Calls acc(L, fmidp, fmidm); returns FTOPUP.
-template <int NLM, int P = 0, typename Acc>
+// D: as in vis_solve.
+template <int NLM, int P = 0, int D = 1, typename Acc>
 __device__ __forceinline__ double ir_solve(int nl, IrState<NLM> &s, const double *__restrict__ cosb,
                                            const double *__restrict__ bk, double rsf,
                            double btop, double bsurf, Acc &acc) {
   double gama_n1 = 0.0, cpm1_n1 = 0.0, cmm1_n1 = 0.0, as1 = 0.0, ds1 = 0.0;
-  // the next layer's state is loaded one layer ahead, into the other buffer
+  // layer n - D's state is loaded while layer n is eliminated, into its own buffer
+  constexpr int NBUF = D + 1;
   struct In {
     Pair wd;
     double cb, bt, bb;
-  } in[2];
-  in[0] = In{s.wd[nl], cosb[nl - 1], bk[2 * nl - 1], bk[2 * nl]};
+  } in[NBUF];
+  // TLEV(2L), TLEV(2L+2); the bottom layer uses TLEV(2L), TLEV(2L+1)
+  auto ld = [&](int m) { return In{s.wd[m], cosb[m - 1], bk[2 * m - 1], m < nl ? bk[2 * m + 1] : bk[2 * m]}; };
+#pragma unroll
+  for (int j = 0; j < D; j++)
+    if (nl - j >= 1) in[j] = ld(nl - j);
   auto elim = [&](auto buf, int n) {
     constexpr int X = decltype(buf)::value;
     const double dtau = in[X].wd.y, w0c = in[X].wd.x, cbc = in[X].cb, btc = in[X].bt, bbc = in[X].bb;
-    if (n > 1) in[1 - X] = In{s.wd[n - 1], cosb[n - 2], bk[2 * n - 3], bk[2 * n - 1]};
-    // TLEV(2L), TLEV(2L+2); the bottom layer uses TLEV(2L), TLEV(2L+1)
+    if (n - D >= 1) in[(X + D) % NBUF] = ld(n - D);
     const IrLayer v = ir_layer<P>(w0c, cbc, dtau, btc, bbc);
     const double lam = v.lam, gama = v.gama, term = v.term, b0 = v.b0, b1 = v.b1;
     const double cp = b0 + b1 * dtau + b1 * term;
@@ -616,18 +660,19 @@ __device__ __forceinline__ double ir_solve(int nl, IrState<NLM> &s, const double
     cpm1_n1 = cpm1;
     cmm1_n1 = cmm1;
   };
-  int n = nl;
-  for (; n >= 2; n -= 2) {
-    elim(Buf0{}, n);
-    elim(Buf1{}, n - 1);
-  }
-  if (n == 1) elim(Buf0{}, 1);
-  // top-down; the next layer's rows and state are loaded one layer ahead, into the other buffer
+  sweep_down<NBUF>(elim, nl);
+  // top-down; layer n + D's rows and state are loaded while layer n is solved, into its own buffer
   struct Sub {
     Pair r1, r2, wd;
     double cb, dm, bt, bb;
-  } sub[2];
-  sub[0] = Sub{s.row[1], s.row[2], s.wd[1], cosb[0], s.dmid[1], bk[1], (1 < nl) ? bk[3] : bk[2]};
+  } sub[NBUF];
+  auto lds = [&](int m) {
+    return Sub{s.row[2 * m - 1], s.row[2 * m], s.wd[m], cosb[m - 1], s.dmid[m], bk[2 * m - 1],
+               m < nl ? bk[2 * m + 1] : bk[2 * m]};
+  };
+#pragma unroll
+  for (int j = 0; j < D; j++)
+    if (1 + j <= nl) sub[j] = lds(1 + j);
   double xkprev = sub[0].r1.y;
   double xk1_1 = 0.0, xk2_1 = 0.0, gama_1 = 0.0, b0_1 = 0.0, b1_1 = 0.0, term_1 = 0.0;
   auto subst = [&](auto buf, int n) {
@@ -635,9 +680,7 @@ __device__ __forceinline__ double ir_solve(int nl, IrState<NLM> &s, const double
     const double as1c = sub[X].r1.x, ds1c = sub[X].r1.y, as2c = sub[X].r2.x, ds2c = sub[X].r2.y;
     const double w0c = sub[X].wd.x, cbc = sub[X].cb, dtc = sub[X].wd.y, dmc = sub[X].dm, btc = sub[X].bt,
                  bbc = sub[X].bb;
-    if (n < nl)
-      sub[1 - X] = Sub{s.row[2 * n + 1], s.row[2 * n + 2], s.wd[n + 1], cosb[n], s.dmid[n + 1], bk[2 * n + 1],
-                       (n + 1 < nl) ? bk[2 * n + 3] : bk[2 * n + 2]};
+    if (n + D <= nl) sub[(X + D) % NBUF] = lds(n + D);
     const double xa = (n == 1) ? ds1c : ds1c - as1c * xkprev;
     const double xb = ds2c - as2c * xa;
     xkprev = xb;
@@ -658,11 +701,7 @@ __device__ __forceinline__ double ir_solve(int nl, IrState<NLM> &s, const double
     acc(n, fmidp, fmidm);
     if (n == 1) { xk1_1 = xk1; xk2_1 = xk2; gama_1 = gama; b0_1 = b0; b1_1 = b1; term_1 = term; }
   };
-  for (n = 1; n + 1 <= nl; n += 2) {
-    subst(Buf0{}, n);
-    subst(Buf1{}, n + 1);
-  }
-  if (n == nl) subst(Buf0{}, nl);
+  sweep_up<NBUF>(subst, nl);
   const double ep = 1.0, em = 1.0;
   const double cpmid = b0_1 + b1_1 * term_1;
   const double cmmid = b0_1 - b1_1 * term_1;
