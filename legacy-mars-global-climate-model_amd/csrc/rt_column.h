@@ -202,19 +202,56 @@ __device__ __forceinline__ bool below_tlimit(const Src &src, const double *tab64
 }
 
 // Continuum-only point: sub-level optical depths straight from the column's
-// profiles (code/optcv.f90:172-173, code/optci.f90:173-174).
+// profiles (code/optcv.f90:172-173, code/optci.f90:173-174).  A sub-level's
+// profile values are fetched a layer ahead, as the gas points' table corners
+// are; the aerosol depths of the last odd and even sub-level are kept for the
+// layer optics (ClearLayerAer), which would otherwise compute them again.
 template <bool VIS>
 struct ClearSrc {
   const Aer &ae;
   int b;
   const double *plev, *tauref, *taurefcld;
   double taugsurf;
-  struct Raw {};
-  __device__ __forceinline__ Raw fetch(int) const { return Raw{}; }
-  __device__ __forceinline__ double finish(int K, const Raw &) {
-    const SubAer s = sub_aerosol<VIS>(ae, b, plev, tauref, taurefcld, K);
+  SubAer last[2];  // [K & 1]
+  struct Raw {
+    double pk, pk1, tr, tc;  // PLEV(K), PLEV(K-1), TAUREF(K), TAUREFCLD(K)
+  };
+  __device__ __forceinline__ Raw fetch(int K) const {
+    return Raw{plev[K - 1], plev[K - 2], tauref[K - 1], taurefcld[K - 1]};
+  }
+  __device__ __forceinline__ double finish(int K, const Raw &r) {
+    const SubAer s = sub_aerosol_of<VIS>(ae, b, K, r.pk, r.pk1, r.tr, r.tc);
+    last[K & 1] = s;
     if (VIS) return s.taeros + s.tray + s.tcloud;
     return s.taeros + s.tcloud;
+  }
+};
+
+// Layer aerosol optics of a continuum point from the sub-levels 2L+1 and 2L+2
+// its source has just finished (layer_aerosol's values), keeping what the
+// solve needs later: COSBAR (VIS) or COSB (IR) of each layer in keep[L-1].
+template <bool VIS>
+struct ClearLayerAer {
+  const ClearSrc<VIS> &src;
+  int nl;
+  double *keep;
+  __device__ __forceinline__ void get(int L, double &scat, double &cosb) const {
+    double tlk2 = 0.0, tck2 = 0.0, tray2 = 0.0;
+    if (L < nl) {
+      const SubAer &a2 = src.last[0];  // K = 2L+2
+      tlk2 = a2.tlk;
+      tck2 = a2.tck;
+      tray2 = a2.tray;
+    } else if (!VIS) {
+      ir_pad(src.ae, src.b, src.tauref, src.taurefcld, 2 * nl + 1, tlk2, tck2);
+    }
+    layer_aerosol_of<VIS>(src.ae, src.b, nl, L, src.last[1], tlk2, tck2, tray2, scat, cosb);
+    keep[L - 1] = VIS ? cosb / (1.0 + cosb) : cosb;  // COSBAR of code/gfluxv.f90:86
+  }
+  __device__ __forceinline__ double scat(int L) const {
+    double sc, cb;
+    get(L, sc, cb);
+    return sc;
   }
 };
 
@@ -631,7 +668,6 @@ __device__ __forceinline__ void clear_block(const DevTables &dt, const RunArgs &
   // layer aerosol optics computed as the optics pass reaches each layer; the
   // solve's COSBAR (VIS) or COSB (IR) kept per layer
   double keep[NLM];
-  const LiveLayerAer<VIS> la{ae, b, nl, pl, tr, tc, keep};
   auto accf = [&](int n, double fp, double fm) {
     rec[(n - 1) * NB] = fp;
     rec[(nl + n - 1) * NB] = fm;
@@ -640,6 +676,7 @@ __device__ __forceinline__ void clear_block(const DevTables &dt, const RunArgs &
     const double mu = a.acosz[col];
     if (!(mu >= F32LIT(1.0e-5))) return;  // code/comp3.f:637
     ClearSrc<true> src{ae, b, pl, tr, tc, 0.0};
+    const ClearLayerAer<true> la{src, nl, keep};
     VisState<NLM> s;
     double cumtau = 0.0;
     const double detau = vis_optics<NLM>(nl, src, la, s, cumtau);
@@ -653,6 +690,7 @@ __device__ __forceinline__ void clear_block(const DevTables &dt, const RunArgs &
   } else {
     const int nw = b - NV;
     ClearSrc<false> src{ae, b, pl, tr, tc, 0.0};
+    const ClearLayerAer<false> la{src, nl, keep};
     IrState<NLM> s;
     ir_optics<NLM>(nl, src, la, s);
     double bk[2 * NLM + 1];

@@ -103,20 +103,26 @@ struct SubAer {
   double tray, taeros, tcloud, tlk, tck;
 };
 
+// ... from the sub-level's profile values: PLEV(K), PLEV(K-1), TAUREF(K), TAUREFCLD(K)
 template <bool VIS>
-__device__ __forceinline__ SubAer sub_aerosol(const Aer &ae, int b, const double *__restrict__ plev,
-                                              const double *__restrict__ tauref,
-                                              const double *__restrict__ taurefcld, int K) {
+__device__ __forceinline__ SubAer sub_aerosol_of(const Aer &ae, int b, int K, double pk, double pk1, double trk,
+                                                 double tck) {
   SubAer s;
-  const double dpr = plev[K - 1] - plev[K - 2];
-  const double tr = tauref[K - 1] / (VIS ? ae.qref_vis(K) : ae.qref_dst(K));
-  const double tc = taurefcld[K - 1] / ae.qref_cld(K);
+  const double dpr = pk - pk1;
+  const double tr = trk / (VIS ? ae.qref_vis(K) : ae.qref_dst(K));
+  const double tc = tck / ae.qref_cld(K);
   s.tray = VIS ? ae.rc->tauray[b] * dpr : 0.0;
   s.taeros = tr * ae.qx_dst(b, K);
   s.tcloud = tc * ae.qx_cld(b, K);
   s.tlk = tr * ae.qs_dst(b, K);
   s.tck = tc * ae.qs_cld(b, K);
   return s;
+}
+template <bool VIS>
+__device__ __forceinline__ SubAer sub_aerosol(const Aer &ae, int b, const double *__restrict__ plev,
+                                              const double *__restrict__ tauref,
+                                              const double *__restrict__ taurefcld, int K) {
+  return sub_aerosol_of<VIS>(ae, b, K, plev[K - 1], plev[K - 2], tauref[K - 1], taurefcld[K - 1]);
 }
 
 // IR padding level L_LEVELS+1 (code/optci.f90:89-95,182-187): TAUREF there is
@@ -129,19 +135,19 @@ __device__ __forceinline__ void ir_pad(const Aer &ae, int b, const double *__res
 }
 
 // Layer scattering optical depth and asymmetry of the continuum channel:
-// VIS code/optcv.f90:194-222, IR code/optci.f90:195-219.
+// VIS code/optcv.f90:194-222, IR code/optci.f90:195-219, from the aerosol
+// depths a of sub-level K = 2L+1 and a2 of K+1 (L < NL; for the bottom IR
+// layer a2 is the padding level's, ir_pad).
 template <bool VIS>
-__device__ __forceinline__ void layer_aerosol(const Aer &ae, int b, const double *__restrict__ plev,
-                              const double *__restrict__ tauref, const double *__restrict__ taurefcld,
-                              int nl, int L, double &scat, double &cosb) {
-  const int nlev = 2 * nl + 1;
+__device__ __forceinline__ void layer_aerosol_of(const Aer &ae, int b, int nl, int L, const SubAer &a,
+                                                 double tlk2, double tck2, double tray2, double &scat,
+                                                 double &cosb) {
   const int K = 2 * L + 1;
   const double gd = ae.g_dst(b, K), gc = ae.g_cld(b, K);
   if (VIS) {
     const double f9999 = F32LIT(0.9999);
-    const SubAer a = sub_aerosol<true>(ae, b, plev, tauref, taurefcld, K);
     if (L < nl) {
-      const SubAer a2 = sub_aerosol<true>(ae, b, plev, tauref, taurefcld, K + 1);
+      const SubAer a2{tray2, 0.0, 0.0, tlk2, tck2};
       const double gd2 = ae.g_dst(b, K + 1), gc2 = ae.g_cld(b, K + 1);
       cosb = (gd * a.tlk + gd2 * a2.tlk + gc * a.tck + gc2 * a2.tck) /
              (a.tray + a2.tray + a.tlk + a2.tlk + a.tck + a2.tck);
@@ -151,23 +157,34 @@ __device__ __forceinline__ void layer_aerosol(const Aer &ae, int b, const double
       scat = a.tlk + a.tck + a.tray * f9999;
     }
   } else {
-    const SubAer a = sub_aerosol<false>(ae, b, plev, tauref, taurefcld, K);
-    double tlk2, tck2;
-    if (K + 1 <= nlev) {
-      const SubAer a2 = sub_aerosol<false>(ae, b, plev, tauref, taurefcld, K + 1);
-      tlk2 = a2.tlk;
-      tck2 = a2.tck;
-    } else {
-      ir_pad(ae, b, tauref, taurefcld, nlev, tlk2, tck2);
-    }
     const double gd2 = ae.g_dst(b, K + 1), gc2 = ae.g_cld(b, K + 1);
     scat = a.tlk + tlk2 + a.tck + tck2;
     cosb = (scat > 0.0) ? (gd * a.tlk + gd2 * tlk2 + gc * a.tck + gc2 * tck2) / scat : 0.0;
   }
 }
+// ... from the column's profiles
+template <bool VIS>
+__device__ __forceinline__ void layer_aerosol(const Aer &ae, int b, const double *__restrict__ plev,
+                              const double *__restrict__ tauref, const double *__restrict__ taurefcld,
+                              int nl, int L, double &scat, double &cosb) {
+  const int nlev = 2 * nl + 1;
+  const int K = 2 * L + 1;
+  const SubAer a = sub_aerosol<VIS>(ae, b, plev, tauref, taurefcld, K);
+  double tlk2 = 0.0, tck2 = 0.0, tray2 = 0.0;
+  if (K + 1 <= nlev) {  // L < NL
+    const SubAer a2 = sub_aerosol<VIS>(ae, b, plev, tauref, taurefcld, K + 1);
+    tlk2 = a2.tlk;
+    tck2 = a2.tck;
+    tray2 = a2.tray;
+  } else if (!VIS) {
+    ir_pad(ae, b, tauref, taurefcld, nlev, tlk2, tck2);
+  }
+  layer_aerosol_of<VIS>(ae, b, nl, L, a, tlk2, tck2, tray2, scat, cosb);
+}
 
 // Layer scattering optical depth and asymmetry (SCAT, COSB of layer L) as the
-// optics pass reads them: from arrays staged per column ...
+// optics pass reads them, from arrays staged per column (the continuum points
+// compute them as the pass reaches each layer: ClearLayerAer).
 struct StagedLayerAer {
   const double *sc, *cb;
   __device__ __forceinline__ void get(int L, double &scat, double &cosb) const {
@@ -177,24 +194,6 @@ struct StagedLayerAer {
   __device__ __forceinline__ double scat(int L) const { return sc[L - 1]; }
 };
 
-// ... or computed when the pass reaches the layer, keeping what the solve
-// needs later: COSBAR (VIS) or COSB (IR) of each layer in keep[L-1].
-template <bool VIS>
-struct LiveLayerAer {
-  const Aer &ae;
-  int b, nl;
-  const double *plev, *tauref, *taurefcld;
-  double *keep;
-  __device__ __forceinline__ void get(int L, double &scat, double &cosb) const {
-    layer_aerosol<VIS>(ae, b, plev, tauref, taurefcld, nl, L, scat, cosb);
-    keep[L - 1] = VIS ? cosb / (1.0 + cosb) : cosb;  // COSBAR of code/gfluxv.f90:86
-  }
-  __device__ __forceinline__ double scat(int L) const {
-    double sc, cb;
-    get(L, sc, cb);
-    return sc;
-  }
-};
 
 // ---------------------------------------------------------------------------
 // One two-stream item.  NLM bounds the radiation layer count (arrays are
