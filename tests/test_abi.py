@@ -1,5 +1,5 @@
 """The C ABI library: it loads without a GPU and exports every function that
-include/mgcm_rt.h declares; status strings; argument checks that need no device."""
+include/mgcm_rt.h and include/mgcm_sg.h declare; status strings; argument checks that need no device."""
 from __future__ import annotations
 
 import ctypes
@@ -11,12 +11,16 @@ import pytest
 from conftest import ROOT
 
 HEADER = os.path.join(ROOT, "include", "mgcm_rt.h")
+HEADERS = (HEADER, os.path.join(ROOT, "include", "mgcm_sg.h"))
 
 
 def declared_functions() -> list:
-    text = open(HEADER).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(mgcm_rt_\w+)\s*\(", text)))
+    """Every function include/mgcm_rt.h and include/mgcm_sg.h declare."""
+    names = set()
+    for h in HEADERS:
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        names |= set(re.findall(r"\b(mgcm_(?:rt|sg)_\w+)\s*\(", text))
+    return sorted(names)
 
 
 def test_header_declares_entry_points():
