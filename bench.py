@@ -82,11 +82,13 @@ def _sample(cols, n):
     return sub
 
 
-def cpu_baseline(seed, cols, nsample):
+def cpu_baseline(seed, cols, nsample, keep=None):
     """The reference's own Fortran (oracle/_ref, built from its sources) on one core.
 
     It runs with the reference's single GWEIGHT set (one SFLUXV/SFLUXI per
     column, its native cost); the GPU time does not depend on the weights.
+    With keep (a dict), its tables, columns and outputs are stored there for
+    the parity check against the device on the same inputs.
     """
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import Reference
@@ -98,8 +100,10 @@ def cpu_baseline(seed, cols, nsample):
     ref.run_grid(t, _sample(cols, 64))
     sub = _sample(cols, nsample)
     t0 = time.perf_counter()
-    ref.run_grid(t, sub)
+    out = ref.run_grid(t, sub)
     dt = time.perf_counter() - t0
+    if keep is not None:
+        keep.update(tables=t, cols=sub, out=out)
     return {"value": sub["ncol"] / dt, "unit": "columns/s", "cores": 1, "kind": "reference",
             "sample": f"first {sub['ncol']} of the {cols['ncol']} seeded columns, {int(cols['nlay'])} layers, "
                       f"reference OPTCV/SFLUXV/OPTCI/SFLUXI (amdflang -O2 build), one core, {dt:.2f} s wall"}
@@ -720,7 +724,19 @@ def main():
 
     if not args.no_cpu:
         try:
-            result["cpu_baseline"] = cpu_baseline(args.seed, cols, args.cpu_sample)
+            refrun = {}
+            result["cpu_baseline"] = cpu_baseline(args.seed, cols, args.cpu_sample, keep=refrun)
+            # the device on the reference's own inputs (its Gauss weights), against its outputs
+            rr = Radiation(refrun["tables"])
+            dr = to_device(alloc_outputs(int(refrun["cols"]["ncol"]), args.nlay))
+            rr.run_device(to_device(refrun["cols"]), dr, FP64, sh)
+            torch.cuda.synchronize()
+            rr.check()
+            result["parity_reference"] = dict(
+                parity({k: dr[k].cpu().numpy() for k in PARITY_KEYS}, refrun["out"]),
+                against=f"the reference's Fortran (oracle/_ref) on the same {refrun['cols']['ncol']} columns, "
+                        "its Gauss weights")
+            del rr, dr
         except (FileNotFoundError, RuntimeError) as e:
             print(f"bench.py: reference build unavailable ({e}); timing the C restatement", file=sys.stderr)
             result["cpu_baseline"] = cpu_port(tables, _sample(cols, args.cpu_sample))
