@@ -117,3 +117,26 @@ def test_mixed_path_on_real_tables(oracle, data_dir):
           f"|hr| median {np.median(np.abs(ref['hr'])):.3e} max {np.abs(ref['hr']).max():.3e} K/sol", flush=True)
     assert err.max() <= 2.0 * floor + 0.05
     assert rms <= 0.05
+
+
+def _reference_built() -> bool:
+    from oracle import REF_SO
+    return os.path.exists(REF_SO)
+
+
+@pytest.mark.skipif(not _reference_built(), reason="oracle/_ref not built (needs the reference's sources)")
+def test_real_tables_match_the_reference_fortran(data_dir):
+    """8,100 seeded columns x 24 layers on the reference's k-distribution (the device-built
+    tables, which equal the reference's bit for bit) against the reference's own Fortran
+    on the same tables and columns: VIS and IR at rel 1e-10 in every column."""
+    from oracle import Reference
+    from mgcm_rt import Radiation, realdata, synth
+    t = realdata.load_tables(data_dir, 24)
+    c = synth.make_columns(35, 8100, 24)
+    got = Radiation(t).run_host(c)
+    want = Reference().run_grid(t, c)
+    ev = column_errors(got, want, VIS_KEYS)
+    ei = column_errors(got, want, IR_KEYS)
+    print(f"vs the reference Fortran: VIS max {ev.max():.1e}, IR max {ei.max():.1e}", flush=True)
+    assert ev.max() <= 1e-10
+    assert ei.max() <= 1e-10, np.where(ei > 1e-10)[0].tolist()

@@ -121,3 +121,32 @@ def test_batched_pinned_tile(tile):
         single = rt.run_host(c)
         for k in KEYS:
             assert np.array_equal(o[k].cpu().numpy(), single[k]), (tile, n, nlay, k)
+
+
+def _reference_built() -> bool:
+    import os
+    from oracle import REF_SO
+    return os.path.exists(REF_SO)
+
+
+@pytest.mark.skipif(not _reference_built(), reason="oracle/_ref not built (needs the reference's sources)")
+def test_reference_grid_every_tile_against_the_reference_fortran():
+    """The reference's own 36 x 60 grid (2,160 columns, 24 layers, its Gauss weights) at
+    every tile against the reference's Fortran on the same columns: rel 1e-10 everywhere,
+    and the three tiles bit-identical."""
+    from oracle import Reference
+    from mgcm_rt import Radiation, synth
+    t = synth.make_tables(36, 24, weights="reference")
+    c = synth.make_columns(36, 2160, 24)
+    want = Reference().run_grid(t, c)
+    rt = Radiation(t)
+    outs = []
+    for tile in (1, 2, 4):
+        rt.set_tile(tile)
+        got = rt.run_host(c)
+        worst = max(rel_err(got[k], want[k]) for k in want if k in got)
+        print("tile", tile, f"{worst:.1e}", flush=True)
+        assert worst <= 1e-10, (tile, worst)
+        outs.append(got)
+    for k in outs[0]:
+        assert np.array_equal(outs[0][k], outs[1][k]) and np.array_equal(outs[0][k], outs[2][k]), k
