@@ -723,23 +723,26 @@ def main():
     result["forms"] = forms
 
     if not args.no_cpu:
+        refrun = {}
         try:
-            refrun = {}
             result["cpu_baseline"] = cpu_baseline(args.seed, cols, args.cpu_sample, keep=refrun)
-            # the device on the reference's own inputs (its Gauss weights), against its outputs
-            rr = Radiation(refrun["tables"])
-            dr = to_device(alloc_outputs(int(refrun["cols"]["ncol"]), args.nlay))
-            rr.run_device(to_device(refrun["cols"]), dr, FP64, sh)
-            torch.cuda.synchronize()
-            rr.check()
-            result["parity_reference"] = dict(
-                parity({k: dr[k].cpu().numpy() for k in PARITY_KEYS}, refrun["out"]),
-                against=f"the reference's Fortran (oracle/_ref) on the same {refrun['cols']['ncol']} columns, "
-                        "its Gauss weights")
-            del rr, dr
         except (FileNotFoundError, RuntimeError) as e:
             print(f"bench.py: reference build unavailable ({e}); timing the C restatement", file=sys.stderr)
             result["cpu_baseline"] = cpu_port(tables, _sample(cols, args.cpu_sample))
+        if refrun:  # the device on the reference's own inputs (its Gauss weights), against its outputs
+            try:
+                rr = Radiation(refrun["tables"])
+                dr = to_device(alloc_outputs(int(refrun["cols"]["ncol"]), args.nlay))
+                rr.run_device(to_device(refrun["cols"]), dr, FP64, sh)
+                torch.cuda.synchronize()
+                rr.check()
+                result["parity_reference"] = dict(
+                    parity({k: dr[k].cpu().numpy() for k in PARITY_KEYS}, refrun["out"]),
+                    against=f"the reference's Fortran (oracle/_ref) on the same {refrun['cols']['ncol']} columns, "
+                            "its Gauss weights")
+                del rr, dr
+            except RuntimeError as e:
+                result["parity_reference"] = {"error": f"{type(e).__name__}: {e}"[:300], "pass": False}
         want = {}
         result["cpu_port"] = cpu_port(tables, cols, keep=want)
         result["parity"] = parity(got, want)
