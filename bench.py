@@ -328,8 +328,8 @@ SWEEP = [(24, 540), (24, 1080), (24, 2160), (24, 4320), (24, 8100), (24, 64800),
 def form_small_grid(args, rt, stream):
     """The reference's own CPU case on the GPU: 2,160 columns (a 36 x 60 grid) x
     24 layers, one single-grid call per step, at each column-kernel tile (1, 2
-    or 4 band slots per column: 4, 2 or 1 columns per wave) and at the
-    library's automatic choice.  Every tile's outputs are compared bit for bit
+    or 4 band slots per column: 4, 2 or 1 columns per wave; 12: one band per
+    wave and a finishing kernel) and at the library's automatic choice.  Every tile's outputs are compared bit for bit
     with the one-slot tile's, and the automatic tile's with the oracle at rel
     1e-10.  At 4 columns per wave this grid is 540 waves, a quarter of the
     device's resident slots.  "sweep" repeats the per-tile timing over SWEEP's
@@ -342,7 +342,7 @@ def form_small_grid(args, rt, stream):
 
     def per_tile(ctx, dc, do, steps):
         res, first, same = {}, None, True
-        for tile in (1, 2, 4):
+        for tile in (1, 2, 4, 12):
             ctx.set_tile(tile)
             ms, _ = timed_steps(lambda: ctx.run_device(dc, do, FP64, sh), steps, 3, stream)
             got = {k: do[k].cpu().numpy() for k in PARITY_KEYS}

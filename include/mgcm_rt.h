@@ -174,16 +174,18 @@ int mgcm_rt_run(mgcm_rt_ctx *ctx, const mgcm_rt_columns *in, const mgcm_rt_fluxe
 
 /* mgcm_rt_run for callers that cannot allocate device memory (code under HIP
  * graph capture, tight serving loops, harnesses that forbid hipMalloc): the
- * same kernels and results bit for bit, with the continuum-point records in a
- * static device array of MGCM_RT_NOALLOC_SCRATCH_BYTES (128 MiB, held by every
- * device the library runs on until the process ends) instead of the context's
- * workspace.  Nothing is allocated, ever.  A grid needing more than the array
- * -- ncol * 12 * (2*nlay+4) * 8 bytes: 26,886 columns at 24 layers, 3,461 at 99
- * -- returns MGCM_RT_ERR_SCRATCH; run it in column slices (every array is per
- * column, so a slice is a pointer offset).  All calls on one device share the
- * array: calls that overlap on a device (any context, any stream) race, so
- * issue them on one stream or synchronise between them.  No counterpart in the
- * reference. */
+ * same kernels and results bit for bit, with the continuum-point records (and
+ * at tile 12 the band records) in a static device array of
+ * MGCM_RT_NOALLOC_SCRATCH_BYTES (128 MiB, held by every device the library
+ * runs on until the process ends) instead of the context's workspace.  Nothing
+ * is allocated, ever.  A grid needing more than the array -- ncol * 12 *
+ * (2*nlay+4) * 8 bytes, plus ncol * 12 * (3*nlay+7) * 8 at tile 12 (the
+ * automatic choice takes tile 12 only where it fits): 26,886 columns at 24
+ * layers, 3,461 at 99 -- returns MGCM_RT_ERR_SCRATCH; run it in column
+ * slices (every array is per column, so a slice is a pointer offset).  All
+ * calls on one device share the array: calls that overlap on a device (any
+ * context, any stream) race, so issue them on one stream or synchronise
+ * between them.  No counterpart in the reference. */
 #define MGCM_RT_NOALLOC_SCRATCH_BYTES (128ull << 20)
 int mgcm_rt_run_noalloc(mgcm_rt_ctx *ctx, const mgcm_rt_columns *in, const mgcm_rt_fluxes *out,
                         int precision, void *stream);
@@ -240,17 +242,21 @@ int mgcm_rt_batched_tuning(const mgcm_rt_ctx *ctx, int maxcls, mgcm_rt_batch_cla
 
 /* The column kernel's tile for single-grid calls (mgcm_rt_run, _noalloc,
  * _host): band_slots = 1 (4 columns per wave, each wave solving one band of a
- * column per 16-lane row at a time), 2 (2 columns, 2 bands at a time) or 4
- * (1 column, 4 bands at a time); 0 (the default) lets the library choose per
- * grid from its column count, layer count and the device's CU count.  Every
- * tile gives the same bits; more slots give a small grid more waves at some
- * extra work per column (empty band slots).  MGCM_RT_ERR_ARG for another
- * value.  No counterpart in the reference (its radiation is serial per
- * column, code/comp3.f:411-415).  A non-zero tile also restricts the batched
- * tuner to that tile's candidates (the next mgcm_rt_run_batched re-plans). */
+ * column per 16-lane row at a time), 2 (2 columns, 2 bands at a time), 4
+ * (1 column, 4 bands at a time) or 12 (all 12 bands of a column at once, each
+ * in its own wave of 4 columns; a finishing kernel adds the bands' sums, and
+ * the workspace grows by 8 (3 (nlay+1) + 4) bytes per band and column); 0
+ * (the default) lets the library choose per grid from its column count,
+ * layer count and the device's CU count.  Every tile gives the same bits;
+ * more slots give a small grid more waves at some extra work per column
+ * (empty band slots, staging repeated per wave).  MGCM_RT_ERR_ARG for
+ * another value.  No counterpart in the reference (its radiation is serial
+ * per column, code/comp3.f:411-415).  A tile of 1, 2 or 4 also restricts the
+ * batched tuner to that tile's candidates (the next mgcm_rt_run_batched
+ * re-plans); 12 leaves the batched tuner its own choice. */
 int mgcm_rt_set_tile(mgcm_rt_ctx *ctx, int band_slots);
-/* The tile a single-grid call of ncol columns x nlay layers runs with (1, 2
- * or 4), or MGCM_RT_ERR_ARG. */
+/* The tile a single-grid call of ncol columns x nlay layers runs with (1, 2,
+ * 4 or 12), or MGCM_RT_ERR_ARG. */
 int mgcm_rt_tile_for(const mgcm_rt_ctx *ctx, int ncol, int nlay);
 
 /* Init-time table builders (host in/out pointers), both equal to the

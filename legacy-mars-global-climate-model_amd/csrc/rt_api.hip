@@ -176,6 +176,7 @@ void fill_args(RunArgs &a, const mgcm_rt_columns *in, const mgcm_rt_fluxes *out,
   a.flag = flag;
   a.bands = 3;
   a.clr = nullptr;
+  a.part = nullptr;
   const mgcm_rt_aerosol *ae = in->aerosol;
   const double *aq[12] = {nullptr};
   if (ae) {
@@ -193,13 +194,13 @@ void fill_args(RunArgs &a, const mgcm_rt_columns *in, const mgcm_rt_fluxes *out,
 
 namespace mgcm {
 hipError_t run_grid(const mgcm_rt_ctx *c, const mgcm_rt_columns *in, const mgcm_rt_fluxes *out, int path,
-                    int bands, double *clr, hipStream_t st) {
+                    int bands, double *ws, int tile, hipStream_t st) {
   RunArgs a;
   fill_args(a, in, out, c->d_flag);
   a.bands = bands;
-  a.clr = clr;
+  a.clr = ws;
+  a.part = tile == TILE_BANDS ? ws + clr_workspace_bytes(in->ncol, in->nlay) / sizeof(double) : nullptr;
   const int p = path & 0xf;
-  const int tile = c->tile ? c->tile : auto_tile(in->ncol, in->nlay, c->ncu);
   hipError_t e = (p == MGCM_RT_FP64)    ? launch_plain(c->dt, a, tile, st)
                  : (p == MGCM_RT_MIXED) ? launch_mixed(c->dt, a, tile, st)
                                         : launch_study(path >> 8, c->dt, a, st);
@@ -220,8 +221,9 @@ extern "C" int mgcm_rt_run(mgcm_rt_ctx *c, const mgcm_rt_columns *in, const mgcm
   s = decode_mode(precision, &path, &bands);
   if (s != MGCM_RT_OK) return s;
   if (in->ncol == 0) return MGCM_RT_OK;
-  const size_t need = clr_workspace_bytes(in->ncol, in->nlay);
-  if (need > c->clr_bytes) {  // grow-only continuum-point workspace
+  const int tile = grid_tile(c, in->ncol, in->nlay);
+  const size_t need = grid_workspace_bytes(in->ncol, in->nlay, tile);
+  if (need > c->clr_bytes) {  // grow-only workspace
     if (c->d_clr) {  // the context's earlier runs were issued on this stream (or have ended)
       if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess) return MGCM_RT_ERR_DEVICE;
       (void)hipFree(c->d_clr);
@@ -231,8 +233,8 @@ extern "C" int mgcm_rt_run(mgcm_rt_ctx *c, const mgcm_rt_columns *in, const mgcm
     if (hipMalloc(&c->d_clr, need) != hipSuccess) return MGCM_RT_ERR_NOMEM;
     c->clr_bytes = need;
   }
-  return run_grid(c, in, out, path, bands, c->d_clr, (hipStream_t)stream) == hipSuccess ? MGCM_RT_OK
-                                                                                        : MGCM_RT_ERR_DEVICE;
+  return run_grid(c, in, out, path, bands, c->d_clr, tile, (hipStream_t)stream) == hipSuccess ? MGCM_RT_OK
+                                                                                              : MGCM_RT_ERR_DEVICE;
 }
 
 extern "C" int mgcm_rt_check(mgcm_rt_ctx *c) {
@@ -279,14 +281,15 @@ extern "C" int mgcm_rt_batched_tuning(const mgcm_rt_ctx *c, int maxcls, mgcm_rt_
 }
 
 extern "C" int mgcm_rt_set_tile(mgcm_rt_ctx *c, int band_slots) {
-  if (!c || !(band_slots == 0 || band_slots == 1 || band_slots == 2 || band_slots == 4)) return MGCM_RT_ERR_ARG;
+  if (!c || !(band_slots == 0 || band_slots == 1 || band_slots == 2 || band_slots == 4 || band_slots == TILE_BANDS))
+    return MGCM_RT_ERR_ARG;
   c->tile = band_slots;
   return MGCM_RT_OK;
 }
 
 extern "C" int mgcm_rt_tile_for(const mgcm_rt_ctx *c, int ncol, int nlay) {
   if (!c || ncol < 0 || nlay < 1 || nlay > MAX_NLAY) return MGCM_RT_ERR_ARG;
-  return c->tile ? c->tile : auto_tile(ncol, nlay, c->ncu);
+  return grid_tile(c, ncol, nlay);
 }
 
 // ---------------------------------------------------------------------------

@@ -260,7 +260,8 @@ int run_batched_impl(mgcm_rt_ctx *c, int ngrids, const mgcm_rt_columns *grids, c
   std::vector<long long> sig;
   sig.reserve(3 + 2 * (size_t)ngrids);
   sig.push_back(precision);
-  sig.push_back(c->tile);
+  const int force_tile = c->tile == TILE_BANDS ? 0 : c->tile;  // single-grid calls only
+  sig.push_back(force_tile);
   sig.push_back(ngrids);
   for (int i = 0; i < ngrids; i++) {
     sig.push_back(grids[i].ncol);
@@ -288,7 +289,7 @@ int run_batched_impl(mgcm_rt_ctx *c, int ngrids, const mgcm_rt_columns *grids, c
         }
     for (int k = 0; k < NCLASS; k++) {
       ClassPlan &p = b->cls[k];
-      if (p.nwork[0] > 0) tune_class(precision, k, c->dt, b, ncu, c->tile, st);
+      if (p.nwork[0] > 0) tune_class(precision, k, c->dt, b, ncu, force_tile, st);
       for (int kind = 1; kind < NKIND; kind++) p.nblocks[kind] = std::min(p.nwork[kind], 16 * ncu);
     }
     // capture the class launches; the argument upload stays outside the graph

@@ -57,6 +57,14 @@ inline size_t stage_bytes(int nlay) {
 __host__ __device__ inline int clr_rec(int nl) { return 2 * nl + 2; }
 inline size_t clr_bytes(int ncol, int nlay) { return (size_t)ncol * NB * clr_rec(nlay + 1) * sizeof(double); }
 
+// Band records in RunArgs::part (the one-band-per-wave tile, TILE_BANDS): per
+// column and band, the band's 16-point sums of the net, upward and downward
+// flux at levels 1..nl, then its top term (NFLUXTOPV or NFLUXTOPI part), its
+// DIFFVT part and its TLIMIT skip mask; written by the band's wave, added in
+// band order by the finishing kernel.
+__host__ __device__ inline int part_rec(int nl) { return 3 * nl + 4; }
+inline size_t part_bytes(int ncol, int nlay) { return (size_t)ncol * NB * part_rec(nlay + 1) * sizeof(double); }
+
 __device__ inline Stage carve(double *p, int nsub, int nl, int nlev) {
   p += mgcm_libm::LDS_EXPTAB_BYTES / sizeof(double);
   Stage S;
@@ -301,16 +309,6 @@ __device__ __forceinline__ void acc_add(double *arr, int n, int g0, double v) {
   for (int j = 0; j < SL; j++) arr[j] = (own && j == slot) ? arr[j] + v : arr[j];
 }
 
-template <int SL>
-__device__ __forceinline__ double acc_get(const double *arr, int n) {
-  const int slot = (n - 1) >> 4;
-  double r = 0.0;
-#pragma unroll
-  for (int j = 0; j < SL; j++)
-    if (j == slot) r = arr[j];
-  return r;
-}
-
 // The row sums of a column's S band slots (rows 16 apart, bands in increasing
 // order) added one after the other, as band after band: on the column's slot-0
 // row the result is ((acc + v_0) + v_1) + ...; the other rows' results are
@@ -342,9 +340,129 @@ struct Tile {
   }
 };
 
+// Tile code of one band per wave: the 4 columns of a wave at one band slot
+// each, each of the NB band rounds in its own wave (column_block with ONE),
+// the bands' sums added by a finishing kernel.
+constexpr int TILE_BANDS = NB;
+
 #ifndef MGCM_FP64_WAVES
 #define MGCM_FP64_WAVES 2
 #endif
+// auto_tile's latency of a one-band wave and cost of the finishing launch, in
+// hundredths of an S = 1 wave's latency
+#ifndef MGCM_BANDS_LATENCY
+#define MGCM_BANDS_LATENCY 13
+#endif
+#ifndef MGCM_BANDS_FINISH
+#define MGCM_BANDS_FINISH 4
+#endif
+
+// Stages 3 and 4 of a wave's columns (the column's lead row; lane g0 holds the
+// levels n with (n-1) % 16 == g0).
+template <int SL>
+__device__ __forceinline__ void column_tail(const DevTables &dt, const RunArgs &a, const Stage &St, Acc<SL> &acc,
+                                            double topv, double difv, double topi, int c, int col, int colc,
+                                            bool valid, bool lead, bool sun, int g0, int nl,
+                                            const double *plev) {
+  const RtConst *rc = dt.c;
+  // 3. continuum point g = 17 of every (column, band): solved by k_rt_clear
+  //    into a.clr, weighted here with FZERO plus the weights of the band's
+  //    TLIMIT-skipped points, in g order, and added band by band
+  //    (code/sfluxv.f90:72-75,121-162, code/sfluxi.f90:72-75,114-143); by the
+  //    column's slot-0 row
+  if (lead) {
+    double wz[NB];
+    for (int b = 0; b < NB; b++) {
+      double fzero = rc->fzero[b];
+      if (!(fzero >= F32LIT(0.99))) {
+        const uint32_t mk = St.skip[c * NB + b];
+        for (int g = 0; g < NGAS; g++)
+          if (mk & (1u << g)) fzero = fzero + (1.0 - rc->fzero[b]) * rc->gw[b][g];
+      }
+      wz[b] = fzero;
+    }
+    const int rec = clr_rec(nl);
+    const double *cr = a.clr + (size_t)colc * NB * rec;
+    if (a.bands & 1) {
+      for (int n = g0 + 1; n <= nl; n += 16) {  // this lane's levels
+        double sn = 0.0, su = 0.0, sd = 0.0;
+        for (int b = 0; b < NV; b++) {
+          const double fp = cr[(n - 1) * NB + b], fm = cr[(nl + n - 1) * NB + b];
+          sn = sn + (sun ? (fp - fm) * wz[b] : 0.0);
+          su = su + (sun ? fp * wz[b] : 0.0);
+          sd = sd + (sun ? fm * wz[b] : 0.0);
+        }
+        acc_add<SL>(acc.vnet, n, g0, sn);
+        acc_add<SL>(acc.vup, n, g0, su);
+        acc_add<SL>(acc.vdn, n, g0, sd);
+      }
+      double st = 0.0, sdf = 0.0;
+      for (int b = 0; b < NV; b++) {
+        st = st + (sun ? cr[2 * nl * NB + b] * wz[b] : 0.0);
+        sdf = sdf + (sun ? cr[(2 * nl + 1) * NB + b] * wz[b] : 0.0);
+      }
+      topv = topv + st;
+      difv = difv + sdf;
+    }
+    if (a.bands & 2) {
+      for (int n = g0 + 1; n <= nl; n += 16) {  // this lane's levels
+        double sn = 0.0, su = 0.0, sd = 0.0;
+        for (int b = NV; b < NB; b++) {
+          const double dw = rc->dwni[b - NV];
+          const double fp = cr[(n - 1) * NB + b], fm = cr[(nl + n - 1) * NB + b];
+          sn = sn + (valid ? (fp - fm) * dw * wz[b] : 0.0);
+          su = su + (valid ? fp * dw * wz[b] : 0.0);
+          sd = sd + (valid ? fm * dw * wz[b] : 0.0);
+        }
+        acc_add<SL>(acc.inet, n, g0, sn);
+        acc_add<SL>(acc.iup, n, g0, su);
+        acc_add<SL>(acc.idn, n, g0, sd);
+      }
+      double st = 0.0;
+      for (int b = NV; b < NB; b++) st = st + (valid ? cr[2 * nl * NB + b] * rc->dwni[b - NV] * wz[b] : 0.0);
+      topi = topi + st;
+    }
+  }
+
+  // 4. outputs, net-flux differences and heating (slot-0 rows)
+  __syncthreads();
+  double *fv = St.t;  // reuse the staging: [C][nl] net VIS, [C][nl] net IR
+  double *fi = St.t + 4 * nl;
+  if (!sun) { topv = 0.0; difv = 0.0; }
+  if (lead) {
+#pragma unroll
+    for (int j = 0; j < SL; j++) {  // this lane's levels n = 16 j + g0 + 1
+      const int n = 16 * j + g0 + 1;
+      if (n <= nl) {
+        const double nv = sun ? acc.vnet[j] : 0.0, uv = sun ? acc.vup[j] : 0.0, dv = sun ? acc.vdn[j] : 0.0;
+        const double ni = acc.inet[j], ui = acc.iup[j], di = acc.idn[j];
+        fv[c * nl + n - 1] = nv;
+        fi[c * nl + n - 1] = ni;
+        if (valid) {
+          const size_t o = (size_t)col * nl + n - 1;
+          a.fmnetv[o] = nv; a.fluxupv[o] = uv; a.fluxdnv[o] = dv;
+          a.fmneti[o] = ni; a.fluxupi[o] = ui; a.fluxdni[o] = di;
+        }
+      }
+    }
+    if (g0 == 0 && valid) {
+      a.nfluxtopv[col] = topv;
+      a.nfluxtopi[col] = topi;
+      a.diffvt[col] = difv;
+    }
+  }
+  __syncthreads();
+  if (a.hr && valid && lead) {
+    for (int l = g0; l <= a.nlay; l += 16) {
+      const int L = l + 1;  // SUNTOT(2L+1), IRTOTAL(2L+1)
+      const double nv = fv[c * nl + L - 1], ni = fi[c * nl + L - 1];
+      const double sunt = sun ? (L == 1 ? nv - topv : nv - fv[c * nl + L - 2]) : 0.0;
+      const double irt = (L == 1) ? ni - topi : ni - fi[c * nl + L - 2];
+      const double dp = (l == 0) ? (plev[0] + plev[1]) : (plev[2 * l + 2] - plev[2 * l]);
+      a.hr[(size_t)col * nl + l] = (((sunt + irt) * GRAV) / ((CPAIR * SCALEP) * dp)) * SECONDS_PER_SOL;
+    }
+  }
+}
 
 // P: the precision policy (P_* bits; 0 = plain FP64, P_TAB32 = the mixed path,
 // whose k-tables are read in FP32 -- half the gather bytes and corner
@@ -353,8 +471,12 @@ struct Tile {
 // The whole radiation call of columns col0..col0+C-1 of grid a, by one wave.
 // The exp table must be in LDS; the staging after it is rewritten.
 // D: how many layers ahead the solves load the item state (vis_solve).
-template <int NLM, int P, int S, int D = 1>
-__device__ __forceinline__ void column_block(const DevTables &dt, const RunArgs &a, int col0, double *smem) {
+// ONE (S = 1 only): round kr alone, its band's sums written to a.part for
+// column_finish instead of stages 3 and 4.
+template <int NLM, int P, int S, int D = 1, bool ONE = false>
+__device__ __forceinline__ void column_block(const DevTables &dt, const RunArgs &a, int col0, double *smem,
+                                             int kr = 0) {
+  static_assert(!ONE || S == 1, "one band per wave: one column per row");
   constexpr bool T32 = (P & P_TAB32) != 0;
   using kt = typename std::conditional<T32, float, double>::type;
   using TL = Tile<S>;
@@ -419,7 +541,8 @@ __device__ __forceinline__ void column_block(const DevTables &dt, const RunArgs 
   };
 
   // 2. k-distribution points, S bands per column per round
-  for (int k = 0; k < TL::RV + TL::RI; k++) {
+  const int k_end = ONE ? kr + 1 : TL::RV + TL::RI;
+  for (int k = ONE ? kr : 0; k < k_end; k++) {
     const bool VISR = k < TL::RV;
     const int bnom = TL::band(k, slot);  // this row's band slot (-1: none)
     const int bact = row_band(k, row);   // ... if solved
@@ -539,106 +662,86 @@ __device__ __forceinline__ void column_block(const DevTables &dt, const RunArgs 
     if (g0 == 0 && bnom >= 0) St.skip[c * NB + bnom] = bact >= 0 ? (uint32_t)((m >> (16 * row)) & 0xffffu) : 0u;
   }
   __syncthreads();
-
-  // 3. continuum point g = 17 of every (column, band): solved by k_rt_clear
-  //    into a.clr, weighted here with FZERO plus the weights of the band's
-  //    TLIMIT-skipped points, in g order, and added band by band
-  //    (code/sfluxv.f90:72-75,121-162, code/sfluxi.f90:72-75,114-143); by the
-  //    column's slot-0 row
-  if (lead) {
-    double wz[NB];
-    for (int b = 0; b < NB; b++) {
-      double fzero = rc->fzero[b];
-      if (!(fzero >= F32LIT(0.99))) {
-        const uint32_t mk = St.skip[c * NB + b];
-        for (int g = 0; g < NGAS; g++)
-          if (mk & (1u << g)) fzero = fzero + (1.0 - rc->fzero[b]) * rc->gw[b][g];
-      }
-      wz[b] = fzero;
-    }
-    const int rec = clr_rec(nl);
-    const double *cr = a.clr + (size_t)colc * NB * rec;
-    if (a.bands & 1) {
-      for (int n = 1; n <= nl; n++) {
-        if (((n - 1) & 15) != g0) continue;
-        double sn = 0.0, su = 0.0, sd = 0.0;
-        for (int b = 0; b < NV; b++) {
-          const double fp = cr[(n - 1) * NB + b], fm = cr[(nl + n - 1) * NB + b];
-          sn = sn + (sun ? (fp - fm) * wz[b] : 0.0);
-          su = su + (sun ? fp * wz[b] : 0.0);
-          sd = sd + (sun ? fm * wz[b] : 0.0);
+  if constexpr (ONE) {
+    // the band's record: sums added to zero, so the finishing kernel's sum in
+    // band order repeats the additions above bit for bit
+    const int b = TL::band(kr, 0);
+    if (valid) {
+      double *r = a.part + ((size_t)col * NB + b) * part_rec(nl);
+      const bool vis = b < NV;
+#pragma unroll
+      for (int j = 0; j < SL; j++) {
+        const int n = 16 * j + g0 + 1;
+        if (n <= nl) {
+          r[n - 1] = vis ? acc.vnet[j] : acc.inet[j];
+          r[nl + n - 1] = vis ? acc.vup[j] : acc.iup[j];
+          r[2 * nl + n - 1] = vis ? acc.vdn[j] : acc.idn[j];
         }
-        acc_add<SL>(acc.vnet, n, g0, sn);
-        acc_add<SL>(acc.vup, n, g0, su);
-        acc_add<SL>(acc.vdn, n, g0, sd);
       }
-      double st = 0.0, sdf = 0.0;
-      for (int b = 0; b < NV; b++) {
-        st = st + (sun ? cr[2 * nl * NB + b] * wz[b] : 0.0);
-        sdf = sdf + (sun ? cr[(2 * nl + 1) * NB + b] * wz[b] : 0.0);
+      if (g0 == 0) {
+        r[3 * nl] = vis ? topv : topi;
+        r[3 * nl + 1] = vis ? difv : 0.0;
+        r[3 * nl + 2] = (double)St.skip[c * NB + b];
       }
-      topv = topv + st;
-      difv = difv + sdf;
     }
-    if (a.bands & 2) {
-      for (int n = 1; n <= nl; n++) {
-        if (((n - 1) & 15) != g0) continue;
-        double sn = 0.0, su = 0.0, sd = 0.0;
-        for (int b = NV; b < NB; b++) {
-          const double dw = rc->dwni[b - NV];
-          const double fp = cr[(n - 1) * NB + b], fm = cr[(nl + n - 1) * NB + b];
-          sn = sn + (valid ? (fp - fm) * dw * wz[b] : 0.0);
-          su = su + (valid ? fp * dw * wz[b] : 0.0);
-          sd = sd + (valid ? fm * dw * wz[b] : 0.0);
-        }
-        acc_add<SL>(acc.inet, n, g0, sn);
-        acc_add<SL>(acc.iup, n, g0, su);
-        acc_add<SL>(acc.idn, n, g0, sd);
-      }
-      double st = 0.0;
-      for (int b = NV; b < NB; b++) st = st + (valid ? cr[2 * nl * NB + b] * rc->dwni[b - NV] * wz[b] : 0.0);
-      topi = topi + st;
-    }
+  } else {
+    column_tail<SL>(dt, a, St, acc, topv, difv, topi, c, col, colc, valid, lead, sun, g0, nl, plev);
   }
+}
 
-  // 4. outputs, net-flux differences and heating (slot-0 rows)
-  __syncthreads();
-  double *fv = St.t;  // reuse the staging: [C][nl] net VIS, [C][nl] net IR
-  double *fi = St.t + 4 * nl;
-  if (!sun) { topv = 0.0; difv = 0.0; }
-  if (lead) {
-    for (int n = 1; n <= nl; n++) {
-      if (((n - 1) & 15) == g0) {
-        const double nv = sun ? acc_get<SL>(acc.vnet, n) : 0.0;
-        const double uv = sun ? acc_get<SL>(acc.vup, n) : 0.0;
-        const double dv = sun ? acc_get<SL>(acc.vdn, n) : 0.0;
-        const double ni = acc_get<SL>(acc.inet, n), ui = acc_get<SL>(acc.iup, n), di = acc_get<SL>(acc.idn, n);
-        fv[c * nl + n - 1] = nv;
-        fi[c * nl + n - 1] = ni;
-        if (valid) {
-          const size_t o = (size_t)col * nl + n - 1;
-          a.fmnetv[o] = nv; a.fluxupv[o] = uv; a.fluxdnv[o] = dv;
-          a.fmneti[o] = ni; a.fluxupi[o] = ui; a.fluxdni[o] = di;
-        }
+// Stages 3 and 4 of the one-band-per-wave tile, for the columns
+// col0..col0+3 (one per 16-lane row, as at S = 1): the band records of
+// column_block<..., ONE = true> added in band order, then column_tail.
+template <int NLM>
+__device__ __forceinline__ void column_finish(const DevTables &dt, const RunArgs &a, int col0, double *smem) {
+  constexpr int SL = (NLM + 15) / 16;
+  const int nl = a.nlay + 1, nlev = 2 * a.nlay + 3, nsub = nlev - 1;
+  const int lane = threadIdx.x, c = lane >> 4, g0 = lane & 15;
+  const int col = col0 + c;
+  const bool valid = col < a.ncol;
+  const int colc = valid ? col : a.ncol - 1;
+  Stage St = carve(smem, nsub, nl, nlev);
+  const double *plev = a.plev + (size_t)colc * nlev;
+  const bool sun = a.acosz[colc] >= F32LIT(1.0e-5);  // code/comp3.f:637
+  Acc<SL> acc;
+#pragma unroll
+  for (int j = 0; j < SL; j++) acc.vnet[j] = acc.vup[j] = acc.vdn[j] = acc.inet[j] = acc.iup[j] = acc.idn[j] = 0.0;
+  double topv = 0.0, difv = 0.0, topi = 0.0;
+  const int pr = part_rec(nl);
+  const double *rc0 = a.part + (size_t)colc * NB * pr;
+#pragma unroll
+  for (int b = 0; b < NV; b++) {
+    const double *r = rc0 + b * pr;
+#pragma unroll
+    for (int j = 0; j < SL; j++) {
+      const int n = 16 * j + g0 + 1;
+      if (n <= nl) {
+        acc.vnet[j] = acc.vnet[j] + r[n - 1];
+        acc.vup[j] = acc.vup[j] + r[nl + n - 1];
+        acc.vdn[j] = acc.vdn[j] + r[2 * nl + n - 1];
       }
     }
-    if (g0 == 0 && valid) {
-      a.nfluxtopv[col] = topv;
-      a.nfluxtopi[col] = topi;
-      a.diffvt[col] = difv;
+    topv = topv + r[3 * nl];
+    difv = difv + r[3 * nl + 1];
+    if (g0 == 0) St.skip[c * NB + b] = (uint32_t)r[3 * nl + 2];
+  }
+#pragma unroll
+  for (int b = NV; b < NB; b++) {
+    const double *r = rc0 + b * pr;
+#pragma unroll
+    for (int j = 0; j < SL; j++) {
+      const int n = 16 * j + g0 + 1;
+      if (n <= nl) {
+        acc.inet[j] = acc.inet[j] + r[n - 1];
+        acc.iup[j] = acc.iup[j] + r[nl + n - 1];
+        acc.idn[j] = acc.idn[j] + r[2 * nl + n - 1];
+      }
     }
+    topi = topi + r[3 * nl];
+    if (g0 == 0) St.skip[c * NB + b] = (uint32_t)r[3 * nl + 2];
   }
   __syncthreads();
-  if (a.hr && valid && lead) {
-    for (int l = g0; l <= a.nlay; l += 16) {
-      const int L = l + 1;  // SUNTOT(2L+1), IRTOTAL(2L+1)
-      const double nv = fv[c * nl + L - 1], ni = fi[c * nl + L - 1];
-      const double sunt = sun ? (L == 1 ? nv - topv : nv - fv[c * nl + L - 2]) : 0.0;
-      const double irt = (L == 1) ? ni - topi : ni - fi[c * nl + L - 2];
-      const double dp = (l == 0) ? (plev[0] + plev[1]) : (plev[2 * l + 2] - plev[2 * l]);
-      a.hr[(size_t)col * nl + l] = (((sunt + irt) * GRAV) / ((CPAIR * SCALEP) * dp)) * SECONDS_PER_SOL;
-    }
-  }
+  column_tail<SL>(dt, a, St, acc, topv, difv, topi, c, col, colc, valid, true, sun, g0, nl, plev);
 }
 
 // The continuum-only point g = 17 of (column, band) items, one per lane: a
@@ -756,6 +859,26 @@ void k_rt(DevTables dt, RunArgs a) {
   column_block<NLM, P, S, state_ahead(column_waves(NLM))>(dt, a, blockIdx.x * Tile<S>::C, smem);
 }
 
+// The one-band-per-wave tile: block b solves band round b / ngroups (at one
+// band slot per column, round k is band k) of columns 4 (b % ngroups) ..
+// +3 and writes the band's records; k_rt_finish then adds them per column.
+// Blocks of one band are adjacent, so the waves in flight share few k-table
+// slices.
+template <int NLM, int P>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(column_waves(NLM), column_waves(NLM))))
+void k_rt_band(DevTables dt, RunArgs a, int ngroups) {
+  extern __shared__ double smem[];
+  mgcm_libm::load_lds_exptab(threadIdx.x, 64);
+  __syncthreads();
+  const int k = (int)blockIdx.x / ngroups, grp = (int)blockIdx.x % ngroups;
+  column_block<NLM, P, 1, state_ahead(column_waves(NLM)), true>(dt, a, grp * 4, smem, k);
+}
+template <int NLM>
+__global__ __launch_bounds__(64) void k_rt_finish(DevTables dt, RunArgs a) {
+  extern __shared__ double smem[];
+  column_finish<NLM>(dt, a, (int)blockIdx.x * 4, smem);
+}
+
 // Many grids of one layer-count class: a persistent grid of blocks walks the
 // work list (grid index, first column of 4) round-robin, largest layer counts
 // first; with S band slots per column each item is S sub-items of 4/S columns.
@@ -788,9 +911,10 @@ void k_rt_batch_w1(DevTables dt, const RunArgs *__restrict__ grids, const uint2 
   batch_items<NLM, P, 1, state_ahead(1)>(dt, grids, work, nwork, smem);
 }
 
-template <int NLM, int P, int S>
-static hipError_t launch_nlm(const DevTables &dt, const RunArgs &a, hipStream_t st) {
-  // continuum points first (their records feed the column kernel's sums)
+// The continuum-point kernels of a grid (their records feed the column
+// kernel's sums).
+template <int NLM>
+static void launch_clear(const DevTables &dt, const RunArgs &a, hipStream_t st) {
   const int nvis = (a.ncol + 8) / 9, nir = (a.ncol + 11) / 12;
   if (a.bands == 3)
     hipLaunchKernelGGL((k_rt_clear2<NLM>), dim3(nvis + nir), dim3(64), mgcm_libm::LDS_EXPTAB_BYTES, st, dt, a, nvis);
@@ -798,6 +922,11 @@ static hipError_t launch_nlm(const DevTables &dt, const RunArgs &a, hipStream_t 
     hipLaunchKernelGGL((k_rt_clear<NLM, true>), dim3(nvis), dim3(64), mgcm_libm::LDS_EXPTAB_BYTES, st, dt, a);
   else if (a.bands & 2)
     hipLaunchKernelGGL((k_rt_clear<NLM, false>), dim3(nir), dim3(64), mgcm_libm::LDS_EXPTAB_BYTES, st, dt, a);
+}
+
+template <int NLM, int P, int S>
+static hipError_t launch_nlm(const DevTables &dt, const RunArgs &a, hipStream_t st) {
+  launch_clear<NLM>(dt, a, st);
   const size_t lds = stage_bytes(a.nlay);
   constexpr int C = Tile<S>::C;
   const unsigned gx = (unsigned)((a.ncol + C - 1) / C);
@@ -819,6 +948,30 @@ hipError_t launch_column(const DevTables &dt, const RunArgs &a, hipStream_t st) 
   return launch_nlm<MAX_NLAY + 1, P, S>(dt, a, st);
 }
 
+// One grid at one band per wave (TILE_BANDS): the continuum-point kernels,
+// NB waves per 4 columns, then the finishing kernel.  a.clr must hold
+// clr_bytes(a.ncol, a.nlay), a.part part_bytes(a.ncol, a.nlay).
+template <int NLM, int P>
+static hipError_t launch_bands_nlm(const DevTables &dt, const RunArgs &a, hipStream_t st) {
+  launch_clear<NLM>(dt, a, st);
+  const size_t lds = stage_bytes(a.nlay);
+  const int ngroups = (a.ncol + 3) / 4;
+  hipLaunchKernelGGL((k_rt_band<NLM, P>), dim3((unsigned)ngroups * (Tile<1>::RV + Tile<1>::RI)), dim3(64), lds, st,
+                     dt, a, ngroups);
+  hipLaunchKernelGGL((k_rt_finish<NLM>), dim3((unsigned)ngroups), dim3(64), lds, st, dt, a);
+  return hipGetLastError();
+}
+template <int P>
+hipError_t launch_bands(const DevTables &dt, const RunArgs &a, hipStream_t st) {
+  const int nl = a.nlay + 1;
+  if (nl <= 17) return launch_bands_nlm<17, P>(dt, a, st);
+  if (nl <= 25) return launch_bands_nlm<25, P>(dt, a, st);
+  if (nl <= 37) return launch_bands_nlm<37, P>(dt, a, st);
+  if (nl <= 49) return launch_bands_nlm<49, P>(dt, a, st);
+  if (nl <= 65) return launch_bands_nlm<65, P>(dt, a, st);
+  return launch_bands_nlm<MAX_NLAY + 1, P>(dt, a, st);
+}
+
 // Resident one-wave blocks per CU for `lds` bytes of LDS at `waves` waves per
 // SIMD (4 SIMDs per CU, 160 KiB of LDS).
 inline int resident_per_cu(size_t lds, int waves) {
@@ -834,17 +987,21 @@ inline int resident_per_cu(size_t lds, int waves) {
 // columns of 24 layers take 0.715, 0.535 and 0.376 ms); the band rounds alone
 // (12, 7, 4) would say 1 : 0.58 : 0.33 -- the staging, the continuum-point
 // sums and the empty band slots do not shrink with S.  Large grids fill the
-// device at S = 1, the least work per column.
-inline int auto_tile(int ncol, int nlay, int ncu) {
+// device at S = 1, the least work per column.  One band per wave (bands_ok:
+// its band records fit the caller's workspace) costs a twelfth of S = 1's
+// wave latency plus its TPINDEX staging, and a finishing launch.
+inline int auto_tile(int ncol, int nlay, int ncu, bool bands_ok = true) {
   const int nl = nlay + 1;
   const int nlm = nl <= 17 ? 17 : nl <= 25 ? 25 : nl <= 37 ? 37 : nl <= 49 ? 49 : nl <= 65 ? 65 : MAX_NLAY + 1;
   const long long slots = (long long)ncu * resident_per_cu(stage_bytes(nlay), column_waves(nlm));
-  constexpr int tiles[3] = {1, 2, 4}, latency[3] = {100, 75, 53};
+  constexpr int tiles[4] = {1, 2, 4, TILE_BANDS}, latency[4] = {100, 75, 53, MGCM_BANDS_LATENCY};
+  constexpr int per_wave[4] = {4, 2, 1, 4};  // columns per wave
+  constexpr int waves_per[4] = {1, 1, 1, TILE_BANDS};  // waves per wave's columns
   int best = 1;
   long long best_cost = -1;
-  for (int i = 0; i < 3; i++) {
-    const long long waves = ((long long)ncol * tiles[i] + 3) / 4;
-    const long long cost = ((waves + slots - 1) / slots) * latency[i];
+  for (int i = 0; i < (bands_ok ? 4 : 3); i++) {
+    const long long waves = ((long long)ncol + per_wave[i] - 1) / per_wave[i] * waves_per[i];
+    const long long cost = ((waves + slots - 1) / slots) * latency[i] + (tiles[i] == TILE_BANDS ? MGCM_BANDS_FINISH : 0);
     if (best_cost < 0 || cost < best_cost) {
       best = tiles[i];
       best_cost = cost;

@@ -32,17 +32,18 @@ struct mgcm_rt_ctx {
   // device address of the no-allocation entry point's scratch array on this
   // context's device (rt_noalloc.hip; not owned: it lives until the process ends)
   double *d_scratch;
-  // column kernel tile of single-grid calls: band slots per column (1, 2, 4;
-  // 0 = chosen per grid by auto_tile) and the device's CU count
+  // column kernel tile of single-grid calls: band slots per column (1, 2, 4,
+  // or TILE_BANDS: one band per wave; 0 = chosen per grid by auto_tile) and
+  // the device's CU count
   int tile;
   int ncu;
 };
 
 namespace mgcm {
-// tile: band slots per column of the column kernel (1, 2 or 4)
+// tile: band slots per column of the column kernel (1, 2, 4 or TILE_BANDS)
 hipError_t launch_plain(const DevTables &dt, const RunArgs &a, int tile, hipStream_t st);
 hipError_t launch_mixed(const DevTables &dt, const RunArgs &a, int tile, hipStream_t st);
-hipError_t launch_plain_tile(const DevTables &dt, const RunArgs &a, int tile, hipStream_t st);  // tile 2, 4
+hipError_t launch_plain_tile(const DevTables &dt, const RunArgs &a, int tile, hipStream_t st);  // 2, 4, bands
 hipError_t launch_mixed_tile(const DevTables &dt, const RunArgs &a, int tile, hipStream_t st);
 // The precision study's column kernels (rt_study.hip): policy = P_* bits;
 // hipErrorInvalidValue outside the 25..36-layer class or for a policy not built.
@@ -59,6 +60,12 @@ hipError_t launch_batch_plain_tile(int cls, int kind, int waves, int tile, const
 hipError_t launch_batch_mixed_tile(int cls, int kind, int waves, int tile, const DevTables &dt, const RunArgs *grids,
                                    const uint2 *work, int nwork, int nblocks, size_t lds, hipStream_t st);
 size_t clr_workspace_bytes(int ncol, int nlay);
+// The workspace of a single-grid call at a tile: the continuum-point records,
+// plus the band records at TILE_BANDS.
+size_t grid_workspace_bytes(int ncol, int nlay, int tile);
+// The tile of a single-grid call: the context's, or auto_tile's choice among
+// the tiles whose workspace fits in max_ws bytes.
+int grid_tile(const mgcm_rt_ctx *ctx, int ncol, int nlay, size_t max_ws = (size_t)-1);
 // Splits the precision argument into the path (MGCM_RT_FP64/MIXED, or
 // MGCM_RT_STUDY with its policy bits in path >> 8) and the band mask;
 // MGCM_RT_ERR_PRECISION for anything else.
@@ -74,10 +81,10 @@ int noalloc_setup(mgcm_rt_ctx *ctx);
 // Waits for stream st only, then reads and clears the context's range flag.
 int check_on_stream(mgcm_rt_ctx *ctx, hipStream_t st);
 int batched_tuning_impl(const mgcm_rt_ctx *ctx, int maxcls, mgcm_rt_batch_class *info);
-// mgcm_rt_run's launches on a grid whose arguments are checked, with a
-// continuum-point workspace of clr_workspace_bytes(in->ncol, in->nlay) bytes.
+// mgcm_rt_run's launches on a grid whose arguments are checked, at a tile,
+// with a workspace ws of grid_workspace_bytes(in->ncol, in->nlay, tile) bytes.
 hipError_t run_grid(const mgcm_rt_ctx *c, const mgcm_rt_columns *in, const mgcm_rt_fluxes *out, int path,
-                    int bands, double *clr, hipStream_t st);
+                    int bands, double *ws, int tile, hipStream_t st);
 void host_free(void *state);
 // DSOLFLUX + DNVFLUX of a grid (rt_surface.hip); dnvflux may be null.
 hipError_t launch_dsolflux(const RtConst *rc, int ncol, const double *sol, const double *acosz,

@@ -295,15 +295,20 @@ int run_host(mgcm_rt_ctx *c, const mgcm_rt_columns *in, const mgcm_rt_fluxes *ou
     if (hipMalloc(&c->d_host, bytes) != hipSuccess) return MGCM_RT_ERR_NOMEM;
     c->host_bytes = bytes;
   }
-  if (clr_workspace_bytes((int)K, in->nlay) > h->clr_bytes) {
+  // each chunk runs at its own tile; the workspace fits the full and the last chunk
+  const int tile_k = grid_tile(c, (int)K, in->nlay), last = (int)(ncol - (nchunk - 1) * K);
+  const int tile_last = grid_tile(c, last, in->nlay);
+  const size_t ws = std::max(grid_workspace_bytes((int)K, in->nlay, tile_k),
+                             grid_workspace_bytes(last, in->nlay, tile_last));
+  if (ws > h->clr_bytes) {
     for (double *&p : h->clr) {
       if (p) (void)hipFree(p);
       p = nullptr;
     }
     h->clr_bytes = 0;
     for (double *&p : h->clr)
-      if (hipMalloc((void **)&p, clr_workspace_bytes((int)K, in->nlay)) != hipSuccess) return MGCM_RT_ERR_NOMEM;
-    h->clr_bytes = clr_workspace_bytes((int)K, in->nlay);
+      if (hipMalloc((void **)&p, ws) != hipSuccess) return MGCM_RT_ERR_NOMEM;
+    h->clr_bytes = ws;
   }
 
   auto chunk_cols = [&](size_t i) { return std::min(K, ncol - i * K); };
@@ -341,7 +346,8 @@ int run_host(mgcm_rt_ctx *c, const mgcm_rt_columns *in, const mgcm_rt_fluxes *ou
     if (hipMemcpyAsync(r, h->pin_in[slot], k * in_pc * 8, hipMemcpyHostToDevice, h->up) != hipSuccess ||
         hipEventRecord(h->ev_up[slot], h->up) != hipSuccess || hipStreamWaitEvent(h->run[slot], h->ev_up[slot], 0) != hipSuccess)
       return MGCM_RT_ERR_DEVICE;
-    if (run_grid(c, &din, &dout, path, bands, h->clr[slot], h->run[slot]) != hipSuccess) return MGCM_RT_ERR_DEVICE;
+    if (run_grid(c, &din, &dout, path, bands, h->clr[slot], k == K ? tile_k : tile_last, h->run[slot]) != hipSuccess)
+      return MGCM_RT_ERR_DEVICE;
     return hipEventRecord(h->ev_run[i & 3], h->run[slot]) == hipSuccess ? MGCM_RT_OK : MGCM_RT_ERR_DEVICE;
   };
   // device -> pinned out-slot i % 2 (free: chunk i-2 has been drained), after

@@ -65,6 +65,7 @@ struct RunArgs {
   // DSOLFLUX's direct solar flux at the surface and DNVFLUX, [ncol], or null
   // (the batched path's DSOLFLUX kernel reads them here)
   double *directsol, *dnvflux;
+  double *part; // one-band-per-wave tile: each band's row sums, [ncol][NB][3*nl+4] (workspace)
 };
 
 struct DevTables {

@@ -57,8 +57,10 @@ extern "C" int mgcm_rt_run_noalloc(mgcm_rt_ctx *c, const mgcm_rt_columns *in, co
   s = decode_mode(precision, &path, &bands);
   if (s != MGCM_RT_OK) return s;
   if (in->ncol == 0) return MGCM_RT_OK;
-  if (clr_workspace_bytes(in->ncol, in->nlay) > NOALLOC_SCRATCH_BYTES) return MGCM_RT_ERR_SCRATCH;
+  const int tile = grid_tile(c, in->ncol, in->nlay, NOALLOC_SCRATCH_BYTES);
+  if (grid_workspace_bytes(in->ncol, in->nlay, tile) > NOALLOC_SCRATCH_BYTES) return MGCM_RT_ERR_SCRATCH;
   if (!c->d_scratch) return MGCM_RT_ERR_DEVICE;  // the address lookup at init failed
-  return run_grid(c, in, out, path, bands, c->d_scratch, (hipStream_t)stream) == hipSuccess ? MGCM_RT_OK
-                                                                                             : MGCM_RT_ERR_DEVICE;
+  return run_grid(c, in, out, path, bands, c->d_scratch, tile, (hipStream_t)stream) == hipSuccess
+             ? MGCM_RT_OK
+             : MGCM_RT_ERR_DEVICE;
 }

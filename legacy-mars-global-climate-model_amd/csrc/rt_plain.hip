@@ -6,6 +6,13 @@
 
 namespace mgcm {
 size_t clr_workspace_bytes(int ncol, int nlay) { return clr_bytes(ncol, nlay); }
+size_t grid_workspace_bytes(int ncol, int nlay, int tile) {
+  return clr_bytes(ncol, nlay) + (tile == TILE_BANDS ? part_bytes(ncol, nlay) : 0);
+}
+int grid_tile(const mgcm_rt_ctx *c, int ncol, int nlay, size_t max_ws) {
+  if (c->tile) return c->tile;
+  return auto_tile(ncol, nlay, c->ncu, clr_bytes(ncol, nlay) + part_bytes(ncol, nlay) <= max_ws);
+}
 
 hipError_t launch_plain(const DevTables &dt, const RunArgs &a, int tile, hipStream_t st) {
   return tile == 1 ? launch_column<0, 1>(dt, a, st) : launch_plain_tile(dt, a, tile, st);

@@ -8,6 +8,7 @@ hipError_t launch_mixed_tile(const DevTables &dt, const RunArgs &a, int tile, hi
   switch (tile) {
     case 2: return launch_column<P_TAB32, 2>(dt, a, st);
     case 4: return launch_column<P_TAB32, 4>(dt, a, st);
+    case TILE_BANDS: return launch_bands<P_TAB32>(dt, a, st);
     default: return hipErrorInvalidValue;
   }
 }

@@ -9,6 +9,7 @@ hipError_t launch_plain_tile(const DevTables &dt, const RunArgs &a, int tile, hi
   switch (tile) {
     case 2: return launch_column<0, 2>(dt, a, st);
     case 4: return launch_column<0, 4>(dt, a, st);
+    case TILE_BANDS: return launch_bands<0>(dt, a, st);
     default: return hipErrorInvalidValue;
   }
 }

@@ -8,7 +8,8 @@
 !                  <case file> <result file> [data directory | -] [history file]
 !
 ! --tile sets the column kernel's tile (mgcm_rt_set_tile: 1, 2 or 4 band
-! slots per column, 0 = the library's choice, the default).
+! slots per column, 12 = one band per wave, 0 = the library's choice, the
+! default).
 !
 ! --dust replaces the case file's TAUREF by the prescribed (Conrath) dust of
 ! DUSTPROFILE + FILLTAUCUM (mgcm_rt_dust_profile).  Dust file (FP64 stream):

@@ -129,7 +129,8 @@ module mgcm_rt_mod
       real(c_double), intent(in) :: sol(*), acosz(*), detau(*), diffvt(*)
       real(c_double), intent(out) :: directsol(*), dnvflux(*)
     end function
-    ! column kernel tile of single-grid calls: 1, 2 or 4 band slots per column, 0 = automatic
+    ! column kernel tile of single-grid calls: 1, 2 or 4 band slots per column, 12 = one band per wave,
+    ! 0 = automatic
     integer(c_int) function mgcm_rt_set_tile(ctx, band_slots) bind(C, name="mgcm_rt_set_tile")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

@@ -315,7 +315,8 @@ class Radiation:
 
     def set_tile(self, band_slots: int) -> None:
         """Column kernel tile of single-grid calls: 1, 2 or 4 band slots per
-        column, 0 = chosen per grid by the library (mgcm_rt_set_tile)."""
+        column, 12 = one band per wave, 0 = chosen per grid by the library
+        (mgcm_rt_set_tile)."""
         _check(self.lib.mgcm_rt_set_tile(self.ctx, int(band_slots)), "mgcm_rt_set_tile")
 
     def tile_for(self, ncol: int, nlay: int) -> int:

@@ -181,7 +181,7 @@ def test_fortran_tiles_equal(tmp_path):
     casefile.write_case(case, t, c, 0)
     outs = {}
     for mode in ("batch", "column"):
-        for tile in (1, 2, 4):
+        for tile in (1, 2, 4, 12):
             res = str(tmp_path / f"res_{mode}_{tile}.bin")
             args = [DRIVER, "--tile", str(tile)] + (["--per-column"] if mode == "column" else []) + [case, res]
             r = subprocess.run(args, capture_output=True, text=True, timeout=300)
@@ -189,7 +189,7 @@ def test_fortran_tiles_equal(tmp_path):
             status, outs[mode, tile] = casefile.read_result(res, 13, 24)
             assert status == 0
     for mode in ("batch", "column"):
-        for tile in (2, 4):
+        for tile in (2, 4, 12):
             for k in casefile.RESULT_KEYS:
                 if mode == "column" and k == "hr":
                     continue

@@ -1,6 +1,7 @@
 """The column kernel's tiles -- 1, 2 or 4 band slots per column, i.e. 4, 2 or 1
-columns per wave -- through the C ABI.  Every tile gives the same bits on every
-output: grids whose column count is no multiple of 4, the reference's edge
+columns per wave, and 12: one band per wave, 4 columns per wave, the bands'
+sums added by a finishing kernel -- through the C ABI.  Every tile gives the
+same bits on every output: grids whose column count is no multiple of 4, the reference's edge
 fixture (sun down and grazing, a TLIMIT-skipped band, thick dust, cloud), one
 part only (VIS or IR), both precisions, the smallest and largest layer
 counts, single-grid and batched calls; the oracle within 1e-10; the automatic
@@ -18,7 +19,8 @@ pytestmark = pytest.mark.gpu
 
 KEYS = ("fmnetv", "fluxupv", "fluxdnv", "fmneti", "fluxupi", "fluxdni", "nfluxtopv", "nfluxtopi",
         "diffvt", "detau", "cumtauv", "hr")
-TILES = (1, 2, 4)
+TILES = (1, 2, 4, 12)
+BATCH_TILES = (1, 2, 4)  # the batched path's tiles; 12 is for single-grid calls
 
 
 def _per_tile(rt, c, precision=0):
@@ -89,17 +91,20 @@ def test_auto_tile_choice():
     t, _ = synth.make_workload(314, 1, 24)
     rt = Radiation(t)
     assert rt.tile_for(64800, 24) == 1
-    assert rt.tile_for(540, 24) == 4
+    assert rt.tile_for(540, 24) == 12
+    assert rt.tile_for(2160, 24) == 12  # the reference's 36 x 60 grid
     picks = [rt.tile_for(n, 24) for n in (100, 540, 1080, 2160, 4320, 8100, 16200, 64800)]
     assert picks == sorted(picks, reverse=True), picks  # fewer slots as grids grow
     rt.set_tile(2)
     assert rt.tile_for(64800, 24) == 2
+    rt.set_tile(12)
+    assert rt.tile_for(64800, 24) == 12
     with pytest.raises(RadiationError) as e:
         rt.set_tile(3)
     assert e.value.status == -1
 
 
-@pytest.mark.parametrize("tile", TILES)
+@pytest.mark.parametrize("tile", BATCH_TILES)
 def test_batched_pinned_tile(tile):
     """The batched kernels of each tile (the tuner restricted to it) against
     single-grid runs, bit for bit, on a ragged batch with partial 4-column items."""
@@ -133,7 +138,7 @@ def _reference_built() -> bool:
 def test_reference_grid_every_tile_against_the_reference_fortran():
     """The reference's own 36 x 60 grid (2,160 columns, 24 layers, its Gauss weights) at
     every tile against the reference's Fortran on the same columns: rel 1e-10 everywhere,
-    and the three tiles bit-identical."""
+    and the tiles bit-identical."""
     from oracle import Reference
     from mgcm_rt import Radiation, synth
     t = synth.make_tables(36, 24, weights="reference")
@@ -141,7 +146,7 @@ def test_reference_grid_every_tile_against_the_reference_fortran():
     want = Reference().run_grid(t, c)
     rt = Radiation(t)
     outs = []
-    for tile in (1, 2, 4):
+    for tile in TILES:
         rt.set_tile(tile)
         got = rt.run_host(c)
         worst = max(rel_err(got[k], want[k]) for k in want if k in got)
@@ -149,4 +154,4 @@ def test_reference_grid_every_tile_against_the_reference_fortran():
         assert worst <= 1e-10, (tile, worst)
         outs.append(got)
     for k in outs[0]:
-        assert np.array_equal(outs[0][k], outs[1][k]) and np.array_equal(outs[0][k], outs[2][k]), k
+        assert all(np.array_equal(outs[0][k], o[k]) for o in outs[1:]), k
