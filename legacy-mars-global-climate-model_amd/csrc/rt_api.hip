@@ -65,6 +65,9 @@ extern "C" int mgcm_rt_init(const mgcm_rt_tables *t, mgcm_rt_ctx **out) {
   RtConst h;
   memset(&h, 0, sizeof(h));
   memcpy(h.pfgasref, t->pfgasref, sizeof(h.pfgasref));
+  h.pf_ascending = 1;
+  for (int n = 2; n < NP - 1; n++)
+    if (!(h.pfgasref[n - 1] <= h.pfgasref[n])) h.pf_ascending = 0;
   memcpy(h.tgasref, t->tgasref, sizeof(h.tgasref));
   memcpy(h.wrefh2o, t->wrefh2o, sizeof(h.wrefh2o));
   for (int b = 0; b < NB; b++)

@@ -83,6 +83,27 @@ __device__ inline Stage carve(double *p, int nsub, int nl, int nlev) {
   return S;
 }
 
+// A staging loop of a wave over t = lane, lane + 64, ... < n, four values of t
+// at a time: ld(t) reads the inputs of all four before use(t, inputs) takes
+// the first, so their memory latencies overlap.
+template <typename Load, typename Use>
+__device__ __forceinline__ void staged_loop(int lane, int n, Load &&ld, Use &&use) {
+  using In = decltype(ld(0));
+  for (int t0 = 0; t0 < n; t0 += 256) {
+    In v[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int t = t0 + lane + 64 * i;
+      if (t < n) v[i] = ld(t);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int t = t0 + lane + 64 * i;
+      if (t < n) use(t, v[i]);
+    }
+  }
+}
+
 // TPINDEX (code/tpindex.f90:72-132) of a sub-level without the four bilinear
 // products; they are formed where used, with the same expressions as
 // code/tpindex.f90:110-113.
@@ -98,9 +119,19 @@ static __device__ void tp_stage(const RtConst *__restrict__ rc, double pw, doubl
   }
   u = (tw - rc->tgasref[mt - 1]) / (rc->tgasref[mt] - rc->tgasref[mt - 1]);
   const double pwl = mgcm_libm::mgcm_log10(pw);
-  mp = NP - 1;
-  for (int n = 2; n <= NP - 1; n++) {
-    if (pwl <= rc->pfgasref[n - 1]) { mp = n - 1; break; }
+  if (rc->pf_ascending) {
+    // MP = N-1 for the first N = 2..L_PINT-1 with PWL <= PFGASREF(N), else
+    // L_PINT-1: on an ascending grid, one plus the number of N that fail the
+    // test (a NaN fails them all, as in the scan)
+    int fail = 0;
+#pragma unroll
+    for (int n = 2; n <= NP - 1; n++) fail += !(pwl <= rc->pfgasref[n - 1]);
+    mp = 1 + fail;
+  } else {
+    mp = NP - 1;
+    for (int n = 2; n <= NP - 1; n++) {
+      if (pwl <= rc->pfgasref[n - 1]) { mp = n - 1; break; }
+    }
   }
   t = (pwl - rc->pfgasref[mp - 1]) / (rc->pfgasref[mp] - rc->pfgasref[mp - 1]);
   nh = 1;
@@ -351,7 +382,7 @@ constexpr int TILE_BANDS = NB;
 // auto_tile's latency of a one-band wave and cost of the finishing launch, in
 // hundredths of an S = 1 wave's latency
 #ifndef MGCM_BANDS_LATENCY
-#define MGCM_BANDS_LATENCY 13
+#define MGCM_BANDS_LATENCY 12
 #endif
 #ifndef MGCM_BANDS_FINISH
 #define MGCM_BANDS_FINISH 4
@@ -494,32 +525,44 @@ __device__ __forceinline__ void column_block(const DevTables &dt, const RunArgs 
   Stage St = carve(smem, nsub, nl, nlev);
 
   // 1. TPINDEX of every sub-level K = 2..L_LEVELS of the C columns
-  for (int t = lane; t < C * nsub; t += 64) {
-    const int cc = t / nsub, K = t % nsub + 2;
-    const int cl = col0 + cc;
-    if (cl >= a.ncol) continue;
-    const size_t o = (size_t)cl * nlev + K - 1;
-    int mt, mp, nh;
-    double tt, uu, wr;
-    const double pm = a.pmid[o], tm = a.tmid[o], qw = a.qh2o[o], p0 = a.plev[o - 1], p1 = a.plev[o];
-    tp_stage(rc, pm, tm, qw, tt, uu, mt, mp, nh, wr);
-    St.t[t] = tt;
-    St.u[t] = uu;
-    St.wr[t] = wr;
-    St.uk[t] = CMK * (p1 - p0);
-    St.base[t] = kcell(0, mt, mp, nh) | (nh == NH ? 1 : 0);
-    // a NaN among the column's inputs sets the range flag: the values this
-    // sub-level reads, plus (once per column) TAUREF/TAUREFCLD at levels 1 and
-    // L_LEVELS+1 and the column scalars (TLEV reaches the flag through the
-    // Planck index)
-    const size_t oa = (size_t)cl * (nlev + 1) + K - 1;
-    bool nan = isnan(pm) || isnan(tm) || isnan(qw) || isnan(p0) || isnan(p1) || isnan(a.tauref[oa]) ||
-               isnan(a.taurefcld[oa]);
-    if (K == 2)
-      nan = nan || isnan(a.tauref[oa - 1]) || isnan(a.taurefcld[oa - 1]) || isnan(a.tauref[oa + nsub]) ||
-            isnan(a.taurefcld[oa + nsub]) || isnan(a.acosz[cl]) || isnan(a.als[cl]) || isnan(a.albi[cl]);
-    if (nan) atomicOr(a.flag, 1);
-  }
+  struct TpIn {
+    double pm, tm, qw, p0, p1, tr, tc;
+  };
+  staged_loop(
+      lane, C * nsub,
+      [&](int t) {
+        const int cl = col0 + t / nsub, K = t % nsub + 2;
+        TpIn v{};
+        if (cl < a.ncol) {
+          const size_t o = (size_t)cl * nlev + K - 1, oa = (size_t)cl * (nlev + 1) + K - 1;
+          v = TpIn{a.pmid[o], a.tmid[o], a.qh2o[o], a.plev[o - 1], a.plev[o], a.tauref[oa], a.taurefcld[oa]};
+        }
+        return v;
+      },
+      [&](int t, const TpIn &v) {
+        const int cl = col0 + t / nsub, K = t % nsub + 2;
+        if (cl >= a.ncol) return;
+        int mt, mp, nh;
+        double tt, uu, wr;
+        tp_stage(rc, v.pm, v.tm, v.qw, tt, uu, mt, mp, nh, wr);
+        St.t[t] = tt;
+        St.u[t] = uu;
+        St.wr[t] = wr;
+        St.uk[t] = CMK * (v.p1 - v.p0);
+        St.base[t] = kcell(0, mt, mp, nh) | (nh == NH ? 1 : 0);
+        // a NaN among the column's inputs sets the range flag: the values this
+        // sub-level reads, plus (once per column) TAUREF/TAUREFCLD at levels 1 and
+        // L_LEVELS+1 and the column scalars (TLEV reaches the flag through the
+        // Planck index)
+        bool nan = isnan(v.pm) || isnan(v.tm) || isnan(v.qw) || isnan(v.p0) || isnan(v.p1) || isnan(v.tr) ||
+                   isnan(v.tc);
+        if (K == 2) {
+          const size_t oa = (size_t)cl * (nlev + 1) + K - 1;
+          nan = nan || isnan(a.tauref[oa - 1]) || isnan(a.taurefcld[oa - 1]) || isnan(a.tauref[oa + nsub]) ||
+                isnan(a.taurefcld[oa + nsub]) || isnan(a.acosz[cl]) || isnan(a.als[cl]) || isnan(a.albi[cl]);
+        }
+        if (nan) atomicOr(a.flag, 1);
+      });
 
   Acc<SL> acc;
 #pragma unroll
@@ -556,15 +599,14 @@ __device__ __forceinline__ void column_block(const DevTables &dt, const RunArgs 
       const int rr = t / nsub, K = t % nsub + 2;
       const int cl = col0 + rr / S, bb = row_band(k, rr);
       if (cl >= a.ncol || bb < 0) continue;
-      const double *pl = a.plev + (size_t)cl * nlev;
-      const double *tr = a.tauref + (size_t)cl * (nlev + 1);
-      const double *tc = a.taurefcld + (size_t)cl * (nlev + 1);
+      const SubIn v = sub_inputs(a.plev + (size_t)cl * nlev, a.tauref + (size_t)cl * (nlev + 1),
+                                 a.taurefcld + (size_t)cl * (nlev + 1), K);
       const Aer ae = column_aerosol(a, rc, cl);
       if (VISR) {
-        const SubAer s = sub_aerosol<true>(ae, bb, pl, tr, tc, K);
+        const SubAer s = sub_aerosol_of<true>(ae, bb, K, v.pk, v.pk1, v.trk, v.tck);
         St.c1[t] = s.tray + s.taeros + s.tcloud;
       } else {
-        const SubAer s = sub_aerosol<false>(ae, bb, pl, tr, tc, K);
+        const SubAer s = sub_aerosol_of<false>(ae, bb, K, v.pk, v.pk1, v.trk, v.tck);
         St.c1[t] = s.taeros;
         St.c2[t] = s.tcloud;
       }
@@ -573,13 +615,12 @@ __device__ __forceinline__ void column_block(const DevTables &dt, const RunArgs 
       const int rr = t / nl, L = t % nl + 1;
       const int cl = col0 + rr / S, bb = row_band(k, rr);
       if (cl >= a.ncol || bb < 0) continue;
-      const double *pl = a.plev + (size_t)cl * nlev;
-      const double *tr = a.tauref + (size_t)cl * (nlev + 1);
-      const double *tc = a.taurefcld + (size_t)cl * (nlev + 1);
+      const LayerIn v = layer_inputs(a.plev + (size_t)cl * nlev, a.tauref + (size_t)cl * (nlev + 1),
+                                     a.taurefcld + (size_t)cl * (nlev + 1), nlev, L);
       const Aer ae = column_aerosol(a, rc, cl);
       double sc, cb;
-      if (VISR) layer_aerosol<true>(ae, bb, pl, tr, tc, nl, L, sc, cb);
-      else layer_aerosol<false>(ae, bb, pl, tr, tc, nl, L, sc, cb);
+      if (VISR) layer_aerosol<true>(ae, bb, nl, L, v, sc, cb);
+      else layer_aerosol<false>(ae, bb, nl, L, v, sc, cb);
       St.scat[t] = sc;
       St.cosb[t] = cb;
       St.cbar[t] = cb / (1.0 + cb);  // COSBAR of the delta-Eddington scaling (code/gfluxv.f90:86)

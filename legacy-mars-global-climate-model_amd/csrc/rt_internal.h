@@ -44,6 +44,7 @@ struct RtConst {
   double qextref_vis;  // QXVDST(K, L_NREFV): VIS dust reference extinction
   double qextref_dst;  // QEXTREFDST: IR dust reference extinction
   double qextref_cld;  // QEXTREFCLD
+  int pf_ascending;    // PFGASREF(2..L_PINT-1) does not decrease (TPINDEX's MP by counting)
 };
 
 // Per-run argument block (device pointers).

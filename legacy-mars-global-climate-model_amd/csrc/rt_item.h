@@ -118,11 +118,13 @@ __device__ __forceinline__ SubAer sub_aerosol_of(const Aer &ae, int b, int K, do
   s.tck = tc * ae.qs_cld(b, K);
   return s;
 }
-template <bool VIS>
-__device__ __forceinline__ SubAer sub_aerosol(const Aer &ae, int b, const double *__restrict__ plev,
-                                              const double *__restrict__ tauref,
-                                              const double *__restrict__ taurefcld, int K) {
-  return sub_aerosol_of<VIS>(ae, b, K, plev[K - 1], plev[K - 2], tauref[K - 1], taurefcld[K - 1]);
+// The profile values one sub-level's aerosol depths read.
+struct SubIn {
+  double pk, pk1, trk, tck;  // PLEV(K), PLEV(K-1), TAUREF(K), TAUREFCLD(K)
+};
+__device__ __forceinline__ SubIn sub_inputs(const double *__restrict__ plev, const double *__restrict__ tauref,
+                                            const double *__restrict__ taurefcld, int K) {
+  return SubIn{plev[K - 1], plev[K - 2], tauref[K - 1], taurefcld[K - 1]};
 }
 
 // IR padding level L_LEVELS+1 (code/optci.f90:89-95,182-187): TAUREF there is
@@ -162,22 +164,33 @@ __device__ __forceinline__ void layer_aerosol_of(const Aer &ae, int b, int nl, i
     cosb = (scat > 0.0) ? (gd * a.tlk + gd2 * tlk2 + gc * a.tck + gc2 * tck2) / scat : 0.0;
   }
 }
-// ... from the column's profiles
+// ... from the profile values of sub-levels K = 2L+1 and K+1 (LayerIn: PLEV(K-1),
+// PLEV(K), PLEV(K+1) where K < NLEV, TAUREF and TAUREFCLD at K and K+1; the IR
+// padding level's are those at K+1 = NLEV+1).
+struct LayerIn {
+  double p0, p1, p2, tr1, tc1, tr2, tc2;
+};
+__device__ __forceinline__ LayerIn layer_inputs(const double *__restrict__ plev, const double *__restrict__ tauref,
+                                                const double *__restrict__ taurefcld, int nlev, int L) {
+  const int K = 2 * L + 1;
+  return LayerIn{plev[K - 2], plev[K - 1], K < nlev ? plev[K] : 0.0, tauref[K - 1], taurefcld[K - 1], tauref[K],
+                 taurefcld[K]};
+}
 template <bool VIS>
-__device__ __forceinline__ void layer_aerosol(const Aer &ae, int b, const double *__restrict__ plev,
-                              const double *__restrict__ tauref, const double *__restrict__ taurefcld,
-                              int nl, int L, double &scat, double &cosb) {
+__device__ __forceinline__ void layer_aerosol(const Aer &ae, int b, int nl, int L, const LayerIn &v, double &scat,
+                                              double &cosb) {
   const int nlev = 2 * nl + 1;
   const int K = 2 * L + 1;
-  const SubAer a = sub_aerosol<VIS>(ae, b, plev, tauref, taurefcld, K);
+  const SubAer a = sub_aerosol_of<VIS>(ae, b, K, v.p1, v.p0, v.tr1, v.tc1);
   double tlk2 = 0.0, tck2 = 0.0, tray2 = 0.0;
   if (K + 1 <= nlev) {  // L < NL
-    const SubAer a2 = sub_aerosol<VIS>(ae, b, plev, tauref, taurefcld, K + 1);
+    const SubAer a2 = sub_aerosol_of<VIS>(ae, b, K + 1, v.p2, v.p1, v.tr2, v.tc2);
     tlk2 = a2.tlk;
     tck2 = a2.tck;
     tray2 = a2.tray;
-  } else if (!VIS) {
-    ir_pad(ae, b, tauref, taurefcld, nlev, tlk2, tck2);
+  } else if (!VIS) {  // ir_pad with TAUREF(NLEV+1), TAUREFCLD(NLEV+1)
+    tlk2 = v.tr2 * ae.qs_dst(b, nlev + 1);
+    tck2 = v.tc2 * ae.qs_cld(b, nlev + 1);
   }
   layer_aerosol_of<VIS>(ae, b, nl, L, a, tlk2, tck2, tray2, scat, cosb);
 }

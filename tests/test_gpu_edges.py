@@ -48,6 +48,37 @@ def test_layer_and_column_extremes(oracle, nlay, ncol):
         assert rel_err(got[k], want[k]) <= 1e-10, (nlay, ncol, k)
 
 
+def test_tpindex_pressure_cells_on_grid_points(oracle):
+    """TPINDEX's pressure cell (code/tpindex.f90:93-106) where a sub-level's log10(PMID)
+    equals a grid value: PFGASREF moved onto the log10 (host libm, which the device's
+    equals bit for bit) of sub-level pressures of the grid, ascending -- the kernel
+    counts the points the pressure fails -- and with the point after one of them set a
+    hair below it, so the grid no longer ascends and the kernel scans as the reference
+    does.  Against the oracle's scan at rel 1e-10."""
+    from mgcm_rt import Radiation, synth
+    t, c = synth.make_workload(75, 64, 24)
+    pf = np.array(t["pfgasref"], dtype=np.float64)
+    _, lg, _ = oracle.libm(np.ascontiguousarray(c["pmid"][:, 1:]).ravel())
+    lg = np.unique(lg)
+    placed = []
+    for i in range(1, len(pf) - 1):  # PFGASREF(2..L_PINT-1), the scanned points
+        inside = lg[(lg > pf[i - 1]) & (lg < pf[i + 1])]
+        if inside.size:
+            pf[i] = inside[np.argmin(np.abs(inside - pf[i]))]
+            placed.append(i)
+    assert len(placed) >= 5 and np.all(np.diff(pf) > 0)
+    i = next(j for j in placed[len(placed) // 2:] if j + 1 < len(pf) - 1)
+    down = pf.copy()
+    down[i + 1] = pf[i] - 2.0 ** -40
+    for label, grid in (("ascending", pf), ("not ascending", down)):
+        tt = dict(t, pfgasref=grid)
+        got = Radiation(tt).run_host(c)
+        want = oracle.run(tt, c)
+        print(label, "max rel", f"{max(rel_err(got[k], want[k]) for k in KEYS):.1e}", flush=True)
+        for k in KEYS:
+            assert rel_err(got[k], want[k]) <= 1e-10, (label, k)
+
+
 def test_empty_grid():
     from mgcm_rt import Radiation, synth
     t, c = synth.make_workload(70, 3, 24)
