@@ -900,18 +900,26 @@ void k_rt(DevTables dt, RunArgs a) {
   column_block<NLM, P, S, state_ahead(column_waves(NLM))>(dt, a, blockIdx.x * Tile<S>::C, smem);
 }
 
-// The one-band-per-wave tile: block b solves band round b / ngroups (at one
-// band slot per column, round k is band k) of columns 4 (b % ngroups) ..
-// +3 and writes the band's records; k_rt_finish then adds them per column.
-// Blocks of one band are adjacent, so the waves in flight share few k-table
-// slices.
+// The one-band-per-wave tile, in one launch with the continuum points (the
+// two are independent until k_rt_finish reads both): blocks below nclear are
+// continuum-point blocks (VIS below nvis, as k_rt_clear2); block nclear + w
+// solves band round w / ngroups (at one band slot per column, round k is band
+// k) of columns 4 (w % ngroups) .. +3 and writes the band's records;
+// k_rt_finish then adds them per column.  Blocks of one band are adjacent, so
+// the waves in flight share few k-table slices.
 template <int NLM, int P>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(column_waves(NLM), column_waves(NLM))))
-void k_rt_band(DevTables dt, RunArgs a, int ngroups) {
+void k_rt_band(DevTables dt, RunArgs a, int ngroups, int nvis, int nclear) {
   extern __shared__ double smem[];
   mgcm_libm::load_lds_exptab(threadIdx.x, 64);
   __syncthreads();
-  const int k = (int)blockIdx.x / ngroups, grp = (int)blockIdx.x % ngroups;
+  const int blk = (int)blockIdx.x;
+  if (blk < nclear) {
+    if (blk < nvis) clear_block<NLM, true>(dt, a, blk * ClearShape<true>::cols);
+    else clear_block<NLM, false>(dt, a, (blk - nvis) * ClearShape<false>::cols);
+    return;
+  }
+  const int w = blk - nclear, k = w / ngroups, grp = w % ngroups;
   column_block<NLM, P, 1, state_ahead(column_waves(NLM)), true>(dt, a, grp * 4, smem, k);
 }
 template <int NLM>
@@ -989,16 +997,17 @@ hipError_t launch_column(const DevTables &dt, const RunArgs &a, hipStream_t st) 
   return launch_nlm<MAX_NLAY + 1, P, S>(dt, a, st);
 }
 
-// One grid at one band per wave (TILE_BANDS): the continuum-point kernels,
-// NB waves per 4 columns, then the finishing kernel.  a.clr must hold
+// One grid at one band per wave (TILE_BANDS): the continuum points and NB
+// waves per 4 columns in one launch, then the finishing kernel.  a.clr must hold
 // clr_bytes(a.ncol, a.nlay), a.part part_bytes(a.ncol, a.nlay).
 template <int NLM, int P>
 static hipError_t launch_bands_nlm(const DevTables &dt, const RunArgs &a, hipStream_t st) {
-  launch_clear<NLM>(dt, a, st);
   const size_t lds = stage_bytes(a.nlay);
   const int ngroups = (a.ncol + 3) / 4;
-  hipLaunchKernelGGL((k_rt_band<NLM, P>), dim3((unsigned)ngroups * (Tile<1>::RV + Tile<1>::RI)), dim3(64), lds, st,
-                     dt, a, ngroups);
+  const int nvis = (a.bands & 1) ? (a.ncol + ClearShape<true>::cols - 1) / ClearShape<true>::cols : 0;
+  const int nir = (a.bands & 2) ? (a.ncol + ClearShape<false>::cols - 1) / ClearShape<false>::cols : 0;
+  hipLaunchKernelGGL((k_rt_band<NLM, P>), dim3((unsigned)(nvis + nir + ngroups * (Tile<1>::RV + Tile<1>::RI))),
+                     dim3(64), lds, st, dt, a, ngroups, nvis, nvis + nir);
   hipLaunchKernelGGL((k_rt_finish<NLM>), dim3((unsigned)ngroups), dim3(64), lds, st, dt, a);
   return hipGetLastError();
 }
