@@ -5,7 +5,8 @@
 // One wave (64 lanes, four 16-lane rows) owns C = 4/S columns for the whole
 // radiation call, S in {1, 2, 4} band slots per column (the tile: C columns x
 // S bands x 16 Gauss points; S = 1 for large grids, more slots give small
-// grids more waves):
+// grids more waves; the smallest grids run each band of 4 columns in a wave of
+// its own, TILE_BANDS, and add the bands in a finishing kernel):
 //   1. stages the C columns' TPINDEX results (band independent) in LDS;
 //   2. in rounds of S bands (the 7 VIS bands first, then the 5 IR bands; a
 //      round is all VIS or all IR): row (c, s) stages band round*S+s of
@@ -704,8 +705,9 @@ __device__ __forceinline__ void column_block(const DevTables &dt, const RunArgs 
   }
   __syncthreads();
   if constexpr (ONE) {
-    // the band's record: sums added to zero, so the finishing kernel's sum in
-    // band order repeats the additions above bit for bit
+    // the band's record: its sums added to zero (0 + v is v, or +0 for v = -0),
+    // so the finishing kernel's sums in band order, which start from +0 and so
+    // are never -0, equal the one-wave kernel's bit for bit
     const int b = TL::band(kr, 0);
     if (valid) {
       double *r = a.part + ((size_t)col * NB + b) * part_rec(nl);
