@@ -175,13 +175,13 @@ int mgcm_rt_run(mgcm_rt_ctx *ctx, const mgcm_rt_columns *in, const mgcm_rt_fluxe
 /* mgcm_rt_run for callers that cannot allocate device memory (code under HIP
  * graph capture, tight serving loops, harnesses that forbid hipMalloc): the
  * same kernels and results bit for bit, with the continuum-point records (and
- * at tile 12 the band records) in a static device array of
- * MGCM_RT_NOALLOC_SCRATCH_BYTES (128 MiB, held by every device the library
+ * at tile 12 the band records and TPINDEX results) in a static device array
+ * of MGCM_RT_NOALLOC_SCRATCH_BYTES (128 MiB, held by every device the library
  * runs on until the process ends) instead of the context's workspace.  Nothing
  * is allocated, ever.  A grid needing more than the array -- ncol * 12 *
- * (2*nlay+4) * 8 bytes, plus ncol * 12 * (3*nlay+7) * 8 at tile 12 (the
- * automatic choice takes tile 12 only where it fits): 26,886 columns at 24
- * layers, 3,461 at 99 -- returns MGCM_RT_ERR_SCRATCH; run it in column
+ * (2*nlay+4) * 8 bytes, plus ncol * (12 (3*nlay+7) + 5 (2*nlay+2)) * 8 at
+ * tile 12 (the automatic choice takes tile 12 only where it fits): 26,886
+ * columns at 24 layers, 3,461 at 99 -- returns MGCM_RT_ERR_SCRATCH; run it in column
  * slices (every array is per column, so a slice is a pointer offset).  All
  * calls on one device share the array: calls that overlap on a device (any
  * context, any stream) race, so issue them on one stream or synchronise
@@ -245,7 +245,8 @@ int mgcm_rt_batched_tuning(const mgcm_rt_ctx *ctx, int maxcls, mgcm_rt_batch_cla
  * column per 16-lane row at a time), 2 (2 columns, 2 bands at a time), 4
  * (1 column, 4 bands at a time) or 12 (all 12 bands of a column at once, each
  * in its own wave of 4 columns; a finishing kernel adds the bands' sums, and
- * the workspace grows by 8 (3 (nlay+1) + 4) bytes per band and column); 0
+ * the workspace grows by (12 (3 nlay + 7) + 5 (2 nlay + 2)) * 8 bytes per
+ * column); 0
  * (the default) lets the library choose per grid from its column count,
  * layer count and the device's CU count.  Every tile gives the same bits;
  * more slots give a small grid more waves at some extra work per column

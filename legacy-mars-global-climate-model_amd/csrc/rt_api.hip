@@ -180,6 +180,7 @@ void fill_args(RunArgs &a, const mgcm_rt_columns *in, const mgcm_rt_fluxes *out,
   a.bands = 3;
   a.clr = nullptr;
   a.part = nullptr;
+  a.tp = nullptr;
   const mgcm_rt_aerosol *ae = in->aerosol;
   const double *aq[12] = {nullptr};
   if (ae) {
@@ -203,6 +204,7 @@ hipError_t run_grid(const mgcm_rt_ctx *c, const mgcm_rt_columns *in, const mgcm_
   a.bands = bands;
   a.clr = ws;
   a.part = tile == TILE_BANDS ? ws + clr_workspace_bytes(in->ncol, in->nlay) / sizeof(double) : nullptr;
+  a.tp = a.part ? a.part + (size_t)in->ncol * NB * part_rec(in->nlay + 1) : nullptr;
   const int p = path & 0xf;
   hipError_t e = (p == MGCM_RT_FP64)    ? launch_plain(c->dt, a, tile, st)
                  : (p == MGCM_RT_MIXED) ? launch_mixed(c->dt, a, tile, st)

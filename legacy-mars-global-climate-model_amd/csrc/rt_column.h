@@ -64,7 +64,13 @@ inline size_t clr_bytes(int ncol, int nlay) { return (size_t)ncol * NB * clr_rec
 // DIFFVT part and its TLIMIT skip mask; written by the band's wave, added in
 // band order by the finishing kernel.
 __host__ __device__ inline int part_rec(int nl) { return 3 * nl + 4; }
-inline size_t part_bytes(int ncol, int nlay) { return (size_t)ncol * NB * part_rec(nlay + 1) * sizeof(double); }
+// ... and TPINDEX's results in RunArgs::tp, computed once per column for the 12
+// band waves: per column T, U, WRATIO, Cmk*dp and the cell offset (as a double),
+// each for the sub-levels 2..L_LEVELS.
+constexpr int TP_FIELDS = 5;
+inline size_t part_bytes(int ncol, int nlay) {
+  return (size_t)ncol * (NB * part_rec(nlay + 1) + TP_FIELDS * (2 * nlay + 2)) * sizeof(double);
+}
 
 __device__ inline Stage carve(double *p, int nsub, int nl, int nlev) {
   p += mgcm_libm::LDS_EXPTAB_BYTES / sizeof(double);
@@ -496,6 +502,40 @@ __device__ __forceinline__ void column_tail(const DevTables &dt, const RunArgs &
   }
 }
 
+// TPINDEX of one sub-level from the column's profiles: the inputs ...
+struct TpIn {
+  double pm, tm, qw, p0, p1, tr, tc;  // PMID, TMID, QH2O, PLEV(K-1), PLEV(K), TAUREF, TAUREFCLD at K
+};
+__device__ __forceinline__ TpIn tp_inputs(const RunArgs &a, int cl, int K, int nlev) {
+  const size_t o = (size_t)cl * nlev + K - 1, oa = (size_t)cl * (nlev + 1) + K - 1;
+  return TpIn{a.pmid[o], a.tmid[o], a.qh2o[o], a.plev[o - 1], a.plev[o], a.tauref[oa], a.taurefcld[oa]};
+}
+// ... and the results: T and U weights, WRATIO, Cmk*dp and the table cell
+// offset (bit 0 set when NH2O = L_REFH2O).  A NaN among the column's inputs
+// sets the range flag: the values this sub-level reads, plus (once per column)
+// TAUREF/TAUREFCLD at levels 1 and L_LEVELS+1 and the column scalars (TLEV
+// reaches the flag through the Planck index).
+struct TpOut {
+  double t, u, wr, uk;
+  int base;
+};
+__device__ __forceinline__ TpOut tp_sublevel(const RtConst *rc, const RunArgs &a, int cl, int K, int nlev,
+                                             const TpIn &v) {
+  TpOut r;
+  int mt, mp, nh;
+  tp_stage(rc, v.pm, v.tm, v.qw, r.t, r.u, mt, mp, nh, r.wr);
+  r.uk = CMK * (v.p1 - v.p0);
+  r.base = kcell(0, mt, mp, nh) | (nh == NH ? 1 : 0);
+  bool nan = isnan(v.pm) || isnan(v.tm) || isnan(v.qw) || isnan(v.p0) || isnan(v.p1) || isnan(v.tr) || isnan(v.tc);
+  if (K == 2) {
+    const size_t oa = (size_t)cl * (nlev + 1) + K - 1, nsub = nlev - 1;
+    nan = nan || isnan(a.tauref[oa - 1]) || isnan(a.taurefcld[oa - 1]) || isnan(a.tauref[oa + nsub]) ||
+          isnan(a.taurefcld[oa + nsub]) || isnan(a.acosz[cl]) || isnan(a.als[cl]) || isnan(a.albi[cl]);
+  }
+  if (nan) atomicOr(a.flag, 1);
+  return r;
+}
+
 // P: the precision policy (P_* bits; 0 = plain FP64, P_TAB32 = the mixed path,
 // whose k-tables are read in FP32 -- half the gather bytes and corner
 // registers -- with all arithmetic, the interpolation included, in FP64).
@@ -525,45 +565,50 @@ __device__ __forceinline__ void column_block(const DevTables &dt, const RunArgs 
   const RtConst *rc = dt.c;
   Stage St = carve(smem, nsub, nl, nlev);
 
-  // 1. TPINDEX of every sub-level K = 2..L_LEVELS of the C columns
-  struct TpIn {
-    double pm, tm, qw, p0, p1, tr, tc;
-  };
-  staged_loop(
-      lane, C * nsub,
-      [&](int t) {
-        const int cl = col0 + t / nsub, K = t % nsub + 2;
-        TpIn v{};
-        if (cl < a.ncol) {
-          const size_t o = (size_t)cl * nlev + K - 1, oa = (size_t)cl * (nlev + 1) + K - 1;
-          v = TpIn{a.pmid[o], a.tmid[o], a.qh2o[o], a.plev[o - 1], a.plev[o], a.tauref[oa], a.taurefcld[oa]};
-        }
-        return v;
-      },
-      [&](int t, const TpIn &v) {
-        const int cl = col0 + t / nsub, K = t % nsub + 2;
-        if (cl >= a.ncol) return;
-        int mt, mp, nh;
-        double tt, uu, wr;
-        tp_stage(rc, v.pm, v.tm, v.qw, tt, uu, mt, mp, nh, wr);
-        St.t[t] = tt;
-        St.u[t] = uu;
-        St.wr[t] = wr;
-        St.uk[t] = CMK * (v.p1 - v.p0);
-        St.base[t] = kcell(0, mt, mp, nh) | (nh == NH ? 1 : 0);
-        // a NaN among the column's inputs sets the range flag: the values this
-        // sub-level reads, plus (once per column) TAUREF/TAUREFCLD at levels 1 and
-        // L_LEVELS+1 and the column scalars (TLEV reaches the flag through the
-        // Planck index)
-        bool nan = isnan(v.pm) || isnan(v.tm) || isnan(v.qw) || isnan(v.p0) || isnan(v.p1) || isnan(v.tr) ||
-                   isnan(v.tc);
-        if (K == 2) {
-          const size_t oa = (size_t)cl * (nlev + 1) + K - 1;
-          nan = nan || isnan(a.tauref[oa - 1]) || isnan(a.taurefcld[oa - 1]) || isnan(a.tauref[oa + nsub]) ||
-                isnan(a.taurefcld[oa + nsub]) || isnan(a.acosz[cl]) || isnan(a.als[cl]) || isnan(a.albi[cl]);
-        }
-        if (nan) atomicOr(a.flag, 1);
-      });
+  // 1. TPINDEX of every sub-level K = 2..L_LEVELS of the C columns (one band
+  //    per wave: read from the results k_tp wrote once per column)
+  if constexpr (ONE) {
+    struct TpRes {
+      double v[TP_FIELDS];
+    };
+    staged_loop(
+        lane, C * nsub,
+        [&](int t) {
+          const int cl = col0 + t / nsub, i = t % nsub;
+          TpRes r{};
+          if (cl < a.ncol) {
+            const double *q = a.tp + (size_t)cl * TP_FIELDS * nsub + i;
+#pragma unroll
+            for (int f = 0; f < TP_FIELDS; f++) r.v[f] = q[f * nsub];
+          }
+          return r;
+        },
+        [&](int t, const TpRes &r) {
+          if (col0 + t / nsub >= a.ncol) return;
+          St.t[t] = r.v[0];
+          St.u[t] = r.v[1];
+          St.wr[t] = r.v[2];
+          St.uk[t] = r.v[3];
+          St.base[t] = (int)r.v[4];
+        });
+  } else {
+    staged_loop(
+        lane, C * nsub,
+        [&](int t) {
+          const int cl = col0 + t / nsub, K = t % nsub + 2;
+          return cl < a.ncol ? tp_inputs(a, cl, K, nlev) : TpIn{};
+        },
+        [&](int t, const TpIn &v) {
+          const int cl = col0 + t / nsub, K = t % nsub + 2;
+          if (cl >= a.ncol) return;
+          const TpOut r = tp_sublevel(rc, a, cl, K, nlev, v);
+          St.t[t] = r.t;
+          St.u[t] = r.u;
+          St.wr[t] = r.wr;
+          St.uk[t] = r.uk;
+          St.base[t] = r.base;
+        });
+  }
 
   Acc<SL> acc;
 #pragma unroll
@@ -902,6 +947,22 @@ void k_rt(DevTables dt, RunArgs a) {
   column_block<NLM, P, S, state_ahead(column_waves(NLM))>(dt, a, blockIdx.x * Tile<S>::C, smem);
 }
 
+// TPINDEX of every sub-level of a grid for the one-band-per-wave tile: one
+// thread per (column, sub-level), results to a.tp.
+static __global__ __launch_bounds__(256) void k_tp(DevTables dt, RunArgs a) {
+  const int nlev = 2 * a.nlay + 3, nsub = nlev - 1;
+  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (long long)a.ncol * nsub) return;
+  const int cl = (int)(idx / nsub), i = (int)(idx % nsub), K = i + 2;
+  const TpOut r = tp_sublevel(dt.c, a, cl, K, nlev, tp_inputs(a, cl, K, nlev));
+  double *q = a.tp + (size_t)cl * TP_FIELDS * nsub + i;  // field f of the sub-level at q[f * nsub]
+  q[0] = r.t;
+  q[nsub] = r.u;
+  q[2 * nsub] = r.wr;
+  q[3 * nsub] = r.uk;
+  q[4 * nsub] = (double)r.base;
+}
+
 // The one-band-per-wave tile, in one launch with the continuum points (the
 // two are independent until k_rt_finish reads both): blocks below nclear are
 // continuum-point blocks (VIS below nvis, as k_rt_clear2); block nclear + w
@@ -999,13 +1060,16 @@ hipError_t launch_column(const DevTables &dt, const RunArgs &a, hipStream_t st) 
   return launch_nlm<MAX_NLAY + 1, P, S>(dt, a, st);
 }
 
-// One grid at one band per wave (TILE_BANDS): the continuum points and NB
-// waves per 4 columns in one launch, then the finishing kernel.  a.clr must hold
-// clr_bytes(a.ncol, a.nlay), a.part part_bytes(a.ncol, a.nlay).
+// One grid at one band per wave (TILE_BANDS): TPINDEX of every sub-level
+// (k_tp), then the continuum points and NB waves per 4 columns in one launch,
+// then the finishing kernel.  a.clr must hold clr_bytes(a.ncol, a.nlay), a.part (with
+// a.tp after the band records) part_bytes(a.ncol, a.nlay).
 template <int NLM, int P>
 static hipError_t launch_bands_nlm(const DevTables &dt, const RunArgs &a, hipStream_t st) {
   const size_t lds = stage_bytes(a.nlay);
   const int ngroups = (a.ncol + 3) / 4;
+  const long long ntp = (long long)a.ncol * (2 * a.nlay + 2);
+  hipLaunchKernelGGL(k_tp, dim3((unsigned)((ntp + 255) / 256)), dim3(256), 0, st, dt, a);
   const int nvis = (a.bands & 1) ? (a.ncol + ClearShape<true>::cols - 1) / ClearShape<true>::cols : 0;
   const int nir = (a.bands & 2) ? (a.ncol + ClearShape<false>::cols - 1) / ClearShape<false>::cols : 0;
   hipLaunchKernelGGL((k_rt_band<NLM, P>), dim3((unsigned)(nvis + nir + ngroups * (Tile<1>::RV + Tile<1>::RI))),

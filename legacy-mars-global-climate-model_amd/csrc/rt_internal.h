@@ -67,6 +67,7 @@ struct RunArgs {
   // (the batched path's DSOLFLUX kernel reads them here)
   double *directsol, *dnvflux;
   double *part; // one-band-per-wave tile: each band's row sums, [ncol][NB][3*nl+4] (workspace)
+  double *tp;   // ... and TPINDEX's results, [ncol][5][2*nl] (workspace)
 };
 
 struct DevTables {
