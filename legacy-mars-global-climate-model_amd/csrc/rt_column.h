@@ -231,18 +231,16 @@ struct GasSrc {
 // TLIMIT test of a gas point (code/sfluxv.f90:72, code/sfluxi.f90:72).  The
 // mixed path's sum over FP32 table values is within a relative 1e-6 of the
 // FP64-table one; closer than 1e-4 to the limit it is recomputed from the FP64
-// table (all lanes of the wave take part, the others keep their decision), so
-// both paths skip the same Gauss points.
+// table by the lanes that are that close (only active lanes: the staged cell
+// offsets of an inactive one, past the grid's last column, are never written),
+// so both paths skip the same Gauss points.
 template <int P, class Src>
 __device__ __forceinline__ bool below_tlimit(const Src &src, const double *tab64, int nlev, double tlim,
                                              bool active) {
   double tg = src.taugsurf;
   if (P & (P_TAB32 | P_INTERP32)) {
     const bool near = active && fabs(tg - tlim) <= 1.0e-4 * tlim;
-    if (__any(near)) {
-      const double t64 = src.taugsurf_fp64(tab64, nlev);
-      if (near) tg = t64;
-    }
+    if (near) tg = src.taugsurf_fp64(tab64, nlev);
   }
   return tg < tlim;
 }
