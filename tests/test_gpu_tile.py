@@ -128,6 +128,28 @@ def test_batched_pinned_tile(tile):
             assert np.array_equal(o[k].cpu().numpy(), single[k]), (tile, n, nlay, k)
 
 
+def test_batched_with_one_band_tile_pinned():
+    """Pinning the one-band tile (single-grid calls only) leaves the batched tuner its own
+    choice among its tiles; the batch still equals the single-grid runs bit for bit."""
+    import torch
+    from mgcm_rt import Radiation, alloc_outputs, synth, to_device
+    shapes = [(37, 16), (101, 24), (23, 36)]
+    t = synth.make_tables(316, 24)
+    cols = [synth.make_columns(3160 + i, n, nlay) for i, (n, nlay) in enumerate(shapes)]
+    rt = Radiation(t)
+    rt.set_tile(12)
+    dc = [to_device(c) for c in cols]
+    do = [to_device(alloc_outputs(n, nlay)) for n, nlay in shapes]
+    rt.run_batched(dc, do, 0, torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    rt.check()
+    assert all(c["tile"] in BATCH_TILES for c in rt.batched_tuning())
+    for (n, nlay), c, o in zip(shapes, cols, do):
+        single = rt.run_host(c)  # at tile 12
+        for k in KEYS:
+            assert np.array_equal(o[k].cpu().numpy(), single[k]), (n, nlay, k)
+
+
 def _reference_built() -> bool:
     import os
     from oracle import REF_SO
