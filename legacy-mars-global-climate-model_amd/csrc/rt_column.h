@@ -24,8 +24,9 @@
 //   4. writes the fluxes and forms SUNTOT/IRTOTAL and the layer heating
 //      (code/comp3.f:677-681,735-738).
 // Between the stages, only the continuum-point records (written by the
-// continuum-point kernel, read in stage 3) go through global memory; the
-// per-item tridiagonal state lives in private (scratch) memory.
+// continuum-point kernel, read in stage 3) go through global memory -- and, at
+// one band per wave, the TPINDEX results and the band records; the per-item
+// tridiagonal state lives in private (scratch) memory.
 #pragma once
 #include <type_traits>
 
