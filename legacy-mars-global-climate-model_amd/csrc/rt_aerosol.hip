@@ -31,6 +31,9 @@ constexpr int OFF_DI = OFF_DV + 3 * NV * NBIN;               // dust IR (3, 5, 2
 constexpr int OFF_CV = OFF_DI + 3 * NI * NBIN;               // cloud VIS (3, 7, 20, 15)
 constexpr int OFF_CI = OFF_CV + 3 * NV * NBIN * NRATIO;      // cloud IR (3, 5, 20, 15)
 constexpr int TABLE_DOUBLES = OFF_CI + 3 * NI * NBIN * NRATIO;
+// The device copy keeps the block's offsets, but each cloud table is held per
+// core ratio: ratio j's (3, nb, 20) slice, laid out as the dust tables are, at
+// OFF_C? + j * 3 * nb * 20, so that a layer's 20 bins of a band are adjacent.
 
 // code/modules.f90:1112-1114 (REAL*4)
 __constant__ float cor_ratio[NRATIO] = {0.1f, 0.2f, 0.25f, 0.3f, 0.35f, 0.4f, 0.45f, 0.5f,
@@ -125,9 +128,10 @@ __global__ __launch_bounds__(AER_BLOCK) void k_opt_aer(const double *__restrict_
         surf[i] = 0.5 * (e1 - e0);
         e0 = e1;
       }
-      const double *tv = tab + (CLOUD ? OFF_CV : OFF_DV), *ti = tab + (CLOUD ? OFF_CI : OFF_DI);
-      const int sv = CLOUD ? NV * NBIN * NRATIO : NV * NBIN, si = CLOUD ? NI * NBIN * NRATIO : NI * NBIN;
-      auto idx = [&](int w, int i) { return CLOUD ? (irap - 1) + NRATIO * (i + NBIN * w) : i + NBIN * w; };
+      const double *tv = tab + (CLOUD ? OFF_CV + (irap - 1) * 3 * NV * NBIN : OFF_DV);
+      const double *ti = tab + (CLOUD ? OFF_CI + (irap - 1) * 3 * NI * NBIN : OFF_DI);
+      constexpr int sv = NV * NBIN, si = NI * NBIN;
+      auto idx = [&](int w, int i) { return i + NBIN * w; };
       double qx6 = 0.0, qxi4 = 0.0, qsi4 = 0.0;
       for (int w = 0; w < NV; w++) {
         double x = 0., sc = 0., g = 0.;
@@ -271,7 +275,17 @@ extern "C" int mgcm_rt_set_aerosol_tables(mgcm_rt_ctx *c, const double *tables) 
     c->d_aer = nullptr;
     return MGCM_RT_ERR_NOMEM;
   }
-  if (hipMemcpy(c->d_aer, tables, sizeof(double) * TABLE_DOUBLES, hipMemcpyHostToDevice) != hipSuccess)
+  // the cloud tables per core ratio on the device (see OFF_CV)
+  std::vector<double> t(tables, tables + TABLE_DOUBLES);
+  for (int j = 0; j < NRATIO; j++)
+    for (int kind = 0; kind < 3; kind++)
+      for (int i = 0; i < NBIN; i++) {
+        for (int w = 0; w < NV; w++)
+          t[OFF_CV + j * 3 * NV * NBIN + i + NBIN * (w + NV * kind)] = tables[OFF_CV + j + NRATIO * (i + NBIN * (w + NV * kind))];
+        for (int w = 0; w < NI; w++)
+          t[OFF_CI + j * 3 * NI * NBIN + i + NBIN * (w + NI * kind)] = tables[OFF_CI + j + NRATIO * (i + NBIN * (w + NI * kind))];
+      }
+  if (hipMemcpy(c->d_aer, t.data(), sizeof(double) * TABLE_DOUBLES, hipMemcpyHostToDevice) != hipSuccess)
     return MGCM_RT_ERR_DEVICE;
   return MGCM_RT_OK;
 }
