@@ -388,9 +388,12 @@ extern "C" int mgcm_rt_opt_cld(mgcm_rt_ctx *c, const mgcm_rt_tracers *in, const 
 
 // ---------------------------------------------------------------------------
 // Prescribed (Conrath) dust: DUSTPROFILE's column profile (code/dustprofile.f:
-// 56-113) and FILLTAUCUM (code/filltaucum.f:33-52), one thread per column.
-// Every operation is IEEE or the glibc-exact exp, so TAUREF equals the
-// reference's bit for bit.
+// 56-113) and FILLTAUCUM (code/filltaucum.f:33-52), one 64-lane wave per
+// column.  The exponentials of the Conrath integral and of the mixing ratios,
+// and the per-level interpolations, are formed across the lanes; the two
+// running sums (the integral and TAUDST) are added by lane 0 in the
+// reference's order.  Every operation is IEEE or the glibc-exact exp, so TAUREF
+// equals the reference's bit for bit.
 namespace mgcm {
 namespace aer {
 constexpr int NPDST_MAX = 128;
@@ -410,54 +413,112 @@ __device__ int jsrchgt(int n, const double *sx, double target) {  // code/jsrchg
 }
 
 __global__ __launch_bounds__(64) void k_dust_tauref(mgcm_rt_dust_columns in, double *tauref_all, double *taudst_out) {
-  const int col = blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= in.ncol) return;
+  __shared__ double s_pr[NPDST_MAX], s_t[NPDST_MAX + 8], s_tau[NPDST_MAX + 8], s_tc[2 * MAX_NLAY + 4];
+  const int col = blockIdx.x, lane = threadIdx.x;
   const int npdst = in.npdst, nlevels = 2 * in.nlay + 3;
-  const double *prdst = in.prdst, rptau = in.rptau, conr = in.conr[col];
-  double qrdst[NPDST_MAX], taudst[NPDST_MAX];
-  double sum = 0., pave;
-  int n;
-  for (n = 2; n <= npdst; n++) {
-    if (prdst[n - 1] < rptau) {
-      pave = 0.5 * (prdst[n - 1] + prdst[n - 2]);
-      sum = sum + mgcm_libm::mgcm_exp(conr * (1. - (rptau / pave))) * (prdst[n - 1] - prdst[n - 2]);
+  const double rptau = in.rptau, conr = in.conr[col];
+  for (int n = lane; n < npdst; n += 64) s_pr[n] = in.prdst[n];
+  __syncthreads();
+  // The integral's loop ends at the first n (2..npdst) with PRDST(n) >= RPTAU, or after npdst.
+  int nb = npdst + 1;
+  for (int n0 = 2; n0 <= npdst; n0 += 64) {
+    const int n = n0 + lane;
+    const unsigned long long m = __ballot(n <= npdst && s_pr[n - 1] >= rptau);
+    if (m) {
+      nb = n0 + __builtin_ctzll(m);
+      break;
     }
-    if (prdst[n - 1] >= rptau) break;
   }
-  pave = 0.5 * (rptau + prdst[n - 2]);
-  sum = sum + mgcm_libm::mgcm_exp(conr * (1. - (rptau / pave))) * (rptau - prdst[n - 2]);
-  const double qrdst0 = in.tautot[col] / sum;
-  for (n = 1; n <= npdst - 1; n++) {
-    if (rptau > prdst[n]) {  // region 1
-      pave = 0.5 * (prdst[n] + prdst[n - 1]);
-      qrdst[n - 1] = qrdst0 * mgcm_libm::mgcm_exp(conr * (1.0 - (rptau / pave)));
+  // E(m) = exp(CONR (1 - RPTAU / PAVE)), PAVE = (PRDST(m) + PRDST(m+1)) / 2 (1-based), for m = 1..npdst-1 with
+  // PRDST(m+1) < RPTAU: the integral's term for n = m + 1 and QRDST(m)'s region 1 use the same exponential.
+  // The lane with m = nb - 1 forms the integral's last term, whose exponential is also that of QRDST(m)'s
+  // region 2 there (PAVE is the same sum, commuted).  The terms n = 2..nb-1 go to s_t[n - 2], padded with
+  // +0 to a multiple of 8: a term the reference skips is +0 here too, which leaves the sum (+0 at the
+  // start, so never -0) unchanged.
+  __shared__ double s_fin;
+  const int nt = nb - 2, ntp = (nt + 7) & ~7;
+  double ex[2], x2[2];
+#pragma unroll
+  for (int c = 0; c < 2; c++) {
+    const int m = 1 + lane + 64 * c;
+    const bool below = m <= npdst - 1 && s_pr[m] < rptau;
+    ex[c] = 0.;
+    x2[c] = 0.;
+    if (below) {
+      const double pave = 0.5 * (s_pr[m] + s_pr[m - 1]);
+      ex[c] = mgcm_libm::mgcm_exp(conr * (1. - (rptau / pave)));
     }
-    if (rptau <= prdst[n] && rptau >= prdst[n - 1]) {  // region 2
-      const double pdif1 = rptau - prdst[n - 1], pdif2 = prdst[n] - rptau;
-      pave = 0.5 * (prdst[n - 1] + rptau);
-      qrdst[n - 1] = qrdst0 * (mgcm_libm::mgcm_exp(conr * (1.0 - (rptau / pave))) * pdif1 + pdif2) /
-                     (prdst[n] - prdst[n - 1]);
+    const bool r2 = m <= npdst - 1 && rptau <= s_pr[m] && rptau >= s_pr[m - 1];
+    if (r2 || m == nb - 1) {
+      const double pave = 0.5 * (s_pr[m - 1] + rptau), pdif1 = rptau - s_pr[m - 1];
+      const double e2 = mgcm_libm::mgcm_exp(conr * (1.0 - (rptau / pave)));
+      if (r2) x2[c] = e2 * pdif1 + (s_pr[m] - rptau);
+      if (m == nb - 1) s_fin = e2 * pdif1;
     }
-    if (rptau < prdst[n - 1]) qrdst[n - 1] = qrdst0;  // region 3
+    if (m - 1 < ntp) s_t[m - 1] = m - 1 < nt && below ? ex[c] * (s_pr[m] - s_pr[m - 1]) : 0.;
   }
-  taudst[0] = 0.0;
-  for (n = 2; n <= npdst; n++) taudst[n - 1] = taudst[n - 2] + qrdst[n - 2] * (prdst[n - 1] - prdst[n - 2]);
+  __syncthreads();
+  if (lane == 0) {
+    double sum = 0.;
+    for (int i = 0; i < ntp; i += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) v[u] = s_t[i + u];
+#pragma unroll
+      for (int u = 0; u < 8; u++) sum = sum + v[u];
+    }
+    sum = sum + s_fin;
+    s_tc[0] = in.tautot[col] / sum;
+  }
+  __syncthreads();
+  const double qrdst0 = s_tc[0];
+  // QRDST(n) in its three regions, then its TAUDST increment QRDST(n) (PRDST(n+1) - PRDST(n)) at s_t[n - 1],
+  // padded with +0 as above.
+  const int np1 = npdst - 1, np1p = (np1 + 7) & ~7;
+#pragma unroll
+  for (int c = 0; c < 2; c++) {
+    const int n = 1 + lane + 64 * c;
+    if (n - 1 < np1p) {
+      double dt = 0.;
+      if (n <= npdst - 1) {
+        double q = 0.;
+        if (rptau > s_pr[n]) q = qrdst0 * ex[c];
+        if (rptau <= s_pr[n] && rptau >= s_pr[n - 1]) q = qrdst0 * x2[c] / (s_pr[n] - s_pr[n - 1]);
+        if (rptau < s_pr[n - 1]) q = qrdst0;
+        dt = q * (s_pr[n] - s_pr[n - 1]);
+      }
+      s_t[n - 1] = dt;
+    }
+  }
+  __syncthreads();
+  if (lane == 0) {
+    double tau = 0.0;
+    s_tau[0] = tau;
+    for (int i = 0; i < np1p; i += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) v[u] = s_t[i + u];
+#pragma unroll
+      for (int u = 0; u < 8; u++) s_tau[i + u + 1] = tau = tau + v[u];
+    }
+  }
+  __syncthreads();
   if (taudst_out)
-    for (n = 0; n < npdst; n++) taudst_out[(size_t)col * npdst + n] = taudst[n];
-  double *tauref = tauref_all + (size_t)col * (nlevels + 1);
-  double prev = 0.0;  // TAUCUM(K-1); TAUCUM(1..3) = 0
-  for (int k = 1; k <= 3; k++) tauref[k - 1] = 0.0;
-  for (int k = 4; k <= nlevels; k++) {
-    const double pstar = in.p[col] * in.sigma[k - 1] + in.ptrop;
-    const double pstar1 = fmax(pstar, prdst[0]);
-    int nstar = jsrchgt(npdst - 1, prdst, pstar1) - 1;
+    for (int n = lane; n < npdst; n += 64) taudst_out[(size_t)col * npdst + n] = s_tau[n];
+  // FILLTAUCUM: TAUCUM at each level k = 4..nlevels by interpolation in TAUDST, then its differences.
+  const double pcol = in.p[col];
+  for (int k = 4 + lane; k <= nlevels; k += 64) {
+    const double pstar = pcol * in.sigma[k - 1] + in.ptrop;
+    const double pstar1 = fmax(pstar, s_pr[0]);
+    int nstar = jsrchgt(npdst - 1, s_pr, pstar1) - 1;
     if (nstar > npdst - 1) nstar = npdst - 1;
-    const double tc = taudst[nstar - 1] + (pstar1 - prdst[nstar - 1]) * (taudst[nstar] - taudst[nstar - 1]) /
-                                              (prdst[nstar] - prdst[nstar - 1]);
-    tauref[k - 1] = tc - prev;
-    prev = tc;
+    s_tc[k] = s_tau[nstar - 1] + (pstar1 - s_pr[nstar - 1]) * (s_tau[nstar] - s_tau[nstar - 1]) /
+                                     (s_pr[nstar] - s_pr[nstar - 1]);
   }
-  tauref[nlevels] = 0.0;
+  __syncthreads();
+  double *tauref = tauref_all + (size_t)col * (nlevels + 1);
+  for (int k = 1 + lane; k <= nlevels + 1; k += 64)
+    tauref[k - 1] = k < 4 || k > nlevels ? 0.0 : s_tc[k] - (k == 4 ? 0.0 : s_tc[k - 1]);
 }
 }  // namespace aer
 }  // namespace mgcm
@@ -476,7 +537,7 @@ extern "C" int mgcm_rt_dust_tauref(const mgcm_rt_dust_columns *in, double *taure
   if (in->nlay < 1 || in->nlay > MAX_NLAY) return MGCM_RT_ERR_NLAY;
   if (in->ncol == 0) return MGCM_RT_OK;
   if (!in->tautot || !in->conr || !in->p || !in->sigma || !in->prdst) return MGCM_RT_ERR_ARG;
-  hipLaunchKernelGGL(k_dust_tauref, dim3((in->ncol + 63) / 64), dim3(64), 0, (hipStream_t)stream, *in, tauref,
+  hipLaunchKernelGGL(k_dust_tauref, dim3(in->ncol), dim3(64), 0, (hipStream_t)stream, *in, tauref,
                      taudst);
   return hipGetLastError() == hipSuccess ? MGCM_RT_OK : MGCM_RT_ERR_DEVICE;
 }

@@ -23,7 +23,8 @@ def test_device_dust_matches_reference_fixture():
         assert np.array_equal(o["taudst"], fx[f"case{ci}_taudst"]), ci
 
 
-@pytest.mark.parametrize("nlay,ncol,seed", [(24, 4096, 61), (36, 2048, 62), (1, 64, 63), (99, 256, 64)])
+@pytest.mark.parametrize("nlay,ncol,seed", [(24, 4096, 61), (36, 2048, 62), (1, 64, 63), (99, 256, 64), (24, 1003, 65),
+                                             (2, 1, 66)])
 def test_device_dust_matches_oracle(oracle, nlay, ncol, seed):
     from mgcm_rt import dust, synth
     rng = np.random.default_rng(seed)
